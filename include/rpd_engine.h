@@ -1,0 +1,191 @@
+/*
+ * rpd_engine.h - C ABI of the MI355X path-trace -> RPF grid -> StatMC ->
+ * adaptive-pass engine.
+ *
+ * The reference renderer (andersonliu1/rpd-statmc-denoiser) has no library
+ * interface: its hot path is the function seam
+ *     render(const RenderConfig&, const Camera&)      path_tracer/main.cpp:1543
+ * over process globals (main.cpp:38-43), followed by
+ *     output_buffers(const FrameBuffers&, ...)        path_tracer/core/buffers.h:197
+ * Each entry point below names the reference function it stands in for.
+ *
+ * Conventions: every function returns an rpd_status (0 = ok, negative = error)
+ * and never throws; rpd_last_error() returns a thread-local message for the
+ * last failure on the calling thread. All pointers are host pointers owned by
+ * the caller; the engine copies what it needs. No GPU-runtime types appear.
+ */
+#ifndef RPD_ENGINE_H
+#define RPD_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rpd_status {
+    RPD_OK = 0,
+    RPD_ERR_INVALID_ARG = -1,
+    RPD_ERR_OUT_OF_MEMORY = -2, /* device allocation failed or admission refused */
+    RPD_ERR_DEVICE = -3,        /* HIP runtime error */
+    RPD_ERR_STATE = -4,         /* call not valid in the object's current state */
+    RPD_ERR_UNSUPPORTED = -5
+} rpd_status;
+
+typedef struct rpd_context rpd_context;
+typedef struct rpd_scene rpd_scene;
+typedef struct rpd_job rpd_job;
+
+/* ---- scene description (path_tracer/core/{triangle,sphere,material,light}.h) ---- */
+enum { RPD_MAT_LAMBERTIAN = 0, RPD_MAT_MICROFACET = 1, RPD_MAT_MIRROR = 2 };
+enum { RPD_MICROFACET_GGX = 0, RPD_MICROFACET_BECKMANN = 1 };
+enum { RPD_LIGHT_RECT = 0, RPD_LIGHT_AREA = 1, RPD_LIGHT_POINT = 2, RPD_LIGHT_DIRECTIONAL = 3 };
+
+typedef struct rpd_triangle { /* core/triangle.h:6-11 */
+    float v0[3], v1[3], v2[3];
+    float normal[3];          /* shading normal as stored by the scene (may be flipped) */
+    float emission[3];
+    int material_id;
+    int light_id;             /* light this emitter is linked to, -1 if none (core/scene.h:41-43) */
+} rpd_triangle;
+
+typedef struct rpd_sphere { /* core/sphere.h:6-11 */
+    float center[3];
+    float radius;
+    float emission[3];
+    int material_id;
+    int light_id;
+} rpd_sphere;
+
+typedef struct rpd_material { /* core/material.h:9-240 */
+    int kind;                 /* RPD_MAT_* */
+    float albedo[3];
+    float roughness;          /* microfacet only */
+    float n1[3], n2[3];       /* microfacet Schlick IORs */
+    int distribution;         /* RPD_MICROFACET_* */
+} rpd_material;
+
+typedef struct rpd_light { /* core/light.h:16-150 */
+    int kind;                 /* RPD_LIGHT_* */
+    float p0[3];              /* rect: corner; area: v0; point: position; directional: direction */
+    float p1[3];              /* rect: width edge; area: v1 */
+    float p2[3];              /* rect: length edge; area: v2 */
+    float normal[3];          /* area: triangle normal (rect computes its own) */
+    float emission[3];        /* radiance, or intensity for point/directional */
+} rpd_light;
+
+/* ---- render configuration: POD mirror of RenderConfig (core/config.h:15-49) ---- */
+typedef struct rpd_render_config {
+    int image_width, image_height, samples_per_pixel;
+    int use_statmc;
+    int rpf_tile_size;
+    int color_window_radius;
+    float color_normal_threshold, color_depth_threshold, color_compat_alpha, color_sigma_max;
+    int var_window_radius;
+    float var_normal_threshold, var_depth_threshold, var_compat_sigma, var_shrinkage_k;
+    int var_iterations;
+    int adaptive_base_samples, adaptive_spp;
+    float adaptive_sigma_max;
+    int adaptive_passes, adaptive_importance_smoothing_radius;
+    float rpf_shrinkage_scale, rpf_confidence_samples;
+} rpd_render_config;
+
+/* Fills `cfg` with the reference defaults (core/config.h:27-48). */
+void rpd_render_config_defaults(rpd_render_config* cfg);
+
+typedef struct rpd_camera { /* inputs of Camera::init, core/camera.h:31-54 */
+    float position[3], direction[3], world_up[3];
+    float fov_degrees;
+    float focus_distance;
+    float aperture;           /* lens diameter; 0 = pinhole */
+} rpd_camera;
+
+/* ---- context ---- */
+/* One context per device; thread-safe for job submission and polling. */
+int rpd_ctx_create(int device, rpd_context** out);
+int rpd_ctx_destroy(rpd_context* ctx);
+const char* rpd_last_error(void);
+/* Device memory the context may hand to jobs (admission budget), bytes. */
+int rpd_ctx_set_memory_budget(rpd_context* ctx, uint64_t bytes);
+
+/* ---- scenes: scenes::load_scene (core/scene_loader.h:11-13) + BVH build (main.cpp:1855) ---- */
+int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, const rpd_sphere* spheres,
+                     int n_spheres, const rpd_material* materials, int n_materials, const rpd_light* lights,
+                     int n_lights, const float environment[3], rpd_scene** out);
+/* Built-in scenes by name (scenes/scene_registry.h:18): "cornell_box". */
+int rpd_scene_builtin(rpd_context* ctx, const char* name, rpd_scene** out);
+int rpd_scene_counts(const rpd_scene* scene, int* n_tris, int* n_spheres, int* n_materials, int* n_lights);
+int rpd_scene_destroy(rpd_scene* scene);
+
+/* ---- whole jobs: render() main.cpp:1543-1564 ---- */
+/* Queues the full pipeline (base pass, adaptive passes, final StatMC) on the
+   job's own stream and returns immediately. */
+int rpd_job_submit(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam,
+                   uint32_t seed, rpd_job** out);
+int rpd_job_poll(rpd_job* job, int* done);
+int rpd_job_wait(rpd_job* job);
+int rpd_job_release(rpd_job* job);
+
+/* ---- step-wise jobs: the reference's stage functions on one job's state ---- */
+/* Allocates a job's device state (buffers.init + pixel_stats + rpf_grid.init,
+   main.cpp:651-654) without queuing work. */
+int rpd_job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam,
+                   uint32_t seed, rpd_job** out);
+/* One sampling pass (main.cpp:649-735 and 1451-1535). counts == NULL samples
+   every pixel samples_per_pixel times; otherwise counts[i] times. statmc != 0
+   seeds each pixel with its finite-sample count, folds Welford statistics and
+   splats RPF samples; statmc == 0 is the uniform render (main.cpp:590-642). */
+int rpd_job_run_pass(rpd_job* job, const int* counts, int statmc);
+/* Adaptive pass: allocation from the current state, then the pass (main.cpp:1451-1538). */
+int rpd_job_run_adaptive_pass(rpd_job* job);
+int rpd_job_recompute_sensitivity(rpd_job* job);    /* main.cpp:741 */
+int rpd_job_statmc_denoise(rpd_job* job);           /* main.cpp:957 */
+int rpd_job_variance_denoise(rpd_job* job);         /* main.cpp:1166 */
+int rpd_job_adaptive_counts(rpd_job* job, int* counts_out); /* main.cpp:1273 */
+int rpd_job_finalize(rpd_job* job);                 /* FrameBuffers::finalize, core/buffers.h:152 */
+int rpd_job_sync(rpd_job* job);
+
+/* ---- buffers (core/buffers.h:197-250 output set and internal state) ---- */
+enum rpd_buffer_id {
+    /* images: float RGB interleaved per pixel */
+    RPD_BUF_RADIANCE = 0,          /* final image: denoised for StatMC, mean radiance otherwise */
+    RPD_BUF_RAW_RADIANCE = 1,      /* StatMC only: mean radiance before reconstruction */
+    RPD_BUF_ALBEDO = 2, RPD_BUF_NORMAL = 3, RPD_BUF_WORLD_POS = 4,
+    RPD_BUF_DENOISED = 5,          /* latest statmc_denoise output */
+    /* scalar float images */
+    RPD_BUF_DEPTH = 10, RPD_BUF_UNCERTAINTY = 11, RPD_BUF_ALPHA = 12, RPD_BUF_VARIANCE_DENOISED = 13,
+    RPD_BUF_VAR_TOTAL = 14, RPD_BUF_VAR_EFF = 15, RPD_BUF_VAR_RATIO = 16,
+    RPD_BUF_SENSITIVITY_CONFIDENCE = 17, RPD_BUF_SENSITIVITY_GRADIENT = 18, RPD_BUF_ADAPTIVE_IMPORTANCE = 19,
+    RPD_BUF_SAMPLE_FRACTION = 20,  /* sample_count / max(sample_count) */
+    /* expanded sensitivity fields (SensitivityTiles::expand_values) */
+    RPD_BUF_SENS_ALL = 30, RPD_BUF_SENS_PIXEL, RPD_BUF_SENS_BRDF, RPD_BUF_SENS_LENS, RPD_BUF_SENS_LIGHT,
+    RPD_BUF_SENS_LIGHT_UV, RPD_BUF_SENS_LIGHT_SELECT, RPD_BUF_SENS_ENVIRONMENT, RPD_BUF_SENS_RR,
+    /* int32 per pixel */
+    RPD_BUF_SAMPLE_COUNT = 50, RPD_BUF_HIT_COUNT = 51, RPD_BUF_PASS_COUNTS = 52,
+    /* raw state, readable and writable */
+    RPD_BUF_STATS = 60,            /* 12 floats/pixel: lum mean, lum M2, n (int bits), color mean[3], sqrt mean[3], sqrt M2[3] */
+    RPD_BUF_SUMS = 61,             /* 13 floats/pixel: radiance[3], albedo[3], normal[3], world_pos[3], depth sums */
+    RPD_BUF_GRID = 62,             /* 440 doubles/node (RPF grid accumulators) */
+    RPD_BUF_TILES = 63,            /* 18 planes of node floats (SensitivityTiles fields) */
+    RPD_BUF_FEATURES = 64,         /* 7 floats/pixel: unit normal[3], depth, albedo[3] */
+    RPD_BUF_RECORDS = 65           /* last pass: 32 floats per sample (read only) */
+};
+/* Size in bytes of a buffer for this job (RECORDS: of the last pass). */
+int rpd_job_buffer_size(rpd_job* job, int buffer_id, size_t* bytes);
+int rpd_job_read(rpd_job* job, int buffer_id, void* dst, size_t bytes);
+int rpd_job_write(rpd_job* job, int buffer_id, const void* src, size_t bytes);
+/* Grid geometry: nodes_x, nodes_y, step, support (core/stats.h:312-318). */
+int rpd_job_grid_dims(rpd_job* job, int dims[4]);
+
+/* Stage timings in ms, measured with HIP events on the job's stream:
+   [0] base render, [1] adaptive renders, [2] sensitivity, [3] StatMC,
+   [4] variance smoothing, [5] allocation, [6] whole job,
+   [7] trace kernel only (all passes), [8] number of trace launches,
+   [9] StatMC window kernel only, [10] grid gather kernels only. */
+int rpd_job_stage_ms(rpd_job* job, double* ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RPD_ENGINE_H */

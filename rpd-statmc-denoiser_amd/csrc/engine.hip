@@ -1,0 +1,1091 @@
+// Host side of the engine: contexts, scenes, jobs and the pass pipeline of
+// render() (path_tracer/main.cpp:1543-1564) driven on one HIP stream per job.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/rpd_engine.h"
+#include "kernels.h"
+
+using namespace rpd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(RPD_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---- host float3 with the reference's evaluation order ----------------------
+struct h3 { float x, y, z; };
+h3 H3(float a, float b, float c) { return {a, b, c}; }
+h3 operator+(h3 a, h3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+h3 operator-(h3 a, h3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+h3 operator-(h3 a) { return {-a.x, -a.y, -a.z}; }
+h3 operator*(float s, h3 a) { return {s * a.x, s * a.y, s * a.z}; }
+h3 operator*(h3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+h3 operator/(h3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+float hdot(h3 a, h3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }
+float hsq(h3 a) { return hdot(a, a); }
+h3 hnorm(h3 a) {
+    const float z = hsq(a);
+    if (z > 0.0f) { const float s = std::sqrt(z); return {a.x / s, a.y / s, a.z / s}; }
+    return a;
+}
+h3 hcross(h3 a, h3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+void put3(float* d, h3 v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; }
+h3 get3(const float* s) { return {s[0], s[1], s[2]}; }
+float lum(h3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+template <class T>
+int dalloc(T** p, size_t count, std::vector<void*>& owned) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    void* q = nullptr;
+    const hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) return fail(RPD_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    owned.push_back(q);
+    *p = static_cast<T*>(q);
+    return RPD_OK;
+}
+
+}  // namespace
+
+struct rpd_context {
+    int device = 0;
+    int num_cus = 0;
+    uint64_t budget = 0;
+    std::atomic<uint64_t> in_use{0};
+    std::mutex mu;
+};
+
+struct rpd_scene {
+    rpd_context* ctx = nullptr;
+    std::vector<void*> owned;
+    DevScene dev{};
+    int n_tris = 0, n_spheres = 0, n_materials = 0, n_lights = 0;
+};
+
+struct rpd_job {
+    rpd_context* ctx = nullptr;
+    rpd_scene* scene = nullptr;
+    rpd_render_config cfg{};
+    rpd::Camera cam{};
+    uint32_t seed = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> owned;
+    uint64_t bytes = 0;
+    int W = 0, H = 0, N = 0;
+    GridGeom geom{};
+    // device state
+    PixelBuffers pb{};
+    Features feat{};
+    float *den = nullptr, *unc = nullptr, *alp = nullptr, *vtot = nullptr, *veff = nullptr, *vrat = nullptr;
+    float *vden = nullptr, *vtmp = nullptr, *imp = nullptr, *sconf = nullptr, *sgrad = nullptr;
+    float *out_rad = nullptr, *out_raw = nullptr, *out_alb = nullptr, *out_nrm = nullptr, *out_pos = nullptr,
+          *out_dep = nullptr;
+    int *counts = nullptr, *offsets = nullptr, *active = nullptr, *totals = nullptr, *work = nullptr;
+    int64_t* scan_scratch = nullptr;
+    uint32_t* tape = nullptr;
+    Records rec{};
+    double* grid = nullptr;
+    float *tiles = nullptr, *node_scratch = nullptr, *filter_scratch = nullptr;
+    void* alloc_scratch = nullptr;
+    bool features_valid = false;
+    bool finalized = false;
+    bool last_pass_statmc = false;
+    int last_pass_capture = 0;
+    // timing
+    struct Span { int stage; hipEvent_t a, b; };
+    std::vector<Span> spans;
+    double stage_ms[11] = {0};
+    // async full-pipeline run
+    std::thread worker;
+    std::atomic<int> done{0};
+    int status = RPD_OK;
+    std::string error;
+};
+
+namespace {
+
+// ---- stage timing -----------------------------------------------------------
+struct StageTimer {
+    rpd_job* j;
+    int stage;
+    hipEvent_t a = nullptr;
+    StageTimer(rpd_job* job, int s) : j(job), stage(s) {
+        if (hipEventCreate(&a) == hipSuccess) hipEventRecord(a, j->stream);
+    }
+    ~StageTimer() {
+        hipEvent_t b = nullptr;
+        if (a && hipEventCreate(&b) == hipSuccess) {
+            hipEventRecord(b, j->stream);
+            j->spans.push_back({stage, a, b});
+        }
+    }
+};
+void collect_spans(rpd_job* j) {
+    for (auto& s : j->spans) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(s.b) == hipSuccess && hipEventElapsedTime(&ms, s.a, s.b) == hipSuccess) {
+            j->stage_ms[s.stage] += ms;
+            if (s.stage == 7) j->stage_ms[8] += 1.0;
+        }
+        hipEventDestroy(s.a);
+        hipEventDestroy(s.b);
+    }
+    j->spans.clear();
+}
+
+// ---- config validation: core/config.h:460-539 -------------------------------
+int validate(const rpd_render_config& c) {
+    auto bad = [](const char* m) { return fail(RPD_ERR_INVALID_ARG, m); };
+    if (c.image_width <= 0 || c.image_height <= 0) return bad("image_width and image_height must be positive");
+    if ((int64_t)c.image_width * c.image_height > (1 << 26)) return bad("image too large (more than 2^26 pixels)");
+    if (c.samples_per_pixel <= 0) return bad("samples_per_pixel must be positive");
+    if (c.rpf_tile_size <= 0) return bad("rpf_tile_size must be positive");
+    if (c.rpf_tile_size > 33) return bad("rpf_tile_size above 33 is not supported (grid step must be <= 16)");
+    if (c.color_window_radius <= 0) return bad("color_window_radius must be positive");
+    if (!std::isfinite(c.color_normal_threshold) || c.color_normal_threshold <= 0.0f || c.color_normal_threshold > 1.0f)
+        return bad("color_normal_threshold must be finite and in (0,1]");
+    if (!std::isfinite(c.color_depth_threshold) || c.color_depth_threshold < 0.0f)
+        return bad("color_depth_threshold must be finite and non-negative");
+    if (!std::isfinite(c.color_compat_alpha) || c.color_compat_alpha <= 0.0f || c.color_compat_alpha >= 1.0f)
+        return bad("color_compat_alpha must be finite and in (0,1)");
+    if (!std::isfinite(c.color_sigma_max) || c.color_sigma_max <= 0.0f) return bad("color_sigma_max must be finite and positive");
+    if (c.var_window_radius <= 0) return bad("var_window_radius must be positive");
+    if (!std::isfinite(c.var_normal_threshold) || c.var_normal_threshold <= 0.0f || c.var_normal_threshold > 1.0f)
+        return bad("var_normal_threshold must be finite and in (0,1]");
+    if (!std::isfinite(c.var_depth_threshold) || c.var_depth_threshold < 0.0f)
+        return bad("var_depth_threshold must be finite and non-negative");
+    if (!std::isfinite(c.var_compat_sigma) || c.var_compat_sigma <= 0.0f) return bad("var_compat_sigma must be finite and positive");
+    if (!std::isfinite(c.var_shrinkage_k) || c.var_shrinkage_k <= 0.0f) return bad("var_shrinkage_k must be finite and positive");
+    if (c.adaptive_base_samples < 0) return bad("adaptive_base_samples must be non-negative");
+    if (c.adaptive_spp < 0) return bad("adaptive_spp must be non-negative");
+    if (c.adaptive_base_samples > c.adaptive_spp) return bad("adaptive_base_samples cannot exceed adaptive_spp");
+    if (!std::isfinite(c.adaptive_sigma_max) || c.adaptive_sigma_max <= 0.0f) return bad("adaptive_sigma_max must be finite and positive");
+    if (c.adaptive_passes < 0) return bad("adaptive_passes must be non-negative");
+    if (c.adaptive_importance_smoothing_radius < 0) return bad("adaptive_importance_smoothing_radius must be non-negative");
+    if (c.var_iterations <= 0) return bad("var_iterations must be positive");
+    if (!std::isfinite(c.rpf_shrinkage_scale) || c.rpf_shrinkage_scale < 0.0f) return bad("rpf_shrinkage_scale must be finite and non-negative");
+    if (!std::isfinite(c.rpf_confidence_samples) || c.rpf_confidence_samples <= 0.0f) return bad("rpf_confidence_samples must be finite and positive");
+    return RPD_OK;
+}
+
+// ---- camera: core/camera.h:31-88 ---------------------------------------------
+int make_camera(const rpd_camera& in, float aspect, rpd::Camera& out) {
+    const h3 dir = get3(in.direction), up_in = get3(in.world_up);
+    if (!std::isfinite(in.fov_degrees) || in.fov_degrees <= 0.0f || in.fov_degrees >= 180.0f)
+        return fail(RPD_ERR_INVALID_ARG, "fov must be finite and in (0,180)");
+    if (!(hsq(dir) > 1e-8f) || !(hsq(up_in) > 1e-8f))
+        return fail(RPD_ERR_INVALID_ARG, "Camera vectors must be finite and direction/up must be nonzero");
+    if (!std::isfinite(in.focus_distance) || in.focus_distance <= 0.0f)
+        return fail(RPD_ERR_INVALID_ARG, "focus_distance must be finite and positive");
+    if (!std::isfinite(in.aperture) || in.aperture < 0.0f)
+        return fail(RPD_ERR_INVALID_ARG, "aperture must be finite and non-negative");
+    const h3 fwd = hnorm(dir);
+    h3 up_hint = hnorm(up_in);
+    h3 right = hcross(fwd, up_hint);
+    if (hsq(right) < 1e-8f) {
+        h3 alt = (std::fabs(fwd.z) < 0.999f) ? H3(0, 0, 1) : H3(0, 1, 0);
+        const h3 d = alt - up_hint;
+        if (hsq(d) <= 1e-8f * 1e-8f * std::min(hsq(alt), hsq(up_hint))) alt = H3(1, 0, 0);
+        up_hint = alt;
+        right = hcross(fwd, up_hint);
+    }
+    const h3 r = hnorm(right);
+    const h3 u = hnorm(hcross(r, fwd));
+    const float theta = float(double(in.fov_degrees) * 3.14159265358979323846 / double(180.0f));
+    const float hh = std::tan(0.5f * theta);
+    const float hw = aspect * hh;
+    const float fd = in.focus_distance;
+    const h3 horiz = (2.0f * fd * hw) * r;
+    const h3 vert = (2.0f * fd * hh) * u;
+    const h3 pos = get3(in.position);
+    const h3 llc = pos + fwd * fd - 0.5f * horiz - 0.5f * vert;
+    put3(out.pos, pos); put3(out.right, r); put3(out.up, u);
+    put3(out.llc, llc); put3(out.horiz, horiz); put3(out.vert, vert);
+    out.lens_radius = 0.5f * in.aperture;
+    return RPD_OK;
+}
+
+// ---- built-in cornell_box: scenes/cornell_box.h:9-109 --------------------------
+struct HostScene {
+    std::vector<rpd_triangle> tris;
+    std::vector<rpd_material> mats;
+    std::vector<rpd_light> lights;
+    float env[3] = {0, 0, 0};
+};
+void build_cornell(HostScene& s) {
+    auto lam = [&](float r, float g, float b) {
+        rpd_material m{};
+        m.kind = RPD_MAT_LAMBERTIAN; m.albedo[0] = r; m.albedo[1] = g; m.albedo[2] = b;
+        m.roughness = 0.5f;
+        s.mats.push_back(m);
+        return int(s.mats.size()) - 1;
+    };
+    const int white = lam(0.78f, 0.78f, 0.78f), red = lam(0.63f, 0.065f, 0.05f), green = lam(0.14f, 0.45f, 0.091f);
+    const int light_mat = lam(1.0f, 1.0f, 1.0f), box_white = lam(0.73f, 0.73f, 0.73f);
+    const float sc = 0.001f;
+    auto P = [&](float x, float y, float z) { return H3(x * sc, y * sc, z * sc); };
+    const h3 center = P(278.0f, 274.4f, 279.6f);
+    const h3 zero = H3(0, 0, 0);
+    auto add_tri = [&](h3 a, h3 b, h3 c, int mat, h3 em, h3 ref, bool toward, int light) {
+        rpd_triangle t{};
+        put3(t.v0, a); put3(t.v1, b); put3(t.v2, c); put3(t.emission, em);
+        const h3 n = hcross(b - a, c - a);
+        h3 nn = hsq(n) > 0.0f ? hnorm(n) : H3(0, 1, 0);
+        const h3 mid = (a + b + c) / 3.0f;
+        const float orient = hdot(nn, ref - mid);
+        if (toward ? (orient < 0.0f) : (orient > 0.0f)) nn = -nn;
+        put3(t.normal, nn);
+        t.material_id = mat;
+        t.light_id = light;
+        s.tris.push_back(t);
+    };
+    auto quad = [&](h3 a, h3 b, h3 c, h3 d, int mat, h3 ref, bool toward) {
+        add_tri(a, b, c, mat, zero, ref, toward, -1);
+        add_tri(a, c, d, mat, zero, ref, toward, -1);
+    };
+    const h3 em = H3(70.0f, 70.0f, 70.0f);
+    const h3 lo = P(213.0f, 548.0f, 227.0f);
+    rpd_light L{};
+    L.kind = RPD_LIGHT_RECT;
+    put3(L.p0, lo); put3(L.p1, P(343.0f, 548.0f, 227.0f) - lo); put3(L.p2, P(213.0f, 548.0f, 332.0f) - lo);
+    put3(L.emission, em);
+    s.lights.push_back(L);
+    const h3 l0 = P(213.0f, 548.0f, 227.0f), l1 = P(343.0f, 548.0f, 227.0f), l2 = P(343.0f, 548.0f, 332.0f),
+             l3 = P(213.0f, 548.0f, 332.0f);
+    add_tri(l0, l2, l1, light_mat, em, center, true, 0);
+    add_tri(l0, l3, l2, light_mat, em, center, true, 0);
+    quad(P(552.8f, 0, 0), P(0, 0, 0), P(0, 0, 559.2f), P(549.6f, 0, 559.2f), white, center, true);
+    quad(P(556.0f, 548.8f, 0), P(556.0f, 548.8f, 559.2f), P(0, 548.8f, 559.2f), P(0, 548.8f, 0), white, center, true);
+    quad(P(549.6f, 0, 559.2f), P(0, 0, 559.2f), P(0, 548.8f, 559.2f), P(556.0f, 548.8f, 559.2f), white, center, true);
+    quad(P(0, 0, 559.2f), P(0, 0, 0), P(0, 548.8f, 0), P(0, 548.8f, 559.2f), green, center, true);
+    quad(P(552.8f, 0, 0), P(549.6f, 0, 559.2f), P(556.0f, 548.8f, 559.2f), P(556.0f, 548.8f, 0), red, center, true);
+    const h3 sb[8] = {P(130, 0, 65), P(82, 0, 225), P(240, 0, 272), P(290, 0, 114),
+                      P(130, 165, 65), P(82, 165, 225), P(240, 165, 272), P(290, 165, 114)};
+    const h3 sbc = (sb[0] + sb[2] + sb[5] + sb[7]) * 0.25f;
+    quad(sb[4], sb[5], sb[6], sb[7], box_white, sbc, false);
+    quad(sb[0], sb[4], sb[7], sb[3], box_white, sbc, false);
+    quad(sb[3], sb[7], sb[6], sb[2], box_white, sbc, false);
+    quad(sb[2], sb[6], sb[5], sb[1], box_white, sbc, false);
+    quad(sb[1], sb[5], sb[4], sb[0], box_white, sbc, false);
+    const h3 tb[8] = {P(423, 0, 247), P(265, 0, 296), P(314, 0, 456), P(472, 0, 406),
+                      P(423, 330, 247), P(265, 330, 296), P(314, 330, 456), P(472, 330, 406)};
+    const h3 tbc = (tb[0] + tb[2] + tb[5] + tb[7]) * 0.25f;
+    quad(tb[4], tb[5], tb[6], tb[7], box_white, tbc, false);
+    quad(tb[0], tb[4], tb[7], tb[3], box_white, tbc, false);
+    quad(tb[3], tb[7], tb[6], tb[2], box_white, tbc, false);
+    quad(tb[2], tb[6], tb[5], tb[1], box_white, tbc, false);
+    quad(tb[1], tb[5], tb[4], tb[0], box_white, tbc, false);
+}
+
+// ---- job pipeline -----------------------------------------------------------------
+void set_error_from(rpd_job* j, int st) {
+    if (st != RPD_OK && j->status == RPD_OK) { j->status = st; j->error = g_err; }
+}
+
+int capture_enabled(const rpd_job* j, int statmc) { return statmc && j->cfg.rpf_shrinkage_scale > 0.0f; }
+
+TraceParams trace_params(rpd_job* j, int statmc) {
+    TraceParams p{};
+    p.scene = j->scene->dev;
+    p.cam = j->cam;
+    p.width = j->W; p.height = j->H;
+    p.capture = capture_enabled(j, statmc);
+    p.plan.counts = j->counts; p.plan.offsets = j->offsets; p.plan.active = j->active;
+    p.plan.totals = j->totals; p.plan.n_pixels = j->N;
+    p.seed_n = statmc ? j->pb.n : nullptr;
+    p.base_seed = j->seed;
+    p.tape = j->tape;
+    p.work_counter = j->work;
+    p.rec = j->rec;
+    return p;
+}
+
+int trace_blocks(rpd_job* j) {
+    static int per_cu = -1;
+    if (per_cu < 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, trace_kernel_ptr(), trace_block_size(), 0) != hipSuccess || nb <= 0) nb = 2;
+        per_cu = nb;
+    }
+    const int by_work = (j->N + trace_block_size() - 1) / trace_block_size();
+    return std::max(1, std::min(by_work, per_cu * std::max(1, j->ctx->num_cus)));
+}
+
+// Sampling pass with the plan already in counts/offsets/active/totals.
+int pass_after_plan(rpd_job* j, int statmc, int stage) {
+    StageTimer tm(j, stage);
+    const TraceParams p = trace_params(j, statmc);
+    launch_fill_i32(j->work, 0, 1, j->stream);
+    launch_tape(p, j->stream);
+    {
+        StageTimer tk(j, 7);
+        launch_trace(p, trace_blocks(j), j->stream);
+    }
+    launch_fold(p.plan, j->rec, j->pb, statmc, j->stream);
+    if (p.capture) {
+        StageTimer tg(j, 10);
+        launch_grid_gather(j->geom, j->W, j->H, p.plan, j->rec, j->grid, j->stream);
+    }
+    j->features_valid = false;
+    j->last_pass_statmc = statmc != 0;
+    j->last_pass_capture = p.capture;
+    HIP_TRY(hipGetLastError());
+    return RPD_OK;
+}
+
+void ensure_features(rpd_job* j) {
+    if (j->features_valid) return;
+    launch_features(j->N, j->pb, j->feat, j->stream);
+    j->features_valid = true;
+}
+
+int do_recompute_sensitivity(rpd_job* j) {
+    StageTimer tm(j, 2);
+    if (j->cfg.rpf_shrinkage_scale <= 0.0f) {
+        launch_sensitivity_zero(j->geom, j->W, j->H, j->tiles, j->sconf, j->sgrad, j->stream);
+    } else {
+        ensure_features(j);
+        launch_sensitivity(j->geom, j->W, j->H, j->cfg.rpf_confidence_samples, j->grid, j->feat, j->node_scratch,
+                           j->tiles, j->sconf, j->sgrad, j->stream);
+    }
+    HIP_TRY(hipGetLastError());
+    return RPD_OK;
+}
+
+int do_statmc(rpd_job* j) {
+    StageTimer tm(j, 3);
+    ensure_features(j);
+    FilterConfig fc;
+    fc.radius = j->cfg.color_window_radius;
+    fc.normal_threshold = j->cfg.color_normal_threshold;
+    fc.depth_threshold = j->cfg.color_depth_threshold;
+    fc.compat_alpha = j->cfg.color_compat_alpha;
+    fc.sigma_max = j->cfg.color_sigma_max;
+    fc.rpd_scale = j->cfg.rpf_shrinkage_scale;
+    launch_statmc(j->W, j->H, fc, j->pb, j->feat, j->geom, j->tiles, j->sgrad, j->filter_scratch, j->den, j->unc,
+                  j->alp, j->vtot, j->veff, j->vrat, j->stream);
+    HIP_TRY(hipGetLastError());
+    return RPD_OK;
+}
+
+int do_variance(rpd_job* j) {
+    StageTimer tm(j, 4);
+    ensure_features(j);
+    VarianceConfig vc;
+    vc.radius = j->cfg.var_window_radius;
+    vc.iterations = j->cfg.var_iterations;
+    vc.normal_threshold = j->cfg.var_normal_threshold;
+    vc.depth_threshold = j->cfg.var_depth_threshold;
+    vc.compat_sigma = j->cfg.var_compat_sigma;
+    vc.shrinkage_k = j->cfg.var_shrinkage_k;
+    vc.sigma_max = j->cfg.adaptive_sigma_max;
+    launch_variance(j->W, j->H, vc, j->pb, j->feat, j->vtmp, j->vden, j->stream);
+    HIP_TRY(hipGetLastError());
+    return RPD_OK;
+}
+
+int do_allocation(rpd_job* j) {
+    StageTimer tm(j, 5);
+    ensure_features(j);
+    AllocConfig ac;
+    ac.adaptive_spp = j->cfg.adaptive_spp;
+    ac.base_samples = j->cfg.adaptive_base_samples;
+    ac.radius = j->cfg.adaptive_importance_smoothing_radius;
+    ac.sigma_max = j->cfg.adaptive_sigma_max;
+    ac.seed = j->seed;
+    launch_allocation(j->W, j->H, ac, j->pb, j->feat, j->vden, j->imp, j->counts, j->alloc_scratch, j->stream);
+    launch_plan_scan(j->N, j->counts, j->offsets, j->active, j->totals, j->scan_scratch, j->rec.capacity, j->stream);
+    HIP_TRY(hipGetLastError());
+    return RPD_OK;
+}
+
+int do_finalize(rpd_job* j, bool statmc) {
+    launch_finalize(j->N, j->pb, statmc ? j->out_raw : j->out_rad, j->out_alb, j->out_nrm, j->out_pos, j->out_dep,
+                    j->stream);
+    if (statmc) HIP_TRY(hipMemcpyAsync(j->out_rad, j->den, sizeof(float) * 3 * j->N, hipMemcpyDeviceToDevice, j->stream));
+    j->finalized = true;
+    HIP_TRY(hipGetLastError());
+    return RPD_OK;
+}
+
+int reset_state(rpd_job* j) {
+    const size_t N = j->N;
+    // zero every accumulator (buffers.init, pixel_stats.assign, rpf_grid.reset)
+    HIP_TRY(hipMemsetAsync(j->pb.radiance, 0, sizeof(float) * 13 * N, j->stream));
+    HIP_TRY(hipMemsetAsync(j->pb.hit_count, 0, sizeof(int) * 2 * N, j->stream));
+    HIP_TRY(hipMemsetAsync(j->pb.lum_mean, 0, sizeof(float) * 12 * N, j->stream));
+    HIP_TRY(hipMemsetAsync(j->den, 0, sizeof(float) * 14 * N, j->stream));
+    HIP_TRY(hipMemsetAsync(j->grid, 0, sizeof(double) * GRID_NODE_DOUBLES * j->geom.nx * j->geom.ny, j->stream));
+    HIP_TRY(hipMemsetAsync(j->tiles, 0, sizeof(float) * SF_COUNT * j->geom.nx * j->geom.ny, j->stream));
+    j->features_valid = false;
+    j->finalized = false;
+    return RPD_OK;
+}
+
+// render(): main.cpp:1543-1564. The StatMC calls whose results the reference
+// overwrites before use (statmc_denoise at 1548 and recompute_sensitivity at
+// 736/1551 when adaptive passes run) are not repeated; every output is
+// identical without them.
+int run_pipeline(rpd_job* j) {
+    int st = reset_state(j);
+    if (st) return st;
+    const bool statmc = j->cfg.use_statmc != 0;
+    if (!statmc) {
+        launch_plan_uniform(j->N, j->cfg.samples_per_pixel, j->counts, j->offsets, j->active, j->totals, j->stream);
+        if ((st = pass_after_plan(j, 0, 0))) return st;
+        return do_finalize(j, false);
+    }
+    launch_plan_uniform(j->N, j->cfg.samples_per_pixel, j->counts, j->offsets, j->active, j->totals, j->stream);
+    if ((st = pass_after_plan(j, 1, 0))) return st;
+    const bool passes = j->cfg.adaptive_passes > 0 && j->cfg.adaptive_spp > 0;
+    if (!passes && (st = do_recompute_sensitivity(j))) return st;
+    for (int i = 0; passes && i < j->cfg.adaptive_passes; ++i) {
+        if ((st = do_variance(j))) return st;
+        if ((st = do_allocation(j))) return st;
+        if ((st = pass_after_plan(j, 1, 1))) return st;
+    }
+    if ((st = do_recompute_sensitivity(j))) return st;
+    if ((st = do_statmc(j))) return st;
+    if ((st = do_variance(j))) return st;
+    return do_finalize(j, true);
+}
+
+int job_alloc(rpd_job* j) {
+    const size_t N = j->N;
+    const size_t nn = size_t(j->geom.nx) * j->geom.ny;
+    auto& o = j->owned;
+    int st;
+#define A(ptr, n) if ((st = dalloc(&(ptr), (n), o))) return st
+    float* sums; A(sums, 13 * N);
+    j->pb.radiance = sums; j->pb.albedo = sums + 3 * N; j->pb.normal = sums + 6 * N; j->pb.world_pos = sums + 9 * N;
+    j->pb.depth = sums + 12 * N;
+    int* ic; A(ic, 2 * N);
+    j->pb.hit_count = ic; j->pb.sample_count = ic + N;
+    float* stats; A(stats, 12 * N);
+    j->pb.lum_mean = stats; j->pb.lum_m2 = stats + N; j->pb.n = reinterpret_cast<int*>(stats + 2 * N);
+    j->pb.color_mean = stats + 3 * N; j->pb.sqrt_mean = stats + 6 * N; j->pb.sqrt_m2 = stats + 9 * N;
+    float* fe; A(fe, 7 * N);
+    j->feat.normal = fe; j->feat.depth = fe + 3 * N; j->feat.albedo = fe + 4 * N;
+    float* misc; A(misc, 14 * N);
+    j->den = misc; j->unc = misc + 3 * N; j->alp = misc + 4 * N; j->vtot = misc + 5 * N; j->veff = misc + 6 * N;
+    j->vrat = misc + 7 * N; j->vden = misc + 8 * N; j->vtmp = misc + 9 * N; j->imp = misc + 10 * N;
+    j->sconf = misc + 11 * N; j->sgrad = misc + 12 * N;
+    float* outs; A(outs, 16 * N);
+    j->out_rad = outs; j->out_raw = outs + 3 * N; j->out_alb = outs + 6 * N; j->out_nrm = outs + 9 * N;
+    j->out_pos = outs + 12 * N; j->out_dep = outs + 15 * N;
+    int* plan; A(plan, 3 * N + 8);
+    j->counts = plan; j->offsets = plan + N; j->active = plan + 2 * N; j->totals = plan + 3 * N;
+    j->work = plan + 3 * N + 4;
+    A(j->scan_scratch, (size_t)plan_scan_scratch_elems(j->N));
+    A(j->tape, N * MT_N);
+    const int per_px = std::max(j->cfg.samples_per_pixel, j->cfg.adaptive_spp + 1);
+    const int64_t cap = int64_t(per_px) * int64_t(N);
+    j->rec.capacity = cap;
+    A(j->rec.r0, cap); A(j->rec.r1, cap); A(j->rec.r2, cap); A(j->rec.r3, cap);
+    A(j->rec.r4, cap); A(j->rec.r5, cap); A(j->rec.r6, cap);
+    A(j->grid, nn * GRID_NODE_DOUBLES);
+    A(j->tiles, nn * SF_COUNT);
+    A(j->node_scratch, nn * 33);
+    A(j->filter_scratch, 32 * N);
+    char* as; A(as, alloc_scratch_bytes(j->N));
+    j->alloc_scratch = as;
+#undef A
+    return RPD_OK;
+}
+
+size_t job_bytes(const rpd_render_config& c) {
+    const size_t N = size_t(c.image_width) * c.image_height;
+    const int step = std::max(1, c.rpf_tile_size / 2);
+    const size_t nn = size_t(std::max(2, (c.image_width + step - 1) / step + 1)) *
+                      size_t(std::max(2, (c.image_height + step - 1) / step + 1));
+    const size_t per_px = std::max(c.samples_per_pixel, c.adaptive_spp + 1);
+    return N * 4 * (13 + 2 + 12 + 7 + 14 + 13 + 1 + 3 + 3 + 32) + N * MT_N * 4 + per_px * N * (6 * 16 + 8) +
+           nn * (GRID_NODE_DOUBLES * 8 + SF_COUNT * 4 + 33 * 4) + alloc_scratch_bytes(int(N));
+}
+
+void job_free(rpd_job* j) {
+    if (j->worker.joinable()) j->worker.join();
+    if (j->stream) hipStreamSynchronize(j->stream);
+    for (auto& s : j->spans) { hipEventDestroy(s.a); hipEventDestroy(s.b); }
+    j->spans.clear();
+    for (void* p : j->owned) hipFree(p);
+    j->owned.clear();
+    if (j->stream) hipStreamDestroy(j->stream);
+    j->stream = nullptr;
+    if (j->ctx) j->ctx->in_use -= j->bytes;
+}
+
+int job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam, uint32_t seed,
+               rpd_job** out) {
+    if (!ctx || !scene || !cfg || !cam || !out) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    int st = validate(*cfg);
+    if (st) return st;
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto j = std::make_unique<rpd_job>();
+    j->ctx = ctx; j->scene = scene; j->cfg = *cfg; j->seed = seed;
+    j->W = cfg->image_width; j->H = cfg->image_height; j->N = j->W * j->H;
+    if ((st = make_camera(*cam, float(j->W) / float(j->H), j->cam))) return st;
+    j->geom.support = cfg->rpf_tile_size;
+    j->geom.step = std::max(1, cfg->rpf_tile_size / 2);
+    j->geom.nx = std::max(2, (j->W + j->geom.step - 1) / j->geom.step + 1);
+    j->geom.ny = std::max(2, (j->H + j->geom.step - 1) / j->geom.step + 1);
+    j->bytes = job_bytes(*cfg);
+    if (ctx->budget && ctx->in_use.load() + j->bytes > ctx->budget)
+        return fail(RPD_ERR_OUT_OF_MEMORY, "job does not fit the context's memory budget");
+    ctx->in_use += j->bytes;
+    HIP_TRY(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+    if ((st = job_alloc(j.get()))) { job_free(j.get()); return st; }
+    if ((st = reset_state(j.get()))) { job_free(j.get()); return st; }
+    *out = j.release();
+    return RPD_OK;
+}
+
+// ---- host-side conversion helpers for reads/writes -----------------------------
+int copy_planes_to_interleaved(rpd_job* j, const float* dsrc, int planes, float* dst) {
+    std::vector<float> tmp(size_t(planes) * j->N);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), dsrc, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, j->stream));
+    HIP_TRY(hipStreamSynchronize(j->stream));
+    for (int i = 0; i < j->N; ++i)
+        for (int c = 0; c < planes; ++c) dst[size_t(i) * planes + c] = tmp[size_t(c) * j->N + i];
+    return RPD_OK;
+}
+int copy_interleaved_to_planes(rpd_job* j, const float* src, int planes, float* ddst) {
+    std::vector<float> tmp(size_t(planes) * j->N);
+    for (int i = 0; i < j->N; ++i)
+        for (int c = 0; c < planes; ++c) tmp[size_t(c) * j->N + i] = src[size_t(i) * planes + c];
+    HIP_TRY(hipMemcpyAsync(ddst, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, j->stream));
+    HIP_TRY(hipStreamSynchronize(j->stream));
+    return RPD_OK;
+}
+float host_lerp(float a, float b, float t) { return a + (b - a) * t; }
+// SensitivityTiles::lookup (core/buffers.h:68-86) on the host for expanded reads
+void expand_tiles(const rpd_job* j, const float* t, float* out) {
+    const GridGeom& g = j->geom;
+    for (int y = 0; y < j->H; ++y)
+        for (int x = 0; x < j->W; ++x) {
+            const float fx = float(x) / float(g.step), fy = float(y) / float(g.step);
+            const int x0 = std::clamp(int(std::floor(fx)), 0, g.nx - 1), y0 = std::clamp(int(std::floor(fy)), 0, g.ny - 1);
+            const int x1 = std::min(x0 + 1, g.nx - 1), y1 = std::min(y0 + 1, g.ny - 1);
+            const float tx = std::clamp(fx - float(x0), 0.0f, 1.0f), ty = std::clamp(fy - float(y0), 0.0f, 1.0f);
+            const float top = host_lerp(t[y0 * g.nx + x0], t[y0 * g.nx + x1], tx);
+            const float bot = host_lerp(t[y1 * g.nx + x0], t[y1 * g.nx + x1], tx);
+            out[size_t(y) * j->W + x] = host_lerp(top, bot, ty);
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+void rpd_render_config_defaults(rpd_render_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->samples_per_pixel = 1;
+    c->rpf_tile_size = 8;
+    c->color_window_radius = 7;
+    c->color_normal_threshold = 0.5f; c->color_depth_threshold = 0.25f;
+    c->color_compat_alpha = 0.05f; c->color_sigma_max = 6.0f;
+    c->var_window_radius = 1;
+    c->var_normal_threshold = 0.5f; c->var_depth_threshold = 0.25f;
+    c->var_compat_sigma = 30.0f; c->var_shrinkage_k = 1e-3f;
+    c->var_iterations = 2;
+    c->adaptive_base_samples = 0; c->adaptive_spp = 0;
+    c->adaptive_sigma_max = 3.0f;
+    c->adaptive_passes = 1; c->adaptive_importance_smoothing_radius = 1;
+    c->rpf_shrinkage_scale = 1.0f; c->rpf_confidence_samples = 128.0f;
+}
+
+const char* rpd_last_error(void) { return g_err.c_str(); }
+
+int rpd_ctx_create(int device, rpd_context** out) {
+    if (!out) return fail(RPD_ERR_INVALID_ARG, "null out");
+    *out = nullptr;
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(RPD_ERR_INVALID_ARG, "device index out of range");
+    HIP_TRY(hipSetDevice(device));
+    auto c = std::make_unique<rpd_context>();
+    c->device = device;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    c->num_cus = prop.multiProcessorCount;
+    *out = c.release();
+    return RPD_OK;
+}
+int rpd_ctx_destroy(rpd_context* ctx) {
+    delete ctx;
+    return RPD_OK;
+}
+int rpd_ctx_set_memory_budget(rpd_context* ctx, uint64_t bytes) {
+    if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
+    ctx->budget = bytes;
+    return RPD_OK;
+}
+
+int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, const rpd_sphere* spheres, int n_spheres,
+                     const rpd_material* mats, int n_mats, const rpd_light* lights, int n_lights, const float env[3],
+                     rpd_scene** out) {
+    if (!ctx || !out || n_tris < 0 || n_spheres < 0 || n_mats <= 0 || n_lights < 0 || !mats)
+        return fail(RPD_ERR_INVALID_ARG, "invalid scene arguments");
+    if ((n_tris && !tris) || (n_spheres && !spheres) || (n_lights && !lights))
+        return fail(RPD_ERR_INVALID_ARG, "null primitive array");
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<DevTriangle> dt(n_tris);
+    for (int i = 0; i < n_tris; ++i) {
+        const rpd_triangle& t = tris[i];
+        if (t.material_id < 0 || t.material_id >= n_mats) return fail(RPD_ERR_INVALID_ARG, "triangle material out of range");
+        std::memcpy(dt[i].v0, t.v0, 12); std::memcpy(dt[i].v1, t.v1, 12); std::memcpy(dt[i].v2, t.v2, 12);
+        std::memcpy(dt[i].n, t.normal, 12);
+        dt[i].material = t.material_id;
+        // core/triangle.h:23-25: an emitter has some |emission| above 1e-8
+        const bool em = std::fabs(t.emission[0]) > 1e-8f || std::fabs(t.emission[1]) > 1e-8f || std::fabs(t.emission[2]) > 1e-8f;
+        dt[i].emitter = em ? 1 : 0;
+        dt[i].light = t.light_id;
+        if (em && (t.light_id < 0 || t.light_id >= n_lights)) return fail(RPD_ERR_INVALID_ARG, "emissive triangle without a light");
+        dt[i].pad = 0;
+    }
+    std::vector<DevSphere> ds(n_spheres);
+    for (int i = 0; i < n_spheres; ++i) {
+        const rpd_sphere& s = spheres[i];
+        if (s.material_id < 0 || s.material_id >= n_mats) return fail(RPD_ERR_INVALID_ARG, "sphere material out of range");
+        std::memcpy(ds[i].c, s.center, 12);
+        ds[i].r = s.radius;
+        ds[i].material = s.material_id;
+        const bool em = std::fabs(s.emission[0]) > 1e-8f || std::fabs(s.emission[1]) > 1e-8f || std::fabs(s.emission[2]) > 1e-8f;
+        ds[i].emitter = em ? 1 : 0;
+        ds[i].light = s.light_id;
+        if (em && (s.light_id < 0 || s.light_id >= n_lights)) return fail(RPD_ERR_INVALID_ARG, "emissive sphere without a light");
+        ds[i].pad = 0;
+    }
+    std::vector<DevMaterial> dm(n_mats);
+    for (int i = 0; i < n_mats; ++i) {
+        std::memset(&dm[i], 0, sizeof(DevMaterial));
+        dm[i].kind = mats[i].kind;
+        dm[i].beckmann = mats[i].distribution == RPD_MICROFACET_BECKMANN;
+        dm[i].roughness = mats[i].roughness;
+        std::memcpy(dm[i].albedo, mats[i].albedo, 12);
+        std::memcpy(dm[i].n1, mats[i].n1, 12);
+        std::memcpy(dm[i].n2, mats[i].n2, 12);
+    }
+    std::vector<DevLight> dl(n_lights);
+    std::vector<float> cdf(n_lights);
+    for (int i = 0; i < n_lights; ++i) {
+        const rpd_light& l = lights[i];
+        DevLight& d = dl[i];
+        std::memset(&d, 0, sizeof(DevLight));
+        d.kind = l.kind;
+        std::memcpy(d.p0, l.p0, 12); std::memcpy(d.p1, l.p1, 12); std::memcpy(d.p2, l.p2, 12);
+        std::memcpy(d.emission, l.emission, 12);
+        const h3 em = get3(l.emission);
+        if (l.kind == RPD_LIGHT_RECT) {  // core/light.h:22-26
+            const h3 c = hcross(get3(l.p1), get3(l.p2));
+            d.area = std::sqrt(hsq(c));
+            put3(d.normal, c / d.area);
+            d.power = d.area * std::max(0.0f, lum(em));
+        } else if (l.kind == RPD_LIGHT_AREA) {  // core/light.h:62-63
+            d.area = 0.5f * std::sqrt(hsq(hcross(get3(l.p1) - get3(l.p0), get3(l.p2) - get3(l.p0))));
+            std::memcpy(d.normal, l.normal, 12);
+            d.power = d.area * std::max(0.0f, lum(em));
+        } else if (l.kind == RPD_LIGHT_POINT || l.kind == RPD_LIGHT_DIRECTIONAL) {
+            d.power = std::max(0.0f, lum(em));
+        } else {
+            return fail(RPD_ERR_INVALID_ARG, "unknown light kind");
+        }
+        cdf[i] = (i == 0) ? d.power : cdf[i - 1] + d.power;  // core/scene.h:31-37
+    }
+    auto sc = std::make_unique<rpd_scene>();
+    sc->ctx = ctx;
+    DevTriangle* pt; DevSphere* ps; DevMaterial* pm; DevLight* pl; float* pc;
+    int st;
+    if ((st = dalloc(&pt, dt.size(), sc->owned)) || (st = dalloc(&ps, ds.size(), sc->owned)) ||
+        (st = dalloc(&pm, dm.size(), sc->owned)) || (st = dalloc(&pl, dl.size(), sc->owned)) ||
+        (st = dalloc(&pc, cdf.size(), sc->owned))) {
+        for (void* p : sc->owned) hipFree(p);
+        return st;
+    }
+    if (n_tris) HIP_TRY(hipMemcpy(pt, dt.data(), dt.size() * sizeof(DevTriangle), hipMemcpyHostToDevice));
+    if (n_spheres) HIP_TRY(hipMemcpy(ps, ds.data(), ds.size() * sizeof(DevSphere), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(pm, dm.data(), dm.size() * sizeof(DevMaterial), hipMemcpyHostToDevice));
+    if (n_lights) {
+        HIP_TRY(hipMemcpy(pl, dl.data(), dl.size() * sizeof(DevLight), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(pc, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    sc->dev.tris = pt; sc->dev.n_tris = n_tris;
+    sc->dev.spheres = ps; sc->dev.n_spheres = n_spheres;
+    sc->dev.materials = pm;
+    sc->dev.lights = pl; sc->dev.n_lights = n_lights;
+    sc->dev.light_cdf = pc;
+    for (int k = 0; k < 3; ++k) sc->dev.env[k] = env ? env[k] : 0.0f;
+    sc->dev.env_on = 0;
+    sc->n_tris = n_tris; sc->n_spheres = n_spheres; sc->n_materials = n_mats; sc->n_lights = n_lights;
+    *out = sc.release();
+    return RPD_OK;
+}
+
+int rpd_scene_builtin(rpd_context* ctx, const char* name, rpd_scene** out) {
+    if (!ctx || !name || !out) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    if (std::string(name) != "cornell_box")
+        return fail(RPD_ERR_UNSUPPORTED, std::string("Unknown scene '") + name + "'. Available scenes: cornell_box");
+    HostScene s;
+    build_cornell(s);
+    return rpd_scene_create(ctx, s.tris.data(), int(s.tris.size()), nullptr, 0, s.mats.data(), int(s.mats.size()),
+                            s.lights.data(), int(s.lights.size()), s.env, out);
+}
+
+int rpd_scene_counts(const rpd_scene* s, int* nt, int* ns, int* nm, int* nl) {
+    if (!s) return fail(RPD_ERR_INVALID_ARG, "null scene");
+    if (nt) *nt = s->n_tris;
+    if (ns) *ns = s->n_spheres;
+    if (nm) *nm = s->n_materials;
+    if (nl) *nl = s->n_lights;
+    return RPD_OK;
+}
+
+int rpd_scene_destroy(rpd_scene* s) {
+    if (!s) return RPD_OK;
+    for (void* p : s->owned) hipFree(p);
+    delete s;
+    return RPD_OK;
+}
+
+int rpd_job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam,
+                   uint32_t seed, rpd_job** out) {
+    return job_create(ctx, scene, cfg, cam, seed, out);
+}
+
+int rpd_job_submit(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam,
+                   uint32_t seed, rpd_job** out) {
+    int st = job_create(ctx, scene, cfg, cam, seed, out);
+    if (st) return st;
+    rpd_job* j = *out;
+    const int dev = ctx->device;
+    j->worker = std::thread([j, dev]() {
+        hipSetDevice(dev);
+        const int s = run_pipeline(j);
+        set_error_from(j, s);
+        hipStreamSynchronize(j->stream);
+        j->done.store(1);
+    });
+    return RPD_OK;
+}
+
+int rpd_job_poll(rpd_job* j, int* done) {
+    if (!j || !done) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    *done = j->worker.joinable() ? j->done.load() : 1;
+    return RPD_OK;
+}
+
+int rpd_job_wait(rpd_job* j) {
+    if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
+    if (j->worker.joinable()) j->worker.join();
+    HIP_TRY(hipStreamSynchronize(j->stream));
+    collect_spans(j);
+    if (j->status != RPD_OK) return fail(j->status, j->error);
+    return RPD_OK;
+}
+
+int rpd_job_release(rpd_job* j) {
+    if (!j) return RPD_OK;
+    job_free(j);
+    delete j;
+    return RPD_OK;
+}
+
+int rpd_job_sync(rpd_job* j) {
+    if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
+    if (j->worker.joinable()) j->worker.join();
+    HIP_TRY(hipStreamSynchronize(j->stream));
+    collect_spans(j);
+    int over = 0;
+    HIP_TRY(hipMemcpy(&over, j->totals + 2, sizeof(int), hipMemcpyDeviceToHost));
+    if (over) return fail(RPD_ERR_STATE, "an adaptive pass asked for more samples than the record capacity");
+    return RPD_OK;
+}
+
+int rpd_job_run_pass(rpd_job* j, const int* counts, int statmc) {
+    if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
+    if (!counts) {
+        launch_plan_uniform(j->N, j->cfg.samples_per_pixel, j->counts, j->offsets, j->active, j->totals, j->stream);
+    } else {
+        for (int i = 0; i < j->N; ++i)
+            if (counts[i] < 0) return fail(RPD_ERR_INVALID_ARG, "negative sample count");
+        HIP_TRY(hipMemcpyAsync(j->counts, counts, sizeof(int) * j->N, hipMemcpyHostToDevice, j->stream));
+        launch_plan_scan(j->N, j->counts, j->offsets, j->active, j->totals, j->scan_scratch, j->rec.capacity, j->stream);
+    }
+    const int st = pass_after_plan(j, statmc ? 1 : 0, counts ? 1 : 0);
+    if (st) return st;
+    return rpd_job_sync(j);
+}
+
+int rpd_job_run_adaptive_pass(rpd_job* j) {
+    if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
+    int st = do_allocation(j);
+    if (!st) st = pass_after_plan(j, 1, 1);
+    if (st) return st;
+    return rpd_job_sync(j);
+}
+
+int rpd_job_recompute_sensitivity(rpd_job* j) {
+    if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
+    const int st = do_recompute_sensitivity(j);
+    return st ? st : rpd_job_sync(j);
+}
+int rpd_job_statmc_denoise(rpd_job* j) {
+    if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
+    const int st = do_statmc(j);
+    return st ? st : rpd_job_sync(j);
+}
+int rpd_job_variance_denoise(rpd_job* j) {
+    if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
+    const int st = do_variance(j);
+    return st ? st : rpd_job_sync(j);
+}
+int rpd_job_adaptive_counts(rpd_job* j, int* out) {
+    if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
+    int st = do_allocation(j);
+    if (st) return st;
+    if ((st = rpd_job_sync(j))) return st;
+    if (out) HIP_TRY(hipMemcpy(out, j->counts, sizeof(int) * j->N, hipMemcpyDeviceToHost));
+    return RPD_OK;
+}
+int rpd_job_finalize(rpd_job* j) {
+    if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
+    const int st = do_finalize(j, j->cfg.use_statmc != 0);
+    return st ? st : rpd_job_sync(j);
+}
+
+int rpd_job_grid_dims(rpd_job* j, int dims[4]) {
+    if (!j || !dims) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    dims[0] = j->geom.nx; dims[1] = j->geom.ny; dims[2] = j->geom.step; dims[3] = j->geom.support;
+    return RPD_OK;
+}
+
+int rpd_job_stage_ms(rpd_job* j, double* ms, int n) {
+    if (!j || !ms) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    if (j->worker.joinable()) j->worker.join();
+    HIP_TRY(hipStreamSynchronize(j->stream));
+    collect_spans(j);
+    double total = 0;
+    for (int k = 0; k < 6; ++k) total += j->stage_ms[k];
+    j->stage_ms[6] = total;
+    for (int k = 0; k < n && k < 11; ++k) ms[k] = j->stage_ms[k];
+    return RPD_OK;
+}
+
+int rpd_job_buffer_size(rpd_job* j, int id, size_t* bytes) {
+    if (!j || !bytes) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    const size_t N = j->N, nn = size_t(j->geom.nx) * j->geom.ny;
+    switch (id) {
+        case RPD_BUF_RADIANCE: case RPD_BUF_RAW_RADIANCE: case RPD_BUF_ALBEDO: case RPD_BUF_NORMAL:
+        case RPD_BUF_WORLD_POS: case RPD_BUF_DENOISED: *bytes = 12 * N; return RPD_OK;
+        case RPD_BUF_STATS: *bytes = 48 * N; return RPD_OK;
+        case RPD_BUF_SUMS: *bytes = 52 * N; return RPD_OK;
+        case RPD_BUF_FEATURES: *bytes = 28 * N; return RPD_OK;
+        case RPD_BUF_GRID: *bytes = 8 * GRID_NODE_DOUBLES * nn; return RPD_OK;
+        case RPD_BUF_TILES: *bytes = 4 * SF_COUNT * nn; return RPD_OK;
+        case RPD_BUF_RECORDS: {
+            int tot[2] = {0, 0};
+            HIP_TRY(hipStreamSynchronize(j->stream));
+            HIP_TRY(hipMemcpy(tot, j->totals, sizeof(tot), hipMemcpyDeviceToHost));
+            *bytes = size_t(tot[1] > 0 ? tot[0] : 0) * 32 * sizeof(float);
+            return RPD_OK;
+        }
+        default:
+            if ((id >= RPD_BUF_DEPTH && id <= RPD_BUF_SAMPLE_FRACTION) || (id >= RPD_BUF_SENS_ALL && id <= RPD_BUF_SENS_RR) ||
+                (id >= RPD_BUF_SAMPLE_COUNT && id <= RPD_BUF_PASS_COUNTS)) {
+                *bytes = 4 * N;
+                return RPD_OK;
+            }
+            return fail(RPD_ERR_INVALID_ARG, "unknown buffer id");
+    }
+}
+
+int rpd_job_read(rpd_job* j, int id, void* dst, size_t bytes) {
+    if (!j || !dst) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    size_t need = 0;
+    int st = rpd_job_buffer_size(j, id, &need);
+    if (st) return st;
+    if (bytes < need) return fail(RPD_ERR_INVALID_ARG, "destination buffer too small");
+    if (j->worker.joinable()) j->worker.join();
+    HIP_TRY(hipStreamSynchronize(j->stream));
+    const size_t N = j->N;
+    float* f = static_cast<float*>(dst);
+    auto scalar = [&](const float* d) -> int {
+        HIP_TRY(hipMemcpy(dst, d, 4 * N, hipMemcpyDeviceToHost));
+        return RPD_OK;
+    };
+    switch (id) {
+        case RPD_BUF_RADIANCE:
+            return copy_planes_to_interleaved(j, j->finalized ? j->out_rad : j->pb.radiance, 3, f);
+        case RPD_BUF_RAW_RADIANCE: return copy_planes_to_interleaved(j, j->out_raw, 3, f);
+        case RPD_BUF_ALBEDO: return copy_planes_to_interleaved(j, j->finalized ? j->out_alb : j->pb.albedo, 3, f);
+        case RPD_BUF_NORMAL: return copy_planes_to_interleaved(j, j->finalized ? j->out_nrm : j->pb.normal, 3, f);
+        case RPD_BUF_WORLD_POS: return copy_planes_to_interleaved(j, j->finalized ? j->out_pos : j->pb.world_pos, 3, f);
+        case RPD_BUF_DENOISED: return copy_planes_to_interleaved(j, j->den, 3, f);
+        case RPD_BUF_DEPTH: return scalar(j->finalized ? j->out_dep : j->pb.depth);
+        case RPD_BUF_UNCERTAINTY: return scalar(j->unc);
+        case RPD_BUF_ALPHA: return scalar(j->alp);
+        case RPD_BUF_VARIANCE_DENOISED: return scalar(j->vden);
+        case RPD_BUF_VAR_TOTAL: return scalar(j->vtot);
+        case RPD_BUF_VAR_EFF: return scalar(j->veff);
+        case RPD_BUF_VAR_RATIO: return scalar(j->vrat);
+        case RPD_BUF_SENSITIVITY_CONFIDENCE: return scalar(j->sconf);
+        case RPD_BUF_SENSITIVITY_GRADIENT: return scalar(j->sgrad);
+        case RPD_BUF_ADAPTIVE_IMPORTANCE: return scalar(j->imp);
+        case RPD_BUF_SAMPLE_COUNT: HIP_TRY(hipMemcpy(dst, j->pb.sample_count, 4 * N, hipMemcpyDeviceToHost)); return RPD_OK;
+        case RPD_BUF_HIT_COUNT: HIP_TRY(hipMemcpy(dst, j->pb.hit_count, 4 * N, hipMemcpyDeviceToHost)); return RPD_OK;
+        case RPD_BUF_PASS_COUNTS: HIP_TRY(hipMemcpy(dst, j->counts, 4 * N, hipMemcpyDeviceToHost)); return RPD_OK;
+        case RPD_BUF_SAMPLE_FRACTION: {  // core/buffers.h:182-195
+            std::vector<int> c(N);
+            HIP_TRY(hipMemcpy(c.data(), j->pb.sample_count, 4 * N, hipMemcpyDeviceToHost));
+            const int mx = std::max(1, N ? *std::max_element(c.begin(), c.end()) : 1);
+            const float inv = 1.0f / float(mx);
+            for (size_t i = 0; i < N; ++i) f[i] = float(std::max(0, c[i])) * inv;
+            return RPD_OK;
+        }
+        case RPD_BUF_STATS: {
+            std::vector<float> s(12 * N);
+            HIP_TRY(hipMemcpy(s.data(), j->pb.lum_mean, 48 * N, hipMemcpyDeviceToHost));
+            const int plane_of[12] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
+            for (size_t i = 0; i < N; ++i)
+                for (int k = 0; k < 12; ++k) f[i * 12 + k] = s[size_t(plane_of[k]) * N + i];
+            return RPD_OK;
+        }
+        case RPD_BUF_SUMS: {
+            std::vector<float> s(13 * N);
+            std::vector<int> ic(2 * N);
+            HIP_TRY(hipMemcpy(s.data(), j->pb.radiance, 52 * N, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < N; ++i)
+                for (int k = 0; k < 13; ++k) f[i * 13 + k] = s[size_t(k) * N + i];
+            return RPD_OK;
+        }
+        case RPD_BUF_FEATURES: {
+            int rc = RPD_OK;
+            ensure_features(j);
+            std::vector<float> s(7 * N);
+            HIP_TRY(hipMemcpyAsync(s.data(), j->feat.normal, 28 * N, hipMemcpyDeviceToHost, j->stream));
+            HIP_TRY(hipStreamSynchronize(j->stream));
+            for (size_t i = 0; i < N; ++i)
+                for (int k = 0; k < 7; ++k) f[i * 7 + k] = s[size_t(k) * N + i];
+            return rc;
+        }
+        case RPD_BUF_GRID:
+            HIP_TRY(hipMemcpy(dst, j->grid, need, hipMemcpyDeviceToHost));
+            return RPD_OK;
+        case RPD_BUF_TILES:
+            HIP_TRY(hipMemcpy(dst, j->tiles, need, hipMemcpyDeviceToHost));
+            return RPD_OK;
+        case RPD_BUF_RECORDS: {
+            const size_t ns = need / (32 * sizeof(float));
+            std::vector<float4> r[6];
+            std::vector<float2> r6(ns);
+            Records& R = j->rec;
+            float4* srcs[6] = {R.r0, R.r1, R.r2, R.r3, R.r4, R.r5};
+            for (int k = 0; k < 6; ++k) {
+                r[k].resize(ns);
+                if (ns) HIP_TRY(hipMemcpy(r[k].data(), srcs[k], ns * sizeof(float4), hipMemcpyDeviceToHost));
+            }
+            if (ns) HIP_TRY(hipMemcpy(r6.data(), R.r6, ns * sizeof(float2), hipMemcpyDeviceToHost));
+            std::vector<int> offs(N), cnts(N);
+            HIP_TRY(hipMemcpy(offs.data(), j->offsets, 4 * N, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(cnts.data(), j->counts, 4 * N, hipMemcpyDeviceToHost));
+            // 32 floats per sample in pixel-major sample order (same layout as the oracle's records)
+            for (size_t p = 0; p < N; ++p) {
+                for (int s = 0; s < cnts[p]; ++s) {
+                    const size_t k = size_t(offs[p]) + s;
+                    float* o = f + k * 32;
+                    std::memset(o, 0, 32 * sizeof(float));
+                    uint32_t fl;
+                    std::memcpy(&fl, &r[0][k].w, 4);
+                    const bool hit = fl & REC_HIT;
+                    o[0] = float(p); o[1] = r[0][k].x; o[2] = r[0][k].y; o[3] = r[0][k].z;
+                    o[4] = (fl & REC_FINITE) ? 1.0f : 0.0f; o[5] = hit ? 1.0f : 0.0f;
+                    if (hit) {
+                        o[6] = r[1][k].x; o[7] = r[1][k].y; o[8] = r[1][k].z; o[15] = r[1][k].w;
+                        o[9] = r[2][k].x; o[10] = r[2][k].y; o[11] = r[2][k].z;
+                        o[12] = r[3][k].x; o[13] = r[3][k].y; o[14] = r[3][k].z;
+                    }
+                    if (j->last_pass_capture) {
+                        o[16] = r[4][k].x; o[17] = r[4][k].y; o[18] = r[4][k].z; o[19] = r[4][k].w;
+                        o[20] = r[5][k].x; o[21] = r[5][k].y; o[22] = r[5][k].z; o[23] = r[5][k].w;
+                        o[24] = r6[k].x; o[25] = r6[k].y;
+                    }
+                    const uint32_t li = (fl >> REC_LIGHT_SHIFT) & 255u;
+                    o[26] = (fl & REC_LSEL) ? float(li) : -1.0f;
+                    o[27] = (fl & REC_RR_SURVIVED) ? 1.0f : 0.0f;
+                    o[28] = float(((fl & REC_PIXEL) ? 1 : 0) | ((fl & REC_LENS) ? 2 : 0) | ((fl & REC_LSEL) ? 4 : 0) |
+                                  ((fl & REC_BRDF) ? 8 : 0) | ((fl & REC_LIGHT) ? 16 : 0) | ((fl & REC_ENV) ? 32 : 0) |
+                                  ((fl & REC_RR) ? 64 : 0));
+                }
+            }
+            return RPD_OK;
+        }
+        default: break;
+    }
+    if (id >= RPD_BUF_SENS_ALL && id <= RPD_BUF_SENS_RR) {
+        const int field[9] = {SF_ALL, SF_PIXEL, SF_BRDF, SF_LENS, SF_LIGHT, SF_LIGHT_UV, SF_LIGHT_SELECT, SF_ENV, SF_RR};
+        const size_t nn = size_t(j->geom.nx) * j->geom.ny;
+        std::vector<float> t(nn);
+        HIP_TRY(hipMemcpy(t.data(), j->tiles + size_t(field[id - RPD_BUF_SENS_ALL]) * nn, 4 * nn, hipMemcpyDeviceToHost));
+        expand_tiles(j, t.data(), f);
+        return RPD_OK;
+    }
+    return fail(RPD_ERR_INVALID_ARG, "buffer is not readable");
+}
+
+int rpd_job_write(rpd_job* j, int id, const void* src, size_t bytes) {
+    if (!j || !src) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    size_t need = 0;
+    int st = rpd_job_buffer_size(j, id, &need);
+    if (st) return st;
+    if (bytes < need) return fail(RPD_ERR_INVALID_ARG, "source buffer too small");
+    HIP_TRY(hipStreamSynchronize(j->stream));
+    const size_t N = j->N;
+    const float* f = static_cast<const float*>(src);
+    j->features_valid = false;
+    switch (id) {
+        case RPD_BUF_STATS: {
+            std::vector<float> s(12 * N);
+            for (size_t i = 0; i < N; ++i)
+                for (int k = 0; k < 12; ++k) s[size_t(k) * N + i] = f[i * 12 + k];
+            HIP_TRY(hipMemcpy(j->pb.lum_mean, s.data(), 48 * N, hipMemcpyHostToDevice));
+            return RPD_OK;
+        }
+        case RPD_BUF_SUMS: {
+            std::vector<float> s(13 * N);
+            for (size_t i = 0; i < N; ++i)
+                for (int k = 0; k < 13; ++k) s[size_t(k) * N + i] = f[i * 13 + k];
+            HIP_TRY(hipMemcpy(j->pb.radiance, s.data(), 52 * N, hipMemcpyHostToDevice));
+            return RPD_OK;
+        }
+        case RPD_BUF_GRID: HIP_TRY(hipMemcpy(j->grid, src, need, hipMemcpyHostToDevice)); return RPD_OK;
+        case RPD_BUF_TILES: HIP_TRY(hipMemcpy(j->tiles, src, need, hipMemcpyHostToDevice)); return RPD_OK;
+        case RPD_BUF_HIT_COUNT: HIP_TRY(hipMemcpy(j->pb.hit_count, src, 4 * N, hipMemcpyHostToDevice)); return RPD_OK;
+        case RPD_BUF_SAMPLE_COUNT: HIP_TRY(hipMemcpy(j->pb.sample_count, src, 4 * N, hipMemcpyHostToDevice)); return RPD_OK;
+        case RPD_BUF_SENSITIVITY_GRADIENT: HIP_TRY(hipMemcpy(j->sgrad, src, 4 * N, hipMemcpyHostToDevice)); return RPD_OK;
+        case RPD_BUF_VARIANCE_DENOISED: HIP_TRY(hipMemcpy(j->vden, src, 4 * N, hipMemcpyHostToDevice)); return RPD_OK;
+        default: return fail(RPD_ERR_INVALID_ARG, "buffer is not writable");
+    }
+}
+
+}  // extern "C"

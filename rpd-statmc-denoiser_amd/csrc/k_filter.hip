@@ -1,0 +1,310 @@
+// StatMC Welch-compatibility colour filter and sample-variance smoothing.
+//
+// Reference: path_tracer/main.cpp:957-1162 (statmc_denoise), 188-231
+// (lookup_t_critical), 292-327 (per-pixel reliable components),
+// core/stats.h:48-91 (shared reliability, relaxation, variance update),
+// main.cpp:1166-1268 (variance_denoise).
+#include "kernels.h"
+
+namespace rpd {
+
+__device__ __forceinline__ float tile_lookup_f(const float* t, GridGeom g, int x, int y) {
+    const float fx = float(x) / float(g.step), fy = float(y) / float(g.step);
+    const int x0 = min(max(int(floorf(fx)), 0), g.nx - 1), y0 = min(max(int(floorf(fy)), 0), g.ny - 1);
+    const int x1 = min(x0 + 1, g.nx - 1), y1 = min(y0 + 1, g.ny - 1);
+    const float tx = clampf(fx - float(x0), 0.0f, 1.0f), ty = clampf(fy - float(y0), 0.0f, 1.0f);
+    const float top = lerpf_(t[y0 * g.nx + x0], t[y0 * g.nx + x1], tx);
+    const float bot = lerpf_(t[y1 * g.nx + x0], t[y1 * g.nx + x1], tx);
+    return lerpf_(top, bot, ty);
+}
+
+// Two-sided t critical value (main.cpp:188-231). Tables for alpha 0.05 and
+// 0.01 (df 1..30, interpolated), otherwise a normal quantile step function.
+__constant__ float c_t05[30] = {12.706f, 4.303f, 3.182f, 2.776f, 2.571f, 2.447f, 2.365f, 2.306f, 2.262f, 2.228f,
+                                2.201f, 2.179f, 2.160f, 2.145f, 2.131f, 2.120f, 2.110f, 2.101f, 2.093f, 2.086f,
+                                2.080f, 2.074f, 2.069f, 2.064f, 2.060f, 2.056f, 2.052f, 2.048f, 2.045f, 2.042f};
+__constant__ float c_t01[30] = {63.657f, 9.925f, 5.841f, 4.604f, 4.032f, 3.707f, 3.499f, 3.355f, 3.250f, 3.169f,
+                                3.106f, 3.055f, 3.012f, 2.977f, 2.947f, 2.921f, 2.898f, 2.878f, 2.861f, 2.845f,
+                                2.831f, 2.819f, 2.807f, 2.797f, 2.787f, 2.779f, 2.771f, 2.763f, 2.756f, 2.750f};
+
+// table selector: 0 = alpha 0.05, 1 = alpha 0.01, 2 = normal quantile
+__device__ __forceinline__ float t_critical(float df, int table, float z) {
+    df = fmaxf(df, 1.0f);
+    if (table < 2 && df < 30.5f) {
+        const float* tab = table == 0 ? c_t05 : c_t01;
+        const int lo = min(max(int(floorf(df)), 1), 30), hi = min(max(int(ceilf(df)), 1), 30);
+        if (lo == hi) return tab[lo - 1];
+        return lerpf_(tab[lo - 1], tab[hi - 1], (df - float(lo)) / float(hi - lo));
+    }
+    if (df < 30.0f) return z * sqrtf(df / fmaxf(1.0f, df - 1.0f));
+    return z;
+}
+
+// Per-pixel filter inputs, packed as 7 float4 planes:
+//   P0 = (unit normal xyz, mean depth; -1 marks a pixel without primary hits)
+//   P1 = (mean albedo rgb, n)
+//   P2 = (sqrt-domain colour mean rgb, clamped luminance variance)
+//   P3 = (stabilised sqrt variance of the mean rgb, variance of the mean)
+//   P4 = (colour mean rgb, 0)
+//   P5 = (reliable components 0..3), P6 = (reliable components 4..5, 0, 0)
+constexpr int FILTER_PLANES = 7;
+
+// sqrt-domain variance of the mean, floored (main.cpp:977-980)
+__global__ void k_sqrt_var_mean(int n, PixelBuffers pb, float* vm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int cnt = pb.n[i];
+    for (int c = 0; c < 3; ++c) {
+        const float sv = cnt <= 1 ? 0.0f : pb.sqrt_m2[c * n + i] / float(cnt - 1);
+        vm[c * n + i] = fmaxf(sv / float(max(1, cnt)), RPD_EPS_SMALL);
+    }
+}
+
+__global__ void k_filter_pack(int W, int H, FilterConfig fc, PixelBuffers pb, Features f, GridGeom g,
+                              const float* tiles, const float* grad, const float* vm, float4* P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = W * H;
+    if (i >= n) return;
+    const int x = i % W, y = i / W;
+    // main.cpp:984-990: max of the value and its 3x3 weighted local average
+    double a0 = 0, a1 = 0, a2 = 0, ws = 0;
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int yy = min(max(y + dy, 0), H - 1);
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int xx = min(max(x + dx, 0), W - 1);
+            const double wt = 1.0 / (1.0 + double(dx * dx + dy * dy));
+            const int q = yy * W + xx;
+            a0 += double(vm[q]) * wt; a1 += double(vm[n + q]) * wt; a2 += double(vm[2 * n + q]) * wt;
+            ws += wt;
+        }
+    }
+    const v3 avg = mk3(float(a0 / ws), float(a1 / ws), float(a2 / ws));
+    const v3 svs = mk3(fmaxf(vm[i], avg.x), fmaxf(vm[n + i], avg.y), fmaxf(vm[2 * n + i], avg.z));
+    const int cnt = pb.n[i];
+    const float var2_max = fc.sigma_max * fc.sigma_max;
+    float var = cnt > 1 ? pb.lum_m2[i] / float(cnt - 1) : 0.0f;
+    if (!isfinite(var)) var = RPD_EPS_SMALL;
+    var = clampf(var, RPD_EPS_SMALL, var2_max);
+    const int ni = max(1, cnt);
+    const float vmean = fmaxf(RPD_EPS_SMALL, var / float(ni));
+    const bool hit = pb.hit_count[i] > 0;
+    P[0 * n + i] = make_float4(f.normal[i], f.normal[n + i], f.normal[2 * n + i], hit ? f.depth[i] : -1.0f);
+    P[1 * n + i] = make_float4(f.albedo[i], f.albedo[n + i], f.albedo[2 * n + i], float(ni));
+    P[2 * n + i] = make_float4(pb.sqrt_mean[i], pb.sqrt_mean[n + i], pb.sqrt_mean[2 * n + i], var);
+    P[3 * n + i] = make_float4(svs.x, svs.y, svs.z, vmean);
+    P[4 * n + i] = make_float4(pb.color_mean[i], pb.color_mean[n + i], pb.color_mean[2 * n + i], 0.0f);
+    // main.cpp:313-327, 305-310: six reliable components attenuated by the gradient
+    const int nn = g.nx * g.ny;
+    const int src[6] = {SF_BRDF, SF_LENS, SF_LIGHT_UV, SF_LIGHT_SELECT, SF_ENV, SF_RR};
+    const float gr = clampf(grad[i], 0.0f, 1.0f);
+    float c[6];
+    for (int k = 0; k < 6; ++k) {
+        const float v = clampf(tile_lookup_f(tiles + src[k] * nn, g, x, y), 0.0f, 1.0f);
+        c[k] = clampf(v * (1.0f - gr), 0.0f, 1.0f);
+    }
+    P[5 * n + i] = make_float4(c[0], c[1], c[2], c[3]);
+    P[6 * n + i] = make_float4(c[4], c[5], 0.0f, 0.0f);
+}
+
+// The window filter, one thread per pixel (main.cpp:998-1127). FP64 per
+// neighbour exactly as the reference: Welch t^2 per channel against the
+// t critical value at Welch-Satterthwaite df, the maximum over channels,
+// RPD relaxation, hard rejection above 25, weight exp(-q/2)/(1+d^2).
+__global__ void __launch_bounds__(256) k_statmc(int W, int H, FilterConfig fc, const float4* __restrict__ P,
+                                               PixelBuffers pb, float* den, float* unc, float* alp, float* vtot,
+                                               float* veff, float* vrat) {
+    const int n = W * H;
+    // 16x16 pixel tiles per workgroup
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int x = blockIdx.x * 16 + tx, y = blockIdx.y * 16 + ty;
+    if (x >= W || y >= H) return;
+    const int i = y * W + x;
+    const float4 p0 = P[i];
+    if (p0.w < 0.0f) {  // no primary hit: keep the mean
+        const int cnt = pb.n[i];
+        den[i] = pb.color_mean[i]; den[n + i] = pb.color_mean[n + i]; den[2 * n + i] = pb.color_mean[2 * n + i];
+        const float var = cnt > 1 ? pb.lum_m2[i] / float(cnt - 1) : 0.0f;
+        unc[i] = var / float(max(1, cnt));
+        alp[i] = 1.0f;
+        return;
+    }
+    int table = 2;
+    if (fabsf(fmaxf(fminf(fc.compat_alpha, 0.999999f), RPD_EPS_SMALL) - 0.05f) < RPD_EPS) table = 0;
+    else if (fabsf(fmaxf(fminf(fc.compat_alpha, 0.999999f), RPD_EPS_SMALL) - 0.01f) < RPD_EPS) table = 1;
+    float z;
+    {
+        const float a = fmaxf(fminf(fc.compat_alpha, 0.999999f), RPD_EPS_SMALL);
+        if (a <= 0.001f) z = 3.29f;
+        else if (a <= 0.005f) z = 2.81f;
+        else if (a <= 0.01f) z = 2.58f;
+        else if (a <= 0.02f) z = 2.33f;
+        else if (a <= 0.05f) z = 1.96f;
+        else if (a <= 0.10f) z = 1.64f;
+        else z = 1.28f;
+    }
+    const v3 ni = mk3(p0.x, p0.y, p0.z);
+    const float di = p0.w;
+    const float4 p1 = P[n + i], p2 = P[2 * n + i], p3 = P[3 * n + i], p4 = P[4 * n + i], p5 = P[5 * n + i],
+                 p6 = P[6 * n + i];
+    const v3 ai = mk3(p1.x, p1.y, p1.z);
+    const int n_i = int(p1.w);
+    const float var_i = p2.w;
+    const float vm_i = p3.w;
+    const float ci[6] = {p5.x, p5.y, p5.z, p5.w, p6.x, p6.y};
+    vtot[i] = var_i;
+    veff[i] = vm_i;
+    vrat[i] = (var_i > RPD_EPS_SMALL) ? (vm_i * float(n_i)) / var_i : 0.0f;
+
+    double ws = 0, wss = 0, wvs = 0, c0 = 0, c1 = 0, c2 = 0, accepted = 0;
+    const int R = fc.radius;
+    for (int dy = -R; dy <= R; ++dy) {
+        const int yy = y + dy;
+        if (yy < 0 || yy >= H) continue;
+        for (int dx = -R; dx <= R; ++dx) {
+            const int xx = x + dx;
+            if (xx < 0 || xx >= W) continue;
+            const int q = yy * W + xx;
+            const float4 q0 = P[q];
+            if (q0.w < 0.0f) continue;
+            if (dot3(ni, mk3(q0.x, q0.y, q0.z)) < fc.normal_threshold) continue;
+            if (relative_depth_delta(di, q0.w) > fc.depth_threshold) continue;
+            const float4 q1 = P[n + q];
+            if (norm3(ai - mk3(q1.x, q1.y, q1.z)) > 0.12f) continue;
+            const float4 q2 = P[2 * n + q], q3 = P[3 * n + q];
+            const int n_j = int(q1.w);
+            double mx = 0.0;
+            const float smi[3] = {p2.x, p2.y, p2.z}, smj[3] = {q2.x, q2.y, q2.z};
+            const float vsi[3] = {p3.x, p3.y, p3.z}, vsj[3] = {q3.x, q3.y, q3.z};
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                const double vi = double(vsi[ch]), vj = double(vsj[ch]);
+                const double vs = vi + vj;
+                const double md = double(smi[ch] - smj[ch]);
+                const double t2 = md * md / fmax(double(RPD_EPS_SMALL), vs);
+                const double den_df = (vi * vi) / fmax(double(n_i - 1), 1.0) + (vj * vj) / fmax(double(n_j - 1), 1.0);
+                const double df = (den_df > double(RPD_EPS_SMALL)) ? (vs * vs) / den_df : double(n_i + n_j - 2);
+                const double tc = double(t_critical(float(df), table, z));
+                mx = fmax(mx, t2 / (tc * tc));
+            }
+            const float4 q5 = P[5 * n + q], q6 = P[6 * n + q];
+            const float cj[6] = {q5.x, q5.y, q5.z, q5.w, q6.x, q6.y};
+            float shared = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) shared = fmaxf(shared, fminf(ci[k], cj[k]));
+            shared = clampf(shared, 0.0f, 1.0f);
+            const float adj = clampf(fc.rpd_scale * shared, 0.0f, 1.0f);
+            const double qv = mx / (1.0 + double(adj));
+            if (qv > 25.0) continue;
+            const double sp = 1.0 / (1.0 + double(dx * dx + dy * dy));
+            const double wj = sp * exp(-0.5 * qv);
+            const float4 q4 = P[4 * n + q];
+            c0 += double(q4.x) * wj; c1 += double(q4.y) * wj; c2 += double(q4.z) * wj;
+            ws += wj;
+            wss += wj * wj;
+            wvs += wj * wj * double(q3.w);
+            if ((dx != 0 || dy != 0) && wj > 0.05 * sp) accepted += 1.0;
+        }
+    }
+    v3 mn = mk3(p4.x, p4.y, p4.z);
+    double vn = double(vm_i);
+    if (ws > double(RPD_EPS_SMALL)) {
+        mn = mk3(float(c0 / ws), float(c1 / ws), float(c2 / ws));
+        vn = wvs / (ws * ws);
+    }
+    const double neff = ws * ws / fmax(double(RPD_EPS_SMALL), wss);
+    const double blend = (accepted >= 2.0 && ws >= 1.1) ? fmin(fmax(1.0 - 1.0 / sqrt(neff), 0.0), 0.8) : 0.0;
+    const float a = float(1.0 - blend);
+    const float bf = float(blend);
+    den[i] = p4.x + (mn.x - p4.x) * bf;
+    den[n + i] = p4.y + (mn.y - p4.y) * bf;
+    den[2 * n + i] = p4.z + (mn.z - p4.z) * bf;
+    unc[i] = float(double(a) * double(a) * double(vm_i) + blend * blend * vn +
+                   2.0 * double(a) * blend * (1.0 / fmax(ws, double(RPD_EPS_SMALL))) * double(vm_i));
+    alp[i] = a;
+}
+
+void launch_statmc(int W, int H, const FilterConfig& fc, const PixelBuffers& pb, const Features& f, GridGeom g,
+                   const float* tiles, const float* grad, float* scratch, float* den, float* unc, float* alp,
+                   float* vtot, float* veff, float* vrat, hipStream_t st) {
+    const int n = W * H;
+    float* vm = scratch;                                           // 3 planes
+    float4* P = reinterpret_cast<float4*>(scratch + 4 * (size_t)n);  // 7 float4 planes (16-byte aligned)
+    hipLaunchKernelGGL(k_sqrt_var_mean, dim3((n + 255) / 256), dim3(256), 0, st, n, pb, vm);
+    hipLaunchKernelGGL(k_filter_pack, dim3((n + 255) / 256), dim3(256), 0, st, W, H, fc, pb, f, g, tiles, grad, vm, P);
+    hipLaunchKernelGGL(k_statmc, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, st, W, H, fc, P, pb, den, unc,
+                       alp, vtot, veff, vrat);
+}
+
+// ---------------------------------------------------------------------------
+// Sample-variance smoothing (main.cpp:1166-1268)
+__global__ void k_var_init(int n, PixelBuffers pb, float maxv, float* cur) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int cnt = pb.n[i];
+    float v = cnt > 1 ? pb.lum_m2[i] / float(cnt - 1) : 0.0f;
+    if (!isfinite(v)) v = RPD_EPS_SMALL;
+    cur[i] = clampf(v, RPD_EPS_SMALL, maxv);
+}
+__global__ void k_var_smooth(int W, int H, VarianceConfig vc, PixelBuffers pb, Features f, const float* src,
+                             float* dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = W * H;
+    if (i >= n) return;
+    if (pb.hit_count[i] == 0) { dst[i] = src[i]; return; }
+    const int x = i % W, y = i / W;
+    const v3 ni = mk3(f.normal[i], f.normal[n + i], f.normal[2 * n + i]);
+    const float di = f.depth[i];
+    const float vi = fmaxf(RPD_EPS_SMALL, src[i]);
+    float ws = 0.0f, wsum = 0.0f;
+    const int R = vc.radius;
+    for (int dy = -R; dy <= R; ++dy) {
+        const int yy = y + dy;
+        if (yy < 0 || yy >= H) continue;
+        for (int dx = -R; dx <= R; ++dx) {
+            const int xx = x + dx;
+            if (xx < 0 || xx >= W || (dx == 0 && dy == 0)) continue;
+            const int q = yy * W + xx;
+            if (pb.hit_count[q] == 0) continue;
+            if (dot3(ni, mk3(f.normal[q], f.normal[n + q], f.normal[2 * n + q])) < vc.normal_threshold) continue;
+            if (relative_depth_delta(di, f.depth[q]) > vc.depth_threshold) continue;
+            const float vj = fmaxf(RPD_EPS_SMALL, src[q]);
+            const float rd = fabsf(vi - vj) / fmaxf(RPD_EPS_SMALL, sqrtf(vi * vj));
+            if (rd > vc.compat_sigma) continue;
+            const float wj = 1.0f / (1.0f + float(dx * dx + dy * dy));
+            wsum += wj * vj;
+            ws += wj;
+        }
+    }
+    float nm = vi;
+    if (ws > RPD_EPS_SMALL) nm = wsum / ws;
+    // core/stats.h:85-91
+    const float blend = clampf(fmaxf(0.0f, nm - vi) / fmaxf(RPD_EPS_SMALL, vi + nm + vc.shrinkage_k), 0.0f, 1.0f);
+    dst[i] = lerpf_(vi, nm, blend);
+}
+__global__ void k_var_final(int n, PixelBuffers pb, float maxv, const float* cur, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int cnt = pb.n[i];
+    float v = cnt > 1 ? pb.lum_m2[i] / float(cnt - 1) : 0.0f;
+    if (!isfinite(v)) v = RPD_EPS_SMALL;
+    v = clampf(v, RPD_EPS_SMALL, maxv);
+    out[i] = fmaxf(cur[i], v);
+}
+
+void launch_variance(int W, int H, const VarianceConfig& vc, const PixelBuffers& pb, const Features& f, float* tmp,
+                     float* out, hipStream_t st) {
+    const int n = W * H;
+    const float maxv = vc.sigma_max * vc.sigma_max;
+    const dim3 grid((n + 255) / 256);
+    // ping-pong between out and tmp; the result of the last iteration ends in `cur`
+    float* cur = out;
+    float* nxt = tmp;
+    hipLaunchKernelGGL(k_var_init, grid, dim3(256), 0, st, n, pb, maxv, cur);
+    for (int it = 0; it < vc.iterations; ++it) {
+        hipLaunchKernelGGL(k_var_smooth, grid, dim3(256), 0, st, W, H, vc, pb, f, cur, nxt);
+        float* t = cur; cur = nxt; nxt = t;
+    }
+    hipLaunchKernelGGL(k_var_final, grid, dim3(256), 0, st, n, pb, maxv, cur, out);
+}
+
+}  // namespace rpd

@@ -1,0 +1,383 @@
+// RPF dependency grid and sensitivity reconstruction.
+//
+// Reference: path_tracer/main.cpp:329-388 (splatting samples into the
+// overlapping grid), core/stats.h:93-329 (binned dependency statistics),
+// main.cpp:741-953 (recompute_sensitivity), core/buffers.h:68-101 (bilinear
+// node -> pixel expansion), main.cpp:94-122 (feature averages).
+#include "kernels.h"
+
+namespace rpd {
+
+// ---------------------------------------------------------------------------
+// Per-pixel feature averages: main.cpp:106-118
+__global__ void k_features(int n, PixelBuffers pb, Features f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int hc = pb.hit_count[i];
+    v3 nr = mk3(pb.normal[i], pb.normal[n + i], pb.normal[2 * n + i]);
+    v3 al = mk3(pb.albedo[i], pb.albedo[n + i], pb.albedo[2 * n + i]);
+    float d = pb.depth[i];
+    if (hc > 0) {
+        const float h = float(hc);
+        nr = nr / h;
+        if (sqnorm3(nr) > RPD_EPS_SMALL) nr = normalized3(nr);
+        d = d / h;
+        al = al / h;
+    }
+    f.normal[i] = nr.x; f.normal[n + i] = nr.y; f.normal[2 * n + i] = nr.z;
+    f.depth[i] = d;
+    f.albedo[i] = al.x; f.albedo[n + i] = al.y; f.albedo[2 * n + i] = al.z;
+}
+void launch_features(int n, const PixelBuffers& pb, Features f, hipStream_t st) {
+    hipLaunchKernelGGL(k_features, dim3((n + 255) / 256), dim3(256), 0, st, n, pb, f);
+}
+
+// ---------------------------------------------------------------------------
+// Grid gather. One workgroup owns one node and walks the pixels of the node's
+// support in row-major order and each pixel's samples in order, which is the
+// order a single sampling thread of the reference feeds that node. Phase 1
+// turns up to 256 (pixel, sample) contributions into (weight, luminance,
+// 13 bin indices) in LDS; phase 2 gives each (accumulator, bin) pair one thread
+// that adds the chunk sequentially in FP64, so every sum is formed in sample
+// order without atomics. The pass totals are then added to the node.
+constexpr int GATHER_BLOCK = 256;
+constexpr int GATHER_MAX_PIX = 1024;  // support window of up to (2*step)^2 pixels, step <= 16
+
+__device__ __forceinline__ int dep_bin(float u, float v) {  // core/stats.h:165-169
+    const int x = min(max(int(u * 4), 0), 3);
+    const int z = min(max(int(v * 4), 0), 3);
+    return z * 4 + x;
+}
+
+__global__ void __launch_bounds__(GATHER_BLOCK) k_grid_gather(GridGeom g, int W, int H, PassPlan plan, Records rec,
+                                                             double* grid) {
+    __shared__ int s_pix[GATHER_MAX_PIX];
+    __shared__ float s_w[GATHER_MAX_PIX];
+    __shared__ int s_pre[GATHER_MAX_PIX + 1];
+    __shared__ float c_w[GATHER_BLOCK];
+    __shared__ float c_y[GATHER_BLOCK];
+    __shared__ signed char c_bin[GRID_ACC][GATHER_BLOCK];
+    __shared__ int s_npix;
+
+    const int node = blockIdx.x;
+    const int gx = node % g.nx, gy = node / g.nx;
+    const int tid = threadIdx.x;
+    const int x_lo = max(0, (gx - 1) * g.step), x_hi = min(W - 1, (gx + 1) * g.step - 1);
+    const int y_lo = max(0, (gy - 1) * g.step), y_hi = min(H - 1, (gy + 1) * g.step - 1);
+    const int wx = x_hi - x_lo + 1, wy = y_hi - y_lo + 1;
+    const int nwin = (wx > 0 && wy > 0) ? wx * wy : 0;
+
+    // support pixels with their node weight (main.cpp:332-349), row-major
+    if (tid == 0) {
+        int k = 0;
+        int pre = 0;
+        for (int q = 0; q < nwin; ++q) {
+            const int x = x_lo + q % wx, y = y_lo + q / wx;
+            const float fx = float(x) / float(g.step), fy = float(y) / float(g.step);
+            const int x0 = min(max(int(floorf(fx)), 0), g.nx - 1), y0 = min(max(int(floorf(fy)), 0), g.ny - 1);
+            const int x1 = min(x0 + 1, g.nx - 1), y1 = min(y0 + 1, g.ny - 1);
+            const float tx = clampf(fx - float(x0), 0.0f, 1.0f), ty = clampf(fy - float(y0), 0.0f, 1.0f);
+            float w;
+            if (x0 == gx && y0 == gy) w = (1.0f - tx) * (1.0f - ty);
+            else if (x1 == gx && y0 == gy) w = tx * (1.0f - ty);
+            else if (x0 == gx && y1 == gy) w = (1.0f - tx) * ty;
+            else if (x1 == gx && y1 == gy) w = tx * ty;
+            else continue;
+            if (w <= RPD_EPS_SMALL) continue;
+            const int p = y * W + x;
+            const int c = plan.totals[1] > 0 ? plan.counts[p] : 0;
+            if (c <= 0) continue;
+            s_pix[k] = p; s_w[k] = w; s_pre[k] = pre;
+            pre += c;
+            ++k;
+        }
+        s_pre[k] = pre;
+        s_npix = k;
+    }
+    __syncthreads();
+    const int npix = s_npix;
+    const int total = s_pre[npix];
+
+    // this thread's accumulator slot: k = tid / 18, b = tid % 18
+    const int acc = tid / 18, bsel = tid % 18;
+    const bool active_t = acc < GRID_ACC && (bsel >= 16 || bsel < grid_acc_bins(acc));
+    double a0 = 0.0, a1 = 0.0;
+
+    const float half = 0.5f * float(g.support);
+    const float left = float(gx * g.step) - half, top = float(gy * g.step) - half;
+
+    for (int base = 0; base < total; base += GATHER_BLOCK) {
+        const int c = base + tid;
+        if (c < total) {
+            int j = 0;  // pixel of contribution c: last j with s_pre[j] <= c
+            int lo = 0, hi = npix - 1;
+            while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (s_pre[mid] <= c) lo = mid; else hi = mid - 1; }
+            j = lo;
+            const int p = s_pix[j];
+            const int64_t slot = int64_t(plan.offsets[p]) + (c - s_pre[j]);
+            const float4 r0 = rec.r0[slot];
+            const uint32_t fl = __float_as_uint(r0.w);
+            signed char bins[GRID_ACC];
+            for (int k = 0; k < GRID_ACC; ++k) bins[k] = -1;
+            float y = 0.0f;
+            if (fl & REC_FINITE) {
+                y = luminance3(mk3(r0.x, r0.y, r0.z));
+                const float4 r4 = rec.r4[slot];
+                const float4 r5 = rec.r5[slot];
+                const float2 r6 = rec.r6[slot];
+                const int x = p % W, yy = p / W;
+                if (fl & REC_PIXEL) {
+                    const float su = clampf((float(x) + r4.x - left) / float(g.support), 0.0f, 1.0f);
+                    const float sv = clampf((float(yy) + r4.y - top) / float(g.support), 0.0f, 1.0f);
+                    const int sb = dep_bin(su, sv);
+                    bins[0] = (signed char)sb;
+                    if (fl & REC_BRDF) bins[2] = (signed char)sb;
+                    if (fl & REC_LENS) bins[4] = (signed char)sb;
+                    if (fl & REC_LIGHT) bins[6] = (signed char)sb;
+                    if (fl & REC_ENV) bins[8] = (signed char)sb;
+                    if (fl & REC_LSEL) bins[10] = (signed char)sb;
+                    if (fl & REC_RR) bins[12] = (signed char)sb;
+                }
+                if (fl & REC_BRDF) bins[1] = (signed char)dep_bin(r5.x, r5.y);
+                if (fl & REC_LENS) bins[3] = (signed char)dep_bin(r4.z, r4.w);
+                if (fl & REC_LIGHT) bins[5] = (signed char)dep_bin(r5.z, r5.w);
+                if (fl & REC_ENV) bins[7] = (signed char)dep_bin(r6.x, r6.y);
+                if (fl & REC_LSEL) bins[9] = (signed char)min(max(int((fl >> REC_LIGHT_SHIFT) & 255u), 0), 15);
+                if (fl & REC_RR) bins[11] = (signed char)((fl & REC_RR_SURVIVED) ? 1 : 0);
+            }
+            c_w[tid] = s_w[j];
+            c_y[tid] = y;
+            for (int k = 0; k < GRID_ACC; ++k) c_bin[k][tid] = bins[k];
+        }
+        __syncthreads();
+        const int nchunk = min(GATHER_BLOCK, total - base);
+        if (active_t) {
+            for (int i = 0; i < nchunk; ++i) {
+                const int bi = c_bin[acc][i];
+                if (bi < 0) continue;
+                const double wd = double(c_w[i]), yd = double(c_y[i]);
+                if (bsel < 16) {
+                    if (bi == bsel) { a0 += wd; a1 += wd * yd; }
+                } else if (bsel == 16) {
+                    a0 += wd; a1 += wd * wd;
+                } else {
+                    a0 += wd * yd; a1 += wd * yd * yd;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (active_t && total > 0) {
+        double* nd = grid + (size_t)node * GRID_NODE_DOUBLES + grid_acc_offset(acc);
+        const int nb = grid_acc_bins(acc);
+        if (bsel < 16) { nd[bsel] += a0; nd[nb + bsel] += a1; }
+        else if (bsel == 16) { nd[2 * nb] += a0; nd[2 * nb + 1] += a1; }
+        else { nd[2 * nb + 2] += a0; nd[2 * nb + 3] += a1; }
+    }
+}
+
+void launch_grid_gather(GridGeom g, int W, int H, const PassPlan& plan, const Records& rec, double* grid,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_grid_gather, dim3(g.nx * g.ny), dim3(GATHER_BLOCK), 0, st, g, W, H, plan, rec, grid);
+}
+
+// ---------------------------------------------------------------------------
+// core/stats.h:115-137: bias-adjusted explained variance of one accumulator
+__device__ inline float binned_sensitivity(const double* a, int nb) {
+    const double kEps = 1e-18;
+    const double ws = a[2 * nb], w2 = a[2 * nb + 1], ys = a[2 * nb + 2], y2 = a[2 * nb + 3];
+    const double ne = w2 > 0.0 ? ws * ws / w2 : 0.0;
+    if (ne <= 2.0) return 0.0f;
+    const double mean = ys / ws;
+    const double tot = y2 - ws * mean * mean;
+    if (tot <= kEps) return 0.0f;
+    int active = 0;
+    double between = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        const double w = a[b];
+        if (w <= kEps) continue;
+        ++active;
+        const double d = a[nb + b] / w - mean;
+        between += w * d * d;
+    }
+    if (active < 2 || ne <= double(active)) return 0.0f;
+    const double eta2 = fmin(fmax(between / tot, 0.0), 1.0);
+    const double adj = fmax(0.0, 1.0 - (1.0 - eta2) * (ne - 1.0) / (ne - active));
+    return float(sqrt(adj));
+}
+__device__ inline double binned_mass(const double* a, int nb) {
+    const double ws = a[2 * nb], w2 = a[2 * nb + 1];
+    return w2 > 0.0 ? ws * ws / w2 : 0.0;
+}
+
+// 3x3 clamped local averages with weights 1/(1+d^2) (main.cpp:156-185)
+__device__ inline float local_avg1(const float* v, int w, int h, int cx, int cy) {
+    double acc = 0.0, ws = 0.0;
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int y = min(max(cy + dy, 0), h - 1);
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int x = min(max(cx + dx, 0), w - 1);
+            const double wt = 1.0 / (1.0 + double(dx * dx + dy * dy));
+            acc += wt * double(v[y * w + x]);
+            ws += wt;
+        }
+    }
+    return ws > 0.0 ? float(acc / ws) : 0.0f;
+}
+__device__ inline v3 local_avg3(const float* v, int n, int w, int h, int cx, int cy) {
+    double a0 = 0, a1 = 0, a2 = 0, ws = 0;
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int y = min(max(cy + dy, 0), h - 1);
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int x = min(max(cx + dx, 0), w - 1);
+            const double wt = 1.0 / (1.0 + double(dx * dx + dy * dy));
+            const int q = y * w + x;
+            a0 += double(v[q]) * wt; a1 += double(v[n + q]) * wt; a2 += double(v[2 * n + q]) * wt;
+            ws += wt;
+        }
+    }
+    if (ws <= 0.0) return mk3(0, 0, 0);
+    return mk3(float(a0 / ws), float(a1 / ws), float(a2 / ws));
+}
+
+// node scratch planes: raw[13], conf[13], normal[3], depth, albedo[3]
+constexpr int NODE_PLANES = 33;
+
+// main.cpp:773-815
+__global__ void k_node_split(GridGeom g, int W, int H, float n0, const double* grid, Features f, float* ns) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nn = g.nx * g.ny;
+    if (idx >= nn) return;
+    const double* nd = grid + (size_t)idx * GRID_NODE_DOUBLES;
+    const double n0d = double(n0);
+    for (int k = 0; k < GRID_ACC; ++k) {
+        const double* a = nd + grid_acc_offset(k);
+        const int nb = grid_acc_bins(k);
+        ns[k * nn + idx] = binned_sensitivity(a, nb);
+        const double m = binned_mass(a, nb);
+        ns[(GRID_ACC + k) * nn + idx] = float(m / (m + n0d));
+    }
+    const int gx = idx % g.nx, gy = idx / g.nx;
+    const int px = min(max(gx * g.step, 0), W - 1), py = min(max(gy * g.step, 0), H - 1);
+    const int np = W * H;
+    v3 nr = local_avg3(f.normal, np, W, H, px, py);
+    if (sqnorm3(nr) > RPD_EPS_SMALL) nr = normalized3(nr);
+    const float d = local_avg1(f.depth, W, H, px, py);
+    const v3 al = local_avg3(f.albedo, np, W, H, px, py);
+    ns[26 * nn + idx] = nr.x; ns[27 * nn + idx] = nr.y; ns[28 * nn + idx] = nr.z;
+    ns[29 * nn + idx] = d;
+    ns[30 * nn + idx] = al.x; ns[31 * nn + idx] = al.y; ns[32 * nn + idx] = al.z;
+}
+
+// main.cpp:817-928: confidence-weighted 3x3 prior per field, shrinkage toward
+// it, then reliable random components relative to their screen comparators.
+__global__ void k_prior(GridGeom g, const float* ns, float* tiles) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nn = g.nx * g.ny;
+    if (idx >= nn) return;
+    const int gx = idx % g.nx, gy = idx / g.nx;
+    const v3 ni = mk3(ns[26 * nn + idx], ns[27 * nn + idx], ns[28 * nn + idx]);
+    const float di = ns[29 * nn + idx];
+    const v3 ai = mk3(ns[30 * nn + idx], ns[31 * nn + idx], ns[32 * nn + idx]);
+    // neighbour weights without the confidence factor are shared by all fields
+    double wgt[9];
+    int nidx[9];
+    for (int dy = -1; dy <= 1; ++dy) {
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int q = (dy + 1) * 3 + (dx + 1);
+            const int sx = gx + dx, sy = gy + dy;
+            if (sx < 0 || sx >= g.nx || sy < 0 || sy >= g.ny) { nidx[q] = -1; wgt[q] = 0.0; continue; }
+            const int si = sy * g.nx + sx;
+            const v3 nj = mk3(ns[26 * nn + si], ns[27 * nn + si], ns[28 * nn + si]);
+            const v3 aj = mk3(ns[30 * nn + si], ns[31 * nn + si], ns[32 * nn + si]);
+            const float sw = 1.0f / (1.0f + float(dx * dx + dy * dy));
+            const float nw = clampf((dot3(ni, nj) + 1.0f) * 0.5f, 0.0f, 1.0f);
+            const float dw = expf(-4.0f * relative_depth_delta(di, ns[29 * nn + si]));
+            const float aw = expf(-2.5f * norm3(ai - aj));
+            nidx[q] = si;
+            wgt[q] = double(sw) * double(nw) * double(dw) * double(aw);
+        }
+    }
+    float fin[GRID_ACC];
+    for (int k = 0; k < GRID_ACC; ++k) {
+        const float* raw = ns + k * nn;
+        const float* cf = ns + (GRID_ACC + k) * nn;
+        double ws = 0.0, acc = 0.0;
+        for (int q = 0; q < 9; ++q) {
+            if (nidx[q] < 0) continue;
+            const double w = wgt[q] * double(fmaxf(0.05f, cf[nidx[q]]));
+            acc += w * double(raw[nidx[q]]);
+            ws += w;
+        }
+        const float prior = (ws > double(RPD_EPS_SMALL)) ? float(acc / ws) : raw[idx];
+        fin[k] = lerpf_(prior, raw[idx], cf[idx]);
+    }
+    // accumulator indices: 0 pixel, 1 brdf, 2 s_brdf, 3 lens, 4 s_lens, 5 light_uv, 6 s_light_uv,
+    // 7 env, 8 s_env, 9 light_select, 10 s_light_select, 11 rr, 12 s_rr
+    const int rnd[6] = {1, 3, 5, 9, 7, 11};
+    const int scr[6] = {2, 4, 6, 10, 8, 12};
+    const int sv[6] = {SF_BRDF, SF_LENS, SF_LIGHT_UV, SF_LIGHT_SELECT, SF_ENV, SF_RR};
+    const int sc[6] = {SF_BRDF_CONF, SF_LENS_CONF, SF_LIGHT_UV_CONF, SF_LIGHT_SELECT_CONF, SF_ENV_CONF, SF_RR_CONF};
+    tiles[SF_PIXEL * nn + idx] = clampf(fin[0], 0.0f, 1.0f);
+    tiles[SF_PIXEL_CONF * nn + idx] = clampf(ns[GRID_ACC * nn + idx], 0.0f, 1.0f);
+    float rel[6], rc[6];
+    for (int k = 0; k < 6; ++k) {
+        const float rv = clampf(fin[rnd[k]], 0.0f, 1.0f);
+        rc[k] = clampf(ns[(GRID_ACC + rnd[k]) * nn + idx], 0.0f, 1.0f);
+        const float relative = rv / fmaxf(RPD_EPS_SMALL, rv + fin[scr[k]]);
+        rel[k] = relative * fminf(rc[k], ns[(GRID_ACC + scr[k]) * nn + idx]);
+        tiles[sc[k] * nn + idx] = rc[k];
+    }
+    int dom = 0;
+    for (int k = 1; k < 6; ++k) if (rel[dom] < rel[k]) dom = k;
+    for (int k = 0; k < 6; ++k) tiles[sv[k] * nn + idx] = rel[k];
+    tiles[SF_LIGHT * nn + idx] = fmaxf(rel[2], rel[3]);
+    tiles[SF_LIGHT_CONF * nn + idx] = rel[2] >= rel[3] ? rc[2] : rc[3];
+    tiles[SF_ALL * nn + idx] = rel[dom];
+    tiles[SF_ALL_CONF * nn + idx] = fminf(rc[dom], ns[(GRID_ACC + scr[dom]) * nn + idx]);
+}
+
+// core/buffers.h:68-86: bilinear lookup of a node field at pixel (x, y)
+__device__ __forceinline__ float tile_lookup(const float* t, GridGeom g, int x, int y) {
+    const float fx = float(x) / float(g.step), fy = float(y) / float(g.step);
+    const int x0 = min(max(int(floorf(fx)), 0), g.nx - 1), y0 = min(max(int(floorf(fy)), 0), g.ny - 1);
+    const int x1 = min(x0 + 1, g.nx - 1), y1 = min(y0 + 1, g.ny - 1);
+    const float tx = clampf(fx - float(x0), 0.0f, 1.0f), ty = clampf(fy - float(y0), 0.0f, 1.0f);
+    const float top = lerpf_(t[y0 * g.nx + x0], t[y0 * g.nx + x1], tx);
+    const float bot = lerpf_(t[y1 * g.nx + x0], t[y1 * g.nx + x1], tx);
+    return lerpf_(top, bot, ty);
+}
+
+// main.cpp:930-947
+__global__ void k_gradient(GridGeom g, int W, int H, const float* tiles, float* conf, float* grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W * H) return;
+    const int x = i % W, y = i / W;
+    const float* all = tiles + SF_ALL * g.nx * g.ny;
+    const float l = tile_lookup(all, g, max(0, x - 1), y), r = tile_lookup(all, g, min(W - 1, x + 1), y);
+    const float u = tile_lookup(all, g, x, max(0, y - 1)), d = tile_lookup(all, g, x, min(H - 1, y + 1));
+    const float gr = sqrtf((r - l) * (r - l) + (d - u) * (d - u));
+    grad[i] = clampf(0.5f * gr, 0.0f, 1.0f);
+    conf[i] = tile_lookup(tiles + SF_ALL_CONF * g.nx * g.ny, g, x, y);
+}
+
+__global__ void k_zero_f32(float* p, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+void launch_sensitivity(GridGeom g, int W, int H, float n0, const double* grid, const Features& f, float* ns,
+                        float* tiles, float* conf, float* grad, hipStream_t st) {
+    const int nn = g.nx * g.ny;
+    hipLaunchKernelGGL(k_node_split, dim3((nn + 127) / 128), dim3(128), 0, st, g, W, H, n0, grid, f, ns);
+    hipLaunchKernelGGL(k_prior, dim3((nn + 127) / 128), dim3(128), 0, st, g, ns, tiles);
+    hipLaunchKernelGGL(k_gradient, dim3((W * H + 255) / 256), dim3(256), 0, st, g, W, H, tiles, conf, grad);
+}
+void launch_sensitivity_zero(GridGeom g, int W, int H, float* tiles, float* conf, float* grad, hipStream_t st) {
+    const int64_t nt = int64_t(SF_COUNT) * g.nx * g.ny, np = int64_t(W) * H;
+    hipLaunchKernelGGL(k_zero_f32, dim3((nt + 255) / 256), dim3(256), 0, st, tiles, nt);
+    hipLaunchKernelGGL(k_zero_f32, dim3((np + 255) / 256), dim3(256), 0, st, conf, np);
+    hipLaunchKernelGGL(k_zero_f32, dim3((np + 255) / 256), dim3(256), 0, st, grad, np);
+}
+
+}  // namespace rpd

@@ -1,0 +1,499 @@
+// Sampling pass: MT19937 tapes, the path-regeneration trace kernel, the pass
+// plan (counts -> slot offsets + active pixel list) and the per-pixel fold.
+//
+// Reference: path_tracer/main.cpp:394-585 (shade_mis, mis_path_trace),
+// 590-737 and 1451-1538 (per-pixel sample loops), core/sampler.h,
+// core/stats.h:25-41 (Welford fold).
+#include "kernels.h"
+
+namespace rpd {
+
+// ---------------------------------------------------------------------------
+// Tape generation: one thread per active pixel seeds std::mt19937 with
+// pixel_sample_seed(idx, n) and performs the first twist. The seeding words
+// x[k] come from the recurrence x[k] = 1812433253 * (x[k-1] ^ x[k-1] >> 30) + k,
+// walked by two cursors (at k+1 and at k+397) so the seeded state is never
+// stored; words k >= 227 of the twist read back the tape words it wrote 227
+// steps earlier.
+__device__ __forceinline__ uint32_t mt_seed_step(uint32_t prev, uint32_t k) {
+    return 1812433253u * (prev ^ (prev >> 30)) + k;
+}
+
+__global__ void __launch_bounds__(256) k_tape(TraceParams p) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= p.plan.totals[1]) return;
+    const int pix = p.plan.active[a];
+    const int n_before = p.seed_n ? p.seed_n[pix] : 0;
+    const uint32_t s = pixel_sample_seed(p.base_seed, pix, n_before);
+    uint32_t* t = p.tape + (size_t)a * MT_N;
+    uint32_t lo = s;               // x[k]
+    uint32_t far = s;              // x[k + 397]
+    for (uint32_t k = 1; k <= (uint32_t)MT_M; ++k) far = mt_seed_step(far, k);
+    const uint32_t x0 = s;
+    for (int k = 0; k < MT_N - MT_M; ++k) {          // k + 397 < 624: far word from the seeded state
+        const uint32_t nxt = mt_seed_step(lo, (uint32_t)(k + 1));
+        t[k] = mt_mix(lo, nxt, far);
+        lo = nxt;
+        if (k + 1 < MT_N - MT_M) far = mt_seed_step(far, (uint32_t)(k + 1 + MT_M));
+    }
+    for (int k = MT_N - MT_M; k < MT_N - 1; ++k) {   // far word is an already twisted word
+        const uint32_t nxt = mt_seed_step(lo, (uint32_t)(k + 1));
+        t[k] = mt_mix(lo, nxt, t[k + MT_M - MT_N]);
+        lo = nxt;
+    }
+    t[MT_N - 1] = mt_mix(lo, t[0], t[MT_M - 1]);
+    (void)x0;
+}
+
+void launch_tape(const TraceParams& p, hipStream_t st) {
+    const int blocks = (p.plan.n_pixels + 255) / 256;
+    hipLaunchKernelGGL(k_tape, dim3(blocks), dim3(256), 0, st, p);
+}
+
+// ---------------------------------------------------------------------------
+// Trace kernel. One lane owns one pixel at a time and draws that pixel's
+// samples in order from its tape, so every random number lands where the
+// reference draws it. Lanes regenerate: a lane whose sample ended starts the
+// pixel's next sample, and a lane whose pixel ended takes the next pixel from
+// a wave-level chunk of the active list. Each loop iteration runs at most one
+// shading step (light + environment + BRDF sampling, shadow rays) and one
+// closest-hit query per lane, with every lane's query in the same code path.
+//
+// Radiance is accumulated forward (throughput times each bounce's direct
+// term); the reference's recursion sums the same terms innermost-first.
+enum : int { PH_NEED = 0, PH_CAMERA = 1, PH_SHADE = 2, PH_TRACE_PRIMARY = 3, PH_TRACE_EXT = 4, PH_DONE = 5, PH_DEAD = 6 };
+
+constexpr int TRACE_BLOCK = 256;
+constexpr int TRACE_CHUNK = 64;
+
+__device__ __forceinline__ uint64_t ballot64(bool b) { return __ballot(b); }
+
+__global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
+    const DevScene& S = p.scene;
+    const int lane = threadIdx.x & 63;
+    const float env_x = S.env[0], env_y = S.env[1], env_z = S.env[2];
+    const v3 env = mk3(env_x, env_y, env_z);
+    const bool env_on = maxc3(env) > 0.0f;
+    const float pdf_env = float(1.0 / (4.0 * RPD_PI));
+
+    // wave-uniform chunk cursor over the active list
+    int c_next = 0, c_end = 0;
+    const int n_active = p.plan.totals[1];
+
+    int phase = PH_NEED;
+    int pix = -1, left = 0;
+    int64_t slot = 0;
+    Draws dr;
+    dr.tape = nullptr; dr.pos = 0; dr.cache = make_uint4(0, 0, 0, 0);
+    // path state
+    v3 ro = mk3(0, 0, 0), rd = mk3(0, 0, 0);
+    v3 T = mk3(1, 1, 1), L = mk3(0, 0, 0), Ld = mk3(0, 0, 0);
+    v3 sp = mk3(0, 0, 0), sn = mk3(0, 0, 0), swo = mk3(0, 0, 0);  // shading point
+    int smat = 0;
+    float bpdf = 0.0f;
+    v3 bw = mk3(0, 0, 0);
+    uint32_t flags = 0;
+    v2 c_pix, c_lens, c_brdf, c_light, c_env;
+    c_pix.x = c_pix.y = c_lens.x = c_lens.y = c_brdf.x = c_brdf.y = 0.0f;
+    c_light.x = c_light.y = c_env.x = c_env.y = 0.0f;
+    uint32_t light_bits = 255u;
+
+    while (true) {
+        // -- finish samples ------------------------------------------------
+        if (phase == PH_DONE) {
+            if (finite3(L)) flags |= REC_FINITE;
+            flags |= (light_bits & 255u) << REC_LIGHT_SHIFT;
+            p.rec.r0[slot] = make_float4(L.x, L.y, L.z, __uint_as_float(flags));
+            if (p.capture) {
+                p.rec.r4[slot] = make_float4(c_pix.x, c_pix.y, c_lens.x, c_lens.y);
+                p.rec.r5[slot] = make_float4(c_brdf.x, c_brdf.y, c_light.x, c_light.y);
+                p.rec.r6[slot] = make_float2(c_env.x, c_env.y);
+            }
+            ++slot;
+            --left;
+            phase = left > 0 ? PH_CAMERA : PH_NEED;
+        }
+        // -- hand out pixels ------------------------------------------------
+        uint64_t need = ballot64(phase == PH_NEED);
+        while (need) {
+            if (c_next >= c_end) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(p.work_counter, TRACE_CHUNK);
+                base = __shfl(base, 0);
+                c_next = base;
+                c_end = min(base + TRACE_CHUNK, n_active);
+                if (c_next >= c_end) break;
+            }
+            const int avail = c_end - c_next;
+            const int rank = __popcll(need & ((1ull << lane) - 1ull));
+            const bool mine = (phase == PH_NEED) && rank < avail;
+            if (mine) {
+                const int a = c_next + rank;
+                pix = p.plan.active[a];
+                left = p.plan.counts[pix];
+                slot = p.plan.offsets[pix];
+                dr.reset(p.tape + (size_t)a * MT_N);
+                phase = PH_CAMERA;
+            }
+            const int taken = min(__popcll(need), avail);
+            c_next += taken;
+            need = ballot64(phase == PH_NEED);
+        }
+        if (phase == PH_NEED) phase = PH_DEAD;
+        if (ballot64(phase != PH_DEAD) == 0) break;
+
+        // -- camera rays: main.cpp:691-705 ----------------------------------
+        if (phase == PH_CAMERA) {
+            const int x = pix % p.width, y = pix / p.width;
+            c_pix = dr.next2();
+            const float u = (float(x) + c_pix.x) / float(p.width);
+            const float v = (float(y) + c_pix.y) / float(p.height);
+            c_lens = dr.next2();
+            v2 lens; lens.x = 0.0f; lens.y = 0.0f;
+            if (p.cam.lens_radius > 0.0f) lens = sample_disk(c_lens);
+            flags = 0;
+            light_bits = 255u;
+            if (p.capture) {
+                flags |= REC_PIXEL;
+                if (p.cam.lens_radius > 0.0f) flags |= REC_LENS;
+            }
+            v3 o = ld3(p.cam.pos);
+            if (p.cam.lens_radius > 0.0f) {
+                const float lx = p.cam.lens_radius * lens.x, ly = p.cam.lens_radius * lens.y;
+                o = o + (lx * ld3(p.cam.right) + ly * ld3(p.cam.up));
+            }
+            const v3 px = ld3(p.cam.llc) + u * ld3(p.cam.horiz) + (1.0f - v) * ld3(p.cam.vert);
+            ro = o;
+            rd = normalized3(px - o);
+            T = mk3(1, 1, 1);
+            L = mk3(0, 0, 0);
+            phase = PH_TRACE_PRIMARY;
+        }
+
+        // -- shading: main.cpp:394-479 ----------------------------------------
+        if (phase == PH_SHADE) {
+            const DevMaterial& m = S.materials[smat];
+            const bool delta_brdf = m.kind == MAT_MIRROR;
+            Ld = mk3(0, 0, 0);
+            const float sel_u = dr.next();
+            int li; float sel_pdf;
+            sample_light(S, sel_u, li, sel_pdf);
+            if (p.capture && !(flags & REC_LSEL) && li >= 0 && li < 16) { flags |= REC_LSEL; light_bits = (uint32_t)li; }
+            if (li >= 0 && (!delta_brdf || light_is_delta(S.lights[li]))) {
+                const DevLight& lt = S.lights[li];
+                const bool is_delta = light_is_delta(lt);
+                const v2 lu = dr.next2();
+                if (p.capture && !(flags & REC_LIGHT) && !is_delta) { flags |= REC_LIGHT; c_light = lu; }
+                v3 wi; float dist;
+                light_sample(lt, sp, lu, wi, dist);
+                if (dist > RPD_EPS_SMALL || is_delta) {
+                    const float pl = light_pdf(S, li, sel_pdf, wi, dist);
+                    if (pl > RPD_EPS_SMALL) {
+                        const float pw = fmaxf(RPD_EPS_SMALL, brdf_pdf(m, swo, wi, sn));
+                        const float wl = is_delta ? 1.0f : pl / (pl + pw);
+                        if (!any_hit(S, offset_origin(sp, sn), wi, dist)) {
+                            const v3 f = brdf_eval(m, swo, wi, sn);
+                            const v3 em = eval_light(lt, -wi, dist);
+                            if (maxc3(em) > 0.0f) {
+                                const float g = delta_brdf ? 1.0f : fmaxf(0.0f, dot3(sn, wi));
+                                const v3 contrib = (vmul(em, f) * g) / pl;
+                                Ld = Ld + wl * contrib;
+                            }
+                        }
+                    }
+                }
+            }
+            const v2 eu = dr.next2();
+            if (env_on && !delta_brdf) {
+                const v3 ed = sample_sphere(eu);
+                if (p.capture && !(flags & REC_ENV)) { flags |= REC_ENV; c_env = eu; }
+                const float bp = brdf_pdf(m, swo, ed, sn);
+                const float we = pdf_env / (pdf_env + bp);
+                if (!any_hit(S, offset_origin(sp, sn), ed, __builtin_inff())) {
+                    const v3 f = brdf_eval(m, swo, ed, sn);
+                    const float c = fmaxf(0.0f, dot3(sn, ed));
+                    Ld = Ld + (we * vmul(env, f)) * (c / pdf_env);
+                }
+            }
+            const v2 bu = dr.next2();
+            if (p.capture && !delta_brdf && !(flags & REC_BRDF)) { flags |= REC_BRDF; c_brdf = bu; }
+            v3 dir; float pdf;
+            brdf_sample(m, swo, sn, bu, dir, pdf);
+            if (pdf > RPD_EPS_SMALL && sqnorm3(dir) > 0.0f) {
+                bpdf = pdf;
+                bw = brdf_sample_weight(m, swo, dir, sn, pdf);
+                ro = offset_origin(sp, sn);
+                rd = dir;
+                phase = PH_TRACE_EXT;
+            } else {
+                L = L + vmul(T, Ld);
+                phase = PH_DONE;
+            }
+        }
+
+        // -- closest hit for camera and extension rays ------------------------
+        if (phase == PH_TRACE_PRIMARY || phase == PH_TRACE_EXT) {
+            const HitRec h = closest_hit(S, ro, rd);
+            bool emitter = false;
+            int li = -1;
+            v3 hn = mk3(0, 0, 0);
+            int hm = 0;
+            if (h.hit) {
+                if (h.sphere) {
+                    const DevSphere& s = S.spheres[h.index];
+                    emitter = s.emitter != 0; li = s.light; hn = h.sn; hm = s.material;
+                } else {
+                    const DevTriangle& t = S.tris[h.index];
+                    emitter = t.emitter != 0; li = t.light; hn = ld3(t.n); hm = t.material;
+                }
+            }
+            if (phase == PH_TRACE_PRIMARY) {
+                // main.cpp:540-585
+                if (!h.hit) {
+                    L = env;
+                    phase = PH_DONE;
+                } else {
+                    const v3 hp = ro + h.t * rd;
+                    flags |= REC_HIT;
+                    p.rec.r1[slot] = make_float4(hn.x, hn.y, hn.z, h.t);
+                    const DevMaterial& hmat = S.materials[hm];
+                    p.rec.r2[slot] = make_float4(hmat.albedo[0], hmat.albedo[1], hmat.albedo[2], 0.0f);
+                    p.rec.r3[slot] = make_float4(hp.x, hp.y, hp.z, 0.0f);
+                    if (emitter) {
+                        L = eval_light(S.lights[li], -rd, h.t);
+                        phase = PH_DONE;
+                    } else {
+                        sp = hp; sn = hn; swo = -normalized3(rd); smat = hm;
+                        phase = PH_SHADE;
+                    }
+                }
+            } else {
+                // main.cpp:480-529
+                const DevMaterial& m = S.materials[smat];
+                const bool delta_brdf = m.kind == MAT_MIRROR;
+                if (h.hit && emitter) {
+                    const DevLight& E = S.lights[li];
+                    float wb = 1.0f;
+                    if (!delta_brdf && !light_is_delta(E)) {
+                        const float plb = light_pdf(S, li, light_select_pdf(S, li), rd, h.t);
+                        if (plb > RPD_EPS_SMALL) wb = bpdf / (bpdf + plb);
+                    }
+                    Ld = Ld + wb * vmul(eval_light(E, -rd, h.t), bw);
+                } else if (!h.hit) {
+                    const float wb = delta_brdf ? 1.0f : bpdf / (bpdf + pdf_env);
+                    Ld = Ld + wb * vmul(env, bw);
+                }
+                L = L + vmul(T, Ld);
+                phase = PH_DONE;
+                if (h.hit && !emitter) {
+                    const float ksi = dr.next();
+                    if (p.capture && !(flags & REC_RR)) { flags |= REC_RR; if (ksi < 0.8f) flags |= REC_RR_SURVIVED; }
+                    if (ksi < 0.8f) {
+                        T = vmul(T, bw) / 0.8f;
+                        sp = ro + h.t * rd;
+                        sn = hn;
+                        swo = -rd;
+                        smat = hm;
+                        phase = PH_SHADE;
+                    }
+                }
+            }
+        }
+    }
+}
+
+int trace_block_size() { return TRACE_BLOCK; }
+const void* trace_kernel_ptr() { return reinterpret_cast<const void*>(&k_trace); }
+
+void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_trace, dim3(n_blocks), dim3(TRACE_BLOCK), 0, st, p);
+}
+
+// ---------------------------------------------------------------------------
+// Pass plans. A uniform pass samples every pixel `count` times; an adaptive
+// pass samples counts[p] times. offsets[p] is the pixel's first sample slot.
+__global__ void k_plan_uniform(int n, int count, int* counts, int* offsets, int* active, int* totals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) { totals[0] = n * count; totals[1] = count > 0 ? n : 0; totals[2] = 0; }
+    if (i >= n) return;
+    counts[i] = count;
+    offsets[i] = i * count;
+    active[i] = i;
+}
+void launch_plan_uniform(int n, int count, int* counts, int* offsets, int* active, int* totals, hipStream_t st) {
+    hipLaunchKernelGGL(k_plan_uniform, dim3((n + 255) / 256), dim3(256), 0, st, n, count, counts, offsets, active,
+                       totals);
+}
+__global__ void k_fill_i32(int* p, int v, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+void launch_fill_i32(int* p, int v, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
+}
+
+// Exclusive scan of (active flag << 32 | count) in three steps: per-block
+// inclusive sums, one block scanning the block totals, then the fix-up that
+// writes offsets and the compacted active list.
+constexpr int SCAN_BLOCK = 1024;
+
+__device__ __forceinline__ int64_t block_incl_scan(int64_t v, int64_t* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = 1; o < SCAN_BLOCK; o <<= 1) {
+        const int64_t add = t >= o ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    return sh[t];
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_blocks(int n, const int* counts, int64_t* block_sums) {
+    __shared__ int64_t sh[SCAN_BLOCK];
+    const int i = blockIdx.x * SCAN_BLOCK + threadIdx.x;
+    int64_t v = 0;
+    if (i < n) { const int c = counts[i]; v = (int64_t(c > 0 ? 1 : 0) << 32) | int64_t(c); }
+    const int64_t s = block_incl_scan(v, sh);
+    if (threadIdx.x == SCAN_BLOCK - 1) block_sums[blockIdx.x] = s;
+}
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_totals(int nb, int64_t* block_sums, int* totals,
+                                                            int64_t capacity) {
+    __shared__ int64_t sh[SCAN_BLOCK];
+    int64_t carry = 0;
+    for (int base = 0; base < nb; base += SCAN_BLOCK) {
+        const int i = base + threadIdx.x;
+        const int64_t v = i < nb ? block_sums[i] : 0;
+        const int64_t s = block_incl_scan(v, sh);
+        if (i < nb) block_sums[i] = carry + s - v;  // exclusive
+        __syncthreads();
+        carry += sh[SCAN_BLOCK - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int64_t samples = carry & 0xffffffffll;
+        const bool over = samples > capacity;
+        totals[0] = int(samples);
+        totals[1] = over ? 0 : int(carry >> 32);
+        totals[2] = over ? 1 : 0;
+    }
+}
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(int n, const int* counts, const int64_t* block_sums,
+                                                           int* offsets, int* active) {
+    __shared__ int64_t sh[SCAN_BLOCK];
+    const int i = blockIdx.x * SCAN_BLOCK + threadIdx.x;
+    int64_t v = 0;
+    int c = 0;
+    if (i < n) { c = counts[i]; v = (int64_t(c > 0 ? 1 : 0) << 32) | int64_t(c); }
+    const int64_t s = block_incl_scan(v, sh) - v + block_sums[blockIdx.x];
+    if (i < n) {
+        offsets[i] = int(s & 0xffffffffll);
+        if (c > 0) active[int(s >> 32)] = i;
+    }
+}
+int64_t plan_scan_scratch_elems(int n) { return (n + SCAN_BLOCK - 1) / SCAN_BLOCK + 1; }
+void launch_plan_scan(int n, const int* counts, int* offsets, int* active, int* totals, int64_t* scratch,
+                      int64_t capacity, hipStream_t st) {
+    const int nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_BLOCK), 0, st, n, counts, scratch);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(SCAN_BLOCK), 0, st, nb, scratch, totals, capacity);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, n, counts, scratch, offsets, active);
+}
+
+// ---------------------------------------------------------------------------
+// Fold a pass's samples into the per-pixel accumulators in sample order
+// (main.cpp:572-576, 707-717; core/stats.h:25-41). Features are summed for every
+// sample with a primary hit, statistics and radiance only for finite samples.
+__global__ void __launch_bounds__(256) k_fold(PassPlan plan, Records rec, PixelBuffers pb, int statmc) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= plan.totals[1]) return;
+    const int i = plan.active[a];
+    const int N = plan.n_pixels;
+    const int cnt = plan.counts[i];
+    const int64_t off = plan.offsets[i];
+    float rr = pb.radiance[i], rg = pb.radiance[N + i], rb = pb.radiance[2 * N + i];
+    float ar = pb.albedo[i], ag = pb.albedo[N + i], ab = pb.albedo[2 * N + i];
+    float nx = pb.normal[i], ny = pb.normal[N + i], nz = pb.normal[2 * N + i];
+    float px = pb.world_pos[i], py = pb.world_pos[N + i], pz = pb.world_pos[2 * N + i];
+    float dep = pb.depth[i];
+    int hits = pb.hit_count[i], scount = pb.sample_count[i];
+    float mean = 0, m2 = 0;
+    int n = 0;
+    v3 cm = mk3(0, 0, 0), sm = mk3(0, 0, 0), s2 = mk3(0, 0, 0);
+    if (statmc) {
+        mean = pb.lum_mean[i]; m2 = pb.lum_m2[i]; n = pb.n[i];
+        cm = mk3(pb.color_mean[i], pb.color_mean[N + i], pb.color_mean[2 * N + i]);
+        sm = mk3(pb.sqrt_mean[i], pb.sqrt_mean[N + i], pb.sqrt_mean[2 * N + i]);
+        s2 = mk3(pb.sqrt_m2[i], pb.sqrt_m2[N + i], pb.sqrt_m2[2 * N + i]);
+    }
+    for (int s = 0; s < cnt; ++s) {
+        const float4 r0 = rec.r0[off + s];
+        const uint32_t fl = __float_as_uint(r0.w);
+        if (fl & REC_HIT) {
+            const float4 r1 = rec.r1[off + s], r2 = rec.r2[off + s], r3 = rec.r3[off + s];
+            ++hits;
+            nx += r1.x; ny += r1.y; nz += r1.z;
+            ar += r2.x; ag += r2.y; ab += r2.z;
+            px += r3.x; py += r3.y; pz += r3.z;
+            dep += r1.w;
+        }
+        if (!(fl & REC_FINITE)) continue;
+        const v3 c = mk3(r0.x, r0.y, r0.z);
+        if (statmc) {
+            const float lum = luminance3(c);
+            const float d = lum - mean;
+            ++n;
+            mean += d / float(n);
+            const float d2 = lum - mean;
+            m2 += d * d2;
+            cm = cm + (c - cm) / float(n);
+            const v3 sc = mk3(sqrtf(fmaxf(0.0f, c.x)), sqrtf(fmaxf(0.0f, c.y)), sqrtf(fmaxf(0.0f, c.z)));
+            const v3 sd = sc - sm;
+            sm = sm + sd / float(n);
+            s2 = s2 + vmul(sd, sc - sm);
+        }
+        rr += c.x; rg += c.y; rb += c.z;
+        ++scount;
+    }
+    pb.radiance[i] = rr; pb.radiance[N + i] = rg; pb.radiance[2 * N + i] = rb;
+    pb.albedo[i] = ar; pb.albedo[N + i] = ag; pb.albedo[2 * N + i] = ab;
+    pb.normal[i] = nx; pb.normal[N + i] = ny; pb.normal[2 * N + i] = nz;
+    pb.world_pos[i] = px; pb.world_pos[N + i] = py; pb.world_pos[2 * N + i] = pz;
+    pb.depth[i] = dep;
+    pb.hit_count[i] = hits;
+    pb.sample_count[i] = scount;
+    if (statmc) {
+        pb.lum_mean[i] = mean; pb.lum_m2[i] = m2; pb.n[i] = n;
+        pb.color_mean[i] = cm.x; pb.color_mean[N + i] = cm.y; pb.color_mean[2 * N + i] = cm.z;
+        pb.sqrt_mean[i] = sm.x; pb.sqrt_mean[N + i] = sm.y; pb.sqrt_mean[2 * N + i] = sm.z;
+        pb.sqrt_m2[i] = s2.x; pb.sqrt_m2[N + i] = s2.y; pb.sqrt_m2[2 * N + i] = s2.z;
+    }
+}
+
+void launch_fold(const PassPlan& plan, const Records& rec, PixelBuffers pb, int statmc, hipStream_t st) {
+    hipLaunchKernelGGL(k_fold, dim3((plan.n_pixels + 255) / 256), dim3(256), 0, st, plan, rec, pb, statmc);
+}
+
+// ---------------------------------------------------------------------------
+// core/buffers.h:152-167: averages over samples and primary hits.
+__global__ void k_finalize(int n, PixelBuffers pb, float* orad, float* oalb, float* onrm, float* opos, float* odep) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float s = float(max(1, pb.sample_count[i])), h = float(max(1, pb.hit_count[i]));
+    for (int c = 0; c < 3; ++c) {
+        orad[c * n + i] = pb.radiance[c * n + i] / s;
+        oalb[c * n + i] = pb.albedo[c * n + i] / h;
+        opos[c * n + i] = pb.world_pos[c * n + i] / h;
+    }
+    v3 nn = mk3(pb.normal[i] / h, pb.normal[n + i] / h, pb.normal[2 * n + i] / h);
+    if (sqnorm3(nn) > RPD_EPS_SMALL) nn = normalized3(nn);
+    onrm[i] = nn.x; onrm[n + i] = nn.y; onrm[2 * n + i] = nn.z;
+    odep[i] = pb.depth[i] / h;
+}
+void launch_finalize(int n, const PixelBuffers& pb, float* orad, float* oalb, float* onrm, float* opos, float* odep,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_finalize, dim3((n + 255) / 256), dim3(256), 0, st, n, pb, orad, oalb, onrm, opos, odep);
+}
+
+}  // namespace rpd

@@ -1,0 +1,156 @@
+// Kernel parameter blocks and launch wrappers shared by the engine's kernel
+// files and its host orchestration (engine.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "rpd_device.h"
+
+namespace rpd {
+
+// Per-sample record, structure of arrays indexed by sample slot. Slots of one
+// pixel are contiguous and in sample order: slot = offsets[pixel] + s.
+//   r0 = (radiance rgb, flag bits)          flags: REC_*
+//   r1 = (primary normal xyz, hit distance)
+//   r2 = (primary albedo rgb, 0)
+//   r3 = (primary position xyz, 0)
+//   r4 = (pixel_u xy, lens_u xy)           canonical [0,1) draws
+//   r5 = (brdf_u xy, light_u xy)
+//   r6 = (environment_u xy)
+enum : uint32_t {
+    REC_FINITE = 1u << 0,
+    REC_HIT = 1u << 1,
+    REC_PIXEL = 1u << 2,
+    REC_LENS = 1u << 3,
+    REC_LSEL = 1u << 4,
+    REC_BRDF = 1u << 5,
+    REC_LIGHT = 1u << 6,
+    REC_ENV = 1u << 7,
+    REC_RR = 1u << 8,
+    REC_RR_SURVIVED = 1u << 9,
+    REC_LIGHT_SHIFT = 16,  // bits 16..23: light index of the first light selection
+};
+struct Records {
+    float4* r0; float4* r1; float4* r2; float4* r3; float4* r4; float4* r5; float2* r6;
+    int64_t capacity;
+};
+
+struct Camera {
+    float pos[3], right[3], up[3], llc[3], horiz[3], vert[3];
+    float lens_radius;
+};
+
+struct PassPlan {            // which pixels a pass samples and where their slots are
+    const int* counts;       // samples per pixel
+    const int* offsets;      // first slot per pixel (exclusive scan of counts)
+    const int* active;       // compacted pixel ids with counts > 0
+    const int* totals;       // device: [0] samples in the pass, [1] active pixels (0 if over capacity)
+    int n_pixels;
+};
+
+struct TraceParams {
+    DevScene scene;
+    Camera cam;
+    int width, height;
+    int capture;             // record random parameters (StatMC with RPD on)
+    PassPlan plan;
+    const int* seed_n;       // per-pixel sample offset used for seeding (finite samples so far), or null for 0
+    uint32_t base_seed;
+    uint32_t* tape;          // n_active * 624 words
+    int* work_counter;       // zero before launch
+    Records rec;
+};
+
+// Per-pixel accumulators (structure of arrays, n_pixels each unless noted)
+struct PixelBuffers {
+    float* radiance;         // 3 planes: r, g, b sums over finite samples
+    float* albedo;           // 3 planes: sums over primary hits
+    float* normal;           // 3 planes
+    float* world_pos;        // 3 planes
+    float* depth;            // hit distance sum
+    int* hit_count;
+    int* sample_count;
+    // Welford statistics (path_tracer/core/stats.h:6-41)
+    float* lum_mean;
+    float* lum_m2;
+    int* n;
+    float* color_mean;       // 3 planes
+    float* sqrt_mean;        // 3 planes
+    float* sqrt_m2;          // 3 planes
+};
+
+struct Features {            // path_tracer/main.cpp:94-122
+    float* normal;           // 3 planes, unit where the sum was non-degenerate
+    float* depth;
+    float* albedo;           // 3 planes
+};
+
+// RPF grid: 440 doubles per node, accumulators in this order, each laid out as
+// weight[bins], sum_y[bins], w_sum, w2_sum, y_sum, y2_sum:
+//   pixel, brdf, screen_brdf, lens, screen_lens, light_uv, screen_light_uv,
+//   environment, screen_environment, light_select(16 bins),
+//   screen_light_select, rr(2 bins), screen_rr
+constexpr int GRID_ACC = 13;
+constexpr int GRID_NODE_DOUBLES = 440;
+__host__ __device__ constexpr int grid_acc_bins(int k) { return k == 11 ? 2 : 16; }
+__host__ __device__ constexpr int grid_acc_offset(int k) { return k <= 11 ? 36 * k : 36 * 11 + 8; }
+
+struct GridGeom { int support, step, nx, ny; };
+
+// Sensitivity tiles over grid nodes (path_tracer/core/buffers.h:18-66), field order:
+enum : int {
+    SF_ALL = 0, SF_ALL_CONF, SF_PIXEL, SF_PIXEL_CONF, SF_BRDF, SF_BRDF_CONF, SF_LENS, SF_LENS_CONF,
+    SF_LIGHT, SF_LIGHT_CONF, SF_LIGHT_UV, SF_LIGHT_UV_CONF, SF_LIGHT_SELECT, SF_LIGHT_SELECT_CONF,
+    SF_ENV, SF_ENV_CONF, SF_RR, SF_RR_CONF, SF_COUNT
+};
+
+struct FilterConfig {
+    int radius;
+    float normal_threshold, depth_threshold, compat_alpha, sigma_max, rpd_scale;
+};
+struct VarianceConfig {
+    int radius, iterations;
+    float normal_threshold, depth_threshold, compat_sigma, shrinkage_k, sigma_max;
+};
+
+// ---- launch wrappers (each enqueues on `st`) ----
+void launch_tape(const TraceParams& p, hipStream_t st);
+void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st);
+int trace_block_size();
+const void* trace_kernel_ptr();
+void launch_fold(const PassPlan& plan, const Records& rec, PixelBuffers pb, int statmc, hipStream_t st);
+void launch_plan_uniform(int n_pixels, int count, int* counts, int* offsets, int* active, int* totals,
+                         hipStream_t st);
+// exclusive scan of counts -> offsets, compaction -> active; writes totals[0] = samples, totals[1] = active.
+// A pass whose samples exceed `capacity` gets totals[1] = 0 and sets totals[2] = 1.
+void launch_plan_scan(int n_pixels, const int* counts, int* offsets, int* active, int* totals,
+                      int64_t* scratch, int64_t capacity, hipStream_t st);
+void launch_fill_i32(int* p, int value, int64_t n, hipStream_t st);
+int64_t plan_scan_scratch_elems(int n_pixels);
+
+void launch_features(int n, const PixelBuffers& pb, Features f, hipStream_t st);
+void launch_grid_gather(GridGeom g, int width, int height, const PassPlan& plan, const Records& rec,
+                        double* grid, hipStream_t st);
+void launch_sensitivity(GridGeom g, int width, int height, float conf_n0, const double* grid, const Features& f,
+                        float* node_scratch, float* tiles, float* confidence, float* gradient, hipStream_t st);
+void launch_sensitivity_zero(GridGeom g, int width, int height, float* tiles, float* confidence, float* gradient,
+                             hipStream_t st);
+void launch_statmc(int width, int height, const FilterConfig& fc, const PixelBuffers& pb, const Features& f,
+                   GridGeom g, const float* tiles, const float* gradient, float* scratch,
+                   float* denoised, float* uncertainty, float* alpha, float* var_total, float* var_eff,
+                   float* var_ratio, hipStream_t st);
+void launch_variance(int width, int height, const VarianceConfig& vc, const PixelBuffers& pb, const Features& f,
+                     float* tmp, float* out, hipStream_t st);
+struct AllocConfig {
+    int adaptive_spp, base_samples, radius;
+    float sigma_max;
+    uint32_t seed;
+};
+// writes counts (per pixel) and importance; scratch must hold alloc_scratch_bytes(n)
+void launch_allocation(int width, int height, const AllocConfig& ac, const PixelBuffers& pb, const Features& f,
+                       const float* variance_denoised, float* importance, int* counts, void* scratch,
+                       hipStream_t st);
+size_t alloc_scratch_bytes(int n_pixels);
+void launch_finalize(int n, const PixelBuffers& pb, float* out_radiance, float* out_albedo, float* out_normal,
+                     float* out_pos, float* out_depth, hipStream_t st);
+
+}  // namespace rpd

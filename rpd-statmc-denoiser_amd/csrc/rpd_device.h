@@ -1,0 +1,489 @@
+// Device-side math, sampler and scene helpers for gfx950.
+//
+// All kernels are compiled with -ffp-contract=off: the reference renderer is
+// built for the x86-64 baseline ISA (no FMA), so every a*b+c there rounds
+// twice and parity needs the same here. Fixed evaluation orders below follow
+// the reference's Eigen fixed-size expressions (3-term sums as x0 + (x1 + x2),
+// per-component division in normalize) and its float/double promotions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define RPD_EPS 1e-4f         // path_tracer/core/common.h:5
+#define RPD_EPS_ANYHIT 5e-4f  // path_tracer/core/common.h:6
+#define RPD_EPS_SMALL 1e-8f   // shared/global.h:4
+#define RPD_PI 3.14159265358979323846
+
+namespace rpd {
+
+struct v2 { float x, y; };
+struct v3 { float x, y, z; };
+
+__device__ __forceinline__ v3 mk3(float a, float b, float c) { v3 r; r.x = a; r.y = b; r.z = c; return r; }
+__device__ __forceinline__ v3 operator+(v3 a, v3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 operator-(v3 a, v3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 operator-(v3 a) { return mk3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ v3 operator*(v3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ v3 operator*(float s, v3 a) { return mk3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ v3 operator/(v3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ v3 vmul(v3 a, v3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ v3 vdiv(v3 a, v3 b) { return mk3(a.x / b.x, a.y / b.y, a.z / b.z); }
+__device__ __forceinline__ float dot3(v3 a, v3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }
+__device__ __forceinline__ float sqnorm3(v3 a) { return a.x * a.x + (a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ float norm3(v3 a) { return sqrtf(sqnorm3(a)); }
+__device__ __forceinline__ v3 normalized3(v3 a) {
+    const float z = sqnorm3(a);
+    if (z > 0.0f) { const float s = sqrtf(z); return mk3(a.x / s, a.y / s, a.z / s); }
+    return a;
+}
+__device__ __forceinline__ v3 cross3(v3 a, v3 b) {
+    return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float maxc3(v3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+__device__ __forceinline__ bool finite3(v3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+__device__ __forceinline__ float comp3(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+__device__ __forceinline__ float lerpf_(float a, float b, float t) { return a + (b - a) * t; }
+__device__ __forceinline__ float luminance3(v3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+// core/common.h:14-35: Pixar revised orthonormal basis
+__device__ __forceinline__ void onb(v3 n, v3& bx, v3& by) {
+    const float sign = copysignf(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    bx = mk3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    by = mk3(b, sign + n.y * n.y * a, -n.y);
+}
+__device__ __forceinline__ v3 local_to_world(v3 v, v3 n) {
+    v3 bx, by; onb(n, bx, by);
+    return v.x * bx + v.y * by + v.z * n;
+}
+__device__ __forceinline__ v3 world_to_local(v3 v, v3 n) {
+    v3 bx, by; onb(n, bx, by);
+    return mk3(dot3(v, bx), dot3(v, by), dot3(v, n));
+}
+__device__ __forceinline__ v3 offset_origin(v3 p, v3 n) { return p + RPD_EPS * n; }
+
+// path_tracer/main.cpp:266-286
+__host__ __device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+__host__ __device__ __forceinline__ float hash_to_unit_float(uint32_t x) {
+    return float(hash_u32(x) & 0x00ffffffu) / float(0x01000000u);
+}
+__host__ __device__ __forceinline__ uint32_t pixel_sample_seed(uint32_t base, int idx, int n) {
+    return hash_u32(base ^ hash_u32(uint32_t(idx) + 0x9e3779b9u) ^ hash_u32(uint32_t(n) + 0x85ebca6bu));
+}
+__device__ __forceinline__ float relative_depth_delta(float a, float b) {
+    return fabsf(a - b) / fmaxf(fmaxf(fabsf(a), fabsf(b)), RPD_EPS_SMALL);
+}
+
+// ---- MT19937 tape -----------------------------------------------------------
+// The reference seeds a std::mt19937 once per pixel per pass and draws
+// std::uniform_real_distribution<float> values from it (core/sampler.h:7-16).
+// A tape holds one pixel's 624-word generator state right after a twist; draws
+// temper one word each, and a twist regenerates the tape in place.
+constexpr int MT_N = 624;
+constexpr int MT_M = 397;
+constexpr uint32_t MT_UPPER = 0x80000000u;
+constexpr uint32_t MT_LOWER = 0x7fffffffu;
+constexpr uint32_t MT_MATRIX = 0x9908b0dfu;
+
+__device__ __forceinline__ uint32_t mt_mix(uint32_t hi_word, uint32_t lo_word, uint32_t far_word) {
+    const uint32_t y = (hi_word & MT_UPPER) | (lo_word & MT_LOWER);
+    return far_word ^ (y >> 1) ^ ((y & 1u) ? MT_MATRIX : 0u);
+}
+__device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
+    y ^= (y >> 11);
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= (y >> 18);
+    return y;
+}
+// libstdc++ generate_canonical<float,24> with a 32-bit engine: float(u)/2^32,
+// a result of 1 is replaced by the largest float below 1.
+__device__ __forceinline__ float mt_to_unit(uint32_t u) {
+    float r = float(u) / 4294967296.0f;
+    return r >= 1.0f ? 0x1.fffffep-1f : r;
+}
+// Regenerates a tape in place (the standard twist over all 624 words).
+__device__ inline void mt_twist(uint32_t* __restrict__ s) {
+    for (int k = 0; k < MT_N - MT_M; ++k) s[k] = mt_mix(s[k], s[k + 1], s[k + MT_M]);
+    for (int k = MT_N - MT_M; k < MT_N - 1; ++k) s[k] = mt_mix(s[k], s[k + 1], s[k + MT_M - MT_N]);
+    s[MT_N - 1] = mt_mix(s[MT_N - 1], s[0], s[MT_M - 1]);
+}
+
+// Per-lane draw cursor over a tape with a 4-word register cache.
+struct Draws {
+    uint32_t* tape;
+    int pos;      // next word index in [0, 624]
+    uint4 cache;  // words [pos & ~3, pos | 3]
+    __device__ __forceinline__ void reset(uint32_t* t) { tape = t; pos = 0; }
+    __device__ __forceinline__ float next() {
+        if (pos == MT_N) {
+            mt_twist(tape);
+            pos = 0;
+            cache = *reinterpret_cast<const uint4*>(tape);
+        } else if ((pos & 3) == 0) {
+            cache = *reinterpret_cast<const uint4*>(tape + pos);
+        }
+        const int k = pos & 3;
+        const uint32_t w = k == 0 ? cache.x : (k == 1 ? cache.y : (k == 2 ? cache.z : cache.w));
+        ++pos;
+        return mt_to_unit(mt_temper(w));
+    }
+    // Vec2f(next1d(), next1d()) as GCC evaluates it: x is the second draw.
+    __device__ __forceinline__ v2 next2() {
+        const float first = next();
+        const float second = next();
+        v2 r; r.x = second; r.y = first; return r;
+    }
+};
+
+// core/sampler.h:22-35
+__device__ __forceinline__ v2 sample_disk(v2 u) {
+    const float ox = 2.0f * u.x - 1.0f, oy = 2.0f * u.y - 1.0f;
+    v2 r; r.x = 0.0f; r.y = 0.0f;
+    if (ox == 0.0f && oy == 0.0f) return r;
+    float theta, rad;
+    if (fabsf(ox) > fabsf(oy)) { rad = ox; theta = float((RPD_PI / 4.0) * double(oy / ox)); }
+    else { rad = oy; theta = float((RPD_PI / 2.0) - (RPD_PI / 4.0) * double(ox / oy)); }
+    r.x = rad * cosf(theta); r.y = rad * sinf(theta);
+    return r;
+}
+// core/sampler.h:37-43
+__device__ __forceinline__ v3 sample_sphere(v2 u) {
+    const float z = 1.0f - 2.0f * u.x;
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    const float phi = float(2.0 * RPD_PI * double(u.y));
+    return mk3(r * cosf(phi), r * sinf(phi), z);
+}
+
+// ---- scene ------------------------------------------------------------------
+enum : int { MAT_LAMBERT = 0, MAT_MICROFACET = 1, MAT_MIRROR = 2 };
+enum : int { LIGHT_RECT = 0, LIGHT_AREA = 1, LIGHT_POINT = 2, LIGHT_DIRECTIONAL = 3 };
+
+struct DevMaterial {  // 48 B
+    int kind, beckmann;
+    float roughness, pad;
+    float albedo[3], n1[3], n2[3], pad2[3];
+};
+struct DevLight {  // 80 B
+    int kind;
+    float area, power, pad;
+    float p0[3], p1[3], p2[3], normal[3], emission[3], pad2;
+};
+// Triangles as 16 floats: v0, v1, v2, normal, then material id, light id
+// (-1 if not an emitter) as integer bits.
+struct DevTriangle { float v0[3], v1[3], v2[3], n[3]; int material, light, emitter, pad; };
+struct DevSphere { float c[3], r; int material, light, emitter, pad; };
+
+struct DevScene {
+    const DevTriangle* tris; int n_tris;
+    const DevSphere* spheres; int n_spheres;
+    const DevMaterial* materials;
+    const DevLight* lights; int n_lights;
+    const float* light_cdf;  // inclusive cumulative power
+    float env[3];
+    int env_on;
+};
+
+__device__ __forceinline__ v3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
+
+struct RayPre {  // watertight-test setup that depends on the ray only
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    float okx, oky, okz;
+};
+__device__ __forceinline__ RayPre ray_pre(v3 o, v3 d) {
+    RayPre r;
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int axis = 0;
+    if (ay > ax && ay > az) axis = 1;
+    if (az > ax && az > ay) axis = 2;
+    int kz = axis, kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
+    if (comp3(d, kz) < 0.0f) { const int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    const float dkz = comp3(d, kz);
+    r.Sx = comp3(d, kx) / dkz;
+    r.Sy = comp3(d, ky) / dkz;
+    r.Sz = 1.0f / dkz;
+    r.okx = comp3(o, kx); r.oky = comp3(o, ky); r.okz = comp3(o, kz);
+    return r;
+}
+__device__ __forceinline__ float pick(const float* v, int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+
+// Watertight ray/triangle test, path_tracer/core/triangle.h:35-104.
+__device__ __forceinline__ bool tri_hit(const DevTriangle& tri, const RayPre& r, float t_min, float t_max, float& t_out) {
+    const float Akx = pick(tri.v0, r.kx) - r.okx, Aky = pick(tri.v0, r.ky) - r.oky, Akz = pick(tri.v0, r.kz) - r.okz;
+    const float Bkx = pick(tri.v1, r.kx) - r.okx, Bky = pick(tri.v1, r.ky) - r.oky, Bkz = pick(tri.v1, r.kz) - r.okz;
+    const float Ckx = pick(tri.v2, r.kx) - r.okx, Cky = pick(tri.v2, r.ky) - r.oky, Ckz = pick(tri.v2, r.kz) - r.okz;
+    const float Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
+    const float Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
+    const float Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+    const float t = T * (1.0f / det);
+    if (t < t_min || t > t_max) return false;
+    t_out = t;
+    return true;
+}
+
+// path_tracer/core/sphere.h:19-54
+__device__ __forceinline__ bool sphere_hit(const DevSphere& s, v3 o, v3 d, float t_min, float t_max, float& t_out, v3& n_out) {
+    const v3 oc = o - ld3(s.c);
+    const float a = sqnorm3(d);
+    const float hb = dot3(oc, d);
+    const float c = sqnorm3(oc) - s.r * s.r;
+    const float disc = hb * hb - a * c;
+    if (disc < 0.0f) return false;
+    const float sd = sqrtf(disc);
+    float t = (-hb - sd) / a;
+    if (t < t_min || t > t_max) {
+        t = (-hb + sd) / a;
+        if (t < t_min || t > t_max) return false;
+    }
+    n_out = normalized3((o + t * d) - ld3(s.c));
+    t_out = t;
+    return true;
+}
+
+struct HitRec {
+    int hit;       // 0 miss
+    int sphere;    // 1 if a sphere
+    int index;
+    float t;
+    v3 sn;         // sphere normal
+};
+
+// Closest hit over all triangles then spheres; ties keep the lower index
+// (path_tracer/core/ray_tracer.h:304-325 combines the two the same way).
+__device__ __forceinline__ HitRec closest_hit(const DevScene& S, v3 o, v3 d) {
+    HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
+    const RayPre rp = ray_pre(o, d);
+    float best = __builtin_inff();
+    int bi = -1;
+    for (int i = 0; i < S.n_tris; ++i) {
+        float t;
+        if (tri_hit(S.tris[i], rp, 0.0f, best, t) && t < best) { best = t; bi = i; }
+    }
+    float sbest = __builtin_inff();
+    int si = -1;
+    v3 sn = mk3(0, 0, 0);
+    for (int i = 0; i < S.n_spheres; ++i) {
+        float t; v3 n;
+        if (sphere_hit(S.spheres[i], o, d, RPD_EPS_SMALL, sbest, t, n) && t < sbest) { sbest = t; si = i; sn = n; }
+    }
+    if (bi >= 0 && (si < 0 || best < sbest)) { h.hit = 1; h.index = bi; h.t = best; }
+    else if (si >= 0) { h.hit = 1; h.sphere = 1; h.index = si; h.t = sbest; h.sn = sn; }
+    return h;
+}
+
+// Occlusion before t_max - 5e-4 (path_tracer/core/ray_tracer.h:65-75, 282-296).
+__device__ __forceinline__ bool any_hit(const DevScene& S, v3 o, v3 d, float t_max) {
+    const float eff = t_max - RPD_EPS_ANYHIT;
+    if (eff <= RPD_EPS_SMALL) return false;
+    const RayPre rp = ray_pre(o, d);
+    bool blocked = false;
+    for (int i = 0; i < S.n_tris; ++i) {
+        float t;
+        if (tri_hit(S.tris[i], rp, 0.0f, eff, t) && t < eff) { blocked = true; break; }
+    }
+    if (blocked) return true;
+    for (int i = 0; i < S.n_spheres; ++i) {
+        float t; v3 n;
+        if (sphere_hit(S.spheres[i], o, d, RPD_EPS_SMALL, eff, t, n)) return true;
+    }
+    return false;
+}
+
+// ---- materials: path_tracer/core/material.h --------------------------------
+__device__ __forceinline__ v3 mf_fresnel(const DevMaterial& m, float c) {
+    c = clampf(c, 0.0f, 1.0f);
+    const v3 n1 = ld3(m.n1), n2 = ld3(m.n2);
+    const v3 dn = n1 - n2, sn = n1 + n2;
+    const v3 r0 = vdiv(vmul(dn, dn), vmul(sn, sn));
+    const float f = powf(1.0f - c, 5.0f);
+    return r0 + (mk3(1.0f, 1.0f, 1.0f) - r0) * f;
+}
+__device__ __forceinline__ float mf_D(const DevMaterial& m, v3 h) {
+    const v3 mm = normalized3(h);
+    const float c = clampf(mm.z, RPD_EPS, 1.0f);
+    const float s2 = fmaxf(0.0f, 1.0f - c * c);
+    const float t2 = s2 / (c * c);
+    const float a = fmaxf(m.roughness, RPD_EPS);
+    if (m.beckmann) {
+        const float rs = a * a;
+        return float(double(expf(-t2 / rs)) / (RPD_PI * double(rs) * double(powf(c, 4.0f))));
+    }
+    const float den = c * c * (a * a - 1.0f) + 1.0f;
+    return float(double(a * a) / (RPD_PI * double(den) * double(den)));
+}
+__device__ __forceinline__ float mf_smith(const DevMaterial& m, v3 v) {
+    const float c = clampf(v.z, RPD_EPS, 1.0f);
+    const float s2 = fmaxf(0.0f, 1.0f - c * c);
+    if (s2 <= RPD_EPS_SMALL) return 1.0f;
+    const float tn = sqrtf(s2) / c;
+    const float a = fmaxf(m.roughness, RPD_EPS);
+    if (m.beckmann) {
+        const float x = 1.0f / (a * tn);
+        if (x >= 1.6f) return 1.0f;
+        return (3.535f * x + 2.181f * x * x) / (1.0f + 2.276f * x + 2.577f * x * x);
+    }
+    const float lambda = (-1.0f + sqrtf(1.0f + a * a * tn * tn)) * 0.5f;
+    return 1.0f / (1.0f + lambda);
+}
+__device__ __forceinline__ float mf_pdf(const DevMaterial& m, v3 wo_w, v3 wi_w, v3 n) {
+    const v3 wo = world_to_local(wo_w, n), wi = world_to_local(wi_w, n);
+    if (wo.z <= RPD_EPS_SMALL || wi.z <= RPD_EPS_SMALL) return 0.0f;
+    const v3 h = normalized3(wo + wi);
+    if (!finite3(h)) return 0.0f;
+    const float ndoth = fmaxf(0.0f, h.z);
+    const float wodoth = fmaxf(0.0f, dot3(wo, h));
+    if (ndoth <= RPD_EPS_SMALL || wodoth <= RPD_EPS_SMALL) return 0.0f;
+    return mf_D(m, h) * ndoth / (4.0f * wodoth);
+}
+__device__ __forceinline__ v3 reflect3(v3 d, v3 n) { return d - (2.0f * dot3(d, n)) * n; }
+
+__device__ inline v3 brdf_eval(const DevMaterial& m, v3 wo_w, v3 wi_w, v3 n) {
+    if (m.kind == MAT_LAMBERT) return ld3(m.albedo) / float(RPD_PI);
+    if (m.kind == MAT_MICROFACET) {
+        const v3 wo = normalized3(world_to_local(wo_w, n)), wi = normalized3(world_to_local(wi_w, n));
+        if (wo.z <= RPD_EPS_SMALL || wi.z <= RPD_EPS_SMALL) return mk3(0, 0, 0);
+        const v3 h = normalized3(wo + wi);
+        if (!finite3(h)) return mk3(0, 0, 0);
+        const float co = clampf(wo.z, 0.0f, 1.0f), ci = clampf(wi.z, 0.0f, 1.0f);
+        const float G = mf_smith(m, wo) * mf_smith(m, wi);
+        const v3 fr = ((mf_fresnel(m, fabsf(dot3(wi, h))) * G) * mf_D(m, h)) / (4.0f * ci * co);
+        return vmul(fr, ld3(m.albedo));
+    }
+    const v3 nn = normalized3(n);
+    if (dot3(wo_w, nn) <= RPD_EPS_SMALL || dot3(wi_w, nn) <= RPD_EPS_SMALL) return mk3(0, 0, 0);
+    const v3 e = normalized3(reflect3(-normalized3(wo_w), nn));
+    const v3 in = normalized3(wi_w);
+    return (sqnorm3(e - in) <= RPD_EPS_SMALL) ? ld3(m.albedo) : mk3(0, 0, 0);
+}
+__device__ inline float brdf_pdf(const DevMaterial& m, v3 wo_w, v3 wi_w, v3 n) {
+    if (m.kind == MAT_LAMBERT) return float(double(fmaxf(0.0f, dot3(wi_w, n))) / RPD_PI);
+    if (m.kind == MAT_MICROFACET) return mf_pdf(m, wo_w, wi_w, n);
+    const v3 nn = normalized3(n);
+    if (dot3(wo_w, nn) <= RPD_EPS_SMALL || dot3(wi_w, nn) <= RPD_EPS_SMALL) return 0.0f;
+    const v3 e = normalized3(reflect3(-normalized3(wo_w), nn));
+    const v3 in = normalized3(wi_w);
+    return (sqnorm3(e - in) <= RPD_EPS_SMALL) ? 1.0f : 0.0f;
+}
+__device__ inline void brdf_sample(const DevMaterial& m, v3 wo_w, v3 n, v2 u, v3& wi, float& pdf) {
+    wi = mk3(0, 0, 0); pdf = 0.0f;
+    if (m.kind == MAT_LAMBERT) {
+        const float r = sqrtf(u.x);
+        const float theta = float(2.0 * RPD_PI * double(u.y));
+        const v3 l = mk3(r * cosf(theta), r * sinf(theta), sqrtf(fmaxf(0.0f, 1.0f - u.x)));
+        wi = local_to_world(l, n);
+        pdf = float(double(l.z) / RPD_PI);
+        return;
+    }
+    if (m.kind == MAT_MICROFACET) {
+        const v3 wo = normalized3(world_to_local(wo_w, n));
+        if (wo.z <= RPD_EPS_SMALL) return;
+        const float u1 = clampf(u.x, RPD_EPS_SMALL, 1.0f - RPD_EPS_SMALL);
+        const float u2 = clampf(u.y, RPD_EPS_SMALL, 1.0f - RPD_EPS_SMALL);
+        const float a = fmaxf(m.roughness, RPD_EPS);
+        v3 wh;
+        if (m.beckmann) {
+            const float r = a * sqrtf(-logf(1.0f - u1));
+            const float phi = float(2.0 * RPD_PI * double(u2));
+            wh = normalized3(mk3(-(r * cosf(phi)), -(r * sinf(phi)), 1.0f));
+        } else {
+            const float phi = float(2.0 * RPD_PI * double(u1));
+            const float ct = sqrtf((1.0f - u2) / (1.0f + (a * a - 1.0f) * u2));
+            const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+            wh = normalized3(mk3(st * cosf(phi), st * sinf(phi), ct));
+        }
+        if (wh.z <= RPD_EPS_SMALL) wh.z = RPD_EPS;
+        const v3 wl = normalized3(-wo + (2.0f * dot3(wo, wh)) * wh);
+        if (wl.z <= RPD_EPS_SMALL) return;
+        wi = local_to_world(wl, n);
+        pdf = mf_pdf(m, wo_w, wi, n);
+        return;
+    }
+    const v3 nn = normalized3(n), wo = normalized3(wo_w);
+    if (dot3(wo, nn) <= RPD_EPS_SMALL) return;
+    const v3 r = normalized3(reflect3(-wo, nn));
+    if (!finite3(r)) return;
+    wi = r; pdf = 1.0f;
+}
+__device__ inline v3 brdf_sample_weight(const DevMaterial& m, v3 wo, v3 wi, v3 n, float pdf) {
+    if (pdf <= RPD_EPS_SMALL) return mk3(0, 0, 0);
+    if (m.kind == MAT_MIRROR) return ld3(m.albedo);
+    return brdf_eval(m, wo, wi, n) * (fmaxf(0.0f, dot3(n, wi)) / pdf);
+}
+
+// ---- lights: path_tracer/core/light.h, core/scene.h --------------------------
+__device__ __forceinline__ bool light_is_delta(const DevLight& l) { return l.kind == LIGHT_POINT || l.kind == LIGHT_DIRECTIONAL; }
+
+__device__ inline void light_sample(const DevLight& l, v3 p, v2 u, v3& wi, float& dist) {
+    wi = mk3(0, 0, 0); dist = 0.0f;
+    if (l.kind == LIGHT_DIRECTIONAL) { wi = ld3(l.p0); dist = __builtin_inff(); return; }
+    v3 target;
+    if (l.kind == LIGHT_RECT) {
+        target = ld3(l.p0) + u.x * ld3(l.p1) + u.y * ld3(l.p2);
+    } else if (l.kind == LIGHT_AREA) {
+        const float su = sqrtf(fmaxf(0.0f, u.x));
+        const float b0 = 1.0f - su, b1 = u.y * su, b2 = 1.0f - b0 - b1;
+        target = b0 * ld3(l.p0) + b1 * ld3(l.p1) + b2 * ld3(l.p2);
+    } else {
+        target = ld3(l.p0);
+    }
+    v3 d = target - p;
+    const float sq = sqnorm3(d);
+    if (sq <= RPD_EPS_SMALL) return;
+    d = normalized3(d);
+    if (l.kind == LIGHT_RECT || l.kind == LIGHT_AREA) {
+        if (dot3(ld3(l.normal), -d) <= RPD_EPS_SMALL) return;
+    }
+    wi = d;
+    dist = sqrtf(sq);
+}
+__device__ inline v3 eval_light(const DevLight& l, v3 dir, float distance) {
+    if (l.kind == LIGHT_RECT || l.kind == LIGHT_AREA)
+        return (dot3(dir, ld3(l.normal)) > 0.0f) ? ld3(l.emission) : mk3(0, 0, 0);
+    if (l.kind == LIGHT_POINT) return ld3(l.emission) / fmaxf(distance * distance, RPD_EPS_SMALL);
+    return ld3(l.emission);
+}
+__device__ __forceinline__ float light_select_pdf(const DevScene& S, int idx) {
+    const float total = S.n_lights > 0 ? S.light_cdf[S.n_lights - 1] : 0.0f;
+    if (total <= RPD_EPS_SMALL) return 0.0f;
+    return S.lights[idx].power / total;
+}
+__device__ __forceinline__ float light_pdf(const DevScene& S, int idx, float sel, v3 dir, float dist) {
+    const DevLight& l = S.lights[idx];
+    float pdf = 1.0f;
+    if (l.kind == LIGHT_RECT || l.kind == LIGHT_AREA) {
+        const float c = dot3(ld3(l.normal), -dir);
+        pdf = (c <= RPD_EPS_SMALL) ? 0.0f : (dist * dist) / (c * l.area);
+    }
+    return sel * pdf;
+}
+// lower_bound over the inclusive power CDF (core/scene.h:54-63)
+__device__ __forceinline__ void sample_light(const DevScene& S, float u, int& idx, float& pdf) {
+    if (S.n_lights == 0) { idx = -1; pdf = 0.0f; return; }
+    const float target = u * S.light_cdf[S.n_lights - 1];
+    int lo = 0, hi = S.n_lights;
+    while (lo < hi) { const int mid = (lo + hi) >> 1; if (S.light_cdf[mid] < target) lo = mid + 1; else hi = mid; }
+    if (lo >= S.n_lights) lo = S.n_lights - 1;
+    idx = lo;
+    pdf = light_select_pdf(S, idx);
+}
+
+}  // namespace rpd

@@ -1,0 +1,250 @@
+"""ctypes binding of the engine's C ABI (include/rpd_engine.h).
+
+Loads lib/librpd_engine.so next to this file and raises if it is missing: there
+is no fallback path. Mirrors the reference renderer's stage functions
+(path_tracer/main.cpp: render, recompute_sensitivity, statmc_denoise,
+variance_denoise, compute_adaptive_sample_counts) on one job's device state.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "librpd_engine.so")
+
+# buffer ids (include/rpd_engine.h)
+BUF = dict(
+    radiance=0, raw_radiance=1, albedo=2, normal=3, world_pos=4, denoised=5,
+    depth=10, uncertainty=11, alpha=12, variance_denoised=13, var_total=14, var_eff=15, var_ratio=16,
+    sensitivity_confidence=17, sensitivity_gradient=18, adaptive_importance=19, sample_fraction=20,
+    sens_all=30, sens_pixel=31, sens_brdf=32, sens_lens=33, sens_light=34, sens_light_uv=35,
+    sens_light_select=36, sens_environment=37, sens_rr=38,
+    sample_count=50, hit_count=51, pass_counts=52,
+    stats=60, sums=61, grid=62, tiles=63, features=64, records=65,
+)
+_RGB = {"radiance", "raw_radiance", "albedo", "normal", "world_pos", "denoised"}
+_INT = {"sample_count", "hit_count", "pass_counts"}
+_WIDTH = {"stats": 12, "sums": 13, "features": 7, "records": 32}
+
+
+class RenderConfig(C.Structure):
+    """POD mirror of RenderConfig (path_tracer/core/config.h:15-49)."""
+    _fields_ = [
+        ("image_width", C.c_int), ("image_height", C.c_int), ("samples_per_pixel", C.c_int),
+        ("use_statmc", C.c_int), ("rpf_tile_size", C.c_int), ("color_window_radius", C.c_int),
+        ("color_normal_threshold", C.c_float), ("color_depth_threshold", C.c_float),
+        ("color_compat_alpha", C.c_float), ("color_sigma_max", C.c_float),
+        ("var_window_radius", C.c_int),
+        ("var_normal_threshold", C.c_float), ("var_depth_threshold", C.c_float),
+        ("var_compat_sigma", C.c_float), ("var_shrinkage_k", C.c_float),
+        ("var_iterations", C.c_int), ("adaptive_base_samples", C.c_int), ("adaptive_spp", C.c_int),
+        ("adaptive_sigma_max", C.c_float),
+        ("adaptive_passes", C.c_int), ("adaptive_importance_smoothing_radius", C.c_int),
+        ("rpf_shrinkage_scale", C.c_float), ("rpf_confidence_samples", C.c_float),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("direction", C.c_float * 3), ("world_up", C.c_float * 3),
+                ("fov_degrees", C.c_float), ("focus_distance", C.c_float), ("aperture", C.c_float)]
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EngineError(f"engine library not built: {LIB_PATH}")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.rpd_last_error.restype = C.c_char_p
+        L.rpd_render_config_defaults.argtypes = [C.POINTER(RenderConfig)]
+        L.rpd_render_config_defaults.restype = None
+        for name, args in [
+            ("rpd_ctx_create", [C.c_int, C.POINTER(vp)]), ("rpd_ctx_destroy", [vp]),
+            ("rpd_ctx_set_memory_budget", [vp, C.c_uint64]),
+            ("rpd_scene_create", [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.POINTER(vp)]),
+            ("rpd_scene_builtin", [vp, C.c_char_p, C.POINTER(vp)]),
+            ("rpd_scene_counts", [vp, vp, vp, vp, vp]),
+            ("rpd_scene_destroy", [vp]),
+            ("rpd_job_create", [vp, vp, C.POINTER(RenderConfig), C.POINTER(Camera), C.c_uint32, C.POINTER(vp)]),
+            ("rpd_job_submit", [vp, vp, C.POINTER(RenderConfig), C.POINTER(Camera), C.c_uint32, C.POINTER(vp)]),
+            ("rpd_job_poll", [vp, C.POINTER(C.c_int)]), ("rpd_job_wait", [vp]), ("rpd_job_release", [vp]),
+            ("rpd_job_run_pass", [vp, vp, C.c_int]), ("rpd_job_run_adaptive_pass", [vp]),
+            ("rpd_job_recompute_sensitivity", [vp]), ("rpd_job_statmc_denoise", [vp]),
+            ("rpd_job_variance_denoise", [vp]), ("rpd_job_adaptive_counts", [vp, vp]),
+            ("rpd_job_finalize", [vp]), ("rpd_job_sync", [vp]),
+            ("rpd_job_buffer_size", [vp, C.c_int, C.POINTER(C.c_size_t)]),
+            ("rpd_job_read", [vp, C.c_int, vp, C.c_size_t]), ("rpd_job_write", [vp, C.c_int, vp, C.c_size_t]),
+            ("rpd_job_grid_dims", [vp, vp]), ("rpd_job_stage_ms", [vp, vp, C.c_int]),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise EngineError(f"engine error {rc}: {lib().rpd_last_error().decode(errors='replace')}")
+
+
+def default_config(**kw):
+    c = RenderConfig()
+    lib().rpd_render_config_defaults(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def make_camera(position=(0.278, 0.2744, -0.8), direction=(0.0, 0.0, 1.0), world_up=(0.0, 1.0, 0.0),
+                fov=38.6, focus_distance=1.0, aperture=0.0):
+    cam = Camera()
+    for i in range(3):
+        cam.position[i] = position[i]
+        cam.direction[i] = direction[i]
+        cam.world_up[i] = world_up[i]
+    cam.fov_degrees = fov
+    cam.focus_distance = focus_distance
+    cam.aperture = aperture
+    return cam
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    def __init__(self, device=0):
+        self.L = lib()
+        h = C.c_void_p()
+        check(self.L.rpd_ctx_create(device, C.byref(h)))
+        self.h = h
+        self._scenes = {}
+
+    def scene(self, name="cornell_box"):
+        if name not in self._scenes:
+            s = C.c_void_p()
+            check(self.L.rpd_scene_builtin(self.h, name.encode(), C.byref(s)))
+            self._scenes[name] = s
+        return self._scenes[name]
+
+    def set_memory_budget(self, nbytes):
+        check(self.L.rpd_ctx_set_memory_budget(self.h, int(nbytes)))
+
+    def close(self):
+        for s in self._scenes.values():
+            self.L.rpd_scene_destroy(s)
+        self._scenes = {}
+        if self.h:
+            self.L.rpd_ctx_destroy(self.h)
+            self.h = None
+
+
+class Job:
+    """One render job's device state; stage methods mirror the reference's functions."""
+
+    def __init__(self, ctx, cfg, cam, seed, scene="cornell_box", submit=False):
+        self.L = ctx.L
+        self.cfg = cfg
+        self.w, self.h_px = cfg.image_width, cfg.image_height
+        h = C.c_void_p()
+        fn = self.L.rpd_job_submit if submit else self.L.rpd_job_create
+        check(fn(ctx.h, ctx.scene(scene), C.byref(cfg), C.byref(cam), seed & 0xFFFFFFFF, C.byref(h)))
+        self.h = h
+        d = np.zeros(4, np.int32)
+        check(self.L.rpd_job_grid_dims(self.h, _ptr(d)))
+        self.nodes_x, self.nodes_y, self.step, self.support = (int(v) for v in d)
+
+    def release(self):
+        if self.h:
+            self.L.rpd_job_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    def wait(self):
+        check(self.L.rpd_job_wait(self.h))
+
+    def poll(self):
+        d = C.c_int()
+        check(self.L.rpd_job_poll(self.h, C.byref(d)))
+        return bool(d.value)
+
+    def run_pass(self, counts=None, statmc=True):
+        if counts is None:
+            check(self.L.rpd_job_run_pass(self.h, None, int(statmc)))
+        else:
+            counts = np.ascontiguousarray(counts, np.int32)
+            assert counts.size == self.w * self.h_px
+            check(self.L.rpd_job_run_pass(self.h, _ptr(counts), int(statmc)))
+
+    def run_adaptive_pass(self):
+        check(self.L.rpd_job_run_adaptive_pass(self.h))
+
+    def recompute_sensitivity(self):
+        check(self.L.rpd_job_recompute_sensitivity(self.h))
+
+    def statmc(self):
+        check(self.L.rpd_job_statmc_denoise(self.h))
+
+    def variance(self):
+        check(self.L.rpd_job_variance_denoise(self.h))
+
+    def adaptive_counts(self):
+        out = np.zeros(self.w * self.h_px, np.int32)
+        check(self.L.rpd_job_adaptive_counts(self.h, _ptr(out)))
+        return out
+
+    def finalize(self):
+        check(self.L.rpd_job_finalize(self.h))
+
+    def stage_ms(self):
+        out = np.zeros(11, np.float64)
+        check(self.L.rpd_job_stage_ms(self.h, _ptr(out), 11))
+        return out
+
+    def nbytes(self, name):
+        n = C.c_size_t()
+        check(self.L.rpd_job_buffer_size(self.h, BUF[name], C.byref(n)))
+        return n.value
+
+    def get(self, name):
+        nb = self.nbytes(name)
+        if name == "grid":
+            out = np.zeros(nb // 8, np.float64)
+        elif name in _INT:
+            out = np.zeros(nb // 4, np.int32)
+        else:
+            out = np.zeros(max(1, nb // 4), np.float32)
+        check(self.L.rpd_job_read(self.h, BUF[name], _ptr(out), nb))
+        if name == "grid":
+            return out.reshape(-1, 440)
+        if name == "tiles":
+            return out.reshape(18, -1)
+        if name in _RGB:
+            return out.reshape(-1, 3)
+        if name in _WIDTH:
+            return out[: nb // 4].reshape(-1, _WIDTH[name])
+        return out
+
+    def set(self, name, arr):
+        if name == "grid":
+            arr = np.ascontiguousarray(arr, np.float64)
+        elif name in _INT:
+            arr = np.ascontiguousarray(arr, np.int32)
+        else:
+            arr = np.ascontiguousarray(arr, np.float32)
+        check(self.L.rpd_job_write(self.h, BUF[name], _ptr(arr), arr.nbytes))

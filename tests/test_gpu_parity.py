@@ -1,0 +1,196 @@
+"""GPU engine vs. the CPU oracle on the same seeded inputs.
+
+Each stage test feeds the engine the oracle's own state (so both sides start
+from identical bits) and compares the stage output; the trace tests compare
+per-sample records of identical RNG streams. Tolerances: random draws, bin
+inputs, flags and sample counts are compared exactly; radiance within 1e-4
+relative per sample, with samples whose path took a different branch at an
+intersection rounding boundary counted as outliers.
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import rpd_engine as eng
+from parity_util import engine_camera, engine_config, rel_err
+
+pytestmark = pytest.mark.gpu
+
+SEED = 1234
+
+
+def _pair(rc, seed=SEED):
+    oj = ob.OracleJob(rc, seed, nthreads=1, keep_records=True)
+    return oj
+
+
+def _gpu_job(ctx, rc, seed=SEED):
+    return eng.Job(ctx, engine_config(rc), engine_camera(rc), seed)
+
+
+def _compare_records(orec, grec):
+    assert orec.shape == grec.shape, (orec.shape, grec.shape)
+    np.testing.assert_array_equal(orec[:, 0], grec[:, 0])              # pixel of every slot
+    np.testing.assert_array_equal(orec[:, 16:20], grec[:, 16:20])      # pixel_u, lens_u: first four draws
+    rad_bad = (rel_err(orec[:, 1:4], grec[:, 1:4]) > 1e-4).any(axis=1)
+    dims_bad = (orec[:, 20:29] != grec[:, 20:29]).any(axis=1)
+    flag_bad = (orec[:, 4:6] != grec[:, 4:6]).any(axis=1)
+    feat_bad = (rel_err(orec[:, 6:16], grec[:, 6:16], floor=1e-3) > 1e-5).any(axis=1)
+    bad = rad_bad | dims_bad | flag_bad | feat_bad
+    return bad, rad_bad, dims_bad, flag_bad, feat_bad
+
+
+@pytest.mark.parametrize("aperture", [0.0, 0.14])
+def test_trace_records_match_oracle(engine_ctx, aperture):
+    rc = ob.config_c0(40, 32)
+    rc.aperture = aperture
+    rc.focus_distance = 1.05 if aperture > 0 else 1.0
+    oj = _pair(rc)
+    oj.run_pass(np.full(40 * 32, rc.spp, np.int32), statmc=True)
+    gj = _gpu_job(engine_ctx, rc)
+    gj.run_pass(None, statmc=True)
+    orec, grec = oj.get("records"), gj.get("records")
+    bad, rad_bad, dims_bad, flag_bad, feat_bad = _compare_records(orec, grec)
+    print(f"samples {len(orec)} outliers {bad.sum()} (radiance {rad_bad.sum()}, dims {dims_bad.sum()}, "
+          f"flags {flag_bad.sum()}, features {feat_bad.sum()})")
+    # samples whose path split at an intersection rounding boundary
+    assert bad.mean() <= 2e-3
+    # the per-sample radiance of agreeing samples is within 1e-4 relative
+    assert rel_err(orec[~bad, 1:4], grec[~bad, 1:4]).max() <= 1e-4
+
+
+def test_trace_uniform_render_matches_oracle(engine_ctx):
+    rc = ob.config_c0(24, 20)
+    rc.use_statmc = 0
+    oj = _pair(rc)
+    oj.run_pass(np.full(24 * 20, rc.spp, np.int32), statmc=False)
+    gj = _gpu_job(engine_ctx, rc)
+    gj.run_pass(None, statmc=False)
+    orec, grec = oj.get("records"), gj.get("records")
+    bad = _compare_records(orec, grec)[0]
+    assert bad.mean() <= 2e-3
+
+
+def test_pass_state_matches_oracle(engine_ctx):
+    rc = ob.config_c0(40, 32)
+    oj = _pair(rc)
+    oj.run_pass(np.full(40 * 32, rc.spp, np.int32), statmc=True)
+    gj = _gpu_job(engine_ctx, rc)
+    gj.run_pass(None, statmc=True)
+    np.testing.assert_array_equal(oj.get("hit_count"), gj.get("hit_count"))
+    np.testing.assert_array_equal(oj.get("sample_count"), gj.get("sample_count"))
+    os_, gs = oj.get("stats"), gj.get("stats")
+    np.testing.assert_array_equal(os_[:, 2].view(np.int32), gs[:, 2].view(np.int32))
+    cols = [0, 3, 4, 5, 6, 7, 8]
+    px_bad = (rel_err(os_[:, cols], gs[:, cols], floor=1e-3) > 1e-4).any(axis=1)
+    print("pixels with stat differences:", px_bad.sum())
+    assert px_bad.mean() <= 0.02
+    og, gg = oj.get("grid"), gj.get("grid")
+    assert og.shape == gg.shape
+    node_bad = (np.abs(og - gg) > 1e-6 * np.maximum(np.abs(og), 1.0)).any(axis=1)
+    print("grid nodes with differences:", node_bad.sum(), "of", len(og))
+    assert node_bad.mean() <= 0.1
+
+
+def _oracle_after_base(rc, seed=SEED):
+    oj = ob.OracleJob(rc, seed, nthreads=1)
+    oj.run_pass(np.full(rc.width * rc.height, rc.spp, np.int32), statmc=True)
+    return oj
+
+
+def _inject_pixel_state(oj, gj):
+    gj.set("stats", oj.get("stats"))
+    sums = np.concatenate([oj.get("radiance"), oj.get("albedo"), oj.get("normal"), oj.get("world_pos"),
+                           oj.get("depth")[:, None]], axis=1)
+    gj.set("sums", sums)
+    gj.set("hit_count", oj.get("hit_count"))
+    gj.set("sample_count", oj.get("sample_count"))
+
+
+def test_sensitivity_stage_identical_inputs(engine_ctx):
+    rc = ob.config_c0(48, 40)
+    oj = _oracle_after_base(rc)
+    oj.recompute_sensitivity()
+    gj = _gpu_job(engine_ctx, rc)
+    _inject_pixel_state(oj, gj)
+    gj.set("grid", oj.get("grid"))
+    gj.recompute_sensitivity()
+    ot, gt = oj.get("tiles"), gj.get("tiles")
+    err = np.abs(ot - gt)
+    print("tile max abs err", err.max())
+    assert err.max() <= 1e-5
+    np.testing.assert_allclose(gj.get("sensitivity_gradient"), oj.get("sensitivity_gradient"), atol=1e-5)
+    np.testing.assert_allclose(gj.get("sensitivity_confidence"), oj.get("sensitivity_confidence"), atol=1e-5)
+
+
+@pytest.mark.parametrize("radius,alpha", [(7, 0.05), (10, 0.01), (3, 0.2)])
+def test_statmc_stage_identical_inputs(engine_ctx, radius, alpha):
+    rc = ob.config_c0(48, 40)
+    rc.color_window_radius = radius
+    rc.color_compat_alpha = alpha
+    oj = _oracle_after_base(rc)
+    oj.recompute_sensitivity()
+    oj.statmc()
+    gj = _gpu_job(engine_ctx, rc)
+    _inject_pixel_state(oj, gj)
+    gj.set("tiles", oj.get("tiles"))
+    gj.set("sensitivity_gradient", oj.get("sensitivity_gradient"))
+    gj.statmc()
+    od, gd = oj.get("denoised"), gj.get("denoised")
+    e = rel_err(od, gd)
+    print("denoised max rel err", e.max(), "pixels > 1e-4:", (e > 1e-4).any(axis=1).sum())
+    assert (e > 1e-4).any(axis=1).mean() <= 1e-3
+    np.testing.assert_allclose(gj.get("alpha"), oj.get("alpha"), atol=1e-5)
+    assert rel_err(oj.get("uncertainty"), gj.get("uncertainty"), floor=1e-9).max() <= 1e-3
+    for name in ("var_total", "var_eff", "var_ratio"):
+        np.testing.assert_array_equal(oj.get(name), gj.get(name))
+
+
+def test_variance_stage_identical_inputs(engine_ctx):
+    rc = ob.config_c0(48, 40)
+    rc.var_window_radius = 3
+    rc.var_iterations = 6
+    rc.var_compat_sigma = 3.0
+    rc.var_shrinkage_k = 0.7
+    oj = _oracle_after_base(rc)
+    oj.variance()
+    gj = _gpu_job(engine_ctx, rc)
+    _inject_pixel_state(oj, gj)
+    gj.variance()
+    np.testing.assert_array_equal(oj.get("variance_denoised"), gj.get("variance_denoised"))
+
+
+@pytest.mark.parametrize("base_samples,radius,spp", [(0, 1, 4), (2, 1, 4), (1, 0, 4), (0, 2, 1)])
+def test_allocation_bit_exact(engine_ctx, base_samples, radius, spp):
+    rc = ob.config_c0(48, 40)
+    rc.adaptive_base_samples = base_samples
+    rc.adaptive_importance_smoothing_radius = radius
+    rc.adaptive_spp = spp
+    oj = _oracle_after_base(rc)
+    oj.variance()
+    oc = oj.adaptive_counts()
+    gj = _gpu_job(engine_ctx, rc)
+    _inject_pixel_state(oj, gj)
+    gj.set("variance_denoised", oj.get("variance_denoised"))
+    gc = gj.adaptive_counts()
+    np.testing.assert_array_equal(oj.get("adaptive_importance"), gj.get("adaptive_importance"))
+    np.testing.assert_array_equal(oc, gc)
+    assert gc.sum() <= rc.adaptive_spp * rc.width * rc.height
+
+
+def test_full_pipeline_c0_small(engine_ctx):
+    rc = ob.config_c0(48, 40)
+    oj = ob.OracleJob(rc, SEED, nthreads=4, skip_dead=True)
+    oj.render()
+    gj = eng.Job(engine_ctx, engine_config(rc), engine_camera(rc), SEED, submit=True)
+    gj.wait()
+    og, gg = oj.get("radiance"), gj.get("radiance")
+    oraw, graw = oj.get("raw_radiance"), gj.get("raw_radiance")
+    e = rel_err(og, gg).max(axis=1)
+    er = rel_err(oraw, graw).max(axis=1)
+    sc_same = (oj.get("sample_count") == gj.get("sample_count")).mean()
+    print(f"denoised px >1e-4: {(e > 1e-4).mean():.4f}, raw px >1e-4: {(er > 1e-4).mean():.4f}, "
+          f"sample counts equal: {sc_same:.4f}")
+    assert sc_same >= 0.98
+    assert (e > 1e-4).mean() <= 0.05
+    assert np.median(e) <= 1e-5
