@@ -143,6 +143,9 @@ int rpd_job_recompute_sensitivity(rpd_job* job);    /* main.cpp:741 */
 int rpd_job_statmc_denoise(rpd_job* job);           /* main.cpp:957 */
 int rpd_job_variance_denoise(rpd_job* job);         /* main.cpp:1166 */
 int rpd_job_adaptive_counts(rpd_job* job, int* counts_out); /* main.cpp:1273 */
+/* The part of compute_adaptive_sample_counts after the importance field is
+   known (main.cpp:1373-1438): clipping, floors, systematic residual draw. */
+int rpd_job_adaptive_counts_from_importance(rpd_job* job, const float* importance, int* counts_out);
 int rpd_job_finalize(rpd_job* job);                 /* FrameBuffers::finalize, core/buffers.h:152 */
 int rpd_job_sync(rpd_job* job);
 

@@ -68,6 +68,18 @@ void orc_adaptive_counts(void* p, int* out) {
     compute_adaptive_counts(*j, j->extra_counts);
     std::memcpy(out, j->extra_counts.data(), j->extra_counts.size() * sizeof(int));
 }
+// counts from a given importance field (its sum formed as std::accumulate does)
+void orc_adaptive_counts_from_importance(void* p, const float* imp, int* out) {
+    Job* j = static_cast<Job*>(p);
+    const int n = j->w * j->h;
+    std::vector<float> v(imp, imp + n);
+    double sum = 0.0;
+    for (float x : v) sum += double(x);
+    const int floor_pp = std::clamp(j->cfg.adaptive_base_samples, 0, j->cfg.adaptive_spp);
+    std::vector<int> counts(n, floor_pp);
+    if (j->cfg.adaptive_spp * n > 0) counts_from_importance(*j, v, sum, counts);
+    std::memcpy(out, counts.data(), size_t(n) * sizeof(int));
+}
 void orc_finalize(void* p) { finalize(*static_cast<Job*>(p)); }
 double orc_stage_ms(void* p, int i) { return static_cast<Job*>(p)->stage_ms[i]; }
 void orc_dims(void* p, int* out) {

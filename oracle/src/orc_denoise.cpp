@@ -475,6 +475,66 @@ void variance_denoise(Job& j) {
     if (j.verbose) std::fprintf(stderr, "Sample-variance denoise completed in %.0f ms\n", ms_since(t0));
 }
 
+// main.cpp:1373-1438: clip-and-redistribute, floors, systematic residual draw.
+// `counts` arrives holding the per-pixel floor; `sum` is the sequential FP64
+// sum of the importance field.
+void counts_from_importance(Job& j, const std::vector<float>& imp, double sum, std::vector<int>& counts) {
+    const int n = j.w * j.h;
+    const int total = j.cfg.adaptive_spp * j.w * j.h;
+    const int floor_pp = std::clamp(j.cfg.adaptive_base_samples, 0, j.cfg.adaptive_spp);
+    const int reserved = floor_pp * j.w * j.h;
+    int remaining = std::max(0, total - reserved);
+    if (sum <= 0.0 || remaining <= 0) return;
+    std::vector<float> desired(n, 0.0f);
+    std::vector<int> active(n);
+    std::iota(active.begin(), active.end(), 0);
+    double rem_imp = sum;
+    const int max_pp = std::max(4, 8 * std::max(1, j.cfg.adaptive_spp));
+    while (!active.empty() && remaining > 0 && rem_imp > double(ORC_EPS_SMALL)) {
+        bool clipped = false;
+        std::vector<int> next;
+        next.reserve(active.size());
+        for (int i : active) {
+            const double share = double(imp[i]) / rem_imp;
+            const double want = share * double(remaining);
+            if (want > double(max_pp)) {
+                desired[i] = float(max_pp);
+                remaining -= max_pp;
+                rem_imp -= double(imp[i]);
+                clipped = true;
+            } else {
+                next.push_back(i);
+            }
+        }
+        if (!clipped) {
+            for (int i : next) desired[i] = float(double(imp[i]) / rem_imp * double(remaining));
+            break;
+        }
+        active.swap(next);
+    }
+    int assigned = 0;
+    std::vector<float> resid(n, 0.0f);
+    double rsum = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const int add = int(std::floor(desired[i]));
+        counts[i] += add;
+        assigned += add;
+        resid[i] = std::max(0.0f, desired[i] - float(add));
+        rsum += double(resid[i]);
+    }
+    const int left = std::max(0, total - reserved - assigned);
+    if (left > 0 && rsum > double(ORC_EPS_SMALL)) {
+        const double off = double(hash_to_unit_float(j.seed ^ 0x4d595df4u));
+        double cum = 0.0;
+        int cur = 0;
+        for (int k = 0; k < left; ++k) {
+            const double target = ((double(k) + off) / double(left)) * rsum;
+            while (cur < n && cum + double(resid[cur]) < target) { cum += double(resid[cur]); ++cur; }
+            if (cur < n) counts[cur]++;
+        }
+    }
+}
+
 // main.cpp:1273-1446
 void compute_adaptive_counts(Job& j, std::vector<int>& counts) {
     const auto t0 = Clock::now();
@@ -531,56 +591,7 @@ void compute_adaptive_counts(Job& j, std::vector<int>& counts) {
         sum = std::accumulate(imp.begin(), imp.end(), 0.0);
     }
     j.adaptive_importance = imp;
-    int remaining = std::max(0, total - reserved);
-    if (sum <= 0.0 || remaining <= 0) return;
-    std::vector<float> desired(n, 0.0f);
-    std::vector<int> active(n);
-    std::iota(active.begin(), active.end(), 0);
-    double rem_imp = sum;
-    const int max_pp = std::max(4, 8 * std::max(1, j.cfg.adaptive_spp));
-    while (!active.empty() && remaining > 0 && rem_imp > double(ORC_EPS_SMALL)) {
-        bool clipped = false;
-        std::vector<int> next;
-        next.reserve(active.size());
-        for (int i : active) {
-            const double share = double(imp[i]) / rem_imp;
-            const double want = share * double(remaining);
-            if (want > double(max_pp)) {
-                desired[i] = float(max_pp);
-                remaining -= max_pp;
-                rem_imp -= double(imp[i]);
-                clipped = true;
-            } else {
-                next.push_back(i);
-            }
-        }
-        if (!clipped) {
-            for (int i : next) desired[i] = float(double(imp[i]) / rem_imp * double(remaining));
-            break;
-        }
-        active.swap(next);
-    }
-    int assigned = 0;
-    std::vector<float> resid(n, 0.0f);
-    double rsum = 0.0;
-    for (int i = 0; i < n; ++i) {
-        const int add = int(std::floor(desired[i]));
-        counts[i] += add;
-        assigned += add;
-        resid[i] = std::max(0.0f, desired[i] - float(add));
-        rsum += double(resid[i]);
-    }
-    const int left = std::max(0, total - reserved - assigned);
-    if (left > 0 && rsum > double(ORC_EPS_SMALL)) {
-        const double off = double(hash_to_unit_float(j.seed ^ 0x4d595df4u));
-        double cum = 0.0;
-        int cur = 0;
-        for (int k = 0; k < left; ++k) {
-            const double target = ((double(k) + off) / double(left)) * rsum;
-            while (cur < n && cum + double(resid[cur]) < target) { cum += double(resid[cur]); ++cur; }
-            if (cur < n) counts[cur]++;
-        }
-    }
+    counts_from_importance(j, imp, sum, counts);
     j.stage_ms[5] += ms_since(t0);
     if (j.verbose) std::fprintf(stderr, "Adaptive sample count computation completed in %.0f ms\n", ms_since(t0));
 }

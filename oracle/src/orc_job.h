@@ -177,6 +177,7 @@ void recompute_sensitivity(Job& j);
 void statmc_denoise(Job& j);
 void variance_denoise(Job& j);
 void compute_adaptive_counts(Job& j, std::vector<int>& counts);
+void counts_from_importance(Job& j, const std::vector<float>& imp, double sum, std::vector<int>& counts);
 void finalize(Job& j);
 void render(Job& j);
 const std::vector<F3>& feature_normals(Job& j);

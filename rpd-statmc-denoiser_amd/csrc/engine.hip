@@ -380,8 +380,12 @@ int do_statmc(rpd_job* j) {
     fc.compat_alpha = j->cfg.color_compat_alpha;
     fc.sigma_max = j->cfg.color_sigma_max;
     fc.rpd_scale = j->cfg.rpf_shrinkage_scale;
-    launch_statmc(j->W, j->H, fc, j->pb, j->feat, j->geom, j->tiles, j->sgrad, j->filter_scratch, j->den, j->unc,
-                  j->alp, j->vtot, j->veff, j->vrat, j->stream);
+    launch_statmc_prepare(j->W, j->H, fc, j->pb, j->feat, j->geom, j->tiles, j->sgrad, j->filter_scratch, j->stream);
+    {
+        StageTimer tw(j, 9);
+        launch_statmc_window(j->W, j->H, fc, j->pb, j->filter_scratch, j->den, j->unc, j->alp, j->vtot, j->veff,
+                             j->vrat, j->stream);
+    }
     HIP_TRY(hipGetLastError());
     return RPD_OK;
 }
@@ -411,10 +415,21 @@ int do_allocation(rpd_job* j) {
     ac.radius = j->cfg.adaptive_importance_smoothing_radius;
     ac.sigma_max = j->cfg.adaptive_sigma_max;
     ac.seed = j->seed;
-    launch_allocation(j->W, j->H, ac, j->pb, j->feat, j->vden, j->imp, j->counts, j->alloc_scratch, j->stream);
+    launch_importance(j->W, j->H, ac, j->pb, j->feat, j->vden, j->imp, j->alloc_scratch, j->stream);
+    launch_alloc_counts(j->N, ac, j->imp, j->counts, j->alloc_scratch, j->stream);
     launch_plan_scan(j->N, j->counts, j->offsets, j->active, j->totals, j->scan_scratch, j->rec.capacity, j->stream);
     HIP_TRY(hipGetLastError());
     return RPD_OK;
+}
+
+AllocConfig alloc_config(const rpd_job* j) {
+    AllocConfig ac;
+    ac.adaptive_spp = j->cfg.adaptive_spp;
+    ac.base_samples = j->cfg.adaptive_base_samples;
+    ac.radius = j->cfg.adaptive_importance_smoothing_radius;
+    ac.sigma_max = j->cfg.adaptive_sigma_max;
+    ac.seed = j->seed;
+    return ac;
 }
 
 int do_finalize(rpd_job* j, bool statmc) {
@@ -435,6 +450,7 @@ int reset_state(rpd_job* j) {
     HIP_TRY(hipMemsetAsync(j->den, 0, sizeof(float) * 14 * N, j->stream));
     HIP_TRY(hipMemsetAsync(j->grid, 0, sizeof(double) * GRID_NODE_DOUBLES * j->geom.nx * j->geom.ny, j->stream));
     HIP_TRY(hipMemsetAsync(j->tiles, 0, sizeof(float) * SF_COUNT * j->geom.nx * j->geom.ny, j->stream));
+    HIP_TRY(hipMemsetAsync(j->counts, 0, sizeof(int) * (3 * N + 8), j->stream));  // plan, totals, work counter
     j->features_valid = false;
     j->finalized = false;
     return RPD_OK;
@@ -868,6 +884,20 @@ int rpd_job_adaptive_counts(rpd_job* j, int* out) {
     if (out) HIP_TRY(hipMemcpy(out, j->counts, sizeof(int) * j->N, hipMemcpyDeviceToHost));
     return RPD_OK;
 }
+int rpd_job_adaptive_counts_from_importance(rpd_job* j, const float* importance, int* out) {
+    if (!j || !importance) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    HIP_TRY(hipMemcpyAsync(j->imp, importance, sizeof(float) * j->N, hipMemcpyHostToDevice, j->stream));
+    {
+        StageTimer tm(j, 5);
+        launch_alloc_counts(j->N, alloc_config(j), j->imp, j->counts, j->alloc_scratch, j->stream);
+    }
+    HIP_TRY(hipGetLastError());
+    const int st = rpd_job_sync(j);
+    if (st) return st;
+    if (out) HIP_TRY(hipMemcpy(out, j->counts, sizeof(int) * j->N, hipMemcpyDeviceToHost));
+    return RPD_OK;
+}
+
 int rpd_job_finalize(rpd_job* j) {
     if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
     const int st = do_finalize(j, j->cfg.use_statmc != 0);
@@ -1025,10 +1055,12 @@ int rpd_job_read(rpd_job* j, int id, void* dst, size_t bytes) {
                         o[9] = r[2][k].x; o[10] = r[2][k].y; o[11] = r[2][k].z;
                         o[12] = r[3][k].x; o[13] = r[3][k].y; o[14] = r[3][k].z;
                     }
-                    if (j->last_pass_capture) {
-                        o[16] = r[4][k].x; o[17] = r[4][k].y; o[18] = r[4][k].z; o[19] = r[4][k].w;
-                        o[20] = r[5][k].x; o[21] = r[5][k].y; o[22] = r[5][k].z; o[23] = r[5][k].w;
-                        o[24] = r6[k].x; o[25] = r6[k].y;
+                    if (j->last_pass_capture) {  // dimensions that were never captured read as zero
+                        if (fl & REC_PIXEL) { o[16] = r[4][k].x; o[17] = r[4][k].y; }
+                        if (fl & REC_LENS) { o[18] = r[4][k].z; o[19] = r[4][k].w; }
+                        if (fl & REC_BRDF) { o[20] = r[5][k].x; o[21] = r[5][k].y; }
+                        if (fl & REC_LIGHT) { o[22] = r[5][k].z; o[23] = r[5][k].w; }
+                        if (fl & REC_ENV) { o[24] = r6[k].x; o[25] = r6[k].y; }
                     }
                     const uint32_t li = (fl >> REC_LIGHT_SHIFT) & 255u;
                     o[26] = (fl & REC_LSEL) ? float(li) : -1.0f;

@@ -1,13 +1,25 @@
 // Adaptive sample allocation (path_tracer/main.cpp:1273-1446).
 //
-// The reference allocates with a sequential FP64 scan: std::accumulate of the
-// importance field, an in-order clip-and-redistribute sweep, floors, and a
-// systematic residual draw against a running FP64 cumulative sum. Counts are
-// reproduced bit for bit here: the two FP64 sums are formed by one lane in
-// index order (other lanes stage the data through LDS), clipping is an
-// ordered find-first over the current sweep state, and the residual draw is a
-// binary search of the sequential prefix sums, which is exactly where the
-// reference's forward cursor stops.
+// The reference allocates with sequential FP64 arithmetic: std::accumulate of
+// the importance field, an in-order clip-and-redistribute sweep, floors, and a
+// systematic residual draw against a running FP64 cumulative sum. Every count
+// is reproduced bit for bit here without running those loops serially:
+//
+//  * Sequential FP64 sums of non-negative floats. S_k = fl(S_{k-1} + x_k) only
+//    depends on S_{k-1} through its binade and, on an exact tie, its parity.
+//    Inside one binade [2^E, 2^(E+1)) every double is a multiple of
+//    u = 2^(E-52), so a step that stays in the binade and is not a tie adds
+//    exactly round(x_k / u) units of u. Steps are classified in parallel from an
+//    approximate prefix (with a margin far above its rounding error); the
+//    integer unit counts of each run of such steps are summed exactly, and one
+//    lane replays only the remaining steps (binade changes, ties, the first
+//    step), in order, with the reference's own double additions.
+//  * Clipping: the first pixel that exceeds the cap under the initial state is
+//    found by a grid-wide search; sweeps continue serially only if one exists.
+//  * Residual draw: sample k lands on the first pixel whose cumulative residual
+//    reaches target_k = ((k + offset) / left) * rsum, so pixel c receives
+//    #{k : target_k <= P_c} - #{k : target_k <= P_{c-1}}, counted per pixel
+//    from the exact prefix sums P.
 #include "kernels.h"
 
 namespace rpd {
@@ -46,7 +58,7 @@ __global__ void k_importance_smooth(int W, int H, int R, PixelBuffers pb, Featur
             const float sw = 1.0f / (1.0f + float(dx * dx + dy * dy));
             const float nw = clampf((dot3(ni, mk3(f.normal[q], f.normal[n + q], f.normal[2 * n + q])) + 1.0f) * 0.5f,
                                     0.0f, 1.0f);
-            const float dw = expf(-4.0f * relative_depth_delta(di, f.depth[q]));
+            const float dw = expf_cr(-4.0f * relative_depth_delta(di, f.depth[q]));
             const double w = double(sw) * double(nw) * double(dw);
             acc += w * double(src[q]);
             ws += w;
@@ -55,111 +67,303 @@ __global__ void k_importance_smooth(int W, int H, int R, PixelBuffers pb, Featur
     dst[i] = (ws > double(RPD_EPS_SMALL)) ? float(acc / ws) : src[i];
 }
 
-constexpr int ALLOC_BLOCK = 1024;
-constexpr int STAGE = 4096;  // floats staged per LDS buffer
+// ---------------------------------------------------------------------------
+// Generic inclusive scans (three launches: block scans, block totals, fix-up).
+constexpr int SB = 1024;
 
-// Sequential FP64 sum (and optionally the running prefix) of a float array in
-// index order. Thread 0 adds; the other threads stage the next chunk into the
-// second LDS buffer and write the previous chunk's prefix values out.
-__device__ double seq_sum(const float* __restrict__ v, int n, double* prefix, float* stage /* 2*STAGE */,
-                          double* pstage /* STAGE */) {
+template <typename T>
+__device__ __forceinline__ T block_scan_incl(T v, T* sh) {
     const int t = threadIdx.x;
-    double s = 0.0;
-    const int nchunks = (n + STAGE - 1) / STAGE;
-    for (int k = t; k < STAGE; k += ALLOC_BLOCK) stage[k] = k < n ? v[k] : 0.0f;
+    sh[t] = v;
     __syncthreads();
-    for (int c = 0; c < nchunks; ++c) {
-        const float* cur = stage + (c & 1) * STAGE;
-        float* nxt = stage + ((c + 1) & 1) * STAGE;
-        const int base = c * STAGE;
-        const int cnt = min(STAGE, n - base);
-        if (t == 0) {
-            if (prefix) {
-                for (int k = 0; k < cnt; ++k) { s += double(cur[k]); pstage[k] = s; }
-            } else {
-                int k = 0;
-                for (; k + 4 <= cnt; k += 4) {
-                    const float a = cur[k], b = cur[k + 1], cc = cur[k + 2], d = cur[k + 3];
-                    s += double(a); s += double(b); s += double(cc); s += double(d);
-                }
-                for (; k < cnt; ++k) s += double(cur[k]);
-            }
-        } else if (c + 1 < nchunks) {
-            const int nb = base + STAGE;
-            for (int k = t - 1; k < STAGE; k += ALLOC_BLOCK - 1) nxt[k] = nb + k < n ? v[nb + k] : 0.0f;
-        }
+    for (int o = 1; o < SB; o <<= 1) {
+        const T add = t >= o ? sh[t - o] : T(0);
         __syncthreads();
-        if (prefix) {
-            for (int k = t; k < cnt; k += ALLOC_BLOCK) prefix[base + k] = pstage[k];
-            __syncthreads();
-        }
+        sh[t] = sh[t] + add;
+        __syncthreads();
     }
-    return s;  // valid in thread 0
+    return sh[t];
+}
+template <typename T>
+__global__ void __launch_bounds__(SB) k_scan_a(int n, T* data, T* bsum) {
+    __shared__ T sh[SB];
+    const int i = blockIdx.x * SB + threadIdx.x;
+    const T v = i < n ? data[i] : T(0);
+    const T s = block_scan_incl(v, sh);
+    if (i < n) data[i] = s;
+    if (threadIdx.x == SB - 1) bsum[blockIdx.x] = s;
+}
+template <typename T>
+__global__ void __launch_bounds__(SB) k_scan_b(int nb, T* bsum) {  // exclusive scan of block totals
+    __shared__ T sh[SB];
+    T carry = T(0);
+    for (int base = 0; base < nb; base += SB) {
+        const int i = base + threadIdx.x;
+        const T v = i < nb ? bsum[i] : T(0);
+        const T s = block_scan_incl(v, sh);
+        if (i < nb) bsum[i] = carry + (s - v);
+        __syncthreads();
+        carry = carry + sh[SB - 1];
+        __syncthreads();
+    }
+}
+template <typename T>
+__global__ void k_scan_c(int n, T* data, const T* bsum) {
+    const int i = blockIdx.x * SB + threadIdx.x;
+    if (i < n && blockIdx.x > 0) data[i] = data[i] + bsum[blockIdx.x];
+}
+template <typename T>
+void scan_inclusive(int n, T* data, T* bsum, hipStream_t st) {
+    const int nb = (n + SB - 1) / SB;
+    hipLaunchKernelGGL(k_scan_a<T>, dim3(nb), dim3(SB), 0, st, n, data, bsum);
+    hipLaunchKernelGGL(k_scan_b<T>, dim3(1), dim3(SB), 0, st, nb, bsum);
+    hipLaunchKernelGGL(k_scan_c<T>, dim3(nb), dim3(SB), 0, st, n, data, bsum);
 }
 
-struct AllocScratch {
-    float* desired;
-    unsigned char* clipped;
-    double* prefix;
-    int* misc;  // [0] status
+// ---------------------------------------------------------------------------
+// Exact sequential FP64 sum of non-negative floats.
+struct SeqSum {
+    int n;
+    const float* x;
+    double* approx;        // n: approximate inclusive prefix, then (optionally) the exact prefix
+    uint64_t* units;       // n: unit counts of simple steps, then their inclusive scan
+    int* serial_pos;       // n: serial flag, then its inclusive scan
+    int* serial_list;      // positions of serial steps, in order
+    int* binade;           // n: binade exponent of simple steps
+    double* seg_add;       // per serial step: exact sum of the simple run before it (+1 for the tail)
+    double* s_at;          // per serial step: S right after it
+    double* total;         // 1: the final sum
+    int* n_serial;         // 1
+    double* bsum_d;
+    uint64_t* bsum_u;
+    int* bsum_i;
 };
 
-__global__ void __launch_bounds__(ALLOC_BLOCK) k_alloc_serial(int n, AllocConfig ac, const float* imp, int* counts,
-                                                             AllocScratch s) {
-    __shared__ float stage[2 * STAGE];
-    __shared__ double pstage[STAGE];
-    __shared__ double sh_d[4];
-    __shared__ int sh_i[4];
-    __shared__ int wave_first[ALLOC_BLOCK / 64];
-    const int t = threadIdx.x;
-    const int total = ac.adaptive_spp * n;
-    const int floor_pp = min(max(ac.base_samples, 0), ac.adaptive_spp);
-    const int reserved = floor_pp * n;
-    for (int i = t; i < n; i += ALLOC_BLOCK) { counts[i] = floor_pp; s.desired[i] = 0.0f; s.clipped[i] = 0; }
-    if (total <= 0) return;
-    const double sum = seq_sum(imp, n, nullptr, stage, pstage);
-    if (t == 0) sh_d[0] = sum;
-    __syncthreads();
-    double rem_imp = sh_d[0];
-    int remaining = max(0, total - reserved);
-    if (rem_imp <= 0.0 || remaining <= 0) return;
-    const int max_pp = max(4, 8 * max(1, ac.adaptive_spp));
+__device__ __forceinline__ int binade_of(double v) {  // floor(log2 v) for v > 0
+    int e;
+    frexp(v, &e);
+    return e - 1;
+}
 
-    // clip-and-redistribute sweeps (main.cpp:1387-1412): every sweep visits the
-    // unclipped pixels in index order; a pixel whose share of the remaining
-    // samples exceeds the cap is clipped and the state it changes applies to
-    // the rest of the sweep, so each clip is the first index at or after the
-    // previous one that satisfies the test under the current state.
+__global__ void k_seq_init(SeqSum s) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < s.n) s.approx[i] = double(s.x[i]);
+}
+
+// The approximate prefix (any summation order) is within (n + 64) * 2^-52 of
+// the sequential one in relative terms, below 2^-24 for up to 2^26 terms.
+#define SEQ_MARGIN 1e-7
+
+__global__ void k_seq_classify(SeqSum s) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.n) return;
+    const float xf = s.x[k];
+    bool serial = (k == 0);
+    uint64_t q = 0;
+    int E = 0;
+    if (!serial) {
+        const double lo_prev = s.approx[k - 1] * (1.0 - SEQ_MARGIN);
+        const double hi_cur = s.approx[k] * (1.0 + SEQ_MARGIN);
+        if (!(lo_prev > 0.0)) {
+            serial = true;                   // running sum may still be zero
+        } else {
+            E = binade_of(lo_prev);
+            const double hi_prev = s.approx[k - 1] * (1.0 + SEQ_MARGIN);
+            // both S_{k-1} (whole interval) and S_k certainly in binade E; this also makes
+            // consecutive simple steps share E even where the approximate prefix is not monotone
+            if (binade_of(hi_cur) != E || binade_of(hi_prev) != E) {
+                serial = true;               // may cross a binade boundary
+            } else if (xf != 0.0f) {
+                // x = M * 2^(fe-24) with integer M < 2^24; in units u = 2^(E-52): x/u = M * 2^(fe-24-E+52)
+                int fe;
+                const float fm = frexpf(xf, &fe);
+                const uint32_t M = uint32_t(ldexpf(fm, 24));
+                const int sh = fe - 24 - E + 52;
+                if (sh >= 0) {
+                    q = uint64_t(M) << sh;                            // < 2^53 since x < 2^(E+1)
+                } else if (sh > -40) {
+                    const int r = -sh;
+                    const uint64_t half = 1ull << (r - 1);
+                    const uint64_t rem = uint64_t(M) & ((1ull << r) - 1ull);
+                    if (rem == half) serial = true;                   // exact tie: parity decides
+                    else q = (uint64_t(M) + half) >> r;
+                }                                                     // else far below half a unit: q = 0
+            }
+        }
+    }
+    s.units[k] = serial ? 0ull : q;
+    s.serial_pos[k] = serial ? 1 : 0;
+    s.binade[k] = E;
+}
+
+__global__ void k_seq_compact(SeqSum s) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.n) return;
+    const int c = s.serial_pos[k];
+    const bool is_serial = (k == 0) ? c == 1 : c != s.serial_pos[k - 1];
+    if (is_serial) s.serial_list[c - 1] = k;
+    if (k == s.n - 1) *s.n_serial = c;
+}
+
+// Run j (0 <= j <= m): the simple steps just before serial step j; run m is the tail.
+__global__ void k_seq_runs(SeqSum s) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = *s.n_serial;
+    if (j > m) return;
+    const int start = (j == 0) ? 0 : s.serial_list[j - 1] + 1;
+    const int end = (j == m) ? s.n : s.serial_list[j];      // exclusive
+    double add = 0.0;
+    if (end > start) {
+        const uint64_t Q = s.units[end - 1] - s.units[start - 1];   // units are zero at serial steps
+        add = ldexp(double(Q), s.binade[start] - 52);              // exact: Q < 2^53
+    }
+    s.seg_add[j] = add;
+}
+
+// The serial chain: one lane adds each run (exactly), then the serial step
+// itself with the reference's rounded addition. Other lanes stage inputs.
+constexpr int SEQ_STAGE = 2048;
+__global__ void __launch_bounds__(SB) k_seq_chain(SeqSum s) {
+    __shared__ double st_add[2][SEQ_STAGE];
+    __shared__ float st_x[2][SEQ_STAGE];
+    __shared__ double st_out[SEQ_STAGE];
+    const int t = threadIdx.x;
+    const int m = *s.n_serial;
+    double S = 0.0;
+    const int nch = (m + SEQ_STAGE - 1) / SEQ_STAGE;
+    for (int k = t; k < SEQ_STAGE; k += SB)
+        if (k < m) { st_add[0][k] = s.seg_add[k]; st_x[0][k] = s.x[s.serial_list[k]]; }
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+        const int base = c * SEQ_STAGE, cnt = min(SEQ_STAGE, m - base), b = c & 1;
+        if (t == 0) {
+            for (int k = 0; k < cnt; ++k) {
+                S = S + st_add[b][k];
+                S = S + double(st_x[b][k]);
+                st_out[k] = S;
+            }
+        } else if (c + 1 < nch) {
+            const int nb = base + SEQ_STAGE;
+            for (int k = t - 1; k < SEQ_STAGE; k += SB - 1)
+                if (nb + k < m) { st_add[b ^ 1][k] = s.seg_add[nb + k]; st_x[b ^ 1][k] = s.x[s.serial_list[nb + k]]; }
+        }
+        __syncthreads();
+        for (int k = t; k < cnt; k += SB) s.s_at[base + k] = st_out[k];
+        __syncthreads();
+    }
+    if (t == 0) *s.total = S + s.seg_add[m];
+}
+
+// exact prefix S_k for every k (into `approx`)
+__global__ void k_seq_prefix(SeqSum s) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.n) return;
+    const int c = s.serial_pos[k];                 // serial steps at or before k
+    const int last = s.serial_list[c - 1];
+    if (last == k) { s.approx[k] = s.s_at[c - 1]; return; }
+    const uint64_t Q = s.units[k] - s.units[last];
+    s.approx[k] = s.s_at[c - 1] + ldexp(double(Q), s.binade[k] - 52);
+}
+
+void seq_sum_launch(const SeqSum& s, bool want_prefix, hipStream_t st) {
+    const int n = s.n;
+    const dim3 g((n + 255) / 256), b(256);
+    hipLaunchKernelGGL(k_seq_init, g, b, 0, st, s);
+    scan_inclusive<double>(n, s.approx, s.bsum_d, st);
+    hipLaunchKernelGGL(k_seq_classify, g, b, 0, st, s);
+    scan_inclusive<uint64_t>(n, s.units, s.bsum_u, st);
+    scan_inclusive<int>(n, s.serial_pos, s.bsum_i, st);
+    hipLaunchKernelGGL(k_seq_compact, g, b, 0, st, s);
+    hipLaunchKernelGGL(k_seq_runs, dim3((n + 1 + 255) / 256), b, 0, st, s);
+    hipLaunchKernelGGL(k_seq_chain, dim3(1), dim3(SB), 0, st, s);
+    if (want_prefix) hipLaunchKernelGGL(k_seq_prefix, g, b, 0, st, s);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation state shared by the kernels below (device memory).
+struct AllocState {
+    double rem_imp;    // remaining importance after clipping
+    int remaining;     // remaining samples after clipping
+    int first_clip;    // first clip of the first sweep (INT_MAX: none)
+    int final_assign;  // 1 if the sweep loop ended by assigning shares
+    int assigned;      // sum of floors
+    int active;        // 0: allocation ended early (counts = floors)
+};
+
+struct AllocBufs {
+    float* desired;
+    unsigned char* clipped;
+    AllocState* st;
+    SeqSum seq;
+};
+
+__global__ void k_alloc_begin(int n, AllocConfig ac, int* counts, AllocBufs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int floor_pp = min(max(ac.base_samples, 0), ac.adaptive_spp);
+    if (i < n) { counts[i] = floor_pp; a.desired[i] = 0.0f; a.clipped[i] = 0; }
+    if (i == 0) {
+        const int total = ac.adaptive_spp * n;
+        const double sum = *a.seq.total;
+        const int remaining = max(0, total - floor_pp * n);
+        a.st->rem_imp = sum;
+        a.st->remaining = remaining;
+        a.st->first_clip = 0x7fffffff;
+        a.st->final_assign = 0;
+        a.st->assigned = 0;
+        a.st->active = (total > 0 && sum > 0.0 && remaining > 0) ? 1 : 0;
+    }
+}
+
+// first index whose share exceeds the cap under the initial state (main.cpp:1391-1398)
+__global__ void k_alloc_first_clip(int n, AllocConfig ac, const float* imp, AllocBufs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !a.st->active) return;
+    const int max_pp = max(4, 8 * max(1, ac.adaptive_spp));
+    const double want = double(imp[i]) / a.st->rem_imp * double(a.st->remaining);
+    if (want > double(max_pp)) atomicMin(&a.st->first_clip, i);
+}
+
+// The clip-and-redistribute sweeps (main.cpp:1387-1412), one workgroup.
+// The first sweep starts at the first clip found by k_alloc_first_clip.
+__global__ void __launch_bounds__(SB) k_alloc_sweeps(int n, AllocConfig ac, const float* imp, AllocBufs a) {
+    __shared__ int wave_first[SB / 64];
+    __shared__ int sh_found;
+    const int t = threadIdx.x;
+    if (!a.st->active) return;
+    const int max_pp = max(4, 8 * max(1, ac.adaptive_spp));
+    double rem_imp = a.st->rem_imp;
+    int remaining = a.st->remaining;
     int active_count = n;
+    const int first = a.st->first_clip;
+    bool final_assign = false;
+    bool first_sweep = true;
     while (active_count > 0 && remaining > 0 && rem_imp > double(RPD_EPS_SMALL)) {
         bool clipped_any = false;
         int pos = 0;
         while (pos < n) {
             int found = 0x7fffffff;
-            for (int base = pos; base < n && found == 0x7fffffff; base += ALLOC_BLOCK) {
-                const int i = base + t;
-                bool hitc = false;
-                if (i < n && !s.clipped[i]) {
-                    const double want = double(imp[i]) / rem_imp * double(remaining);
-                    hitc = want > double(max_pp);
+            if (first_sweep && pos == 0) {
+                found = first;                 // no earlier index clips under the initial state
+            } else {
+                for (int base = pos; base < n && found == 0x7fffffff; base += SB) {
+                    const int i = base + t;
+                    bool hitc = false;
+                    if (i < n && !a.clipped[i]) hitc = double(imp[i]) / rem_imp * double(remaining) > double(max_pp);
+                    const uint64_t b = __ballot(hitc);
+                    if ((t & 63) == 0) wave_first[t >> 6] = b ? (base + (t & ~63) + __ffsll((long long)b) - 1) : 0x7fffffff;
+                    __syncthreads();
+                    if (t == 0) {
+                        int f = 0x7fffffff;
+                        for (int w = 0; w < SB / 64; ++w) f = min(f, wave_first[w]);
+                        sh_found = f;
+                    }
+                    __syncthreads();
+                    found = sh_found;
+                    __syncthreads();
                 }
-                const uint64_t b = __ballot(hitc);
-                if ((t & 63) == 0) wave_first[t >> 6] = b ? (base + (t & ~63) + __ffsll((long long)b) - 1) : 0x7fffffff;
-                __syncthreads();
-                if (t == 0) {
-                    int f = 0x7fffffff;
-                    for (int w = 0; w < ALLOC_BLOCK / 64; ++w) f = min(f, wave_first[w]);
-                    sh_i[0] = f;
-                }
-                __syncthreads();
-                found = sh_i[0];
-                __syncthreads();
             }
             if (found == 0x7fffffff) break;
-            if (t == 0) {
-                s.desired[found] = float(max_pp);
-                s.clipped[found] = 1;
-            }
+            if (t == 0) { a.desired[found] = float(max_pp); a.clipped[found] = 1; }
             remaining -= max_pp;
             rem_imp -= double(imp[found]);
             --active_count;
@@ -167,77 +371,130 @@ __global__ void __launch_bounds__(ALLOC_BLOCK) k_alloc_serial(int n, AllocConfig
             pos = found + 1;
             __syncthreads();
         }
-        if (!clipped_any) {
-            for (int i = t; i < n; i += ALLOC_BLOCK)
-                if (!s.clipped[i]) s.desired[i] = float(double(imp[i]) / rem_imp * double(remaining));
-            break;
-        }
+        first_sweep = false;
+        if (!clipped_any) { final_assign = true; break; }
     }
-    __syncthreads();
-
-    // floors and residuals (main.cpp:1414-1423)
-    int assigned_local = 0;
-    for (int i = t; i < n; i += ALLOC_BLOCK) {
-        const float d = s.desired[i];
-        const int add = int(floorf(d));
-        counts[i] += add;
-        assigned_local += add;
-        s.desired[i] = fmaxf(0.0f, d - float(add));  // residual weight
-    }
-    __shared__ int red[ALLOC_BLOCK];
-    red[t] = assigned_local;
-    __syncthreads();
-    for (int o = ALLOC_BLOCK / 2; o > 0; o >>= 1) {
-        if (t < o) red[t] += red[t + o];
-        __syncthreads();
-    }
-    const int assigned = red[0];
-    __syncthreads();
-    const double rsum_t0 = seq_sum(s.desired, n, s.prefix, stage, pstage);
-    if (t == 0) sh_d[1] = rsum_t0;
-    __syncthreads();
-    const double rsum = sh_d[1];
-    const int left = max(0, total - reserved - assigned);
-    if (left <= 0 || !(rsum > double(RPD_EPS_SMALL))) return;
-    // systematic residual sampling (main.cpp:1425-1438)
-    const double off = double(hash_to_unit_float(ac.seed ^ 0x4d595df4u));
-    for (int k = t; k < left; k += ALLOC_BLOCK) {
-        const double target = ((double(k) + off) / double(left)) * rsum;
-        int lo = 0, hi = n;  // first c with prefix[c] >= target
-        while (lo < hi) { const int mid = (lo + hi) >> 1; if (s.prefix[mid] < target) lo = mid + 1; else hi = mid; }
-        if (lo < n) atomicAdd(&counts[lo], 1);
+    if (t == 0) {
+        a.st->rem_imp = rem_imp;
+        a.st->remaining = remaining;
+        a.st->final_assign = final_assign ? 1 : 0;
     }
 }
+
+// shares, floors and residuals (main.cpp:1403-1408, 1414-1423)
+__global__ void k_alloc_floor(int n, const float* imp, int* counts, AllocBufs a, float* resid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!a.st->active) { resid[i] = 0.0f; return; }
+    float d = a.desired[i];
+    if (a.st->final_assign && !a.clipped[i]) d = float(double(imp[i]) / a.st->rem_imp * double(a.st->remaining));
+    const int add = int(floorf(d));
+    counts[i] += add;
+    resid[i] = fmaxf(0.0f, d - float(add));
+    if (add) atomicAdd(&a.st->assigned, add);
+}
+
+// systematic residual sampling (main.cpp:1425-1438), counted per pixel
+__device__ __forceinline__ double sys_target(int k, double off, int left, double rsum) {
+    return ((double(k) + off) / double(left)) * rsum;
+}
+__device__ __forceinline__ int targets_at_most(double v, double off, int left, double rsum) {
+    const double est = v / rsum * double(left) - off;
+    int k;
+    if (!(est > 0.0)) k = 0;
+    else if (est >= double(left)) k = left;
+    else k = int(est);
+    while (k < left && sys_target(k, off, left, rsum) <= v) ++k;
+    while (k > 0 && sys_target(k - 1, off, left, rsum) > v) --k;
+    return k;
+}
+__global__ void k_alloc_systematic(int n, AllocConfig ac, int* counts, AllocBufs a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n || !a.st->active) return;
+    const int total = ac.adaptive_spp * n;
+    const int floor_pp = min(max(ac.base_samples, 0), ac.adaptive_spp);
+    const int left = max(0, total - floor_pp * n - a.st->assigned);
+    const double rsum = *a.seq.total;
+    if (left <= 0 || !(rsum > double(RPD_EPS_SMALL))) return;
+    const double off = double(hash_to_unit_float(ac.seed ^ 0x4d595df4u));
+    const double* P = a.seq.approx;   // exact prefix of the residuals
+    const int hi = targets_at_most(P[c], off, left, rsum);
+    const int lo = c > 0 ? targets_at_most(P[c - 1], off, left, rsum) : 0;
+    if (hi > lo) counts[c] += hi - lo;
+}
+
+// ---------------------------------------------------------------------------
+static size_t al(size_t b) { return (b + 255) & ~size_t(255); }
 
 size_t alloc_scratch_bytes(int n) {
-    size_t b = 0;
-    b += ((size_t)n * sizeof(float) + 255) & ~size_t(255);
-    b += ((size_t)n + 255) & ~size_t(255);
-    b += ((size_t)n * sizeof(double) + 255) & ~size_t(255);
-    b += 256;
-    b += ((size_t)n * sizeof(float) + 255) & ~size_t(255);  // smoothing buffer
-    return b;
+    const size_t N = size_t(n) + 1;
+    const size_t nb = N / SB + 2;
+    return al(N * 4) + al(N) + al(sizeof(AllocState)) + al(N * 4) + al(N * 4) + al(N * 8) + al(N * 8) +
+           al(N * 4) + al(N * 4) + al(N * 4) + al(N * 8) + al(N * 8) + al(8) + al(4) + al(nb * 8) + al(nb * 8) +
+           al(nb * 4);
 }
 
-void launch_allocation(int W, int H, const AllocConfig& ac, const PixelBuffers& pb, const Features& f,
-                       const float* vden, float* imp, int* counts, void* scratch, hipStream_t st) {
-    const int n = W * H;
+struct Carved {
+    AllocBufs a;
+    float* resid;
+    float* tmp;
+};
+static Carved carve(void* scratch, int n) {
+    const size_t N = size_t(n) + 1;
+    const size_t nb = N / SB + 2;
     char* p = static_cast<char*>(scratch);
-    AllocScratch s;
-    s.desired = reinterpret_cast<float*>(p); p += ((size_t)n * sizeof(float) + 255) & ~size_t(255);
-    s.clipped = reinterpret_cast<unsigned char*>(p); p += ((size_t)n + 255) & ~size_t(255);
-    s.prefix = reinterpret_cast<double*>(p); p += ((size_t)n * sizeof(double) + 255) & ~size_t(255);
-    s.misc = reinterpret_cast<int*>(p); p += 256;
-    float* tmp = reinterpret_cast<float*>(p);
+    auto take = [&](size_t bytes) { char* q = p; p += al(bytes); return q; };
+    Carved c;
+    c.a.desired = reinterpret_cast<float*>(take(N * 4));
+    c.a.clipped = reinterpret_cast<unsigned char*>(take(N));
+    c.a.st = reinterpret_cast<AllocState*>(take(sizeof(AllocState)));
+    c.resid = reinterpret_cast<float*>(take(N * 4));
+    c.tmp = reinterpret_cast<float*>(take(N * 4));
+    SeqSum& s = c.a.seq;
+    s.n = n;
+    s.x = nullptr;
+    s.approx = reinterpret_cast<double*>(take(N * 8));
+    s.units = reinterpret_cast<uint64_t*>(take(N * 8));
+    s.serial_pos = reinterpret_cast<int*>(take(N * 4));
+    s.serial_list = reinterpret_cast<int*>(take(N * 4));
+    s.binade = reinterpret_cast<int*>(take(N * 4));
+    s.seg_add = reinterpret_cast<double*>(take(N * 8));
+    s.s_at = reinterpret_cast<double*>(take(N * 8));
+    s.total = reinterpret_cast<double*>(take(8));
+    s.n_serial = reinterpret_cast<int*>(take(4));
+    s.bsum_d = reinterpret_cast<double*>(take(nb * 8));
+    s.bsum_u = reinterpret_cast<uint64_t*>(take(nb * 8));
+    s.bsum_i = reinterpret_cast<int*>(take(nb * 4));
+    return c;
+}
+
+void launch_importance(int W, int H, const AllocConfig& ac, const PixelBuffers& pb, const Features& f,
+                       const float* vden, float* imp, void* scratch, hipStream_t st) {
+    const int n = W * H;
+    const Carved c = carve(scratch, n);
     const dim3 grid((n + 255) / 256);
-    if (ac.adaptive_spp > 0) {
-        hipLaunchKernelGGL(k_importance, grid, dim3(256), 0, st, n, pb, vden, ac.sigma_max, imp);
-        if (ac.radius > 0) {
-            hipLaunchKernelGGL(k_importance_smooth, grid, dim3(256), 0, st, W, H, ac.radius, pb, f, imp, tmp);
-            hipMemcpyAsync(imp, tmp, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st);
-        }
+    if (ac.adaptive_spp <= 0) return;
+    hipLaunchKernelGGL(k_importance, grid, dim3(256), 0, st, n, pb, vden, ac.sigma_max, imp);
+    if (ac.radius > 0) {
+        hipLaunchKernelGGL(k_importance_smooth, grid, dim3(256), 0, st, W, H, ac.radius, pb, f, imp, c.tmp);
+        hipMemcpyAsync(imp, c.tmp, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st);
     }
-    hipLaunchKernelGGL(k_alloc_serial, dim3(1), dim3(ALLOC_BLOCK), 0, st, n, ac, imp, counts, s);
+}
+
+void launch_alloc_counts(int n, const AllocConfig& ac, const float* imp, int* counts, void* scratch, hipStream_t st) {
+    Carved c = carve(scratch, n);
+    const dim3 g((n + 255) / 256), b(256);
+    // sum of the importance field as std::accumulate forms it (main.cpp:1370)
+    c.a.seq.x = imp;
+    seq_sum_launch(c.a.seq, false, st);
+    hipLaunchKernelGGL(k_alloc_begin, g, b, 0, st, n, ac, counts, c.a);
+    hipLaunchKernelGGL(k_alloc_first_clip, g, b, 0, st, n, ac, imp, c.a);
+    hipLaunchKernelGGL(k_alloc_sweeps, dim3(1), dim3(SB), 0, st, n, ac, imp, c.a);
+    hipLaunchKernelGGL(k_alloc_floor, g, b, 0, st, n, imp, counts, c.a, c.resid);
+    // residual prefix sums as the reference's running `cumulative` forms them
+    c.a.seq.x = c.resid;
+    seq_sum_launch(c.a.seq, true, st);
+    hipLaunchKernelGGL(k_alloc_systematic, g, b, 0, st, n, ac, counts, c.a);
 }
 
 }  // namespace rpd

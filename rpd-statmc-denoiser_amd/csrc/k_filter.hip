@@ -223,14 +223,17 @@ __global__ void __launch_bounds__(256) k_statmc(int W, int H, FilterConfig fc, c
     alp[i] = a;
 }
 
-void launch_statmc(int W, int H, const FilterConfig& fc, const PixelBuffers& pb, const Features& f, GridGeom g,
-                   const float* tiles, const float* grad, float* scratch, float* den, float* unc, float* alp,
-                   float* vtot, float* veff, float* vrat, hipStream_t st) {
+void launch_statmc_prepare(int W, int H, const FilterConfig& fc, const PixelBuffers& pb, const Features& f,
+                           GridGeom g, const float* tiles, const float* grad, float* scratch, hipStream_t st) {
     const int n = W * H;
     float* vm = scratch;                                           // 3 planes
     float4* P = reinterpret_cast<float4*>(scratch + 4 * (size_t)n);  // 7 float4 planes (16-byte aligned)
     hipLaunchKernelGGL(k_sqrt_var_mean, dim3((n + 255) / 256), dim3(256), 0, st, n, pb, vm);
     hipLaunchKernelGGL(k_filter_pack, dim3((n + 255) / 256), dim3(256), 0, st, W, H, fc, pb, f, g, tiles, grad, vm, P);
+}
+void launch_statmc_window(int W, int H, const FilterConfig& fc, const PixelBuffers& pb, const float* scratch,
+                          float* den, float* unc, float* alp, float* vtot, float* veff, float* vrat, hipStream_t st) {
+    const float4* P = reinterpret_cast<const float4*>(scratch + 4 * (size_t)W * H);
     hipLaunchKernelGGL(k_statmc, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, st, W, H, fc, P, pb, den, unc,
                        alp, vtot, veff, vrat);
 }

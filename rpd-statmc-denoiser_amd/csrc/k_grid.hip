@@ -292,8 +292,8 @@ __global__ void k_prior(GridGeom g, const float* ns, float* tiles) {
             const v3 aj = mk3(ns[30 * nn + si], ns[31 * nn + si], ns[32 * nn + si]);
             const float sw = 1.0f / (1.0f + float(dx * dx + dy * dy));
             const float nw = clampf((dot3(ni, nj) + 1.0f) * 0.5f, 0.0f, 1.0f);
-            const float dw = expf(-4.0f * relative_depth_delta(di, ns[29 * nn + si]));
-            const float aw = expf(-2.5f * norm3(ai - aj));
+            const float dw = expf_cr(-4.0f * relative_depth_delta(di, ns[29 * nn + si]));
+            const float aw = expf_cr(-2.5f * norm3(ai - aj));
             nidx[q] = si;
             wgt[q] = double(sw) * double(nw) * double(dw) * double(aw);
         }

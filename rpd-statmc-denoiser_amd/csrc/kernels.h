@@ -134,10 +134,13 @@ void launch_sensitivity(GridGeom g, int width, int height, float conf_n0, const 
                         float* node_scratch, float* tiles, float* confidence, float* gradient, hipStream_t st);
 void launch_sensitivity_zero(GridGeom g, int width, int height, float* tiles, float* confidence, float* gradient,
                              hipStream_t st);
-void launch_statmc(int width, int height, const FilterConfig& fc, const PixelBuffers& pb, const Features& f,
-                   GridGeom g, const float* tiles, const float* gradient, float* scratch,
-                   float* denoised, float* uncertainty, float* alpha, float* var_total, float* var_eff,
-                   float* var_ratio, hipStream_t st);
+// StatMC in two steps: per-pixel packing of the filter inputs into `scratch`,
+// then the window kernel reading them.
+void launch_statmc_prepare(int width, int height, const FilterConfig& fc, const PixelBuffers& pb, const Features& f,
+                           GridGeom g, const float* tiles, const float* gradient, float* scratch, hipStream_t st);
+void launch_statmc_window(int width, int height, const FilterConfig& fc, const PixelBuffers& pb,
+                          const float* scratch, float* denoised, float* uncertainty, float* alpha, float* var_total,
+                          float* var_eff, float* var_ratio, hipStream_t st);
 void launch_variance(int width, int height, const VarianceConfig& vc, const PixelBuffers& pb, const Features& f,
                      float* tmp, float* out, hipStream_t st);
 struct AllocConfig {
@@ -145,10 +148,12 @@ struct AllocConfig {
     float sigma_max;
     uint32_t seed;
 };
-// writes counts (per pixel) and importance; scratch must hold alloc_scratch_bytes(n)
-void launch_allocation(int width, int height, const AllocConfig& ac, const PixelBuffers& pb, const Features& f,
-                       const float* variance_denoised, float* importance, int* counts, void* scratch,
-                       hipStream_t st);
+// importance field (main.cpp:1295-1371) into `importance`
+void launch_importance(int width, int height, const AllocConfig& ac, const PixelBuffers& pb, const Features& f,
+                       const float* variance_denoised, float* importance, void* scratch, hipStream_t st);
+// counts from the importance field (main.cpp:1373-1438); scratch holds alloc_scratch_bytes(n)
+void launch_alloc_counts(int n_pixels, const AllocConfig& ac, const float* importance, int* counts, void* scratch,
+                         hipStream_t st);
 size_t alloc_scratch_bytes(int n_pixels);
 void launch_finalize(int n, const PixelBuffers& pb, float* out_radiance, float* out_albedo, float* out_normal,
                      float* out_pos, float* out_depth, hipStream_t st);

@@ -80,6 +80,7 @@ def lib():
             ("rpd_job_run_pass", [vp, vp, C.c_int]), ("rpd_job_run_adaptive_pass", [vp]),
             ("rpd_job_recompute_sensitivity", [vp]), ("rpd_job_statmc_denoise", [vp]),
             ("rpd_job_variance_denoise", [vp]), ("rpd_job_adaptive_counts", [vp, vp]),
+            ("rpd_job_adaptive_counts_from_importance", [vp, vp, vp]),
             ("rpd_job_finalize", [vp]), ("rpd_job_sync", [vp]),
             ("rpd_job_buffer_size", [vp, C.c_int, C.POINTER(C.c_size_t)]),
             ("rpd_job_read", [vp, C.c_int, vp, C.c_size_t]), ("rpd_job_write", [vp, C.c_int, vp, C.c_size_t]),
@@ -206,6 +207,13 @@ class Job:
     def adaptive_counts(self):
         out = np.zeros(self.w * self.h_px, np.int32)
         check(self.L.rpd_job_adaptive_counts(self.h, _ptr(out)))
+        return out
+
+    def counts_from_importance(self, importance):
+        imp = np.ascontiguousarray(importance, np.float32)
+        assert imp.size == self.w * self.h_px
+        out = np.zeros(self.w * self.h_px, np.int32)
+        check(self.L.rpd_job_adaptive_counts_from_importance(self.h, _ptr(imp), _ptr(out)))
         return out
 
     def finalize(self):

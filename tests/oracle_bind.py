@@ -109,6 +109,7 @@ def lib():
         L.orc_job_flags.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.orc_pass.argtypes = [vp, C.c_void_p, C.c_int]
         L.orc_adaptive_counts.argtypes = [vp, C.c_void_p]
+        L.orc_adaptive_counts_from_importance.argtypes = [vp, C.c_void_p, C.c_void_p]
         L.orc_stage_ms.argtypes = [vp, C.c_int]
         L.orc_stage_ms.restype = C.c_double
         L.orc_dims.argtypes = [vp, C.c_void_p]
@@ -213,6 +214,12 @@ class OracleJob:
     def adaptive_counts(self):
         out = np.zeros(self.w * self.ht, np.int32)
         self.L.orc_adaptive_counts(self.h, ptr(out))
+        return out
+
+    def counts_from_importance(self, importance):
+        imp = np.ascontiguousarray(importance, np.float32)
+        out = np.zeros(self.w * self.ht, np.int32)
+        self.L.orc_adaptive_counts_from_importance(self.h, ptr(imp), ptr(out))
         return out
 
     def finalize(self):
