@@ -511,7 +511,7 @@ int job_alloc(rpd_job* j) {
     j->counts = plan; j->offsets = plan + N; j->active = plan + 2 * N; j->totals = plan + 3 * N;
     j->work = plan + 3 * N + 4;
     A(j->scan_scratch, (size_t)plan_scan_scratch_elems(j->N));
-    A(j->tape, N * MT_N);
+    A(j->tape, ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_N);
     const int per_px = std::max(j->cfg.samples_per_pixel, j->cfg.adaptive_spp + 1);
     const int64_t cap = int64_t(per_px) * int64_t(N);
     j->rec.capacity = cap;
@@ -533,7 +533,7 @@ size_t job_bytes(const rpd_render_config& c) {
     const size_t nn = size_t(std::max(2, (c.image_width + step - 1) / step + 1)) *
                       size_t(std::max(2, (c.image_height + step - 1) / step + 1));
     const size_t per_px = std::max(c.samples_per_pixel, c.adaptive_spp + 1);
-    return N * 4 * (13 + 2 + 12 + 7 + 14 + 13 + 1 + 3 + 3 + 32) + N * MT_N * 4 + per_px * N * (6 * 16 + 8) +
+    return N * 4 * (13 + 2 + 12 + 7 + 14 + 13 + 1 + 3 + 3 + 32) + ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_N * 4 + per_px * N * (6 * 16 + 8) +
            nn * (GRID_NODE_DOUBLES * 8 + SF_COUNT * 4 + 33 * 4) + alloc_scratch_bytes(int(N));
 }
 

@@ -25,24 +25,31 @@ __global__ void __launch_bounds__(256) k_tape(TraceParams p) {
     const int pix = p.plan.active[a];
     const int n_before = p.seed_n ? p.seed_n[pix] : 0;
     const uint32_t s = pixel_sample_seed(p.base_seed, pix, n_before);
-    uint32_t* t = p.tape + (size_t)a * MT_N;
+    uint32_t* t = p.tape + tape_base(a);
     uint32_t lo = s;               // x[k]
-    uint32_t far = s;              // x[k + 397]
+    uint32_t far = s;              // x[k + 397] while k + 397 < 624
     for (uint32_t k = 1; k <= (uint32_t)MT_M; ++k) far = mt_seed_step(far, k);
-    const uint32_t x0 = s;
-    for (int k = 0; k < MT_N - MT_M; ++k) {          // k + 397 < 624: far word from the seeded state
-        const uint32_t nxt = mt_seed_step(lo, (uint32_t)(k + 1));
-        t[k] = mt_mix(lo, nxt, far);
-        lo = nxt;
-        if (k + 1 < MT_N - MT_M) far = mt_seed_step(far, (uint32_t)(k + 1 + MT_M));
+    // four words per store: the 64 lanes of a wave write one contiguous KiB
+    for (int c = 0; c < MT_N / 4; ++c) {
+        uint32_t w4[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = 4 * c + i;
+            if (k < MT_N - MT_M) {                       // far word from the seeded state
+                const uint32_t nxt = mt_seed_step(lo, (uint32_t)(k + 1));
+                w4[i] = mt_mix(lo, nxt, far);
+                lo = nxt;
+                if (k + 1 < MT_N - MT_M) far = mt_seed_step(far, (uint32_t)(k + 1 + MT_M));
+            } else if (k < MT_N - 1) {                   // far word is an already twisted word
+                const uint32_t nxt = mt_seed_step(lo, (uint32_t)(k + 1));
+                w4[i] = mt_mix(lo, nxt, t[tape_word(k + MT_M - MT_N)]);
+                lo = nxt;
+            } else {                                     // last word wraps to the new word 0
+                w4[i] = mt_mix(lo, t[tape_word(0)], t[tape_word(MT_M - 1)]);
+            }
+        }
+        *reinterpret_cast<uint4*>(t + tape_word(4 * c)) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
     }
-    for (int k = MT_N - MT_M; k < MT_N - 1; ++k) {   // far word is an already twisted word
-        const uint32_t nxt = mt_seed_step(lo, (uint32_t)(k + 1));
-        t[k] = mt_mix(lo, nxt, t[k + MT_M - MT_N]);
-        lo = nxt;
-    }
-    t[MT_N - 1] = mt_mix(lo, t[0], t[MT_M - 1]);
-    (void)x0;
 }
 
 void launch_tape(const TraceParams& p, hipStream_t st) {
@@ -132,7 +139,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
                 pix = p.plan.active[a];
                 left = p.plan.counts[pix];
                 slot = p.plan.offsets[pix];
-                dr.reset(p.tape + (size_t)a * MT_N);
+                dr.reset(p.tape + tape_base(a));
                 phase = PH_CAMERA;
             }
             const int taken = min(__popcll(need), avail);

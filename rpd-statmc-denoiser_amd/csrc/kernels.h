@@ -55,7 +55,7 @@ struct TraceParams {
     PassPlan plan;
     const int* seed_n;       // per-pixel sample offset used for seeding (finite samples so far), or null for 0
     uint32_t base_seed;
-    uint32_t* tape;          // n_active * 624 words
+    uint32_t* tape;          // ceil(n/64)*64 tapes of 624 words, layout of tape_base/tape_word
     int* work_counter;       // zero before launch
     Records rec;
 };

@@ -111,18 +111,30 @@ __device__ __forceinline__ float mt_to_unit(uint32_t u) {
     float r = float(u) / 4294967296.0f;
     return r >= 1.0f ? 0x1.fffffep-1f : r;
 }
+// Tapes are stored in groups of 64: word w of tape a lives at
+//   (a / 64) * 624 * 64 + (w / 4) * 256 + (a % 64) * 4 + (w % 4)
+// so 64 consecutive tapes write one contiguous KiB per 4 words when they are
+// generated together, and one tape still reads 4 consecutive words as a uint4.
+constexpr int MT_GROUP = 64;
+__host__ __device__ __forceinline__ size_t tape_base(size_t a) {
+    return (a / MT_GROUP) * size_t(MT_N * MT_GROUP) + (a % MT_GROUP) * 4;
+}
+__device__ __forceinline__ int tape_word(int w) { return (w >> 2) * (4 * MT_GROUP) + (w & 3); }
+
 // Regenerates a tape in place (the standard twist over all 624 words).
 __device__ inline void mt_twist(uint32_t* __restrict__ s) {
-    for (int k = 0; k < MT_N - MT_M; ++k) s[k] = mt_mix(s[k], s[k + 1], s[k + MT_M]);
-    for (int k = MT_N - MT_M; k < MT_N - 1; ++k) s[k] = mt_mix(s[k], s[k + 1], s[k + MT_M - MT_N]);
-    s[MT_N - 1] = mt_mix(s[MT_N - 1], s[0], s[MT_M - 1]);
+    for (int k = 0; k < MT_N - MT_M; ++k)
+        s[tape_word(k)] = mt_mix(s[tape_word(k)], s[tape_word(k + 1)], s[tape_word(k + MT_M)]);
+    for (int k = MT_N - MT_M; k < MT_N - 1; ++k)
+        s[tape_word(k)] = mt_mix(s[tape_word(k)], s[tape_word(k + 1)], s[tape_word(k + MT_M - MT_N)]);
+    s[tape_word(MT_N - 1)] = mt_mix(s[tape_word(MT_N - 1)], s[tape_word(0)], s[tape_word(MT_M - 1)]);
 }
 
 // Per-lane draw cursor over a tape with a 4-word register cache.
 struct Draws {
-    uint32_t* tape;
-    int pos;      // next word index in [0, 624]
-    uint4 cache;  // words [pos & ~3, pos | 3]
+    uint32_t* tape;  // tape_base of this pixel's tape
+    int pos;         // next word index in [0, 624]
+    uint4 cache;     // words [pos & ~3, pos | 3]
     __device__ __forceinline__ void reset(uint32_t* t) { tape = t; pos = 0; }
     __device__ __forceinline__ float next() {
         if (pos == MT_N) {
@@ -130,7 +142,7 @@ struct Draws {
             pos = 0;
             cache = *reinterpret_cast<const uint4*>(tape);
         } else if ((pos & 3) == 0) {
-            cache = *reinterpret_cast<const uint4*>(tape + pos);
+            cache = *reinterpret_cast<const uint4*>(tape + tape_word(pos));
         }
         const int k = pos & 3;
         const uint32_t w = k == 0 ? cache.x : (k == 1 ? cache.y : (k == 2 ? cache.z : cache.w));
