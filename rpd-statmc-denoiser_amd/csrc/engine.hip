@@ -78,6 +78,7 @@ struct rpd_scene {
     std::vector<void*> owned;
     DevScene dev{};
     int n_tris = 0, n_spheres = 0, n_materials = 0, n_lights = 0;
+    int trace_per_cu = 0;  // resident trace blocks per CU, set on first use
 };
 
 struct rpd_job {
@@ -319,12 +320,8 @@ TraceParams trace_params(rpd_job* j, int statmc) {
 }
 
 int trace_blocks(rpd_job* j) {
-    static int per_cu = -1;
-    if (per_cu < 0) {
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, trace_kernel_ptr(), trace_block_size(), 0) != hipSuccess || nb <= 0) nb = 2;
-        per_cu = nb;
-    }
+    if (j->scene->trace_per_cu <= 0) j->scene->trace_per_cu = trace_blocks_per_cu(j->scene->n_tris);
+    const int per_cu = j->scene->trace_per_cu;
     const int by_work = (j->N + trace_block_size() - 1) / trace_block_size();
     return std::max(1, std::min(by_work, per_cu * std::max(1, j->ctx->num_cus)));
 }
@@ -731,22 +728,38 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
     }
     auto sc = std::make_unique<rpd_scene>();
     sc->ctx = ctx;
-    DevTriangle* pt; DevSphere* ps; DevMaterial* pm; DevLight* pl; float* pc;
+    // triangle vertices in the three cyclic axis orders of the intersector
+    std::vector<float4> perm(size_t(n_tris) * 9);
+    for (int i = 0; i < n_tris; ++i) {
+        for (int kz = 0; kz < 3; ++kz) {
+            const int kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
+            const float* v[3] = {dt[i].v0, dt[i].v1, dt[i].v2};
+            float f[12] = {};
+            for (int k = 0; k < 3; ++k) { f[3 * k] = v[k][kx]; f[3 * k + 1] = v[k][ky]; f[3 * k + 2] = v[k][kz]; }
+            float4* e = &perm[tri_perm_entry(i, kz)];
+            for (int q = 0; q < 3; ++q) e[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
+        }
+    }
+    DevTriangle* pt; DevSphere* ps; DevMaterial* pm; DevLight* pl; float* pc; float4* pp;
     int st;
-    if ((st = dalloc(&pt, dt.size(), sc->owned)) || (st = dalloc(&ps, ds.size(), sc->owned)) ||
+    if ((st = dalloc(&pt, dt.size(), sc->owned)) || (st = dalloc(&pp, perm.size(), sc->owned)) ||
+        (st = dalloc(&ps, ds.size(), sc->owned)) ||
         (st = dalloc(&pm, dm.size(), sc->owned)) || (st = dalloc(&pl, dl.size(), sc->owned)) ||
         (st = dalloc(&pc, cdf.size(), sc->owned))) {
         for (void* p : sc->owned) hipFree(p);
         return st;
     }
-    if (n_tris) HIP_TRY(hipMemcpy(pt, dt.data(), dt.size() * sizeof(DevTriangle), hipMemcpyHostToDevice));
+    if (n_tris) {
+        HIP_TRY(hipMemcpy(pt, dt.data(), dt.size() * sizeof(DevTriangle), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(pp, perm.data(), perm.size() * sizeof(float4), hipMemcpyHostToDevice));
+    }
     if (n_spheres) HIP_TRY(hipMemcpy(ps, ds.data(), ds.size() * sizeof(DevSphere), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(pm, dm.data(), dm.size() * sizeof(DevMaterial), hipMemcpyHostToDevice));
     if (n_lights) {
         HIP_TRY(hipMemcpy(pl, dl.data(), dl.size() * sizeof(DevLight), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(pc, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice));
     }
-    sc->dev.tris = pt; sc->dev.n_tris = n_tris;
+    sc->dev.tris = pt; sc->dev.n_tris = n_tris; sc->dev.tri_perm = pp;
     sc->dev.spheres = ps; sc->dev.n_spheres = n_spheres;
     sc->dev.materials = pm;
     sc->dev.lights = pl; sc->dev.n_lights = n_lights;

@@ -62,189 +62,73 @@ void launch_tape(const TraceParams& p, hipStream_t st) {
 // samples in order from its tape, so every random number lands where the
 // reference draws it. Lanes regenerate: a lane whose sample ended starts the
 // pixel's next sample, and a lane whose pixel ended takes the next pixel from
-// a wave-level chunk of the active list. Each loop iteration runs at most one
-// shading step (light + environment + BRDF sampling, shadow rays) and one
-// closest-hit query per lane, with every lane's query in the same code path.
+// a wave-level chunk of the active list.
+//
+// Every loop iteration resolves the lane's last ray query, advances the lane
+// through the path stages (camera, light sample, environment sample, BRDF
+// sample) until it issues its next query, and then runs one intersection loop
+// for all lanes at once: camera and extension rays as closest-hit queries,
+// both shadow rays of a bounce as occlusion queries. All random draws of a
+// bounce are independent of the shadow-ray outcomes, so splitting a bounce
+// over three queries keeps the draw order of main.cpp:394-533.
 //
 // Radiance is accumulated forward (throughput times each bounce's direct
 // term); the reference's recursion sums the same terms innermost-first.
-enum : int { PH_NEED = 0, PH_CAMERA = 1, PH_SHADE = 2, PH_TRACE_PRIMARY = 3, PH_TRACE_EXT = 4, PH_DONE = 5, PH_DEAD = 6 };
+enum : int { ST_NEED = 0, ST_CAMERA = 1, ST_LIGHT = 2, ST_ENV = 3, ST_BRDF = 4, ST_DONE = 5, ST_DEAD = 6 };
+enum : int { Q_NONE = 0, Q_PRIMARY = 1, Q_EXT = 2, Q_SHADOW_L = 3, Q_SHADOW_E = 4 };
 
 constexpr int TRACE_BLOCK = 256;
 constexpr int TRACE_CHUNK = 64;
 
 __device__ __forceinline__ uint64_t ballot64(bool b) { return __ballot(b); }
+__device__ __forceinline__ bool is_zero3(v3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
 
+// Triangle tables up to this many triangles are staged in LDS.
+constexpr int TRACE_LDS_TRIS = 256;
+
+template <bool kLds>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
     const DevScene& S = p.scene;
+    extern __shared__ float4 s_tri[];
+    if (kLds) {
+        for (int k = threadIdx.x; k < S.n_tris * 9; k += TRACE_BLOCK) s_tri[k] = S.tri_perm[k];
+        __syncthreads();
+    }
+    const float4* tp = kLds ? s_tri : S.tri_perm;
     const int lane = threadIdx.x & 63;
-    const float env_x = S.env[0], env_y = S.env[1], env_z = S.env[2];
-    const v3 env = mk3(env_x, env_y, env_z);
+    const v3 env = mk3(S.env[0], S.env[1], S.env[2]);
     const bool env_on = maxc3(env) > 0.0f;
     const float pdf_env = float(1.0 / (4.0 * RPD_PI));
+    const float inf = __builtin_inff();
 
     // wave-uniform chunk cursor over the active list
     int c_next = 0, c_end = 0;
     const int n_active = p.plan.totals[1];
 
-    int phase = PH_NEED;
+    int st = ST_NEED, q = Q_NONE;
     int pix = -1, left = 0;
     int64_t slot = 0;
     Draws dr;
     dr.tape = nullptr; dr.pos = 0; dr.cache = make_uint4(0, 0, 0, 0);
-    // path state
-    v3 ro = mk3(0, 0, 0), rd = mk3(0, 0, 0);
+    v3 ro = mk3(0, 0, 0), rd = mk3(0, 0, 0);   // current query ray
+    float qt = inf;                             // query range
     v3 T = mk3(1, 1, 1), L = mk3(0, 0, 0), Ld = mk3(0, 0, 0);
     v3 sp = mk3(0, 0, 0), sn = mk3(0, 0, 0), swo = mk3(0, 0, 0);  // shading point
-    int smat = 0;
+    v3 cw = mk3(0, 0, 0);                       // shadow-ray contribution, or BRDF weight of the extension ray
     float bpdf = 0.0f;
-    v3 bw = mk3(0, 0, 0);
-    uint32_t flags = 0;
-    v2 c_pix, c_lens, c_brdf, c_light, c_env;
-    c_pix.x = c_pix.y = c_lens.x = c_lens.y = c_brdf.x = c_brdf.y = 0.0f;
-    c_light.x = c_light.y = c_env.x = c_env.y = 0.0f;
-    uint32_t light_bits = 255u;
+    int smat = 0;
+    uint32_t flags = 0, light_bits = 255u;
+    HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
 
     while (true) {
-        // -- finish samples ------------------------------------------------
-        if (phase == PH_DONE) {
-            if (finite3(L)) flags |= REC_FINITE;
-            flags |= (light_bits & 255u) << REC_LIGHT_SHIFT;
-            p.rec.r0[slot] = make_float4(L.x, L.y, L.z, __uint_as_float(flags));
-            if (p.capture) {
-                p.rec.r4[slot] = make_float4(c_pix.x, c_pix.y, c_lens.x, c_lens.y);
-                p.rec.r5[slot] = make_float4(c_brdf.x, c_brdf.y, c_light.x, c_light.y);
-                p.rec.r6[slot] = make_float2(c_env.x, c_env.y);
-            }
-            ++slot;
-            --left;
-            phase = left > 0 ? PH_CAMERA : PH_NEED;
-        }
-        // -- hand out pixels ------------------------------------------------
-        uint64_t need = ballot64(phase == PH_NEED);
-        while (need) {
-            if (c_next >= c_end) {
-                int base = 0;
-                if (lane == 0) base = atomicAdd(p.work_counter, TRACE_CHUNK);
-                base = __shfl(base, 0);
-                c_next = base;
-                c_end = min(base + TRACE_CHUNK, n_active);
-                if (c_next >= c_end) break;
-            }
-            const int avail = c_end - c_next;
-            const int rank = __popcll(need & ((1ull << lane) - 1ull));
-            const bool mine = (phase == PH_NEED) && rank < avail;
-            if (mine) {
-                const int a = c_next + rank;
-                pix = p.plan.active[a];
-                left = p.plan.counts[pix];
-                slot = p.plan.offsets[pix];
-                dr.reset(p.tape + tape_base(a));
-                phase = PH_CAMERA;
-            }
-            const int taken = min(__popcll(need), avail);
-            c_next += taken;
-            need = ballot64(phase == PH_NEED);
-        }
-        if (phase == PH_NEED) phase = PH_DEAD;
-        if (ballot64(phase != PH_DEAD) == 0) break;
-
-        // -- camera rays: main.cpp:691-705 ----------------------------------
-        if (phase == PH_CAMERA) {
-            const int x = pix % p.width, y = pix / p.width;
-            c_pix = dr.next2();
-            const float u = (float(x) + c_pix.x) / float(p.width);
-            const float v = (float(y) + c_pix.y) / float(p.height);
-            c_lens = dr.next2();
-            v2 lens; lens.x = 0.0f; lens.y = 0.0f;
-            if (p.cam.lens_radius > 0.0f) lens = sample_disk(c_lens);
-            flags = 0;
-            light_bits = 255u;
-            if (p.capture) {
-                flags |= REC_PIXEL;
-                if (p.cam.lens_radius > 0.0f) flags |= REC_LENS;
-            }
-            v3 o = ld3(p.cam.pos);
-            if (p.cam.lens_radius > 0.0f) {
-                const float lx = p.cam.lens_radius * lens.x, ly = p.cam.lens_radius * lens.y;
-                o = o + (lx * ld3(p.cam.right) + ly * ld3(p.cam.up));
-            }
-            const v3 px = ld3(p.cam.llc) + u * ld3(p.cam.horiz) + (1.0f - v) * ld3(p.cam.vert);
-            ro = o;
-            rd = normalized3(px - o);
-            T = mk3(1, 1, 1);
-            L = mk3(0, 0, 0);
-            phase = PH_TRACE_PRIMARY;
-        }
-
-        // -- shading: main.cpp:394-479 ----------------------------------------
-        if (phase == PH_SHADE) {
-            const DevMaterial& m = S.materials[smat];
-            const bool delta_brdf = m.kind == MAT_MIRROR;
-            Ld = mk3(0, 0, 0);
-            const float sel_u = dr.next();
-            int li; float sel_pdf;
-            sample_light(S, sel_u, li, sel_pdf);
-            if (p.capture && !(flags & REC_LSEL) && li >= 0 && li < 16) { flags |= REC_LSEL; light_bits = (uint32_t)li; }
-            if (li >= 0 && (!delta_brdf || light_is_delta(S.lights[li]))) {
-                const DevLight& lt = S.lights[li];
-                const bool is_delta = light_is_delta(lt);
-                const v2 lu = dr.next2();
-                if (p.capture && !(flags & REC_LIGHT) && !is_delta) { flags |= REC_LIGHT; c_light = lu; }
-                v3 wi; float dist;
-                light_sample(lt, sp, lu, wi, dist);
-                if (dist > RPD_EPS_SMALL || is_delta) {
-                    const float pl = light_pdf(S, li, sel_pdf, wi, dist);
-                    if (pl > RPD_EPS_SMALL) {
-                        const float pw = fmaxf(RPD_EPS_SMALL, brdf_pdf(m, swo, wi, sn));
-                        const float wl = is_delta ? 1.0f : pl / (pl + pw);
-                        if (!any_hit(S, offset_origin(sp, sn), wi, dist)) {
-                            const v3 f = brdf_eval(m, swo, wi, sn);
-                            const v3 em = eval_light(lt, -wi, dist);
-                            if (maxc3(em) > 0.0f) {
-                                const float g = delta_brdf ? 1.0f : fmaxf(0.0f, dot3(sn, wi));
-                                const v3 contrib = (vmul(em, f) * g) / pl;
-                                Ld = Ld + wl * contrib;
-                            }
-                        }
-                    }
-                }
-            }
-            const v2 eu = dr.next2();
-            if (env_on && !delta_brdf) {
-                const v3 ed = sample_sphere(eu);
-                if (p.capture && !(flags & REC_ENV)) { flags |= REC_ENV; c_env = eu; }
-                const float bp = brdf_pdf(m, swo, ed, sn);
-                const float we = pdf_env / (pdf_env + bp);
-                if (!any_hit(S, offset_origin(sp, sn), ed, __builtin_inff())) {
-                    const v3 f = brdf_eval(m, swo, ed, sn);
-                    const float c = fmaxf(0.0f, dot3(sn, ed));
-                    Ld = Ld + (we * vmul(env, f)) * (c / pdf_env);
-                }
-            }
-            const v2 bu = dr.next2();
-            if (p.capture && !delta_brdf && !(flags & REC_BRDF)) { flags |= REC_BRDF; c_brdf = bu; }
-            v3 dir; float pdf;
-            brdf_sample(m, swo, sn, bu, dir, pdf);
-            if (pdf > RPD_EPS_SMALL && sqnorm3(dir) > 0.0f) {
-                bpdf = pdf;
-                bw = brdf_sample_weight(m, swo, dir, sn, pdf);
-                ro = offset_origin(sp, sn);
-                rd = dir;
-                phase = PH_TRACE_EXT;
-            } else {
-                L = L + vmul(T, Ld);
-                phase = PH_DONE;
-            }
-        }
-
-        // -- closest hit for camera and extension rays ------------------------
-        if (phase == PH_TRACE_PRIMARY || phase == PH_TRACE_EXT) {
-            const HitRec h = closest_hit(S, ro, rd);
+        // -- resolve the last query -------------------------------------------
+        if (q == Q_SHADOW_L || q == Q_SHADOW_E) {
+            if (!h.hit) Ld = Ld + cw;
+            st = q == Q_SHADOW_L ? ST_ENV : ST_BRDF;
+        } else if (q == Q_PRIMARY || q == Q_EXT) {
             bool emitter = false;
-            int li = -1;
+            int li = -1, hm = 0;
             v3 hn = mk3(0, 0, 0);
-            int hm = 0;
             if (h.hit) {
                 if (h.sphere) {
                     const DevSphere& s = S.spheres[h.index];
@@ -254,11 +138,11 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
                     emitter = t.emitter != 0; li = t.light; hn = ld3(t.n); hm = t.material;
                 }
             }
-            if (phase == PH_TRACE_PRIMARY) {
+            if (q == Q_PRIMARY) {
                 // main.cpp:540-585
                 if (!h.hit) {
                     L = env;
-                    phase = PH_DONE;
+                    st = ST_DONE;
                 } else {
                     const v3 hp = ro + h.t * rd;
                     flags |= REC_HIT;
@@ -268,10 +152,10 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
                     p.rec.r3[slot] = make_float4(hp.x, hp.y, hp.z, 0.0f);
                     if (emitter) {
                         L = eval_light(S.lights[li], -rd, h.t);
-                        phase = PH_DONE;
+                        st = ST_DONE;
                     } else {
                         sp = hp; sn = hn; swo = -normalized3(rd); smat = hm;
-                        phase = PH_SHADE;
+                        st = ST_LIGHT;
                     }
                 }
             } else {
@@ -285,35 +169,205 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
                         const float plb = light_pdf(S, li, light_select_pdf(S, li), rd, h.t);
                         if (plb > RPD_EPS_SMALL) wb = bpdf / (bpdf + plb);
                     }
-                    Ld = Ld + wb * vmul(eval_light(E, -rd, h.t), bw);
+                    Ld = Ld + wb * vmul(eval_light(E, -rd, h.t), cw);
                 } else if (!h.hit) {
                     const float wb = delta_brdf ? 1.0f : bpdf / (bpdf + pdf_env);
-                    Ld = Ld + wb * vmul(env, bw);
+                    Ld = Ld + wb * vmul(env, cw);
                 }
                 L = L + vmul(T, Ld);
-                phase = PH_DONE;
+                st = ST_DONE;
                 if (h.hit && !emitter) {
                     const float ksi = dr.next();
                     if (p.capture && !(flags & REC_RR)) { flags |= REC_RR; if (ksi < 0.8f) flags |= REC_RR_SURVIVED; }
                     if (ksi < 0.8f) {
-                        T = vmul(T, bw) / 0.8f;
+                        T = vmul(T, cw) / 0.8f;
                         sp = ro + h.t * rd;
                         sn = hn;
                         swo = -rd;
                         smat = hm;
-                        phase = PH_SHADE;
+                        st = ST_LIGHT;
                     }
                 }
             }
         }
+        q = Q_NONE;
+
+        // -- finish samples ------------------------------------------------
+        if (st == ST_DONE) {
+            if (finite3(L)) flags |= REC_FINITE;
+            flags |= (light_bits & 255u) << REC_LIGHT_SHIFT;
+            p.rec.r0[slot] = make_float4(L.x, L.y, L.z, __uint_as_float(flags));
+            ++slot;
+            --left;
+            st = left > 0 ? ST_CAMERA : ST_NEED;
+        }
+        // -- hand out pixels ------------------------------------------------
+        uint64_t need = ballot64(st == ST_NEED);
+        while (need) {
+            if (c_next >= c_end) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(p.work_counter, TRACE_CHUNK);
+                base = __shfl(base, 0);
+                c_next = base;
+                c_end = min(base + TRACE_CHUNK, n_active);
+                if (c_next >= c_end) break;
+            }
+            const int avail = c_end - c_next;
+            const int rank = __popcll(need & ((1ull << lane) - 1ull));
+            const bool mine = (st == ST_NEED) && rank < avail;
+            if (mine) {
+                const int a = c_next + rank;
+                pix = p.plan.active[a];
+                left = p.plan.counts[pix];
+                slot = p.plan.offsets[pix];
+                dr.reset(p.tape + tape_base(a));
+                st = ST_CAMERA;
+            }
+            const int taken = min(__popcll(need), avail);
+            c_next += taken;
+            need = ballot64(st == ST_NEED);
+        }
+        if (st == ST_NEED) st = ST_DEAD;
+        if (ballot64(st != ST_DEAD) == 0) break;
+
+        // -- camera rays: main.cpp:691-705 ----------------------------------
+        if (st == ST_CAMERA) {
+            const int x = pix % p.width, y = pix / p.width;
+            const v2 c_pix = dr.next2();
+            const float u = (float(x) + c_pix.x) / float(p.width);
+            const float v = (float(y) + c_pix.y) / float(p.height);
+            const v2 c_lens = dr.next2();
+            v2 lens; lens.x = 0.0f; lens.y = 0.0f;
+            if (p.cam.lens_radius > 0.0f) lens = sample_disk(c_lens);
+            flags = 0;
+            light_bits = 255u;
+            if (p.capture) {
+                flags |= REC_PIXEL;
+                if (p.cam.lens_radius > 0.0f) flags |= REC_LENS;
+                p.rec.r4[slot] = make_float4(c_pix.x, c_pix.y, c_lens.x, c_lens.y);
+            }
+            v3 o = ld3(p.cam.pos);
+            if (p.cam.lens_radius > 0.0f) {
+                const float lx = p.cam.lens_radius * lens.x, ly = p.cam.lens_radius * lens.y;
+                o = o + (lx * ld3(p.cam.right) + ly * ld3(p.cam.up));
+            }
+            const v3 px = ld3(p.cam.llc) + u * ld3(p.cam.horiz) + (1.0f - v) * ld3(p.cam.vert);
+            ro = o;
+            rd = normalized3(px - o);
+            qt = inf;
+            T = mk3(1, 1, 1);
+            L = mk3(0, 0, 0);
+            q = Q_PRIMARY;
+        }
+
+        // -- light sample of a bounce: main.cpp:405-448 ------------------------
+        if (st == ST_LIGHT) {
+            const DevMaterial& m = S.materials[smat];
+            const bool delta_brdf = m.kind == MAT_MIRROR;
+            Ld = mk3(0, 0, 0);
+            ro = offset_origin(sp, sn);
+            st = ST_ENV;
+            const float sel_u = dr.next();
+            int li; float sel_pdf;
+            sample_light(S, sel_u, li, sel_pdf);
+            if (p.capture && !(flags & REC_LSEL) && li >= 0 && li < 16) { flags |= REC_LSEL; light_bits = (uint32_t)li; }
+            if (li >= 0 && (!delta_brdf || light_is_delta(S.lights[li]))) {
+                const DevLight& lt = S.lights[li];
+                const bool is_delta = light_is_delta(lt);
+                const v2 lu = dr.next2();
+                if (p.capture && !(flags & REC_LIGHT) && !is_delta) {
+                    flags |= REC_LIGHT;
+                    reinterpret_cast<float2*>(&p.rec.r5[slot])[1] = make_float2(lu.x, lu.y);
+                }
+                v3 wi; float dist;
+                light_sample(lt, sp, lu, wi, dist);
+                if (dist > RPD_EPS_SMALL || is_delta) {
+                    const float pl = light_pdf(S, li, sel_pdf, wi, dist);
+                    if (pl > RPD_EPS_SMALL) {
+                        const float pw = fmaxf(RPD_EPS_SMALL, brdf_pdf(m, swo, wi, sn));
+                        const float wl = is_delta ? 1.0f : pl / (pl + pw);
+                        const v3 em = eval_light(lt, -wi, dist);
+                        if (maxc3(em) > 0.0f) {
+                            const v3 f = brdf_eval(m, swo, wi, sn);
+                            const float g = delta_brdf ? 1.0f : fmaxf(0.0f, dot3(sn, wi));
+                            const v3 contrib = (vmul(em, f) * g) / pl;
+                            cw = wl * contrib;
+                            const float eff = dist - RPD_EPS_ANYHIT;  // ray_tracer.h:65-75
+                            if (eff <= RPD_EPS_SMALL) {
+                                Ld = Ld + cw;
+                            } else if (!is_zero3(cw)) {
+                                rd = wi; qt = eff; q = Q_SHADOW_L; st = ST_LIGHT;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // -- environment sample: main.cpp:450-462 -------------------------------
+        if (st == ST_ENV && q == Q_NONE) {
+            const DevMaterial& m = S.materials[smat];
+            const bool delta_brdf = m.kind == MAT_MIRROR;
+            st = ST_BRDF;
+            const v2 eu = dr.next2();
+            if (env_on && !delta_brdf) {
+                const v3 ed = sample_sphere(eu);
+                if (p.capture && !(flags & REC_ENV)) {
+                    flags |= REC_ENV;
+                    p.rec.r6[slot] = make_float2(eu.x, eu.y);
+                }
+                const float bp = brdf_pdf(m, swo, ed, sn);
+                const float we = pdf_env / (pdf_env + bp);
+                const v3 f = brdf_eval(m, swo, ed, sn);
+                const float c = fmaxf(0.0f, dot3(sn, ed));
+                cw = (we * vmul(env, f)) * (c / pdf_env);
+                if (!is_zero3(cw)) { rd = ed; qt = inf; q = Q_SHADOW_E; st = ST_ENV; }
+            }
+        }
+        // -- BRDF sample: main.cpp:464-479 --------------------------------------
+        if (st == ST_BRDF && q == Q_NONE) {
+            const DevMaterial& m = S.materials[smat];
+            const bool delta_brdf = m.kind == MAT_MIRROR;
+            const v2 bu = dr.next2();
+            if (p.capture && !delta_brdf && !(flags & REC_BRDF)) {
+                flags |= REC_BRDF;
+                reinterpret_cast<float2*>(&p.rec.r5[slot])[0] = make_float2(bu.x, bu.y);
+            }
+            v3 dir; float pdf;
+            brdf_sample(m, swo, sn, bu, dir, pdf);
+            if (pdf > RPD_EPS_SMALL && sqnorm3(dir) > 0.0f) {
+                bpdf = pdf;
+                cw = brdf_sample_weight(m, swo, dir, sn, pdf);
+                rd = dir; qt = inf; q = Q_EXT;
+            } else {
+                L = L + vmul(T, Ld);
+                st = ST_DONE;   // no query: the sample is written next iteration
+            }
+        }
+
+        // -- one intersection query per lane ----------------------------------
+        if (q != Q_NONE) h = trace_query(S, tp, ro, rd, qt, q >= Q_SHADOW_L);
     }
 }
-
 int trace_block_size() { return TRACE_BLOCK; }
-const void* trace_kernel_ptr() { return reinterpret_cast<const void*>(&k_trace); }
+
+static size_t trace_lds_bytes(int n_tris) {
+    return n_tris <= TRACE_LDS_TRIS ? size_t(n_tris) * 9 * sizeof(float4) : 0;
+}
+
+int trace_blocks_per_cu(int n_tris) {
+    const size_t lds = trace_lds_bytes(n_tris);
+    const void* fn = lds ? reinterpret_cast<const void*>(&k_trace<true>) : reinterpret_cast<const void*>(&k_trace<false>);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, TRACE_BLOCK, lds) != hipSuccess || nb <= 0) nb = 1;
+    return nb;
+}
 
 void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st) {
-    hipLaunchKernelGGL(k_trace, dim3(n_blocks), dim3(TRACE_BLOCK), 0, st, p);
+    const size_t lds = trace_lds_bytes(p.scene.n_tris);
+    if (lds)
+        hipLaunchKernelGGL(k_trace<true>, dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
+    else
+        hipLaunchKernelGGL(k_trace<false>, dim3(n_blocks), dim3(TRACE_BLOCK), 0, st, p);
 }
 
 // ---------------------------------------------------------------------------

@@ -116,7 +116,7 @@ struct VarianceConfig {
 void launch_tape(const TraceParams& p, hipStream_t st);
 void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st);
 int trace_block_size();
-const void* trace_kernel_ptr();
+int trace_blocks_per_cu(int n_tris);  // resident trace blocks per CU for a scene of n_tris triangles
 void launch_fold(const PassPlan& plan, const Records& rec, PixelBuffers pb, int statmc, hipStream_t st);
 void launch_plan_uniform(int n_pixels, int count, int* counts, int* offsets, int* active, int* totals,
                          hipStream_t st);

@@ -121,13 +121,22 @@ __host__ __device__ __forceinline__ size_t tape_base(size_t a) {
 }
 __device__ __forceinline__ int tape_word(int w) { return (w >> 2) * (4 * MT_GROUP) + (w & 3); }
 
-// Regenerates a tape in place (the standard twist over all 624 words).
+// Regenerates a tape in place (the standard twist over all 624 words) and
+// returns its first four words. Kept out of line: a draw site needs it once
+// per 624 draws, and inlining it at every draw site would multiply the trace
+// kernel's code size past the instruction cache.
+__device__ __noinline__ uint4 mt_twist_reload(uint32_t* __restrict__ s);
 __device__ inline void mt_twist(uint32_t* __restrict__ s) {
     for (int k = 0; k < MT_N - MT_M; ++k)
         s[tape_word(k)] = mt_mix(s[tape_word(k)], s[tape_word(k + 1)], s[tape_word(k + MT_M)]);
     for (int k = MT_N - MT_M; k < MT_N - 1; ++k)
         s[tape_word(k)] = mt_mix(s[tape_word(k)], s[tape_word(k + 1)], s[tape_word(k + MT_M - MT_N)]);
     s[tape_word(MT_N - 1)] = mt_mix(s[tape_word(MT_N - 1)], s[tape_word(0)], s[tape_word(MT_M - 1)]);
+}
+
+__device__ __noinline__ uint4 mt_twist_reload(uint32_t* __restrict__ s) {
+    mt_twist(s);
+    return *reinterpret_cast<const uint4*>(s);
 }
 
 // Per-lane draw cursor over a tape with a 4-word register cache.
@@ -138,9 +147,8 @@ struct Draws {
     __device__ __forceinline__ void reset(uint32_t* t) { tape = t; pos = 0; }
     __device__ __forceinline__ float next() {
         if (pos == MT_N) {
-            mt_twist(tape);
+            cache = mt_twist_reload(tape);
             pos = 0;
-            cache = *reinterpret_cast<const uint4*>(tape);
         } else if ((pos & 3) == 0) {
             cache = *reinterpret_cast<const uint4*>(tape + tape_word(pos));
         }
@@ -197,6 +205,7 @@ struct DevSphere { float c[3], r; int material, light, emitter, pad; };
 
 struct DevScene {
     const DevTriangle* tris; int n_tris;
+    const float4* tri_perm;  // n_tris * 3 entries of 3 float4, see tri_perm_entry
     const DevSphere* spheres; int n_spheres;
     const DevMaterial* materials;
     const DevLight* lights; int n_lights;
@@ -207,20 +216,30 @@ struct DevScene {
 
 __device__ __forceinline__ v3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
 
-struct RayPre {  // watertight-test setup that depends on the ray only
-    int kx, ky, kz;
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), as in
+// path_tracer/core/triangle.h:35-104.
+//
+// The test works in a ray space whose z axis kz is the ray's dominant axis.
+// The reference also swaps kx and ky when d[kz] < 0 to keep the winding; that
+// swap exchanges the sheared x and y coordinates, which negates U, V, W, det
+// and T exactly (IEEE a - b is exactly -(b - a)), so t and the hit decision do
+// not depend on it. Only the three cyclic orders (kx, ky, kz) =
+// (kz+1, kz+2, kz) mod 3 are used, and the triangle vertices are stored
+// pre-permuted for each of them: one table lookup replaces nine per-lane
+// component selects.
+struct RayPre {
+    int kz;
     float Sx, Sy, Sz;
     float okx, oky, okz;
 };
 __device__ __forceinline__ RayPre ray_pre(v3 o, v3 d) {
     RayPre r;
     const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-    int axis = 0;
-    if (ay > ax && ay > az) axis = 1;
-    if (az > ax && az > ay) axis = 2;
-    int kz = axis, kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
-    if (comp3(d, kz) < 0.0f) { const int t = kx; kx = ky; ky = t; }
-    r.kx = kx; r.ky = ky; r.kz = kz;
+    int kz = 0;
+    if (ay > ax && ay > az) kz = 1;
+    if (az > ax && az > ay) kz = 2;
+    const int kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
+    r.kz = kz;
     const float dkz = comp3(d, kz);
     r.Sx = comp3(d, kx) / dkz;
     r.Sy = comp3(d, ky) / dkz;
@@ -228,13 +247,16 @@ __device__ __forceinline__ RayPre ray_pre(v3 o, v3 d) {
     r.okx = comp3(o, kx); r.oky = comp3(o, ky); r.okz = comp3(o, kz);
     return r;
 }
-__device__ __forceinline__ float pick(const float* v, int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
 
-// Watertight ray/triangle test, path_tracer/core/triangle.h:35-104.
-__device__ __forceinline__ bool tri_hit(const DevTriangle& tri, const RayPre& r, float t_min, float t_max, float& t_out) {
-    const float Akx = pick(tri.v0, r.kx) - r.okx, Aky = pick(tri.v0, r.ky) - r.oky, Akz = pick(tri.v0, r.kz) - r.okz;
-    const float Bkx = pick(tri.v1, r.kx) - r.okx, Bky = pick(tri.v1, r.ky) - r.oky, Bkz = pick(tri.v1, r.kz) - r.okz;
-    const float Ckx = pick(tri.v2, r.kx) - r.okx, Cky = pick(tri.v2, r.ky) - r.oky, Ckz = pick(tri.v2, r.kz) - r.okz;
+// Table entry (tri, kz): v0, v1, v2 with components in the order
+// (kz+1, kz+2, kz) mod 3, packed as 9 floats in 3 float4 (3 floats padding).
+__host__ __device__ __forceinline__ int tri_perm_entry(int tri, int kz) { return (tri * 3 + kz) * 3; }
+
+__device__ __forceinline__ bool tri_hit(float4 q0, float4 q1, float4 q2, const RayPre& r, float t_min, float t_max,
+                                        float& t_out) {
+    const float Akx = q0.x - r.okx, Aky = q0.y - r.oky, Akz = q0.z - r.okz;
+    const float Bkx = q0.w - r.okx, Bky = q1.x - r.oky, Bkz = q1.y - r.okz;
+    const float Ckx = q1.z - r.okx, Cky = q1.w - r.oky, Ckz = q2.x - r.okz;
     const float Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
     const float Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
     const float Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
@@ -283,45 +305,39 @@ struct HitRec {
     v3 sn;         // sphere normal
 };
 
-// Closest hit over all triangles then spheres; ties keep the lower index
-// (path_tracer/core/ray_tracer.h:304-325 combines the two the same way).
-__device__ __forceinline__ HitRec closest_hit(const DevScene& S, v3 o, v3 d) {
+// One ray query over all triangles, then all spheres.
+//
+// Closest-hit mode (t_max = inf) is closest_hit_combined
+// (path_tracer/core/ray_tracer.h:304-325): the nearest triangle, the nearest
+// sphere root above 1e-8, and the triangle wins only if it is strictly nearer;
+// ties keep the lower index. Occlusion mode is any_hit_combined
+// (ray_tracer.h:65-75, 282-296, 326-344) with t_max already reduced by 5e-4:
+// a triangle blocks when 0 <= t < t_max, a sphere when its root is in
+// [1e-8, t_max]. In occlusion mode `hit` reports whether the ray is blocked.
+// `tp` is the pre-permuted triangle table (tri_perm_entry), in LDS or HBM.
+__device__ __forceinline__ HitRec trace_query(const DevScene& S, const float4* tp, v3 o, v3 d, float t_max,
+                                              bool occlusion) {
     HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
     const RayPre rp = ray_pre(o, d);
-    float best = __builtin_inff();
+    float best = t_max;
     int bi = -1;
-    for (int i = 0; i < S.n_tris; ++i) {
+    const float4* e = tp + tri_perm_entry(0, rp.kz);
+    for (int i = 0; i < S.n_tris; ++i, e += 9) {
         float t;
-        if (tri_hit(S.tris[i], rp, 0.0f, best, t) && t < best) { best = t; bi = i; }
+        if (tri_hit(e[0], e[1], e[2], rp, 0.0f, best, t) && t < best) { best = t; bi = i; }
     }
-    float sbest = __builtin_inff();
+    float sbest = t_max;
     int si = -1;
     v3 sn = mk3(0, 0, 0);
     for (int i = 0; i < S.n_spheres; ++i) {
         float t; v3 n;
-        if (sphere_hit(S.spheres[i], o, d, RPD_EPS_SMALL, sbest, t, n) && t < sbest) { sbest = t; si = i; sn = n; }
+        if (sphere_hit(S.spheres[i], o, d, RPD_EPS_SMALL, sbest, t, n) && (t < sbest || occlusion)) {
+            sbest = t; si = i; sn = n;
+        }
     }
     if (bi >= 0 && (si < 0 || best < sbest)) { h.hit = 1; h.index = bi; h.t = best; }
     else if (si >= 0) { h.hit = 1; h.sphere = 1; h.index = si; h.t = sbest; h.sn = sn; }
     return h;
-}
-
-// Occlusion before t_max - 5e-4 (path_tracer/core/ray_tracer.h:65-75, 282-296).
-__device__ __forceinline__ bool any_hit(const DevScene& S, v3 o, v3 d, float t_max) {
-    const float eff = t_max - RPD_EPS_ANYHIT;
-    if (eff <= RPD_EPS_SMALL) return false;
-    const RayPre rp = ray_pre(o, d);
-    bool blocked = false;
-    for (int i = 0; i < S.n_tris; ++i) {
-        float t;
-        if (tri_hit(S.tris[i], rp, 0.0f, eff, t) && t < eff) { blocked = true; break; }
-    }
-    if (blocked) return true;
-    for (int i = 0; i < S.n_spheres; ++i) {
-        float t; v3 n;
-        if (sphere_hit(S.spheres[i], o, d, RPD_EPS_SMALL, eff, t, n)) return true;
-    }
-    return false;
 }
 
 // ---- materials: path_tracer/core/material.h --------------------------------
