@@ -40,6 +40,9 @@ void launch_features(int n, const PixelBuffers& pb, Features f, hipStream_t st) 
 // 13 bin indices) in LDS; phase 2 gives each (accumulator, bin) pair one thread
 // that adds the chunk sequentially in FP64, so every sum is formed in sample
 // order without atomics. The pass totals are then added to the node.
+//
+// The support list is built by all threads at once (weights, counts, then an
+// ordered block scan), and phase 2 reads four contributions per LDS access.
 constexpr int GATHER_BLOCK = 256;
 constexpr int GATHER_MAX_PIX = 1024;  // support window of up to (2*step)^2 pixels, step <= 16
 
@@ -49,15 +52,34 @@ __device__ __forceinline__ int dep_bin(float u, float v) {  // core/stats.h:165-
     return z * 4 + x;
 }
 
+// Inclusive block scan of one int per thread (GATHER_BLOCK threads).
+__device__ __forceinline__ int gather_scan(int v, int* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = 1; o < GATHER_BLOCK; o <<= 1) {
+        const int add = t >= o ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    const int r = sh[t];
+    __syncthreads();
+    return r;
+}
+
 __global__ void __launch_bounds__(GATHER_BLOCK) k_grid_gather(GridGeom g, int W, int H, PassPlan plan, Records rec,
                                                              double* grid) {
     __shared__ int s_pix[GATHER_MAX_PIX];
     __shared__ float s_w[GATHER_MAX_PIX];
     __shared__ int s_pre[GATHER_MAX_PIX + 1];
-    __shared__ float c_w[GATHER_BLOCK];
-    __shared__ float c_y[GATHER_BLOCK];
-    __shared__ signed char c_bin[GRID_ACC][GATHER_BLOCK];
-    __shared__ int s_npix;
+    __shared__ int s_scan[GATHER_BLOCK];
+    __shared__ float4 c_w4[GATHER_BLOCK / 4];
+    __shared__ float4 c_y4[GATHER_BLOCK / 4];
+    __shared__ uint32_t c_bin4[GRID_ACC][GATHER_BLOCK / 4];
+    float* c_w = reinterpret_cast<float*>(c_w4);
+    float* c_y = reinterpret_cast<float*>(c_y4);
+    signed char* c_bin = reinterpret_cast<signed char*>(c_bin4);
 
     const int node = blockIdx.x;
     const int gx = node % g.nx, gy = node / g.nx;
@@ -66,37 +88,40 @@ __global__ void __launch_bounds__(GATHER_BLOCK) k_grid_gather(GridGeom g, int W,
     const int y_lo = max(0, (gy - 1) * g.step), y_hi = min(H - 1, (gy + 1) * g.step - 1);
     const int wx = x_hi - x_lo + 1, wy = y_hi - y_lo + 1;
     const int nwin = (wx > 0 && wy > 0) ? wx * wy : 0;
+    const bool any_active = plan.totals[1] > 0;
 
     // support pixels with their node weight (main.cpp:332-349), row-major
-    if (tid == 0) {
-        int k = 0;
-        int pre = 0;
-        for (int q = 0; q < nwin; ++q) {
+    int npix = 0, total = 0;
+    for (int q0 = 0; q0 < nwin; q0 += GATHER_BLOCK) {
+        const int q = q0 + tid;
+        float w = 0.0f;
+        int c = 0, p = 0;
+        if (q < nwin) {
             const int x = x_lo + q % wx, y = y_lo + q / wx;
             const float fx = float(x) / float(g.step), fy = float(y) / float(g.step);
             const int x0 = min(max(int(floorf(fx)), 0), g.nx - 1), y0 = min(max(int(floorf(fy)), 0), g.ny - 1);
             const int x1 = min(x0 + 1, g.nx - 1), y1 = min(y0 + 1, g.ny - 1);
             const float tx = clampf(fx - float(x0), 0.0f, 1.0f), ty = clampf(fy - float(y0), 0.0f, 1.0f);
-            float w;
             if (x0 == gx && y0 == gy) w = (1.0f - tx) * (1.0f - ty);
             else if (x1 == gx && y0 == gy) w = tx * (1.0f - ty);
             else if (x0 == gx && y1 == gy) w = (1.0f - tx) * ty;
             else if (x1 == gx && y1 == gy) w = tx * ty;
-            else continue;
-            if (w <= RPD_EPS_SMALL) continue;
-            const int p = y * W + x;
-            const int c = plan.totals[1] > 0 ? plan.counts[p] : 0;
-            if (c <= 0) continue;
-            s_pix[k] = p; s_w[k] = w; s_pre[k] = pre;
-            pre += c;
-            ++k;
+            p = y * W + x;
+            if (w > RPD_EPS_SMALL && any_active) c = max(plan.counts[p], 0);
         }
-        s_pre[k] = pre;
-        s_npix = k;
+        const bool keep = c > 0;
+        const int kpos = gather_scan(keep ? 1 : 0, s_scan) - (keep ? 1 : 0);
+        const int kcnt = s_scan[GATHER_BLOCK - 1];
+        __syncthreads();
+        const int cpos = gather_scan(c, s_scan) - c;
+        const int ccnt = s_scan[GATHER_BLOCK - 1];
+        if (keep) { s_pix[npix + kpos] = p; s_w[npix + kpos] = w; s_pre[npix + kpos] = total + cpos; }
+        npix += kcnt;
+        total += ccnt;
+        __syncthreads();
     }
+    if (tid == 0) s_pre[npix] = total;
     __syncthreads();
-    const int npix = s_npix;
-    const int total = s_pre[npix];
 
     // this thread's accumulator slot: k = tid / 18, b = tid % 18
     const int acc = tid / 18, bsel = tid % 18;
@@ -108,18 +133,18 @@ __global__ void __launch_bounds__(GATHER_BLOCK) k_grid_gather(GridGeom g, int W,
 
     for (int base = 0; base < total; base += GATHER_BLOCK) {
         const int c = base + tid;
+        signed char bins[GRID_ACC];
+        for (int k = 0; k < GRID_ACC; ++k) bins[k] = -1;
+        float y = 0.0f, wc = 0.0f;
         if (c < total) {
-            int j = 0;  // pixel of contribution c: last j with s_pre[j] <= c
-            int lo = 0, hi = npix - 1;
+            int lo = 0, hi = npix - 1;  // pixel of contribution c: last j with s_pre[j] <= c
             while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (s_pre[mid] <= c) lo = mid; else hi = mid - 1; }
-            j = lo;
+            const int j = lo;
             const int p = s_pix[j];
+            wc = s_w[j];
             const int64_t slot = int64_t(plan.offsets[p]) + (c - s_pre[j]);
             const float4 r0 = rec.r0[slot];
             const uint32_t fl = __float_as_uint(r0.w);
-            signed char bins[GRID_ACC];
-            for (int k = 0; k < GRID_ACC; ++k) bins[k] = -1;
-            float y = 0.0f;
             if (fl & REC_FINITE) {
                 y = luminance3(mk3(r0.x, r0.y, r0.z));
                 const float4 r4 = rec.r4[slot];
@@ -145,23 +170,30 @@ __global__ void __launch_bounds__(GATHER_BLOCK) k_grid_gather(GridGeom g, int W,
                 if (fl & REC_LSEL) bins[9] = (signed char)min(max(int((fl >> REC_LIGHT_SHIFT) & 255u), 0), 15);
                 if (fl & REC_RR) bins[11] = (signed char)((fl & REC_RR_SURVIVED) ? 1 : 0);
             }
-            c_w[tid] = s_w[j];
-            c_y[tid] = y;
-            for (int k = 0; k < GRID_ACC; ++k) c_bin[k][tid] = bins[k];
         }
+        // entries past the end of the chunk carry bin -1 and add nothing
+        c_w[tid] = wc;
+        c_y[tid] = y;
+        for (int k = 0; k < GRID_ACC; ++k) c_bin[k * GATHER_BLOCK + tid] = bins[k];
         __syncthreads();
-        const int nchunk = min(GATHER_BLOCK, total - base);
+        const int nquad = (min(GATHER_BLOCK, total - base) + 3) >> 2;
         if (active_t) {
-            for (int i = 0; i < nchunk; ++i) {
-                const int bi = c_bin[acc][i];
-                if (bi < 0) continue;
-                const double wd = double(c_w[i]), yd = double(c_y[i]);
-                if (bsel < 16) {
-                    if (bi == bsel) { a0 += wd; a1 += wd * yd; }
-                } else if (bsel == 16) {
-                    a0 += wd; a1 += wd * wd;
-                } else {
-                    a0 += wd * yd; a1 += wd * yd * yd;
+            for (int i4 = 0; i4 < nquad; ++i4) {
+                const uint32_t b4 = c_bin4[acc][i4];
+                const float4 w4 = c_w4[i4], y4 = c_y4[i4];
+                const float wv[4] = {w4.x, w4.y, w4.z, w4.w}, yv[4] = {y4.x, y4.y, y4.z, y4.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int bi = int(int8_t(b4 >> (8 * e)));
+                    if (bi < 0) continue;
+                    const double wd = double(wv[e]), yd = double(yv[e]);
+                    if (bsel < 16) {
+                        if (bi == bsel) { a0 += wd; a1 += wd * yd; }
+                    } else if (bsel == 16) {
+                        a0 += wd; a1 += wd * wd;
+                    } else {
+                        a0 += wd * yd; a1 += wd * yd * yd;
+                    }
                 }
             }
         }
