@@ -26,7 +26,16 @@ sys.path.insert(0, os.path.join(REPO, "rpd-statmc-denoiser_amd"))
 import rpd_engine as eng  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+# FP32 operations of one watertight ray/triangle test (core/triangle.h:35-104),
+# counting each +, -, *, / once: 9 vertex offsets, 6 shears (mul+sub = 12),
+# U/V/W (6 mul + 3 sub), det (2 add), T (6 mul + 2 add), reciprocal and t (2),
+# range tests (2).
+TRI_TEST_FLOPS = 9 + 12 + 9 + 2 + 8 + 2 + 2
+# One ray/sphere test (core/sphere.h:19-54): oc (3), a, b, c (15), disc (3),
+# sqrt and root (4), range tests (2).
+SPHERE_TEST_FLOPS = 3 + 15 + 3 + 4 + 2
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "k_trace_cornell1024_c1_traffic.md")
 
 WORKLOAD = dict(width=1024, height=1024, spp=16, adaptive_spp=4, adaptive_passes=3, color_window_radius=10,
                 var_iterations=3, aperture=0.14, focus_distance=1.05, seed=1234)
@@ -43,12 +52,30 @@ def job_camera():
     return eng.make_camera(fov=38.6, focus_distance=WORKLOAD["focus_distance"], aperture=WORKLOAD["aperture"])
 
 
-def statmc_fp64_ops(radius, n_pixels, accepted_fraction=1.0):
-    """Algorithmic FP64 operations of one StatMC window pass: per visited
-    neighbour 3 channels x (t^2: 3, df: 7, ratio: 2) plus compatibility,
-    relaxation, weight and accumulation (~20, exp counted as 1)."""
-    per_neighbour = 3 * 12 + 20
-    return n_pixels * (2 * radius + 1) ** 2 * per_neighbour * accepted_fraction
+def trace_traffic_bytes():
+    """HBM bytes per trace launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-kernel calibrated), or None if it is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+def calibrate_copy(nbytes=1 << 30):
+    """A plain device copy of known size, for scaling the FETCH_SIZE and
+    WRITE_SIZE counters to bytes (run under rocprofv3 --pmc)."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": nbytes, "launches": 3}))
 
 
 def cpu_baseline(seconds_hint=20.0):
@@ -78,10 +105,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--size", type=int, default=WORKLOAD["width"])
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this pool has one MI355X; "
-              "multi-GPU sharding of this workload is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): this workload runs on a single "
+              "MI355X; sharding it over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
     w = h = args.size
     ctx = eng.Context(0)
@@ -91,6 +123,7 @@ def main():
         j.wait()
         j.release()
     samples = 0
+    queries = 0
     stage = np.zeros(11)
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -98,6 +131,8 @@ def main():
         j.wait()
         samples += int(j.get("sample_count").sum())
         stage += j.stage_ms()
+        cnt = j.counters()
+        queries += cnt["queries"]
         j.release()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
@@ -105,7 +140,6 @@ def main():
     # denoised MP/s over the StatMC-side stages (sensitivity, StatMC, variance, allocation)
     denoise_ms = (stage[2] + stage[3] + stage[4] + stage[5]) / args.steps
     trace_ms_total = stage[7] / args.steps
-    statmc_kernel_ms = stage[9] / max(1.0, args.steps) if stage[9] > 0 else None
     result = {
         "metric": "Msamples/s & denoised MP/s; p50/p99 job latency vs plain at equal relMSE",
         "value": samples / wall / 1e6,
@@ -124,13 +158,19 @@ def main():
                      "trace_kernel": trace_ms_total, "grid_gather": stage[10] / args.steps},
         "samples_per_job": samples // args.steps,
     }
-    # roofline of the StatMC window kernel: FP64 VALU-bound
-    if statmc_kernel_ms:
-        ops = statmc_fp64_ops(WORKLOAD["color_window_radius"], n_px)
-        achieved = ops / (statmc_kernel_ms / 1e3) / 1e12
-        result["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                              "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                              "kernel": "k_statmc (FP64 VALU; fp64 vector peak)"}
+    # roofline of the dominant kernel, the trace kernel: FP32 VALU-bound.
+    # Algorithmic work = ray queries x (triangles x TRI_TEST_FLOPS + spheres x
+    # SPHERE_TEST_FLOPS), over the HIP-event time of its launches.
+    launches = stage[8]
+    if launches > 0 and stage[7] > 0:
+        flops = queries * (cnt["triangles"] * TRI_TEST_FLOPS + cnt["spheres"] * SPHERE_TEST_FLOPS)
+        achieved = flops / (stage[7] / 1e3) / 1e12
+        result["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": achieved / FP32_PEAK_TFLOPS, "traffic": trace_traffic_bytes(),
+                              "kernel": "k_trace (FP32 VALU, no MFMA; fp32 peak)",
+                              "avg_launch_ms": stage[7] / launches,
+                              "ray_queries_per_job": queries // args.steps,
+                              "ray_queries_per_s": queries / (stage[7] / 1e3)}
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result))

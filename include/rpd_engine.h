@@ -188,6 +188,12 @@ int rpd_job_grid_dims(rpd_job* job, int dims[4]);
    [9] StatMC window kernel only, [10] grid gather kernels only. */
 int rpd_job_stage_ms(rpd_job* job, double* ms, int n);
 
+/* Work counters of the job so far: [0] ray queries traced (camera, extension
+   and shadow rays), [1] path samples traced, [2] scene triangles, [3] scene
+   spheres. Used to express the trace kernel's work in intersection tests.
+   No counterpart in the reference (it reports stage times only). */
+int rpd_job_counters(rpd_job* job, uint64_t* out, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,6 +100,7 @@ struct rpd_job {
     float *out_rad = nullptr, *out_raw = nullptr, *out_alb = nullptr, *out_nrm = nullptr, *out_pos = nullptr,
           *out_dep = nullptr;
     int *counts = nullptr, *offsets = nullptr, *active = nullptr, *totals = nullptr, *work = nullptr;
+    unsigned long long* query_count = nullptr;  // [0] ray queries over all passes
     int64_t* scan_scratch = nullptr;
     uint32_t* tape = nullptr;
     Records rec{};
@@ -315,6 +316,7 @@ TraceParams trace_params(rpd_job* j, int statmc) {
     p.base_seed = j->seed;
     p.tape = j->tape;
     p.work_counter = j->work;
+    p.query_count = j->query_count;
     p.rec = j->rec;
     return p;
 }
@@ -448,6 +450,7 @@ int reset_state(rpd_job* j) {
     HIP_TRY(hipMemsetAsync(j->grid, 0, sizeof(double) * GRID_NODE_DOUBLES * j->geom.nx * j->geom.ny, j->stream));
     HIP_TRY(hipMemsetAsync(j->tiles, 0, sizeof(float) * SF_COUNT * j->geom.nx * j->geom.ny, j->stream));
     HIP_TRY(hipMemsetAsync(j->counts, 0, sizeof(int) * (3 * N + 8), j->stream));  // plan, totals, work counter
+    HIP_TRY(hipMemsetAsync(j->query_count, 0, sizeof(unsigned long long) * 2, j->stream));
     j->features_valid = false;
     j->finalized = false;
     return RPD_OK;
@@ -508,6 +511,7 @@ int job_alloc(rpd_job* j) {
     j->counts = plan; j->offsets = plan + N; j->active = plan + 2 * N; j->totals = plan + 3 * N;
     j->work = plan + 3 * N + 4;
     A(j->scan_scratch, (size_t)plan_scan_scratch_elems(j->N));
+    A(j->query_count, 2);
     A(j->tape, ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_N);
     const int per_px = std::max(j->cfg.samples_per_pixel, j->cfg.adaptive_spp + 1);
     const int64_t cap = int64_t(per_px) * int64_t(N);
@@ -932,6 +936,17 @@ int rpd_job_stage_ms(rpd_job* j, double* ms, int n) {
     for (int k = 0; k < 6; ++k) total += j->stage_ms[k];
     j->stage_ms[6] = total;
     for (int k = 0; k < n && k < 11; ++k) ms[k] = j->stage_ms[k];
+    return RPD_OK;
+}
+
+int rpd_job_counters(rpd_job* j, uint64_t* out, int n) {
+    if (!j || !out) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    if (j->worker.joinable()) j->worker.join();
+    HIP_TRY(hipStreamSynchronize(j->stream));
+    unsigned long long q[2] = {0, 0};
+    HIP_TRY(hipMemcpy(q, j->query_count, sizeof(q), hipMemcpyDeviceToHost));
+    const uint64_t v[4] = {q[0], q[1], uint64_t(j->scene->n_tris), uint64_t(j->scene->n_spheres)};
+    for (int k = 0; k < n && k < 4; ++k) out[k] = v[k];
     return RPD_OK;
 }
 

@@ -95,6 +95,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
         __syncthreads();
     }
     const float4* tp = kLds ? s_tri : S.tri_perm;
+    if (p.query_count && blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(p.query_count + 1, (unsigned long long)p.plan.totals[0]);  // samples of this pass
     const int lane = threadIdx.x & 63;
     const v3 env = mk3(S.env[0], S.env[1], S.env[2]);
     const bool env_on = maxc3(env) > 0.0f;
@@ -119,6 +121,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
     int smat = 0;
     uint32_t flags = 0, light_bits = 255u;
     HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
+    uint32_t n_queries = 0;
 
     while (true) {
         // -- resolve the last query -------------------------------------------
@@ -228,7 +231,14 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
             need = ballot64(st == ST_NEED);
         }
         if (st == ST_NEED) st = ST_DEAD;
-        if (ballot64(st != ST_DEAD) == 0) break;
+        if (ballot64(st != ST_DEAD) == 0) {
+            if (p.query_count) {
+                unsigned long long tot = n_queries;
+                for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+                if (lane == 0) atomicAdd(p.query_count, tot);
+            }
+            break;
+        }
 
         // -- camera rays: main.cpp:691-705 ----------------------------------
         if (st == ST_CAMERA) {
@@ -345,7 +355,10 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
         }
 
         // -- one intersection query per lane ----------------------------------
-        if (q != Q_NONE) h = trace_query(S, tp, ro, rd, qt, q >= Q_SHADOW_L);
+        if (q != Q_NONE) {
+            h = trace_query(S, tp, ro, rd, qt, q >= Q_SHADOW_L);
+            ++n_queries;
+        }
     }
 }
 int trace_block_size() { return TRACE_BLOCK; }

@@ -57,6 +57,7 @@ struct TraceParams {
     uint32_t base_seed;
     uint32_t* tape;          // ceil(n/64)*64 tapes of 624 words, layout of tape_base/tape_word
     int* work_counter;       // zero before launch
+    unsigned long long* query_count;  // ray queries issued (closest + occlusion), accumulated; may be null
     Records rec;
 };
 

@@ -85,6 +85,7 @@ def lib():
             ("rpd_job_buffer_size", [vp, C.c_int, C.POINTER(C.c_size_t)]),
             ("rpd_job_read", [vp, C.c_int, vp, C.c_size_t]), ("rpd_job_write", [vp, C.c_int, vp, C.c_size_t]),
             ("rpd_job_grid_dims", [vp, vp]), ("rpd_job_stage_ms", [vp, vp, C.c_int]),
+            ("rpd_job_counters", [vp, vp, C.c_int]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -223,6 +224,12 @@ class Job:
         out = np.zeros(11, np.float64)
         check(self.L.rpd_job_stage_ms(self.h, _ptr(out), 11))
         return out
+
+    def counters(self):
+        """(ray queries, path samples, triangles, spheres) traced by this job so far."""
+        out = np.zeros(4, np.uint64)
+        check(self.L.rpd_job_counters(self.h, _ptr(out), 4))
+        return dict(queries=int(out[0]), samples=int(out[1]), triangles=int(out[2]), spheres=int(out[3]))
 
     def nbytes(self, name):
         n = C.c_size_t()
