@@ -107,6 +107,9 @@ struct rpd_job {
     double* grid = nullptr;
     float *tiles = nullptr, *node_scratch = nullptr, *filter_scratch = nullptr;
     void* alloc_scratch = nullptr;
+    // this job as an image of a batch (device copies for the batch of one)
+    ImageDesc desc{};
+    Batch batch1{};
     bool features_valid = false;
     bool finalized = false;
     bool last_pass_statmc = false;
@@ -338,10 +341,11 @@ int pass_after_plan(rpd_job* j, int statmc, int stage) {
         StageTimer tk(j, 7);
         launch_trace(p, trace_blocks(j), j->stream);
     }
-    launch_fold(p.plan, j->rec, j->pb, statmc, j->stream);
+    launch_fold(j->batch1, statmc, j->stream);
     if (p.capture) {
         StageTimer tg(j, 10);
-        launch_grid_gather(j->geom, j->W, j->H, p.plan, j->rec, j->grid, j->stream);
+        launch_batch_queues_reset(j->batch1, j->stream);
+        launch_grid_gather(j->batch1, j->stream);
     }
     j->features_valid = false;
     j->last_pass_statmc = statmc != 0;
@@ -352,7 +356,7 @@ int pass_after_plan(rpd_job* j, int statmc, int stage) {
 
 void ensure_features(rpd_job* j) {
     if (j->features_valid) return;
-    launch_features(j->N, j->pb, j->feat, j->stream);
+    launch_features(j->batch1, j->stream);
     j->features_valid = true;
 }
 
@@ -362,8 +366,7 @@ int do_recompute_sensitivity(rpd_job* j) {
         launch_sensitivity_zero(j->geom, j->W, j->H, j->tiles, j->sconf, j->sgrad, j->stream);
     } else {
         ensure_features(j);
-        launch_sensitivity(j->geom, j->W, j->H, j->cfg.rpf_confidence_samples, j->grid, j->feat, j->node_scratch,
-                           j->tiles, j->sconf, j->sgrad, j->stream);
+        launch_sensitivity(j->batch1, j->stream);
     }
     HIP_TRY(hipGetLastError());
     return RPD_OK;
@@ -372,18 +375,11 @@ int do_recompute_sensitivity(rpd_job* j) {
 int do_statmc(rpd_job* j) {
     StageTimer tm(j, 3);
     ensure_features(j);
-    FilterConfig fc;
-    fc.radius = j->cfg.color_window_radius;
-    fc.normal_threshold = j->cfg.color_normal_threshold;
-    fc.depth_threshold = j->cfg.color_depth_threshold;
-    fc.compat_alpha = j->cfg.color_compat_alpha;
-    fc.sigma_max = j->cfg.color_sigma_max;
-    fc.rpd_scale = j->cfg.rpf_shrinkage_scale;
-    launch_statmc_prepare(j->W, j->H, fc, j->pb, j->feat, j->geom, j->tiles, j->sgrad, j->filter_scratch, j->stream);
+    launch_statmc_prepare(j->batch1, j->stream);
     {
         StageTimer tw(j, 9);
-        launch_statmc_window(j->W, j->H, fc, j->pb, j->filter_scratch, j->den, j->unc, j->alp, j->vtot, j->veff,
-                             j->vrat, j->stream);
+        launch_batch_queues_reset(j->batch1, j->stream);
+        launch_statmc_window(j->batch1, j->stream);
     }
     HIP_TRY(hipGetLastError());
     return RPD_OK;
@@ -484,6 +480,52 @@ int run_pipeline(rpd_job* j) {
     return do_finalize(j, true);
 }
 
+// The job's image descriptor (everything a denoise-stage kernel reads) and
+// the batch of one that the job's own stages launch with.
+ImageDesc make_desc(const rpd_job* j) {
+    ImageDesc d{};
+    d.W = j->W; d.H = j->H; d.N = j->N;
+    d.g = j->geom;
+    d.plan.counts = j->counts; d.plan.offsets = j->offsets; d.plan.active = j->active;
+    d.plan.totals = j->totals; d.plan.n_pixels = j->N;
+    d.rec = j->rec;
+    d.pb = j->pb;
+    d.feat = j->feat;
+    d.grid = j->grid;
+    d.node_scratch = j->node_scratch;
+    d.tiles = j->tiles;
+    d.conf = j->sconf;
+    d.grad = j->sgrad;
+    d.filter_scratch = j->filter_scratch;
+    d.den = j->den; d.unc = j->unc; d.alp = j->alp; d.vtot = j->vtot; d.veff = j->veff; d.vrat = j->vrat;
+    d.fc.radius = j->cfg.color_window_radius;
+    d.fc.normal_threshold = j->cfg.color_normal_threshold;
+    d.fc.depth_threshold = j->cfg.color_depth_threshold;
+    d.fc.compat_alpha = j->cfg.color_compat_alpha;
+    d.fc.sigma_max = j->cfg.color_sigma_max;
+    d.fc.rpd_scale = j->cfg.rpf_shrinkage_scale;
+    d.conf_n0 = j->cfg.rpf_confidence_samples;
+    return d;
+}
+
+int build_batch1(rpd_job* j, ImageDesc* dd, int64_t* dp, unsigned int* dq) {
+    j->desc = make_desc(j);
+    int64_t pf[BK_KINDS * 2];
+    for (int k = 0; k < BK_KINDS; ++k) {
+        pf[2 * k] = 0;
+        pf[2 * k + 1] = image_items(k, j->W, j->H, j->geom);
+        j->batch1.total[k] = pf[2 * k + 1];
+    }
+    HIP_TRY(hipMemcpy(dd, &j->desc, sizeof(ImageDesc), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dp, pf, sizeof(pf), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(dq, 0, sizeof(unsigned int) * BK_KINDS * BATCH_QUEUES));
+    j->batch1.img = dd;
+    j->batch1.prefix = dp;
+    j->batch1.queues = dq;
+    j->batch1.n = 1;
+    return RPD_OK;
+}
+
 int job_alloc(rpd_job* j) {
     const size_t N = j->N;
     const size_t nn = size_t(j->geom.nx) * j->geom.ny;
@@ -524,8 +566,11 @@ int job_alloc(rpd_job* j) {
     A(j->filter_scratch, 32 * N);
     char* as; A(as, alloc_scratch_bytes(j->N));
     j->alloc_scratch = as;
+    ImageDesc* dd; A(dd, 1);
+    int64_t* dp; A(dp, BK_KINDS * 2);
+    unsigned int* dq; A(dq, BK_KINDS * BATCH_QUEUES);
 #undef A
-    return RPD_OK;
+    return build_batch1(j, dd, dp, dq);
 }
 
 size_t job_bytes(const rpd_render_config& c) {

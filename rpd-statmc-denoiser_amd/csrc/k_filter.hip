@@ -6,6 +6,8 @@
 // main.cpp:1166-1268 (variance_denoise).
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace rpd {
 
 __device__ __forceinline__ float tile_lookup_f(const float* t, GridGeom g, int x, int y) {
@@ -50,9 +52,14 @@ __device__ __forceinline__ float t_critical(float df, int table, float z) {
 constexpr int FILTER_PLANES = 7;
 
 // sqrt-domain variance of the mean, floored (main.cpp:977-980)
-__global__ void k_sqrt_var_mean(int n, PixelBuffers pb, float* vm) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void k_sqrt_var_mean(Batch b) {
+    const int64_t item = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (item >= b.total[BK_PIXEL]) return;
+    int64_t li;
+    const ImageDesc& d = b.img[batch_find(b, BK_PIXEL, item, li)];
+    const int i = int(li), n = d.N;
+    const PixelBuffers& pb = d.pb;
+    float* vm = d.filter_scratch;
     const int cnt = pb.n[i];
     for (int c = 0; c < 3; ++c) {
         const float sv = cnt <= 1 ? 0.0f : pb.sqrt_m2[c * n + i] / float(cnt - 1);
@@ -60,11 +67,20 @@ __global__ void k_sqrt_var_mean(int n, PixelBuffers pb, float* vm) {
     }
 }
 
-__global__ void k_filter_pack(int W, int H, FilterConfig fc, PixelBuffers pb, Features f, GridGeom g,
-                              const float* tiles, const float* grad, const float* vm, float4* P) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = W * H;
-    if (i >= n) return;
+__global__ void k_filter_pack(Batch b) {
+    const int64_t item = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (item >= b.total[BK_PIXEL]) return;
+    int64_t li;
+    const ImageDesc& d = b.img[batch_find(b, BK_PIXEL, item, li)];
+    const int W = d.W, H = d.H, n = d.N, i = int(li);
+    const FilterConfig fc = d.fc;
+    const PixelBuffers& pb = d.pb;
+    const Features& f = d.feat;
+    const GridGeom g = d.g;
+    const float* tiles = d.tiles;
+    const float* grad = d.grad;
+    const float* vm = d.filter_scratch;
+    float4* P = reinterpret_cast<float4*>(d.filter_scratch + 4 * size_t(n));
     const int x = i % W, y = i / W;
     // main.cpp:984-990: max of the value and its 3x3 weighted local average
     double a0 = 0, a1 = 0, a2 = 0, ws = 0;
@@ -110,14 +126,12 @@ __global__ void k_filter_pack(int W, int H, FilterConfig fc, PixelBuffers pb, Fe
 // neighbour exactly as the reference: Welch t^2 per channel against the
 // t critical value at Welch-Satterthwaite df, the maximum over channels,
 // RPD relaxation, hard rejection above 25, weight exp(-q/2)/(1+d^2).
-__global__ void __launch_bounds__(256) k_statmc(int W, int H, FilterConfig fc, const float4* __restrict__ P,
-                                               PixelBuffers pb, float* den, float* unc, float* alp, float* vtot,
-                                               float* veff, float* vrat) {
-    const int n = W * H;
-    // 16x16 pixel tiles per workgroup
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int x = blockIdx.x * 16 + tx, y = blockIdx.y * 16 + ty;
-    if (x >= W || y >= H) return;
+__device__ void statmc_pixel(const ImageDesc& d, const int x, const int y) {
+    const int W = d.W, H = d.H, n = d.N;
+    const FilterConfig fc = d.fc;
+    const float4* __restrict__ P = reinterpret_cast<const float4*>(d.filter_scratch + 4 * size_t(n));
+    const PixelBuffers& pb = d.pb;
+    float *den = d.den, *unc = d.unc, *alp = d.alp, *vtot = d.vtot, *veff = d.veff, *vrat = d.vrat;
     const int i = y * W + x;
     const float4 p0 = P[i];
     if (p0.w < 0.0f) {  // no primary hit: keep the mean
@@ -223,19 +237,43 @@ __global__ void __launch_bounds__(256) k_statmc(int W, int H, FilterConfig fc, c
     alp[i] = a;
 }
 
-void launch_statmc_prepare(int W, int H, const FilterConfig& fc, const PixelBuffers& pb, const Features& f,
-                           GridGeom g, const float* tiles, const float* grad, float* scratch, hipStream_t st) {
-    const int n = W * H;
-    float* vm = scratch;                                           // 3 planes
-    float4* P = reinterpret_cast<float4*>(scratch + 4 * (size_t)n);  // 7 float4 planes (16-byte aligned)
-    hipLaunchKernelGGL(k_sqrt_var_mean, dim3((n + 255) / 256), dim3(256), 0, st, n, pb, vm);
-    hipLaunchKernelGGL(k_filter_pack, dim3((n + 255) / 256), dim3(256), 0, st, W, H, fc, pb, f, g, tiles, grad, vm, P);
+// Persistent StatMC: blocks take 16x16 tiles of any image of the batch from
+// the XCD-partitioned work ranges; one thread per pixel of the tile.
+__global__ void __launch_bounds__(256) k_statmc(Batch b) {
+    __shared__ int64_t s_item;
+    int cursor = 0;
+    const int tx = threadIdx.x & (STATMC_TILE - 1), ty = threadIdx.x / STATMC_TILE;
+    while (true) {
+        if (threadIdx.x == 0) s_item = batch_take(b, BK_TILE, cursor);
+        __syncthreads();
+        const int64_t item = s_item;
+        __syncthreads();
+        if (item < 0) break;
+        int64_t lt;
+        const ImageDesc& d = b.img[batch_find(b, BK_TILE, item, lt)];
+        const int tiles_x = (d.W + STATMC_TILE - 1) / STATMC_TILE;
+        const int x = int(lt % tiles_x) * STATMC_TILE + tx, y = int(lt / tiles_x) * STATMC_TILE + ty;
+        if (x < d.W && y < d.H) statmc_pixel(d, x, y);
+    }
 }
-void launch_statmc_window(int W, int H, const FilterConfig& fc, const PixelBuffers& pb, const float* scratch,
-                          float* den, float* unc, float* alp, float* vtot, float* veff, float* vrat, hipStream_t st) {
-    const float4* P = reinterpret_cast<const float4*>(scratch + 4 * (size_t)W * H);
-    hipLaunchKernelGGL(k_statmc, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, st, W, H, fc, P, pb, den, unc,
-                       alp, vtot, veff, vrat);
+
+void launch_statmc_prepare(const Batch& b, hipStream_t st) {
+    const int64_t np = b.total[BK_PIXEL];
+    hipLaunchKernelGGL(k_sqrt_var_mean, dim3((np + 255) / 256), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_filter_pack, dim3((np + 255) / 256), dim3(256), 0, st, b);
+}
+void launch_statmc_window(const Batch& b, hipStream_t st) {
+    static int per_cu = 0, cus = 0;
+    if (per_cu <= 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_statmc), 256, 0) !=
+                hipSuccess || per_cu <= 0) per_cu = 1;
+        int dev = 0;
+        hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
+    }
+    const int64_t blocks = std::min<int64_t>(b.total[BK_TILE], int64_t(per_cu) * cus);
+    if (blocks <= 0) return;
+    hipLaunchKernelGGL(k_statmc, dim3(unsigned(blocks)), dim3(256), 0, st, b);
 }
 
 // ---------------------------------------------------------------------------

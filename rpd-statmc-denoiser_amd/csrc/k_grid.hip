@@ -6,30 +6,45 @@
 // node -> pixel expansion), main.cpp:94-122 (feature averages).
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace rpd {
 
 // ---------------------------------------------------------------------------
 // Per-pixel feature averages: main.cpp:106-118
-__global__ void k_features(int n, PixelBuffers pb, Features f) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void k_features(Batch b) {
+    const int64_t item = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (item >= b.total[BK_PIXEL]) return;
+    int64_t li;
+    const ImageDesc& d = b.img[batch_find(b, BK_PIXEL, item, li)];
+    const int i = int(li), n = d.N;
+    const PixelBuffers& pb = d.pb;
+    const Features& f = d.feat;
     const int hc = pb.hit_count[i];
     v3 nr = mk3(pb.normal[i], pb.normal[n + i], pb.normal[2 * n + i]);
     v3 al = mk3(pb.albedo[i], pb.albedo[n + i], pb.albedo[2 * n + i]);
-    float d = pb.depth[i];
+    float dd = pb.depth[i];
     if (hc > 0) {
         const float h = float(hc);
         nr = nr / h;
         if (sqnorm3(nr) > RPD_EPS_SMALL) nr = normalized3(nr);
-        d = d / h;
+        dd = dd / h;
         al = al / h;
     }
     f.normal[i] = nr.x; f.normal[n + i] = nr.y; f.normal[2 * n + i] = nr.z;
-    f.depth[i] = d;
+    f.depth[i] = dd;
     f.albedo[i] = al.x; f.albedo[n + i] = al.y; f.albedo[2 * n + i] = al.z;
 }
-void launch_features(int n, const PixelBuffers& pb, Features f, hipStream_t st) {
-    hipLaunchKernelGGL(k_features, dim3((n + 255) / 256), dim3(256), 0, st, n, pb, f);
+void launch_features(const Batch& b, hipStream_t st) {
+    hipLaunchKernelGGL(k_features, dim3((b.total[BK_PIXEL] + 255) / 256), dim3(256), 0, st, b);
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_queues_zero(unsigned int* q) {
+    if (threadIdx.x < BK_KINDS * BATCH_QUEUES) q[threadIdx.x] = 0u;
+}
+void launch_batch_queues_reset(const Batch& b, hipStream_t st) {
+    hipLaunchKernelGGL(k_queues_zero, dim3(1), dim3(64), 0, st, b.queues);
 }
 
 // ---------------------------------------------------------------------------
@@ -68,20 +83,34 @@ __device__ __forceinline__ int gather_scan(int v, int* sh) {
     return r;
 }
 
-__global__ void __launch_bounds__(GATHER_BLOCK) k_grid_gather(GridGeom g, int W, int H, PassPlan plan, Records rec,
-                                                             double* grid) {
-    __shared__ int s_pix[GATHER_MAX_PIX];
-    __shared__ float s_w[GATHER_MAX_PIX];
-    __shared__ int s_pre[GATHER_MAX_PIX + 1];
-    __shared__ int s_scan[GATHER_BLOCK];
-    __shared__ float4 c_w4[GATHER_BLOCK / 4];
-    __shared__ float4 c_y4[GATHER_BLOCK / 4];
-    __shared__ uint32_t c_bin4[GRID_ACC][GATHER_BLOCK / 4];
-    float* c_w = reinterpret_cast<float*>(c_w4);
-    float* c_y = reinterpret_cast<float*>(c_y4);
-    signed char* c_bin = reinterpret_cast<signed char*>(c_bin4);
+struct GatherLds {
+    int pix[GATHER_MAX_PIX];
+    float w[GATHER_MAX_PIX];
+    int pre[GATHER_MAX_PIX + 1];
+    int scan[GATHER_BLOCK];
+    float4 w4[GATHER_BLOCK / 4];
+    float4 y4[GATHER_BLOCK / 4];
+    uint32_t bin4[GRID_ACC][GATHER_BLOCK / 4];
+    int64_t item;
+};
 
-    const int node = blockIdx.x;
+// One node of one image; called by all threads of the block.
+__device__ void gather_node(const ImageDesc& d, const int node, GatherLds& L) {
+    const GridGeom g = d.g;
+    const int W = d.W, H = d.H;
+    const PassPlan& plan = d.plan;
+    const Records& rec = d.rec;
+    double* grid = d.grid;
+    int* s_pix = L.pix;
+    float* s_w = L.w;
+    int* s_pre = L.pre;
+    int* s_scan = L.scan;
+    float4* c_w4 = L.w4;
+    float4* c_y4 = L.y4;
+    float* c_w = reinterpret_cast<float*>(L.w4);
+    float* c_y = reinterpret_cast<float*>(L.y4);
+    signed char* c_bin = reinterpret_cast<signed char*>(L.bin4);
+
     const int gx = node % g.nx, gy = node / g.nx;
     const int tid = threadIdx.x;
     const int x_lo = max(0, (gx - 1) * g.step), x_hi = min(W - 1, (gx + 1) * g.step - 1);
@@ -179,7 +208,7 @@ __global__ void __launch_bounds__(GATHER_BLOCK) k_grid_gather(GridGeom g, int W,
         const int nquad = (min(GATHER_BLOCK, total - base) + 3) >> 2;
         if (active_t) {
             for (int i4 = 0; i4 < nquad; ++i4) {
-                const uint32_t b4 = c_bin4[acc][i4];
+                const uint32_t b4 = L.bin4[acc][i4];
                 const float4 w4 = c_w4[i4], y4 = c_y4[i4];
                 const float wv[4] = {w4.x, w4.y, w4.z, w4.w}, yv[4] = {y4.x, y4.y, y4.z, y4.w};
 #pragma unroll
@@ -208,9 +237,33 @@ __global__ void __launch_bounds__(GATHER_BLOCK) k_grid_gather(GridGeom g, int W,
     }
 }
 
-void launch_grid_gather(GridGeom g, int W, int H, const PassPlan& plan, const Records& rec, double* grid,
-                        hipStream_t st) {
-    hipLaunchKernelGGL(k_grid_gather, dim3(g.nx * g.ny), dim3(GATHER_BLOCK), 0, st, g, W, H, plan, rec, grid);
+__global__ void __launch_bounds__(GATHER_BLOCK) k_grid_gather(Batch b) {
+    __shared__ GatherLds L;
+    int cursor = 0;
+    while (true) {
+        if (threadIdx.x == 0) L.item = batch_take(b, BK_NODE, cursor);
+        __syncthreads();
+        const int64_t item = L.item;
+        __syncthreads();
+        if (item < 0) break;
+        int64_t node;
+        const ImageDesc& d = b.img[batch_find(b, BK_NODE, item, node)];
+        gather_node(d, int(node), L);
+    }
+}
+
+void launch_grid_gather(const Batch& b, hipStream_t st) {
+    static int per_cu = 0, cus = 0;
+    if (per_cu <= 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_grid_gather),
+                                                         GATHER_BLOCK, 0) != hipSuccess || per_cu <= 0) per_cu = 1;
+        int dev = 0;
+        hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
+    }
+    const int64_t blocks = std::min<int64_t>(b.total[BK_NODE], int64_t(per_cu) * cus);
+    if (blocks <= 0) return;
+    hipLaunchKernelGGL(k_grid_gather, dim3(unsigned(blocks)), dim3(GATHER_BLOCK), 0, st, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -276,10 +329,19 @@ __device__ inline v3 local_avg3(const float* v, int n, int w, int h, int cx, int
 constexpr int NODE_PLANES = 33;
 
 // main.cpp:773-815
-__global__ void k_node_split(GridGeom g, int W, int H, float n0, const double* grid, Features f, float* ns) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_node_split(Batch b) {
+    const int64_t item = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (item >= b.total[BK_NODE]) return;
+    int64_t li;
+    const ImageDesc& dsc = b.img[batch_find(b, BK_NODE, item, li)];
+    const GridGeom g = dsc.g;
+    const int W = dsc.W, H = dsc.H;
+    const float n0 = dsc.conf_n0;
+    const double* grid = dsc.grid;
+    const Features& f = dsc.feat;
+    float* ns = dsc.node_scratch;
+    const int idx = int(li);
     const int nn = g.nx * g.ny;
-    if (idx >= nn) return;
     const double* nd = grid + (size_t)idx * GRID_NODE_DOUBLES;
     const double n0d = double(n0);
     for (int k = 0; k < GRID_ACC; ++k) {
@@ -303,10 +365,16 @@ __global__ void k_node_split(GridGeom g, int W, int H, float n0, const double* g
 
 // main.cpp:817-928: confidence-weighted 3x3 prior per field, shrinkage toward
 // it, then reliable random components relative to their screen comparators.
-__global__ void k_prior(GridGeom g, const float* ns, float* tiles) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_prior(Batch b) {
+    const int64_t item = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (item >= b.total[BK_NODE]) return;
+    int64_t li;
+    const ImageDesc& dsc = b.img[batch_find(b, BK_NODE, item, li)];
+    const GridGeom g = dsc.g;
+    const float* ns = dsc.node_scratch;
+    float* tiles = dsc.tiles;
+    const int idx = int(li);
     const int nn = g.nx * g.ny;
-    if (idx >= nn) return;
     const int gx = idx % g.nx, gy = idx / g.nx;
     const v3 ni = mk3(ns[26 * nn + idx], ns[27 * nn + idx], ns[28 * nn + idx]);
     const float di = ns[29 * nn + idx];
@@ -381,9 +449,17 @@ __device__ __forceinline__ float tile_lookup(const float* t, GridGeom g, int x, 
 }
 
 // main.cpp:930-947
-__global__ void k_gradient(GridGeom g, int W, int H, const float* tiles, float* conf, float* grad) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= W * H) return;
+__global__ void k_gradient(Batch b) {
+    const int64_t item = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (item >= b.total[BK_PIXEL]) return;
+    int64_t li;
+    const ImageDesc& dsc = b.img[batch_find(b, BK_PIXEL, item, li)];
+    const GridGeom g = dsc.g;
+    const int W = dsc.W, H = dsc.H;
+    const float* tiles = dsc.tiles;
+    float* conf = dsc.conf;
+    float* grad = dsc.grad;
+    const int i = int(li);
     const int x = i % W, y = i / W;
     const float* all = tiles + SF_ALL * g.nx * g.ny;
     const float l = tile_lookup(all, g, max(0, x - 1), y), r = tile_lookup(all, g, min(W - 1, x + 1), y);
@@ -398,12 +474,11 @@ __global__ void k_zero_f32(float* p, int64_t n) {
     if (i < n) p[i] = 0.0f;
 }
 
-void launch_sensitivity(GridGeom g, int W, int H, float n0, const double* grid, const Features& f, float* ns,
-                        float* tiles, float* conf, float* grad, hipStream_t st) {
-    const int nn = g.nx * g.ny;
-    hipLaunchKernelGGL(k_node_split, dim3((nn + 127) / 128), dim3(128), 0, st, g, W, H, n0, grid, f, ns);
-    hipLaunchKernelGGL(k_prior, dim3((nn + 127) / 128), dim3(128), 0, st, g, ns, tiles);
-    hipLaunchKernelGGL(k_gradient, dim3((W * H + 255) / 256), dim3(256), 0, st, g, W, H, tiles, conf, grad);
+void launch_sensitivity(const Batch& b, hipStream_t st) {
+    const int64_t nn = b.total[BK_NODE], np = b.total[BK_PIXEL];
+    hipLaunchKernelGGL(k_node_split, dim3((nn + 127) / 128), dim3(128), 0, st, b);
+    hipLaunchKernelGGL(k_prior, dim3((nn + 127) / 128), dim3(128), 0, st, b);
+    hipLaunchKernelGGL(k_gradient, dim3((np + 255) / 256), dim3(256), 0, st, b);
 }
 void launch_sensitivity_zero(GridGeom g, int W, int H, float* tiles, float* conf, float* grad, hipStream_t st) {
     const int64_t nt = int64_t(SF_COUNT) * g.nx * g.ny, np = int64_t(W) * H;

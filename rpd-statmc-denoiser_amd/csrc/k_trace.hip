@@ -479,8 +479,15 @@ void launch_plan_scan(int n, const int* counts, int* offsets, int* active, int* 
 // Fold a pass's samples into the per-pixel accumulators in sample order
 // (main.cpp:572-576, 707-717; core/stats.h:25-41). Features are summed for every
 // sample with a primary hit, statistics and radiance only for finite samples.
-__global__ void __launch_bounds__(256) k_fold(PassPlan plan, Records rec, PixelBuffers pb, int statmc) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_fold(Batch b, int statmc) {
+    const int64_t item = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (item >= b.total[BK_PIXEL]) return;
+    int64_t la;
+    const ImageDesc& d = b.img[batch_find(b, BK_PIXEL, item, la)];
+    const PassPlan& plan = d.plan;
+    const Records& rec = d.rec;
+    const PixelBuffers& pb = d.pb;
+    const int a = int(la);
     if (a >= plan.totals[1]) return;
     const int i = plan.active[a];
     const int N = plan.n_pixels;
@@ -545,8 +552,8 @@ __global__ void __launch_bounds__(256) k_fold(PassPlan plan, Records rec, PixelB
     }
 }
 
-void launch_fold(const PassPlan& plan, const Records& rec, PixelBuffers pb, int statmc, hipStream_t st) {
-    hipLaunchKernelGGL(k_fold, dim3((plan.n_pixels + 255) / 256), dim3(256), 0, st, plan, rec, pb, statmc);
+void launch_fold(const Batch& b, int statmc, hipStream_t st) {
+    hipLaunchKernelGGL(k_fold, dim3((b.total[BK_PIXEL] + 255) / 256), dim3(256), 0, st, b, statmc);
 }
 
 // ---------------------------------------------------------------------------

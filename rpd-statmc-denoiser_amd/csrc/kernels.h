@@ -113,12 +113,80 @@ struct VarianceConfig {
     float normal_threshold, depth_threshold, compat_sigma, shrinkage_k, sigma_max;
 };
 
+// ---- batches of images ------------------------------------------------------
+// The denoising stages (per-pixel fold, features, grid gather, sensitivity,
+// StatMC) run over a batch of images in one launch per stage; a single image
+// is a batch of one. Work items of all images are laid out back to back: for
+// each item kind an exclusive prefix over the images maps a global item to
+// (image, local item).
+struct ImageDesc {
+    int W, H, N;
+    GridGeom g;
+    PassPlan plan;
+    Records rec;
+    PixelBuffers pb;
+    Features feat;
+    double* grid;            // GRID_NODE_DOUBLES per node
+    float* node_scratch;     // 33 planes over nodes
+    float* tiles;            // SF_COUNT planes over nodes
+    float* conf;             // per-pixel sensitivity confidence
+    float* grad;             // per-pixel sensitivity gradient
+    float* filter_scratch;   // 32 floats per pixel: 3 planes of variance of the mean + 7 float4 planes
+    float *den, *unc, *alp, *vtot, *veff, *vrat;
+    FilterConfig fc;
+    float conf_n0;
+};
+enum : int { BK_PIXEL = 0, BK_NODE = 1, BK_TILE = 2, BK_KINDS = 3 };
+constexpr int STATMC_TILE = 16;     // StatMC tiles are 16x16 pixels
+constexpr int BATCH_QUEUES = 8;     // one work range per XCD for persistent kernels
+struct Batch {
+    const ImageDesc* img;            // device array, n entries
+    const int64_t* prefix;           // device: BK_KINDS rows of n + 1 exclusive prefix sums
+    unsigned int* queues;            // device: BK_KINDS * BATCH_QUEUES work counters
+    int n;
+    int64_t total[BK_KINDS];         // host copies of the last prefix entries (launch sizing)
+};
+__host__ __device__ inline int64_t image_items(int kind, int W, int H, GridGeom g) {
+    return kind == BK_PIXEL ? int64_t(W) * H
+         : kind == BK_NODE  ? int64_t(g.nx) * g.ny
+                            : int64_t((W + STATMC_TILE - 1) / STATMC_TILE) * ((H + STATMC_TILE - 1) / STATMC_TILE);
+}
+// image of global item `item` of `kind` and its local index
+__device__ __forceinline__ int batch_find(const Batch& b, int kind, int64_t item, int64_t& local) {
+    const int64_t* pf = b.prefix + kind * (b.n + 1);
+    int lo = 0, hi = b.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pf[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    local = item - pf[lo];
+    return lo;
+}
+
+// Persistent work distribution. The items of one kind are split into
+// BATCH_QUEUES contiguous ranges; a block starts on range blockIdx.x % 8 (the
+// XCD the dispatcher places it on, so neighbouring items share that XCD's L2)
+// and moves to the following ranges when its own is used up. Called by one
+// thread; `cursor` carries the range the block last took from.
+__device__ inline int64_t batch_take(const Batch& b, int kind, int& cursor) {
+    const int64_t total = b.total[kind];
+    for (; cursor < BATCH_QUEUES; ++cursor) {
+        const int q = (int(blockIdx.x) + cursor) % BATCH_QUEUES;
+        const int64_t lo = total * q / BATCH_QUEUES, hi = total * (q + 1) / BATCH_QUEUES;
+        if (lo >= hi) continue;
+        const int64_t v = atomicAdd(b.queues + kind * BATCH_QUEUES + q, 1u);
+        if (lo + v < hi) return lo + v;
+    }
+    return -1;
+}
+
 // ---- launch wrappers (each enqueues on `st`) ----
 void launch_tape(const TraceParams& p, hipStream_t st);
 void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st);
 int trace_block_size();
 int trace_blocks_per_cu(int n_tris);  // resident trace blocks per CU for a scene of n_tris triangles
-void launch_fold(const PassPlan& plan, const Records& rec, PixelBuffers pb, int statmc, hipStream_t st);
+// per-pixel fold of the current pass's records of every image (Welford statistics when statmc)
+void launch_fold(const Batch& b, int statmc, hipStream_t st);
 void launch_plan_uniform(int n_pixels, int count, int* counts, int* offsets, int* active, int* totals,
                          hipStream_t st);
 // exclusive scan of counts -> offsets, compaction -> active; writes totals[0] = samples, totals[1] = active.
@@ -128,20 +196,17 @@ void launch_plan_scan(int n_pixels, const int* counts, int* offsets, int* active
 void launch_fill_i32(int* p, int value, int64_t n, hipStream_t st);
 int64_t plan_scan_scratch_elems(int n_pixels);
 
-void launch_features(int n, const PixelBuffers& pb, Features f, hipStream_t st);
-void launch_grid_gather(GridGeom g, int width, int height, const PassPlan& plan, const Records& rec,
-                        double* grid, hipStream_t st);
-void launch_sensitivity(GridGeom g, int width, int height, float conf_n0, const double* grid, const Features& f,
-                        float* node_scratch, float* tiles, float* confidence, float* gradient, hipStream_t st);
+void launch_features(const Batch& b, hipStream_t st);
+void launch_grid_gather(const Batch& b, hipStream_t st);
+void launch_sensitivity(const Batch& b, hipStream_t st);
 void launch_sensitivity_zero(GridGeom g, int width, int height, float* tiles, float* confidence, float* gradient,
                              hipStream_t st);
-// StatMC in two steps: per-pixel packing of the filter inputs into `scratch`,
-// then the window kernel reading them.
-void launch_statmc_prepare(int width, int height, const FilterConfig& fc, const PixelBuffers& pb, const Features& f,
-                           GridGeom g, const float* tiles, const float* gradient, float* scratch, hipStream_t st);
-void launch_statmc_window(int width, int height, const FilterConfig& fc, const PixelBuffers& pb,
-                          const float* scratch, float* denoised, float* uncertainty, float* alpha, float* var_total,
-                          float* var_eff, float* var_ratio, hipStream_t st);
+// StatMC in two steps: per-pixel packing of the filter inputs into the
+// images' filter scratch, then the window kernel reading them.
+void launch_statmc_prepare(const Batch& b, hipStream_t st);
+void launch_statmc_window(const Batch& b, hipStream_t st);
+// zero the persistent-kernel work counters of a batch
+void launch_batch_queues_reset(const Batch& b, hipStream_t st);
 void launch_variance(int width, int height, const VarianceConfig& vc, const PixelBuffers& pb, const Features& f,
                      float* tmp, float* out, hipStream_t st);
 struct AllocConfig {
