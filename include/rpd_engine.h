@@ -188,6 +188,75 @@ int rpd_job_grid_dims(rpd_job* job, int dims[4]);
    [9] StatMC window kernel only, [10] grid gather kernels only. */
 int rpd_job_stage_ms(rpd_job* job, double* ms, int n);
 
+/* ---- denoising caller-supplied or synthetic sample records ----------------
+   The reference folds each traced sample into the pixel statistics, features
+   and RPF grid inside its sample loop (main.cpp:707-717, 329-376) and then
+   denoises (recompute_sensitivity, main.cpp:741; statmc_denoise, 957). These
+   entry points run the same fold and denoise on per-sample records that come
+   from the caller (or from the device-side synthetic generator) instead of
+   from the path tracer. */
+enum rpd_sample_flags {
+    RPD_SAMPLE_FINITE = 1u << 0,        /* set by the engine from the radiance */
+    RPD_SAMPLE_HIT = 1u << 1,           /* primary hit: normal, depth, albedo, position valid */
+    RPD_SAMPLE_PIXEL = 1u << 2,         /* rpf::Sample validity bits (core/stats.h:331-399) */
+    RPD_SAMPLE_LENS = 1u << 3,
+    RPD_SAMPLE_LIGHT_SELECT = 1u << 4,
+    RPD_SAMPLE_BRDF = 1u << 5,
+    RPD_SAMPLE_LIGHT = 1u << 6,
+    RPD_SAMPLE_ENV = 1u << 7,
+    RPD_SAMPLE_RR = 1u << 8,
+    RPD_SAMPLE_RR_SURVIVED = 1u << 9
+};
+/* One path sample: radiance, primary-hit features (main.cpp:572-576) and the
+   first-occurrence random parameters of rpf::Sample. */
+typedef struct rpd_sample_record {
+    float radiance[3];
+    uint32_t flags;       /* rpd_sample_flags */
+    float normal[3];
+    float depth;          /* primary hit distance */
+    float albedo[3];
+    float position[3];
+    float pixel_u[2];
+    float lens_u[2];
+    float brdf_u[2];
+    float light_u[2];
+    float env_u[2];
+    int32_t light_index;  /* light selected first; bins clamp it to [0, 15] */
+} rpd_sample_record;
+
+/* Synthetic records: square regions of region_size pixels, each a plane with
+   its own unit normal, depth uniform in [1, 20] and albedo uniform in
+   [0.1, 0.9]; lognormal radiance exp(log_mean + log_sigma * z) tinted by the
+   albedo, a firefly_fraction of samples scaled by firefly_scale; ten random
+   dimensions uniform in [0, 1], three of them (lens x, brdf x, light x) tied to
+   z through a Gaussian copula with the given correlation. */
+typedef struct rpd_synth_config {
+    uint32_t seed;
+    int samples_per_pixel;
+    int region_size;
+    float log_mean, log_sigma, firefly_fraction, firefly_scale, correlation;
+} rpd_synth_config;
+void rpd_synth_config_defaults(rpd_synth_config* c); /* mu -1, sigma 1.5, 2% fireflies x100, rho 0.6, 32 px */
+
+/* A job for denoising records only: no scene, no camera, no sampling tape;
+   room for max_samples_per_pixel samples of every pixel per pass. */
+int rpd_job_create_denoiser(rpd_context* ctx, const rpd_render_config* cfg, int max_samples_per_pixel,
+                            rpd_job** out);
+/* Loads one pass of records: counts[p] samples of pixel p (row-major), the
+   records in pixel-major order, n = sum of counts. */
+int rpd_job_load_records(rpd_job* job, const int* counts, const rpd_sample_record* records, size_t n);
+/* Writes one pass of synthetic records on the device (uniform samples per pixel). */
+int rpd_job_synthesize_records(rpd_job* job, const rpd_synth_config* sc);
+/* Folds the loaded pass into the pixel statistics, features and RPF grid,
+   then recompute_sensitivity and statmc_denoise; RPD_BUF_DENOISED then holds
+   the result. */
+int rpd_job_denoise_records(rpd_job* job);
+/* The same as rpd_job_denoise_records for n jobs at once: each stage is one
+   launch over all images (persistent kernels for the grid gather and the
+   StatMC window). Results are identical to denoising each job alone.
+   gpu_ms (optional) receives the device time of the whole batch. */
+int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu_ms);
+
 /* Work counters of the job so far: [0] ray queries traced (camera, extension
    and shadow rays), [1] path samples traced, [2] scene triangles, [3] scene
    spheres. Used to express the trace kernel's work in intersection tests.

@@ -61,6 +61,29 @@ void orc_render(void* p) { render(*static_cast<Job*>(p)); }
 // one sampling pass (counts per pixel); statmc=1 seeds with the running finite count
 void orc_pass(void* p, const int* counts, int statmc) { render_pass(*static_cast<Job*>(p), counts, statmc != 0, statmc != 0); }
 void orc_recompute_sensitivity(void* p) { recompute_sensitivity(*static_cast<Job*>(p)); }
+// Records in the 32-float layout of orc_get("records"); word 28 holds the
+// validity bits (1 pixel, 2 lens, 4 light select, 8 brdf, 16 light, 32 env, 64 rr).
+void orc_accumulate_records(void* p, const int* counts, const float* recs, long n) {
+    Job* j = static_cast<Job*>(p);
+    std::vector<SampleRecord> v(static_cast<size_t>(n));
+    for (long i = 0; i < n; ++i) {
+        const float* o = recs + i * 32;
+        SampleRecord& r = v[static_cast<size_t>(i)];
+        r.pixel = int(o[0]);
+        r.radiance = {o[1], o[2], o[3]};
+        r.finite = int(o[4]); r.hit = int(o[5]);
+        r.normal = {o[6], o[7], o[8]}; r.albedo = {o[9], o[10], o[11]}; r.position = {o[12], o[13], o[14]};
+        r.depth = o[15];
+        RpfSample& s = r.rpf;
+        s.pixel_u = {o[16], o[17]}; s.lens_u = {o[18], o[19]}; s.brdf_u = {o[20], o[21]};
+        s.light_u = {o[22], o[23]}; s.env_u = {o[24], o[25]};
+        s.light_index = int(o[26]); s.rr_survived = o[27] != 0.0f;
+        const int f = int(o[28]);
+        s.pixel_valid = f & 1; s.lens_valid = f & 2; s.light_select_valid = f & 4; s.brdf_valid = f & 8;
+        s.light_valid = f & 16; s.env_valid = f & 32; s.rr_valid = f & 64;
+    }
+    accumulate_records(*j, counts, v.data());
+}
 void orc_statmc(void* p) { statmc_denoise(*static_cast<Job*>(p)); }
 void orc_variance(void* p) { variance_denoise(*static_cast<Job*>(p)); }
 void orc_adaptive_counts(void* p, int* out) {

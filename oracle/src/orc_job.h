@@ -173,6 +173,9 @@ struct Job {
 
 void job_init_buffers(Job& j);
 void render_pass(Job& j, const int* counts, bool statmc, bool seed_with_n);
+// Folds given per-sample records (pixel-major, counts[p] per pixel) exactly as
+// a StatMC pass folds the samples it traces (main.cpp:707-717).
+void accumulate_records(Job& j, const int* counts, const SampleRecord* recs);
 void recompute_sensitivity(Job& j);
 void statmc_denoise(Job& j);
 void variance_denoise(Job& j);

@@ -313,4 +313,35 @@ void render_pass(Job& j, const int* counts, bool statmc, bool seed_with_n) {
     j.sens_valid = false;
 }
 
+// main.cpp:649-737 with the traced sample replaced by a given record: hit
+// features for every sample, then only finite samples enter the RPF grid, the
+// Welford statistics and the radiance sums. One thread, pixel-major order.
+void accumulate_records(Job& j, const int* counts, const SampleRecord* recs) {
+    const bool capture = j.cfg.rpf_shrinkage_scale > 0.0f;
+    size_t k = 0;
+    for (int y = 0; y < j.h; ++y) {
+        for (int x = 0; x < j.w; ++x) {
+            const int idx = y * j.w + x;
+            for (int s = 0; s < counts[idx]; ++s, ++k) {
+                const SampleRecord& rec = recs[k];
+                if (rec.hit) {
+                    j.hit_count[idx]++;
+                    j.normal[idx] += rec.normal;
+                    j.albedo[idx] += rec.albedo;
+                    j.world_pos[idx] += rec.position;
+                    j.depth[idx] += rec.depth;
+                }
+                const F3 L = rec.radiance;
+                if (!all_finite(L)) continue;
+                if (capture) accumulate_rpf(j.grid, x, y, rec.rpf, luminance(L));
+                accumulate_sample(j.stats[idx], L);
+                j.radiance[idx] += L;
+                j.sample_count[idx]++;
+            }
+        }
+    }
+    j.feat_valid = false;
+    j.sens_valid = false;
+}
+
 }  // namespace orc

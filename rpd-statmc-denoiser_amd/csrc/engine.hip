@@ -71,6 +71,14 @@ struct rpd_context {
     uint64_t budget = 0;
     std::atomic<uint64_t> in_use{0};
     std::mutex mu;
+    // rpd_denoise_batch: descriptor arrays (grown on demand), stream and timing events
+    std::mutex batch_mu;
+    rpd::ImageDesc* b_img = nullptr;
+    int64_t* b_prefix = nullptr;
+    unsigned int* b_queues = nullptr;
+    int b_cap = 0;
+    hipStream_t b_stream = nullptr;
+    hipEvent_t b_ev[2] = {nullptr, nullptr};
 };
 
 struct rpd_scene {
@@ -107,6 +115,8 @@ struct rpd_job {
     double* grid = nullptr;
     float *tiles = nullptr, *node_scratch = nullptr, *filter_scratch = nullptr;
     void* alloc_scratch = nullptr;
+    bool denoiser = false;   // created for denoising records: no scene, camera or tape
+    int max_spp = 0;         // record capacity per pixel of a denoiser job
     // this job as an image of a batch (device copies for the batch of one)
     ImageDesc desc{};
     Batch batch1{};
@@ -554,8 +564,9 @@ int job_alloc(rpd_job* j) {
     j->work = plan + 3 * N + 4;
     A(j->scan_scratch, (size_t)plan_scan_scratch_elems(j->N));
     A(j->query_count, 2);
-    A(j->tape, ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_N);
-    const int per_px = std::max(j->cfg.samples_per_pixel, j->cfg.adaptive_spp + 1);
+    if (j->denoiser) A(j->tape, 1);
+    else A(j->tape, ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_N);
+    const int per_px = j->denoiser ? j->max_spp : std::max(j->cfg.samples_per_pixel, j->cfg.adaptive_spp + 1);
     const int64_t cap = int64_t(per_px) * int64_t(N);
     j->rec.capacity = cap;
     A(j->rec.r0, cap); A(j->rec.r1, cap); A(j->rec.r2, cap); A(j->rec.r3, cap);
@@ -573,14 +584,17 @@ int job_alloc(rpd_job* j) {
     return build_batch1(j, dd, dp, dq);
 }
 
-size_t job_bytes(const rpd_render_config& c) {
+// Device bytes of a job; a denoiser job (max_spp > 0) has no tape and room
+// for max_spp records per pixel.
+size_t job_bytes(const rpd_render_config& c, int max_spp = 0) {
     const size_t N = size_t(c.image_width) * c.image_height;
     const int step = std::max(1, c.rpf_tile_size / 2);
     const size_t nn = size_t(std::max(2, (c.image_width + step - 1) / step + 1)) *
                       size_t(std::max(2, (c.image_height + step - 1) / step + 1));
-    const size_t per_px = std::max(c.samples_per_pixel, c.adaptive_spp + 1);
-    return N * 4 * (13 + 2 + 12 + 7 + 14 + 13 + 1 + 3 + 3 + 32) + ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_N * 4 + per_px * N * (6 * 16 + 8) +
-           nn * (GRID_NODE_DOUBLES * 8 + SF_COUNT * 4 + 33 * 4) + alloc_scratch_bytes(int(N));
+    const size_t per_px = max_spp > 0 ? size_t(max_spp) : size_t(std::max(c.samples_per_pixel, c.adaptive_spp + 1));
+    const size_t tape = max_spp > 0 ? 0 : ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_N * 4;
+    return N * 4 * (13 + 2 + 12 + 7 + 14 + 13 + 1 + 3 + 3 + 32) + tape + per_px * N * (6 * 16 + 8) +
+           nn * (GRID_NODE_DOUBLES * 8 + SF_COUNT * 4 + 33 * 4) + alloc_scratch_bytes(int(N)) + sizeof(ImageDesc) + 512;
 }
 
 void job_free(rpd_job* j) {
@@ -619,6 +633,55 @@ int job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg,
     if ((st = reset_state(j.get()))) { job_free(j.get()); return st; }
     *out = j.release();
     return RPD_OK;
+}
+
+int job_create_denoiser(rpd_context* ctx, const rpd_render_config* cfg, int max_spp, rpd_job** out) {
+    if (!ctx || !cfg || !out) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (max_spp <= 0) return fail(RPD_ERR_INVALID_ARG, "max_samples_per_pixel must be positive");
+    int st = validate(*cfg);
+    if (st) return st;
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto j = std::make_unique<rpd_job>();
+    j->ctx = ctx; j->scene = nullptr; j->cfg = *cfg; j->seed = 0;
+    j->denoiser = true; j->max_spp = max_spp;
+    j->W = cfg->image_width; j->H = cfg->image_height; j->N = j->W * j->H;
+    j->geom.support = cfg->rpf_tile_size;
+    j->geom.step = std::max(1, cfg->rpf_tile_size / 2);
+    j->geom.nx = std::max(2, (j->W + j->geom.step - 1) / j->geom.step + 1);
+    j->geom.ny = std::max(2, (j->H + j->geom.step - 1) / j->geom.step + 1);
+    j->bytes = job_bytes(*cfg, max_spp);
+    if (ctx->budget && ctx->in_use.load() + j->bytes > ctx->budget)
+        return fail(RPD_ERR_OUT_OF_MEMORY, "job does not fit the context's memory budget");
+    ctx->in_use += j->bytes;
+    HIP_TRY(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+    if ((st = job_alloc(j.get()))) { job_free(j.get()); return st; }
+    if ((st = reset_state(j.get()))) { job_free(j.get()); return st; }
+    *out = j.release();
+    return RPD_OK;
+}
+
+bool capture_records(const rpd_job* j) { return j->cfg.rpf_shrinkage_scale > 0.0f; }
+
+// Fold of the loaded pass + RPF grid (main.cpp:707-717), then
+// recompute_sensitivity and statmc_denoise, on the job's own stream.
+int do_denoise_records(rpd_job* j) {
+    {
+        StageTimer tm(j, 0);
+        launch_fold(j->batch1, 1, j->stream);
+        if (capture_records(j)) {
+            StageTimer tg(j, 10);
+            launch_batch_queues_reset(j->batch1, j->stream);
+            launch_grid_gather(j->batch1, j->stream);
+        }
+    }
+    j->features_valid = false;
+    j->last_pass_statmc = true;
+    j->last_pass_capture = capture_records(j);
+    int st = do_recompute_sensitivity(j);
+    if (!st) st = do_statmc(j);
+    HIP_TRY(hipGetLastError());
+    return st;
 }
 
 // ---- host-side conversion helpers for reads/writes -----------------------------
@@ -694,6 +757,13 @@ int rpd_ctx_create(int device, rpd_context** out) {
     return RPD_OK;
 }
 int rpd_ctx_destroy(rpd_context* ctx) {
+    if (!ctx) return RPD_OK;
+    if (ctx->b_stream) hipStreamSynchronize(ctx->b_stream);
+    if (ctx->b_img) hipFree(ctx->b_img);
+    if (ctx->b_prefix) hipFree(ctx->b_prefix);
+    if (ctx->b_queues) hipFree(ctx->b_queues);
+    for (hipEvent_t e : ctx->b_ev) if (e) hipEventDestroy(e);
+    if (ctx->b_stream) hipStreamDestroy(ctx->b_stream);
     delete ctx;
     return RPD_OK;
 }
@@ -902,6 +972,7 @@ int rpd_job_sync(rpd_job* j) {
 
 int rpd_job_run_pass(rpd_job* j, const int* counts, int statmc) {
     if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
+    if (!j->scene) return fail(RPD_ERR_STATE, "job has no scene (it was created for denoising records)");
     if (!counts) {
         launch_plan_uniform(j->N, j->cfg.samples_per_pixel, j->counts, j->offsets, j->active, j->totals, j->stream);
     } else {
@@ -917,6 +988,7 @@ int rpd_job_run_pass(rpd_job* j, const int* counts, int statmc) {
 
 int rpd_job_run_adaptive_pass(rpd_job* j) {
     if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
+    if (!j->scene) return fail(RPD_ERR_STATE, "job has no scene (it was created for denoising records)");
     int st = do_allocation(j);
     if (!st) st = pass_after_plan(j, 1, 1);
     if (st) return st;
@@ -966,6 +1038,167 @@ int rpd_job_finalize(rpd_job* j) {
     return st ? st : rpd_job_sync(j);
 }
 
+void rpd_synth_config_defaults(rpd_synth_config* c) {
+    if (!c) return;
+    c->seed = 1;
+    c->samples_per_pixel = 16;
+    c->region_size = 32;
+    c->log_mean = -1.0f;
+    c->log_sigma = 1.5f;
+    c->firefly_fraction = 0.02f;
+    c->firefly_scale = 100.0f;
+    c->correlation = 0.6f;
+}
+
+int rpd_job_create_denoiser(rpd_context* ctx, const rpd_render_config* cfg, int max_samples_per_pixel,
+                            rpd_job** out) {
+    return job_create_denoiser(ctx, cfg, max_samples_per_pixel, out);
+}
+
+static_assert(sizeof(rpd_sample_record) == 100, "rpd_sample_record is 25 four-byte words");
+
+int rpd_job_load_records(rpd_job* j, const int* counts, const rpd_sample_record* recs, size_t n) {
+    if (!j || !counts || (n > 0 && !recs)) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    uint64_t total = 0;
+    for (int i = 0; i < j->N; ++i) {
+        if (counts[i] < 0) return fail(RPD_ERR_INVALID_ARG, "negative sample count");
+        total += uint64_t(counts[i]);
+    }
+    if (total != n) return fail(RPD_ERR_INVALID_ARG, "record count does not match the sum of counts");
+    if (int64_t(n) > j->rec.capacity) return fail(RPD_ERR_INVALID_ARG, "more records than the job's capacity");
+    // records are given in pixel-major order, which is the slot order of the plan
+    std::vector<float4> r[6];
+    for (auto& v : r) v.resize(n);
+    std::vector<float2> r6(n);
+    for (size_t k = 0; k < n; ++k) {
+        const rpd_sample_record& a = recs[k];
+        uint32_t fl = a.flags & 0x3feu;  // the caller's validity bits; FINITE comes from the radiance
+        if (std::isfinite(a.radiance[0]) && std::isfinite(a.radiance[1]) && std::isfinite(a.radiance[2]))
+            fl |= REC_FINITE;
+        fl |= uint32_t(std::clamp(a.light_index, 0, 255)) << REC_LIGHT_SHIFT;
+        float fw;
+        std::memcpy(&fw, &fl, 4);
+        r[0][k] = make_float4(a.radiance[0], a.radiance[1], a.radiance[2], fw);
+        r[1][k] = make_float4(a.normal[0], a.normal[1], a.normal[2], a.depth);
+        r[2][k] = make_float4(a.albedo[0], a.albedo[1], a.albedo[2], 0.0f);
+        r[3][k] = make_float4(a.position[0], a.position[1], a.position[2], 0.0f);
+        r[4][k] = make_float4(a.pixel_u[0], a.pixel_u[1], a.lens_u[0], a.lens_u[1]);
+        r[5][k] = make_float4(a.brdf_u[0], a.brdf_u[1], a.light_u[0], a.light_u[1]);
+        r6[k] = make_float2(a.env_u[0], a.env_u[1]);
+    }
+    HIP_TRY(hipMemcpyAsync(j->counts, counts, sizeof(int) * j->N, hipMemcpyHostToDevice, j->stream));
+    launch_plan_scan(j->N, j->counts, j->offsets, j->active, j->totals, j->scan_scratch, j->rec.capacity, j->stream);
+    float4* dst[6] = {j->rec.r0, j->rec.r1, j->rec.r2, j->rec.r3, j->rec.r4, j->rec.r5};
+    if (n) {
+        for (int k = 0; k < 6; ++k)
+            HIP_TRY(hipMemcpyAsync(dst[k], r[k].data(), n * sizeof(float4), hipMemcpyHostToDevice, j->stream));
+        HIP_TRY(hipMemcpyAsync(j->rec.r6, r6.data(), n * sizeof(float2), hipMemcpyHostToDevice, j->stream));
+    }
+    j->last_pass_capture = 1;  // records carry their own validity bits
+    HIP_TRY(hipGetLastError());
+    return rpd_job_sync(j);
+}
+
+int rpd_job_synthesize_records(rpd_job* j, const rpd_synth_config* sc) {
+    if (!j || !sc) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    if (sc->samples_per_pixel <= 0) return fail(RPD_ERR_INVALID_ARG, "samples_per_pixel must be positive");
+    if (int64_t(sc->samples_per_pixel) * j->N > j->rec.capacity)
+        return fail(RPD_ERR_INVALID_ARG, "more samples than the job's record capacity");
+    if (sc->region_size <= 0) return fail(RPD_ERR_INVALID_ARG, "region_size must be positive");
+    if (!(sc->correlation >= -1.0f && sc->correlation <= 1.0f)) return fail(RPD_ERR_INVALID_ARG, "correlation must be in [-1, 1]");
+    launch_plan_uniform(j->N, sc->samples_per_pixel, j->counts, j->offsets, j->active, j->totals, j->stream);
+    SynthParams sp;
+    sp.seed = sc->seed; sp.W = j->W; sp.H = j->H; sp.spp = sc->samples_per_pixel; sp.region = sc->region_size;
+    sp.mu = sc->log_mean; sp.sigma = sc->log_sigma; sp.ff_frac = sc->firefly_fraction; sp.ff_scale = sc->firefly_scale;
+    sp.rho = sc->correlation;
+    launch_synth(sp, j->rec, j->stream);
+    j->last_pass_capture = 1;
+    HIP_TRY(hipGetLastError());
+    return rpd_job_sync(j);
+}
+
+int rpd_job_denoise_records(rpd_job* j) {
+    if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
+    const int st = do_denoise_records(j);
+    return st ? st : rpd_job_sync(j);
+}
+
+int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu_ms) {
+    if (!ctx || !jobs || n <= 0) return fail(RPD_ERR_INVALID_ARG, "null or empty batch");
+    for (int k = 0; k < n; ++k) {
+        if (!jobs[k]) return fail(RPD_ERR_INVALID_ARG, "null job in batch");
+        if (jobs[k]->ctx != ctx) return fail(RPD_ERR_INVALID_ARG, "job of another context in batch");
+        for (int m = 0; m < k; ++m)
+            if (jobs[m] == jobs[k]) return fail(RPD_ERR_INVALID_ARG, "job listed twice in batch");
+    }
+    std::lock_guard<std::mutex> lk(ctx->batch_mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    // inputs were produced on the jobs' own streams
+    std::vector<rpd_job*> batched, alone;
+    for (int k = 0; k < n; ++k) {
+        if (jobs[k]->worker.joinable()) jobs[k]->worker.join();
+        HIP_TRY(hipStreamSynchronize(jobs[k]->stream));
+        (capture_records(jobs[k]) ? batched : alone).push_back(jobs[k]);
+    }
+    if (!ctx->b_stream) {
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->b_stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&ctx->b_ev[0]));
+        HIP_TRY(hipEventCreate(&ctx->b_ev[1]));
+        HIP_TRY(hipMalloc(&ctx->b_queues, sizeof(unsigned int) * BK_KINDS * BATCH_QUEUES));
+    }
+    const int nb = int(batched.size());
+    if (nb > ctx->b_cap) {
+        if (ctx->b_img) HIP_TRY(hipFree(ctx->b_img));
+        if (ctx->b_prefix) HIP_TRY(hipFree(ctx->b_prefix));
+        ctx->b_img = nullptr; ctx->b_prefix = nullptr; ctx->b_cap = 0;
+        HIP_TRY(hipMalloc(&ctx->b_img, sizeof(ImageDesc) * nb));
+        HIP_TRY(hipMalloc(&ctx->b_prefix, sizeof(int64_t) * BK_KINDS * (nb + 1)));
+        ctx->b_cap = nb;
+    }
+    hipStream_t s = ctx->b_stream;
+    HIP_TRY(hipEventRecord(ctx->b_ev[0], s));
+    if (nb > 0) {
+        std::vector<ImageDesc> descs(nb);
+        std::vector<int64_t> pf(size_t(BK_KINDS) * (nb + 1), 0);
+        Batch b{};
+        for (int k = 0; k < nb; ++k) descs[k] = batched[k]->desc;
+        for (int kind = 0; kind < BK_KINDS; ++kind) {
+            for (int k = 0; k < nb; ++k)
+                pf[size_t(kind) * (nb + 1) + k + 1] =
+                    pf[size_t(kind) * (nb + 1) + k] + image_items(kind, descs[k].W, descs[k].H, descs[k].g);
+            b.total[kind] = pf[size_t(kind) * (nb + 1) + nb];
+        }
+        HIP_TRY(hipMemcpyAsync(ctx->b_img, descs.data(), sizeof(ImageDesc) * nb, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(ctx->b_prefix, pf.data(), sizeof(int64_t) * pf.size(), hipMemcpyHostToDevice, s));
+        b.img = ctx->b_img; b.prefix = ctx->b_prefix; b.queues = ctx->b_queues; b.n = nb;
+        launch_fold(b, 1, s);
+        launch_batch_queues_reset(b, s);
+        launch_grid_gather(b, s);
+        launch_features(b, s);
+        launch_sensitivity(b, s);
+        launch_statmc_prepare(b, s);
+        launch_batch_queues_reset(b, s);
+        launch_statmc_window(b, s);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(ctx->b_ev[1], s));
+    HIP_TRY(hipEventSynchronize(ctx->b_ev[1]));
+    for (rpd_job* j : batched) {
+        j->features_valid = true;
+        j->last_pass_statmc = true;
+        j->last_pass_capture = true;
+    }
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->b_ev[0], ctx->b_ev[1]));
+    // images without RPF capture (rpf_shrinkage_scale <= 0) take the per-image path
+    for (rpd_job* j : alone) {
+        const int st = rpd_job_denoise_records(j);
+        if (st) return st;
+    }
+    if (gpu_ms) *gpu_ms = double(ms);
+    return RPD_OK;
+}
+
 int rpd_job_grid_dims(rpd_job* j, int dims[4]) {
     if (!j || !dims) return fail(RPD_ERR_INVALID_ARG, "null argument");
     dims[0] = j->geom.nx; dims[1] = j->geom.ny; dims[2] = j->geom.step; dims[3] = j->geom.support;
@@ -990,7 +1223,8 @@ int rpd_job_counters(rpd_job* j, uint64_t* out, int n) {
     HIP_TRY(hipStreamSynchronize(j->stream));
     unsigned long long q[2] = {0, 0};
     HIP_TRY(hipMemcpy(q, j->query_count, sizeof(q), hipMemcpyDeviceToHost));
-    const uint64_t v[4] = {q[0], q[1], uint64_t(j->scene->n_tris), uint64_t(j->scene->n_spheres)};
+    const uint64_t v[4] = {q[0], q[1], j->scene ? uint64_t(j->scene->n_tris) : 0,
+                           j->scene ? uint64_t(j->scene->n_spheres) : 0};
     for (int k = 0; k < n && k < 4; ++k) out[k] = v[k];
     return RPD_OK;
 }

@@ -205,6 +205,13 @@ void launch_sensitivity_zero(GridGeom g, int width, int height, float* tiles, fl
 // images' filter scratch, then the window kernel reading them.
 void launch_statmc_prepare(const Batch& b, hipStream_t st);
 void launch_statmc_window(const Batch& b, hipStream_t st);
+// synthetic per-sample records (denoise-only workload), slot = pixel * spp + sample
+struct SynthParams {
+    uint32_t seed;
+    int W, H, spp, region;
+    float mu, sigma, ff_frac, ff_scale, rho;
+};
+void launch_synth(const SynthParams& sp, const Records& rec, hipStream_t st);
 // zero the persistent-kernel work counters of a batch
 void launch_batch_queues_reset(const Batch& b, hipStream_t st);
 void launch_variance(int width, int height, const VarianceConfig& vc, const PixelBuffers& pb, const Features& f,

@@ -50,6 +50,52 @@ class Camera(C.Structure):
                 ("fov_degrees", C.c_float), ("focus_distance", C.c_float), ("aperture", C.c_float)]
 
 
+class SynthConfig(C.Structure):
+    """rpd_synth_config: synthetic per-sample records of the denoise-only workload."""
+    _fields_ = [("seed", C.c_uint32), ("samples_per_pixel", C.c_int), ("region_size", C.c_int),
+                ("log_mean", C.c_float), ("log_sigma", C.c_float), ("firefly_fraction", C.c_float),
+                ("firefly_scale", C.c_float), ("correlation", C.c_float)]
+
+
+# rpd_sample_record as a numpy structured type (100 bytes)
+SAMPLE_RECORD = np.dtype([
+    ("radiance", np.float32, 3), ("flags", np.uint32), ("normal", np.float32, 3), ("depth", np.float32),
+    ("albedo", np.float32, 3), ("position", np.float32, 3), ("pixel_u", np.float32, 2), ("lens_u", np.float32, 2),
+    ("brdf_u", np.float32, 2), ("light_u", np.float32, 2), ("env_u", np.float32, 2), ("light_index", np.int32)])
+assert SAMPLE_RECORD.itemsize == 100
+
+
+def records_from_rows(rows):
+    """rpd_sample_record array from 32-float record rows (the RPD_BUF_RECORDS layout:
+    pixel, rgb, finite, hit, normal, albedo, position, depth, pixel_u, lens_u, brdf_u,
+    light_u, env_u, light index, rr survived, validity bits)."""
+    rows = np.asarray(rows, np.float32)
+    r = np.zeros(rows.shape[0], SAMPLE_RECORD)
+    r["radiance"] = rows[:, 1:4]
+    r["normal"] = rows[:, 6:9]
+    r["albedo"] = rows[:, 9:12]
+    r["position"] = rows[:, 12:15]
+    r["depth"] = rows[:, 15]
+    r["pixel_u"] = rows[:, 16:18]
+    r["lens_u"] = rows[:, 18:20]
+    r["brdf_u"] = rows[:, 20:22]
+    r["light_u"] = rows[:, 22:24]
+    r["env_u"] = rows[:, 24:26]
+    r["light_index"] = np.maximum(rows[:, 26], 0).astype(np.int32)
+    v = rows[:, 28].astype(np.uint32)
+    fl = np.where(rows[:, 5] != 0, 2, 0).astype(np.uint32)
+    fl |= np.where(v & 1, 4, 0).astype(np.uint32)        # pixel
+    fl |= np.where(v & 2, 8, 0).astype(np.uint32)        # lens
+    fl |= np.where(v & 4, 16, 0).astype(np.uint32)       # light select
+    fl |= np.where(v & 8, 32, 0).astype(np.uint32)       # brdf
+    fl |= np.where(v & 16, 64, 0).astype(np.uint32)      # light uv
+    fl |= np.where(v & 32, 128, 0).astype(np.uint32)     # environment
+    fl |= np.where(v & 64, 256, 0).astype(np.uint32)     # russian roulette
+    fl |= np.where(rows[:, 27] != 0, 512, 0).astype(np.uint32)
+    r["flags"] = fl
+    return r
+
+
 class EngineError(RuntimeError):
     pass
 
@@ -67,6 +113,8 @@ def lib():
         L.rpd_last_error.restype = C.c_char_p
         L.rpd_render_config_defaults.argtypes = [C.POINTER(RenderConfig)]
         L.rpd_render_config_defaults.restype = None
+        L.rpd_synth_config_defaults.argtypes = [C.POINTER(SynthConfig)]
+        L.rpd_synth_config_defaults.restype = None
         for name, args in [
             ("rpd_ctx_create", [C.c_int, C.POINTER(vp)]), ("rpd_ctx_destroy", [vp]),
             ("rpd_ctx_set_memory_budget", [vp, C.c_uint64]),
@@ -86,6 +134,11 @@ def lib():
             ("rpd_job_read", [vp, C.c_int, vp, C.c_size_t]), ("rpd_job_write", [vp, C.c_int, vp, C.c_size_t]),
             ("rpd_job_grid_dims", [vp, vp]), ("rpd_job_stage_ms", [vp, vp, C.c_int]),
             ("rpd_job_counters", [vp, vp, C.c_int]),
+            ("rpd_job_create_denoiser", [vp, C.POINTER(RenderConfig), C.c_int, C.POINTER(vp)]),
+            ("rpd_job_load_records", [vp, vp, vp, C.c_size_t]),
+            ("rpd_job_synthesize_records", [vp, C.POINTER(SynthConfig)]),
+            ("rpd_job_denoise_records", [vp]),
+            ("rpd_denoise_batch", [vp, vp, C.c_int, C.POINTER(C.c_double)]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -151,16 +204,36 @@ class Context:
             self.h = None
 
 
-class Job:
-    """One render job's device state; stage methods mirror the reference's functions."""
+def synth_config(**kw):
+    c = SynthConfig()
+    lib().rpd_synth_config_defaults(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
 
-    def __init__(self, ctx, cfg, cam, seed, scene="cornell_box", submit=False):
+
+def denoise_batch(ctx, jobs):
+    """rpd_denoise_batch over `jobs`; returns the batch's device time in ms."""
+    arr = (C.c_void_p * len(jobs))(*[j.h for j in jobs])
+    ms = C.c_double()
+    check(ctx.L.rpd_denoise_batch(ctx.h, arr, len(jobs), C.byref(ms)))
+    return ms.value
+
+
+class Job:
+    """One render job's device state; stage methods mirror the reference's functions.
+    denoiser_spp > 0 creates a job that denoises given records instead (no scene)."""
+
+    def __init__(self, ctx, cfg, cam, seed, scene="cornell_box", submit=False, denoiser_spp=0):
         self.L = ctx.L
         self.cfg = cfg
         self.w, self.h_px = cfg.image_width, cfg.image_height
         h = C.c_void_p()
-        fn = self.L.rpd_job_submit if submit else self.L.rpd_job_create
-        check(fn(ctx.h, ctx.scene(scene), C.byref(cfg), C.byref(cam), seed & 0xFFFFFFFF, C.byref(h)))
+        if denoiser_spp > 0:
+            check(self.L.rpd_job_create_denoiser(ctx.h, C.byref(cfg), int(denoiser_spp), C.byref(h)))
+        else:
+            fn = self.L.rpd_job_submit if submit else self.L.rpd_job_create
+            check(fn(ctx.h, ctx.scene(scene), C.byref(cfg), C.byref(cam), seed & 0xFFFFFFFF, C.byref(h)))
         self.h = h
         d = np.zeros(4, np.int32)
         check(self.L.rpd_job_grid_dims(self.h, _ptr(d)))
@@ -192,6 +265,18 @@ class Job:
             counts = np.ascontiguousarray(counts, np.int32)
             assert counts.size == self.w * self.h_px
             check(self.L.rpd_job_run_pass(self.h, _ptr(counts), int(statmc)))
+
+    def load_records(self, counts, records):
+        counts = np.ascontiguousarray(counts, np.int32)
+        recs = np.ascontiguousarray(records, SAMPLE_RECORD)
+        assert counts.size == self.w * self.h_px
+        check(self.L.rpd_job_load_records(self.h, _ptr(counts), _ptr(recs), recs.shape[0]))
+
+    def synthesize_records(self, sc):
+        check(self.L.rpd_job_synthesize_records(self.h, C.byref(sc)))
+
+    def denoise_records(self):
+        check(self.L.rpd_job_denoise_records(self.h))
 
     def run_adaptive_pass(self):
         check(self.L.rpd_job_run_adaptive_pass(self.h))

@@ -117,6 +117,7 @@ def lib():
         L.orc_get.restype = C.c_long
         L.orc_set.argtypes = [vp, C.c_char_p, C.c_void_p]
         L.orc_set.restype = C.c_long
+        L.orc_accumulate_records.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_long]
         L.orc_num_records.argtypes = [vp]
         L.orc_num_records.restype = C.c_long
         L.orc_hash_u32.argtypes = [C.c_uint32]
@@ -201,6 +202,13 @@ class OracleJob:
         counts = np.ascontiguousarray(counts, np.int32)
         assert counts.size == self.w * self.ht
         self.L.orc_pass(self.h, ptr(counts), int(statmc))
+
+    def accumulate_records(self, counts, records32):
+        """Fold given records (32-float rows, pixel-major) like one StatMC pass."""
+        counts = np.ascontiguousarray(counts, np.int32)
+        rec = np.ascontiguousarray(records32, np.float32)
+        assert counts.size == self.w * self.ht and rec.shape[0] == int(counts.sum())
+        self.L.orc_accumulate_records(self.h, ptr(counts), ptr(rec), rec.shape[0])
 
     def recompute_sensitivity(self):
         self.L.orc_recompute_sensitivity(self.h)

@@ -344,3 +344,22 @@ def test_no_hit_pixels_keep_mean_and_alpha_one():
     assert np.all(j.get("hit_count") == 0)
     assert np.all(j.get("alpha") == 1.0)
     assert np.all(j.get("radiance") == 0.0)
+
+
+def test_record_fold_equals_traced_fold():
+    """The oracle's fold of given records (used to check denoising of records)
+    leaves exactly the state its own traced StatMC pass leaves."""
+    rc = ob.config_c0(24, 20)
+    rc.aperture = 0.14
+    rc.focus_distance = 1.05
+    counts = np.full(24 * 20, 3, np.int32)
+    a = ob.OracleJob(rc, 77, nthreads=1, keep_records=True)
+    a.run_pass(counts, statmc=True)
+    b = ob.OracleJob(rc, 77)
+    b.accumulate_records(counts, a.get("records"))
+    for name in ("stats", "grid", "hit_count", "sample_count", "radiance", "normal", "albedo", "depth"):
+        x, y = a.get(name), b.get(name)
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), name
+    a.recompute_sensitivity(); a.statmc()
+    b.recompute_sensitivity(); b.statmc()
+    assert np.array_equal(a.get("denoised"), b.get("denoised"))
