@@ -239,7 +239,9 @@ typedef struct rpd_synth_config {
 void rpd_synth_config_defaults(rpd_synth_config* c); /* mu -1, sigma 1.5, 2% fireflies x100, rho 0.6, 32 px */
 
 /* A job for denoising records only: no scene, no camera, no sampling tape;
-   room for max_samples_per_pixel samples of every pixel per pass. */
+   room for max_samples_per_pixel samples of every pixel per pass. It keeps no
+   per-sample hit positions (the denoiser does not read them), so its
+   RPD_BUF_WORLD_POS reads zero. */
 int rpd_job_create_denoiser(rpd_context* ctx, const rpd_render_config* cfg, int max_samples_per_pixel,
                             rpd_job** out);
 /* Loads one pass of records: counts[p] samples of pixel p (row-major), the
@@ -256,6 +258,21 @@ int rpd_job_denoise_records(rpd_job* job);
    StatMC window). Results are identical to denoising each job alone.
    gpu_ms (optional) receives the device time of the whole batch. */
 int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu_ms);
+
+/* Clears a job's accumulated state (pixel statistics, features, RPF grid,
+   outputs) as a fresh job has it; loaded records and the pass plan stay.
+   No counterpart in the reference (a process renders one image). */
+int rpd_job_reset(rpd_job* job);
+
+/* StatMC window tile shapes: 0 = 16x16, 1 = 32x8, 2 = 8x32, 3 = 64x4 pixels.
+   The autotuner times every shape on a synthetic width x height image per
+   colour window radius and keeps the fastest for that radius in the context;
+   later StatMC launches of the context use it (results do not depend on the
+   shape). No counterpart in the reference. */
+int rpd_ctx_autotune_filter(rpd_context* ctx, const int* radii, int n, int width, int height,
+                            int samples_per_pixel);
+int rpd_ctx_filter_tuning(rpd_context* ctx, int radius, int* shape, int* tile_w, int* tile_h, float ms[4]);
+int rpd_ctx_set_filter_shape(rpd_context* ctx, int radius, int shape);
 
 /* Work counters of the job so far: [0] ray queries traced (camera, extension
    and shadow rays), [1] path samples traced, [2] scene triangles, [3] scene

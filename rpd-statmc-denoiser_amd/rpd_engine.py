@@ -139,6 +139,10 @@ def lib():
             ("rpd_job_synthesize_records", [vp, C.POINTER(SynthConfig)]),
             ("rpd_job_denoise_records", [vp]),
             ("rpd_denoise_batch", [vp, vp, C.c_int, C.POINTER(C.c_double)]),
+            ("rpd_job_reset", [vp]),
+            ("rpd_ctx_autotune_filter", [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+            ("rpd_ctx_filter_tuning", [vp, C.c_int, vp, vp, vp, vp]),
+            ("rpd_ctx_set_filter_shape", [vp, C.c_int, C.c_int]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -191,6 +195,21 @@ class Context:
             check(self.L.rpd_scene_builtin(self.h, name.encode(), C.byref(s)))
             self._scenes[name] = s
         return self._scenes[name]
+
+    def autotune_filter(self, radii, width=512, height=512, spp=16):
+        """Time every StatMC tile shape per radius; the context keeps the fastest."""
+        r = np.ascontiguousarray(radii, np.int32)
+        check(self.L.rpd_ctx_autotune_filter(self.h, _ptr(r), r.size, width, height, spp))
+        return {int(k): self.filter_tuning(int(k)) for k in r}
+
+    def filter_tuning(self, radius):
+        shape, tw, th = C.c_int(), C.c_int(), C.c_int()
+        ms = np.zeros(4, np.float32)
+        check(self.L.rpd_ctx_filter_tuning(self.h, radius, C.byref(shape), C.byref(tw), C.byref(th), _ptr(ms)))
+        return {"shape": shape.value, "tile": [tw.value, th.value], "ms_per_shape": [float(x) for x in ms]}
+
+    def set_filter_shape(self, radius, shape):
+        check(self.L.rpd_ctx_set_filter_shape(self.h, radius, shape))
 
     def set_memory_budget(self, nbytes):
         check(self.L.rpd_ctx_set_memory_budget(self.h, int(nbytes)))
@@ -277,6 +296,9 @@ class Job:
 
     def denoise_records(self):
         check(self.L.rpd_job_denoise_records(self.h))
+
+    def reset(self):
+        check(self.L.rpd_job_reset(self.h))
 
     def run_adaptive_pass(self):
         check(self.L.rpd_job_run_adaptive_pass(self.h))
