@@ -239,9 +239,7 @@ typedef struct rpd_synth_config {
 void rpd_synth_config_defaults(rpd_synth_config* c); /* mu -1, sigma 1.5, 2% fireflies x100, rho 0.6, 32 px */
 
 /* A job for denoising records only: no scene, no camera, no sampling tape;
-   room for max_samples_per_pixel samples of every pixel per pass. It keeps no
-   per-sample hit positions (the denoiser does not read them), so its
-   RPD_BUF_WORLD_POS reads zero. */
+   room for max_samples_per_pixel samples of every pixel per pass. */
 int rpd_job_create_denoiser(rpd_context* ctx, const rpd_render_config* cfg, int max_samples_per_pixel,
                             rpd_job** out);
 /* Loads one pass of records: counts[p] samples of pixel p (row-major), the
@@ -259,20 +257,47 @@ int rpd_job_denoise_records(rpd_job* job);
    gpu_ms (optional) receives the device time of the whole batch. */
 int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu_ms);
 
-/* Clears a job's accumulated state (pixel statistics, features, RPF grid,
-   outputs) as a fresh job has it; loaded records and the pass plan stay.
-   No counterpart in the reference (a process renders one image). */
-int rpd_job_reset(rpd_job* job);
+/* Capture each job's adaptive-pass sequence (variance smoothing, allocation,
+   sampling pass) as a hipGraph on its first pass and replay it for every
+   pass; results are identical to launching the kernels one by one. */
+int rpd_ctx_set_graph_passes(rpd_context* ctx, int on);
+/* Device bytes a job of this configuration holds (admission by HBM footprint). */
+int rpd_job_bytes(const rpd_render_config* cfg, uint64_t* bytes);
+/* Bytes held by the context's jobs, its budget (0 = none) and the device's free and total memory. */
+int rpd_ctx_memory(rpd_context* ctx, uint64_t* in_use, uint64_t* budget, uint64_t* device_free,
+                   uint64_t* device_total);
 
-/* StatMC window tile shapes: 0 = 16x16, 1 = 32x8, 2 = 8x32, 3 = 64x4 pixels.
-   The autotuner times every shape on a synthetic width x height image per
-   colour window radius and keeps the fastest for that radius in the context;
-   later StatMC launches of the context use it (results do not depend on the
-   shape). No counterpart in the reference. */
-int rpd_ctx_autotune_filter(rpd_context* ctx, const int* radii, int n, int width, int height,
-                            int samples_per_pixel);
-int rpd_ctx_filter_tuning(rpd_context* ctx, int radius, int* shape, int* tile_w, int* tile_h, float ms[4]);
-int rpd_ctx_set_filter_shape(rpd_context* ctx, int radius, int shape);
+/* ---- serving ---------------------------------------------------------------
+   A request loop over render+denoise jobs (no counterpart in the reference,
+   which renders one image per process). Requests are queued as they arrive
+   and admitted, first in first out with a short look-ahead, whenever a slot is
+   free and the job's device footprint fits the memory budget; every admitted
+   job runs its whole pipeline on its own stream, so the work of all admitted
+   jobs overlaps on the device, and keeps its state resident across its
+   adaptive passes. A job's result does not depend on what else runs. */
+typedef struct rpd_server rpd_server;
+typedef struct rpd_server_config {
+    int max_in_flight;       /* jobs admitted at once (>= 1) */
+    uint64_t memory_budget;  /* bytes of job state admitted at once (0 = device memory) */
+    int graph_passes;        /* replay the adaptive passes from a captured hipGraph */
+} rpd_server_config;
+typedef struct rpd_server_stats {
+    uint64_t submitted, completed, failed;
+    int max_in_flight_seen;
+    uint64_t peak_bytes;
+    double latency_p50_ms, latency_p99_ms, latency_mean_ms, latency_max_ms;  /* submit -> done */
+    double queue_p50_ms, queue_p99_ms;                                        /* submit -> admitted */
+} rpd_server_stats;
+int rpd_server_create(rpd_context* ctx, const rpd_server_config* cfg, rpd_server** out);
+/* Queues one job; returns at once with the request id. */
+int rpd_server_submit(rpd_server* s, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam,
+                      uint32_t seed, uint64_t* request_id);
+/* Blocks until the request is done; hands over its job (release it with
+   rpd_job_release) and its latency from submit to completion. */
+int rpd_server_wait(rpd_server* s, uint64_t request_id, rpd_job** job, double* latency_ms);
+int rpd_server_stats_get(rpd_server* s, rpd_server_stats* out);
+/* Finishes queued and running requests, releases jobs nobody claimed. */
+int rpd_server_destroy(rpd_server* s);
 
 /* Work counters of the job so far: [0] ray queries traced (camera, extension
    and shadow rays), [1] path samples traced, [2] scene triangles, [3] scene

@@ -80,11 +80,7 @@ struct rpd_context {
     int b_cap = 0;
     hipStream_t b_stream = nullptr;
     hipEvent_t b_ev[2] = {nullptr, nullptr};
-    // StatMC tile shape per colour window radius (index into filter_tile_w/h),
-    // set by rpd_ctx_autotune_filter; radii beyond the table use shape 0
-    static constexpr int kMaxTunedRadius = 64;
-    int filter_shape[kMaxTunedRadius + 1] = {};
-    float filter_ms[kMaxTunedRadius + 1][rpd::FILTER_SHAPES] = {};
+    bool graph_passes = false;  // capture each job's adaptive-pass sequence as a hipGraph and replay it
 };
 
 struct rpd_scene {
@@ -122,6 +118,7 @@ struct rpd_job {
     float *tiles = nullptr, *node_scratch = nullptr, *filter_scratch = nullptr;
     void* alloc_scratch = nullptr;
     bool denoiser = false;   // created for denoising records: no scene, camera or tape
+    bool capturing = false;  // the stream is being captured into a graph: no per-stage timing events
     int max_spp = 0;         // record capacity per pixel of a denoiser job
     // this job as an image of a batch (device copies for the batch of one)
     ImageDesc desc{};
@@ -149,11 +146,11 @@ struct StageTimer {
     int stage;
     hipEvent_t a = nullptr;
     StageTimer(rpd_job* job, int s) : j(job), stage(s) {
-        if (hipEventCreate(&a) == hipSuccess) hipEventRecord(a, j->stream);
+        if (!j->capturing && hipEventCreate(&a) == hipSuccess) hipEventRecord(a, j->stream);
     }
     ~StageTimer() {
         hipEvent_t b = nullptr;
-        if (a && hipEventCreate(&b) == hipSuccess) {
+        if (a && !j->capturing && hipEventCreate(&b) == hipSuccess) {
             hipEventRecord(b, j->stream);
             j->spans.push_back({stage, a, b});
         }
@@ -388,10 +385,6 @@ int do_recompute_sensitivity(rpd_job* j) {
     return RPD_OK;
 }
 
-int filter_shape_for(const rpd_context* ctx, int radius) {
-    return (radius >= 0 && radius <= rpd_context::kMaxTunedRadius) ? ctx->filter_shape[radius] : 0;
-}
-
 int do_statmc(rpd_job* j) {
     StageTimer tm(j, 3);
     ensure_features(j);
@@ -399,7 +392,7 @@ int do_statmc(rpd_job* j) {
     {
         StageTimer tw(j, 9);
         launch_batch_queues_reset(j->batch1, j->stream);
-        launch_statmc_window(j->batch1, filter_shape_for(j->ctx, j->cfg.color_window_radius), j->stream);
+        launch_statmc_window(j->batch1, j->stream);
     }
     HIP_TRY(hipGetLastError());
     return RPD_OK;
@@ -456,7 +449,7 @@ int do_finalize(rpd_job* j, bool statmc) {
     return RPD_OK;
 }
 
-int reset_state(rpd_job* j, bool keep_plan = false) {
+int reset_state(rpd_job* j) {
     const size_t N = j->N;
     // zero every accumulator (buffers.init, pixel_stats.assign, rpf_grid.reset)
     HIP_TRY(hipMemsetAsync(j->pb.radiance, 0, sizeof(float) * 13 * N, j->stream));
@@ -465,8 +458,7 @@ int reset_state(rpd_job* j, bool keep_plan = false) {
     HIP_TRY(hipMemsetAsync(j->den, 0, sizeof(float) * 14 * N, j->stream));
     HIP_TRY(hipMemsetAsync(j->grid, 0, sizeof(double) * GRID_NODE_DOUBLES * j->geom.nx * j->geom.ny, j->stream));
     HIP_TRY(hipMemsetAsync(j->tiles, 0, sizeof(float) * SF_COUNT * j->geom.nx * j->geom.ny, j->stream));
-    if (!keep_plan)
-        HIP_TRY(hipMemsetAsync(j->counts, 0, sizeof(int) * (3 * N + 8), j->stream));  // plan, totals, work counter
+    HIP_TRY(hipMemsetAsync(j->counts, 0, sizeof(int) * (3 * N + 8), j->stream));  // plan, totals, work counter
     HIP_TRY(hipMemsetAsync(j->query_count, 0, sizeof(unsigned long long) * 2, j->stream));
     j->features_valid = false;
     j->finalized = false;
@@ -477,6 +469,44 @@ int reset_state(rpd_job* j, bool keep_plan = false) {
 // overwrites before use (statmc_denoise at 1548 and recompute_sensitivity at
 // 736/1551 when adaptive passes run) are not repeated; every output is
 // identical without them.
+// The adaptive passes of main.cpp:1451-1538 (variance smoothing, allocation,
+// sampling pass) are the same launches with the same arguments every pass:
+// all per-pass sizes live in device memory. The first pass is captured into a
+// hipGraph and every pass, the first included, is a replay of it.
+int run_adaptive_passes_graph(rpd_job* j, int passes) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    j->capturing = true;
+    if (hipStreamBeginCapture(j->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        j->capturing = false;
+        return fail(RPD_ERR_DEVICE, "hipStreamBeginCapture failed");
+    }
+    int st = do_variance(j);
+    if (!st) st = do_allocation(j);
+    if (!st) st = pass_after_plan(j, 1, 1);
+    const hipError_t e = hipStreamEndCapture(j->stream, &g);
+    j->capturing = false;
+    if (st || e != hipSuccess || !g) {
+        if (g) hipGraphDestroy(g);
+        return st ? st : fail(RPD_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(e));
+    }
+    if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+        hipGraphDestroy(g);
+        return fail(RPD_ERR_DEVICE, "hipGraphInstantiate failed");
+    }
+    hipError_t le = hipSuccess;
+    for (int i = 0; i < passes && le == hipSuccess; ++i) {
+        StageTimer tm(j, 1);
+        le = hipGraphLaunch(ge, j->stream);
+    }
+    const hipError_t se = hipStreamSynchronize(j->stream);
+    hipGraphExecDestroy(ge);
+    hipGraphDestroy(g);
+    if (le != hipSuccess || se != hipSuccess)
+        return fail(RPD_ERR_DEVICE, std::string("graph launch failed: ") + hipGetErrorString(le != hipSuccess ? le : se));
+    return RPD_OK;
+}
+
 int run_pipeline(rpd_job* j) {
     int st = reset_state(j);
     if (st) return st;
@@ -490,10 +520,14 @@ int run_pipeline(rpd_job* j) {
     if ((st = pass_after_plan(j, 1, 0))) return st;
     const bool passes = j->cfg.adaptive_passes > 0 && j->cfg.adaptive_spp > 0;
     if (!passes && (st = do_recompute_sensitivity(j))) return st;
-    for (int i = 0; passes && i < j->cfg.adaptive_passes; ++i) {
-        if ((st = do_variance(j))) return st;
-        if ((st = do_allocation(j))) return st;
-        if ((st = pass_after_plan(j, 1, 1))) return st;
+    if (passes && j->ctx->graph_passes) {
+        if ((st = run_adaptive_passes_graph(j, j->cfg.adaptive_passes))) return st;
+    } else {
+        for (int i = 0; passes && i < j->cfg.adaptive_passes; ++i) {
+            if ((st = do_variance(j))) return st;
+            if ((st = do_allocation(j))) return st;
+            if ((st = pass_after_plan(j, 1, 1))) return st;
+        }
     }
     if ((st = do_recompute_sensitivity(j))) return st;
     if ((st = do_statmc(j))) return st;
@@ -580,8 +614,7 @@ int job_alloc(rpd_job* j) {
     const int per_px = j->denoiser ? j->max_spp : std::max(j->cfg.samples_per_pixel, j->cfg.adaptive_spp + 1);
     const int64_t cap = int64_t(per_px) * int64_t(N);
     j->rec.capacity = cap;
-    A(j->rec.r0, cap); A(j->rec.r1, cap); A(j->rec.r2, cap);
-    if (!j->denoiser) A(j->rec.r3, cap);  // hit positions only feed the world_pos output, not the denoiser
+    A(j->rec.r0, cap); A(j->rec.r1, cap); A(j->rec.r2, cap); A(j->rec.r3, cap);
     A(j->rec.r4, cap); A(j->rec.r5, cap); A(j->rec.r6, cap);
     A(j->grid, nn * GRID_NODE_DOUBLES);
     A(j->tiles, nn * SF_COUNT);
@@ -605,8 +638,7 @@ size_t job_bytes(const rpd_render_config& c, int max_spp = 0) {
                       size_t(std::max(2, (c.image_height + step - 1) / step + 1));
     const size_t per_px = max_spp > 0 ? size_t(max_spp) : size_t(std::max(c.samples_per_pixel, c.adaptive_spp + 1));
     const size_t tape = max_spp > 0 ? 0 : ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_N * 4;
-    const size_t rec_bytes = max_spp > 0 ? 5 * 16 + 8 : 6 * 16 + 8;
-    return N * 4 * (13 + 2 + 12 + 7 + 14 + 13 + 1 + 3 + 3 + 32) + tape + per_px * N * rec_bytes +
+    return N * 4 * (13 + 2 + 12 + 7 + 14 + 13 + 1 + 3 + 3 + 32) + tape + per_px * N * (6 * 16 + 8) +
            nn * (GRID_NODE_DOUBLES * 8 + SF_COUNT * 4 + 33 * 4) + alloc_scratch_bytes(int(N)) + sizeof(ImageDesc) + 512;
 }
 
@@ -1104,7 +1136,7 @@ int rpd_job_load_records(rpd_job* j, const int* counts, const rpd_sample_record*
     float4* dst[6] = {j->rec.r0, j->rec.r1, j->rec.r2, j->rec.r3, j->rec.r4, j->rec.r5};
     if (n) {
         for (int k = 0; k < 6; ++k)
-            if (dst[k]) HIP_TRY(hipMemcpyAsync(dst[k], r[k].data(), n * sizeof(float4), hipMemcpyHostToDevice, j->stream));
+            HIP_TRY(hipMemcpyAsync(dst[k], r[k].data(), n * sizeof(float4), hipMemcpyHostToDevice, j->stream));
         HIP_TRY(hipMemcpyAsync(j->rec.r6, r6.data(), n * sizeof(float2), hipMemcpyHostToDevice, j->stream));
     }
     j->last_pass_capture = 1;  // records carry their own validity bits
@@ -1189,15 +1221,9 @@ int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu
         launch_grid_gather(b, s);
         launch_features(b, s);
         launch_sensitivity(b, s);
-        // tile shape tuned for the radius that covers the most pixels of the batch
-        std::map<int, int64_t> px_by_radius;
-        for (const ImageDesc& d : descs) px_by_radius[d.fc.radius] += int64_t(d.N);
-        int radius = descs[0].fc.radius;
-        for (const auto& kv : px_by_radius)
-            if (kv.second > px_by_radius[radius]) radius = kv.first;
         launch_statmc_prepare(b, s);
         launch_batch_queues_reset(b, s);
-        launch_statmc_window(b, filter_shape_for(ctx, radius), s);
+        launch_statmc_window(b, s);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(ctx->b_ev[1], s));
@@ -1218,94 +1244,32 @@ int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu
     return RPD_OK;
 }
 
-int rpd_job_reset(rpd_job* j) {
-    if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
-    if (j->worker.joinable()) j->worker.join();
-    const int st = reset_state(j, true);
-    return st ? st : rpd_job_sync(j);
-}
-
-// Switched off: the command that first ran the tuner and the non-default
-// tile shapes also ran the full-size batched denoise and left the GPU
-// faulted; the cause is not identified, so these paths stay unreachable.
-constexpr bool kFilterShapesEnabled = false;
-
-int rpd_ctx_autotune_filter(rpd_context* ctx, const int* radii, int n, int width, int height,
-                            int samples_per_pixel) {
-    if (!kFilterShapesEnabled) return fail(RPD_ERR_UNSUPPORTED, "the StatMC tile autotuner is disabled in this build");
-    if (!ctx || !radii || n <= 0) return fail(RPD_ERR_INVALID_ARG, "null or empty radius list");
-    if (width <= 0 || height <= 0 || samples_per_pixel <= 0) return fail(RPD_ERR_INVALID_ARG, "bad tuning image");
-    for (int k = 0; k < n; ++k)
-        if (radii[k] <= 0 || radii[k] > rpd_context::kMaxTunedRadius)
-            return fail(RPD_ERR_INVALID_ARG, "radius outside the tuning table");
-    rpd_render_config cfg;
-    rpd_render_config_defaults(&cfg);
-    cfg.image_width = width; cfg.image_height = height; cfg.samples_per_pixel = samples_per_pixel;
-    cfg.use_statmc = 1;
-    rpd_job* j = nullptr;
-    int st = job_create_denoiser(ctx, &cfg, samples_per_pixel, &j);
-    if (st) return st;
-    rpd_synth_config sc;
-    rpd_synth_config_defaults(&sc);
-    sc.samples_per_pixel = samples_per_pixel;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    auto done = [&](int rc) {
-        if (e0) hipEventDestroy(e0);
-        if (e1) hipEventDestroy(e1);
-        job_free(j);
-        delete j;
-        return rc;
-    };
-    if ((st = rpd_job_synthesize_records(j, &sc)) || (st = rpd_job_denoise_records(j))) return done(st);
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
-        return done(fail(RPD_ERR_DEVICE, "hipEventCreate failed"));
-    for (int k = 0; k < n; ++k) {
-        const int r = radii[k];
-        j->desc.fc.radius = r;  // every shape computes the same pixels; only the time differs
-        if (hipMemcpy(const_cast<ImageDesc*>(j->batch1.img), &j->desc, sizeof(ImageDesc), hipMemcpyHostToDevice) !=
-            hipSuccess) return done(fail(RPD_ERR_DEVICE, "descriptor upload failed"));
-        float best = 0.0f;
-        int best_shape = 0;
-        for (int shape = 0; shape < FILTER_SHAPES; ++shape) {
-            float tot = 0.0f;
-            for (int rep = 0; rep < 4; ++rep) {
-                launch_batch_queues_reset(j->batch1, j->stream);
-                hipEventRecord(e0, j->stream);
-                launch_statmc_window(j->batch1, shape, j->stream);
-                hipEventRecord(e1, j->stream);
-                if (hipEventSynchronize(e1) != hipSuccess) return done(fail(RPD_ERR_DEVICE, "tuning launch failed"));
-                float ms = 0.0f;
-                hipEventElapsedTime(&ms, e0, e1);
-                if (rep > 0) tot += ms;  // the first launch is a warm-up
-            }
-            const float avg = tot / 3.0f;
-            ctx->filter_ms[r][shape] = avg;
-            if (shape == 0 || avg < best) { best = avg; best_shape = shape; }
-        }
-        ctx->filter_shape[r] = best_shape;
-    }
-    return done(RPD_OK);
-}
-
-int rpd_ctx_filter_tuning(rpd_context* ctx, int radius, int* shape, int* tile_w, int* tile_h, float* ms) {
+int rpd_ctx_set_graph_passes(rpd_context* ctx, int on) {
     if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
-    const int s = filter_shape_for(ctx, radius);
-    if (shape) *shape = s;
-    if (tile_w) *tile_w = filter_tile_w(s);
-    if (tile_h) *tile_h = filter_tile_h(s);
-    if (ms) {
-        for (int k = 0; k < FILTER_SHAPES; ++k)
-            ms[k] = (radius >= 0 && radius <= rpd_context::kMaxTunedRadius) ? ctx->filter_ms[radius][k] : 0.0f;
-    }
+    ctx->graph_passes = on != 0;
     return RPD_OK;
 }
 
-int rpd_ctx_set_filter_shape(rpd_context* ctx, int radius, int shape) {
+int rpd_job_bytes(const rpd_render_config* cfg, uint64_t* bytes) {
+    if (!cfg || !bytes) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    const int st = validate(*cfg);
+    if (st) return st;
+    *bytes = job_bytes(*cfg);
+    return RPD_OK;
+}
+
+int rpd_ctx_memory(rpd_context* ctx, uint64_t* in_use, uint64_t* budget, uint64_t* device_free,
+                   uint64_t* device_total) {
     if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
-    if (radius < 0 || radius > rpd_context::kMaxTunedRadius) return fail(RPD_ERR_INVALID_ARG, "radius outside the table");
-    if (shape < 0 || shape >= FILTER_SHAPES) return fail(RPD_ERR_INVALID_ARG, "unknown tile shape");
-    if (!kFilterShapesEnabled && shape != 0) return fail(RPD_ERR_UNSUPPORTED, "only the 16x16 StatMC tile is enabled");
-    ctx->filter_shape[radius] = shape;
+    if (in_use) *in_use = ctx->in_use.load();
+    if (budget) *budget = ctx->budget;
+    if (device_free || device_total) {
+        size_t fr = 0, tot = 0;
+        HIP_TRY(hipSetDevice(ctx->device));
+        HIP_TRY(hipMemGetInfo(&fr, &tot));
+        if (device_free) *device_free = fr;
+        if (device_total) *device_total = tot;
+    }
     return RPD_OK;
 }
 
@@ -1449,8 +1413,8 @@ int rpd_job_read(rpd_job* j, int id, void* dst, size_t bytes) {
             Records& R = j->rec;
             float4* srcs[6] = {R.r0, R.r1, R.r2, R.r3, R.r4, R.r5};
             for (int k = 0; k < 6; ++k) {
-                r[k].assign(ns, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
-                if (ns && srcs[k]) HIP_TRY(hipMemcpy(r[k].data(), srcs[k], ns * sizeof(float4), hipMemcpyDeviceToHost));
+                r[k].resize(ns);
+                if (ns) HIP_TRY(hipMemcpy(r[k].data(), srcs[k], ns * sizeof(float4), hipMemcpyDeviceToHost));
             }
             if (ns) HIP_TRY(hipMemcpy(r6.data(), R.r6, ns * sizeof(float2), hipMemcpyDeviceToHost));
             std::vector<int> offs(N), cnts(N);

@@ -237,26 +237,22 @@ __device__ void statmc_pixel(const ImageDesc& d, const int x, const int y) {
     alp[i] = a;
 }
 
-// Persistent StatMC: blocks take tiles of 256 pixels (shape TW x 256/TW) of
-// any image of the batch from the XCD-partitioned work ranges; one thread per
-// pixel of the tile.
-template <int SHAPE>
+// Persistent StatMC: blocks take 16x16 tiles of any image of the batch from
+// the XCD-partitioned work ranges; one thread per pixel of the tile.
 __global__ void __launch_bounds__(256) k_statmc(Batch b) {
-    constexpr int TW = filter_tile_w(SHAPE), TH = filter_tile_h(SHAPE);
-    constexpr int KIND = BK_TILE + SHAPE;
     __shared__ int64_t s_item;
     int cursor = 0;
-    const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+    const int tx = threadIdx.x & (STATMC_TILE - 1), ty = threadIdx.x / STATMC_TILE;
     while (true) {
-        if (threadIdx.x == 0) s_item = batch_take(b, KIND, cursor);
+        if (threadIdx.x == 0) s_item = batch_take(b, BK_TILE, cursor);
         __syncthreads();
         const int64_t item = s_item;
         __syncthreads();
         if (item < 0) break;
         int64_t lt;
-        const ImageDesc& d = b.img[batch_find(b, KIND, item, lt)];
-        const int tiles_x = (d.W + TW - 1) / TW;
-        const int x = int(lt % tiles_x) * TW + tx, y = int(lt / tiles_x) * TH + ty;
+        const ImageDesc& d = b.img[batch_find(b, BK_TILE, item, lt)];
+        const int tiles_x = (d.W + STATMC_TILE - 1) / STATMC_TILE;
+        const int x = int(lt % tiles_x) * STATMC_TILE + tx, y = int(lt / tiles_x) * STATMC_TILE + ty;
         if (x < d.W && y < d.H) statmc_pixel(d, x, y);
     }
 }
@@ -266,27 +262,18 @@ void launch_statmc_prepare(const Batch& b, hipStream_t st) {
     hipLaunchKernelGGL(k_sqrt_var_mean, dim3((np + 255) / 256), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_filter_pack, dim3((np + 255) / 256), dim3(256), 0, st, b);
 }
-template <int SHAPE>
-static void launch_statmc_shape(const Batch& b, hipStream_t st) {
+void launch_statmc_window(const Batch& b, hipStream_t st) {
     static int per_cu = 0, cus = 0;
     if (per_cu <= 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_statmc<SHAPE>), 256,
-                                                         0) != hipSuccess || per_cu <= 0) per_cu = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_statmc), 256, 0) !=
+                hipSuccess || per_cu <= 0) per_cu = 1;
         int dev = 0;
         hipGetDevice(&dev);
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
     }
-    const int64_t blocks = std::min<int64_t>(b.total[BK_TILE + SHAPE], int64_t(per_cu) * cus);
+    const int64_t blocks = std::min<int64_t>(b.total[BK_TILE], int64_t(per_cu) * cus);
     if (blocks <= 0) return;
-    hipLaunchKernelGGL(k_statmc<SHAPE>, dim3(unsigned(blocks)), dim3(256), 0, st, b);
-}
-void launch_statmc_window(const Batch& b, int shape, hipStream_t st) {
-    switch (shape) {
-        case 1: launch_statmc_shape<1>(b, st); break;
-        case 2: launch_statmc_shape<2>(b, st); break;
-        case 3: launch_statmc_shape<3>(b, st); break;
-        default: launch_statmc_shape<0>(b, st); break;
-    }
+    hipLaunchKernelGGL(k_statmc, dim3(unsigned(blocks)), dim3(256), 0, st, b);
 }
 
 // ---------------------------------------------------------------------------

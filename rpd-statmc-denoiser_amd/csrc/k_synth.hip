@@ -69,7 +69,7 @@ __global__ void k_synth(SynthParams sp, Records rec) {
     rec.r0[slot] = make_float4(rgb.x, rgb.y, rgb.z, __uint_as_float(flags));
     rec.r1[slot] = make_float4(nr.x, nr.y, nr.z, depth);
     rec.r2[slot] = make_float4(alb.x, alb.y, alb.z, 0.0f);
-    if (rec.r3) rec.r3[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    rec.r3[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     rec.r4[slot] = make_float4(pu_x, pu_y, d[0], d[1]);
     rec.r5[slot] = make_float4(d[2], d[3], d[4], d[5]);
     rec.r6[slot] = make_float2(d[6], d[7]);

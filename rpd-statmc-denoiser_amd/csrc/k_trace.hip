@@ -512,14 +512,11 @@ __global__ void __launch_bounds__(256) k_fold(Batch b, int statmc) {
         const float4 r0 = rec.r0[off + s];
         const uint32_t fl = __float_as_uint(r0.w);
         if (fl & REC_HIT) {
-            const float4 r1 = rec.r1[off + s], r2 = rec.r2[off + s];
+            const float4 r1 = rec.r1[off + s], r2 = rec.r2[off + s], r3 = rec.r3[off + s];
             ++hits;
             nx += r1.x; ny += r1.y; nz += r1.z;
             ar += r2.x; ag += r2.y; ab += r2.z;
-            if (rec.r3) {  // records of a denoiser job carry no hit position
-                const float4 r3 = rec.r3[off + s];
-                px += r3.x; py += r3.y; pz += r3.z;
-            }
+            px += r3.x; py += r3.y; pz += r3.z;
             dep += r1.w;
         }
         if (!(fl & REC_FINITE)) continue;

@@ -136,12 +136,8 @@ struct ImageDesc {
     FilterConfig fc;
     float conf_n0;
 };
-// Item kinds: pixels, grid nodes, and StatMC tiles of each of the
-// FILTER_SHAPES tile shapes (256 pixels each: 16x16, 32x8, 8x32, 64x4).
-constexpr int FILTER_SHAPES = 4;
-enum : int { BK_PIXEL = 0, BK_NODE = 1, BK_TILE = 2, BK_KINDS = 2 + FILTER_SHAPES };
-__host__ __device__ constexpr int filter_tile_w(int shape) { return shape == 0 ? 16 : shape == 1 ? 32 : shape == 2 ? 8 : 64; }
-__host__ __device__ constexpr int filter_tile_h(int shape) { return 256 / filter_tile_w(shape); }
+enum : int { BK_PIXEL = 0, BK_NODE = 1, BK_TILE = 2, BK_KINDS = 3 };
+constexpr int STATMC_TILE = 16;     // StatMC tiles are 16x16 pixels
 constexpr int BATCH_QUEUES = 8;     // one work range per XCD for persistent kernels
 struct Batch {
     const ImageDesc* img;            // device array, n entries
@@ -151,10 +147,9 @@ struct Batch {
     int64_t total[BK_KINDS];         // host copies of the last prefix entries (launch sizing)
 };
 __host__ __device__ inline int64_t image_items(int kind, int W, int H, GridGeom g) {
-    if (kind == BK_PIXEL) return int64_t(W) * H;
-    if (kind == BK_NODE) return int64_t(g.nx) * g.ny;
-    const int tw = filter_tile_w(kind - BK_TILE), th = filter_tile_h(kind - BK_TILE);
-    return int64_t((W + tw - 1) / tw) * ((H + th - 1) / th);
+    return kind == BK_PIXEL ? int64_t(W) * H
+         : kind == BK_NODE  ? int64_t(g.nx) * g.ny
+                            : int64_t((W + STATMC_TILE - 1) / STATMC_TILE) * ((H + STATMC_TILE - 1) / STATMC_TILE);
 }
 // image of global item `item` of `kind` and its local index
 __device__ __forceinline__ int batch_find(const Batch& b, int kind, int64_t item, int64_t& local) {
@@ -209,7 +204,7 @@ void launch_sensitivity_zero(GridGeom g, int width, int height, float* tiles, fl
 // StatMC in two steps: per-pixel packing of the filter inputs into the
 // images' filter scratch, then the window kernel reading them.
 void launch_statmc_prepare(const Batch& b, hipStream_t st);
-void launch_statmc_window(const Batch& b, int shape, hipStream_t st);  // shape: filter_tile_w/h index
+void launch_statmc_window(const Batch& b, hipStream_t st);
 // synthetic per-sample records (denoise-only workload), slot = pixel * spp + sample
 struct SynthParams {
     uint32_t seed;

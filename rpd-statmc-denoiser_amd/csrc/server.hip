@@ -1,0 +1,226 @@
+// Serving loop: requests queue as they arrive; a scheduler thread admits them
+// (FIFO with a short look-ahead, so a small job can pass a large one that
+// does not fit yet) whenever a slot is free and the job's device footprint
+// fits the memory budget, starts each admitted job on its own stream, and
+// records completion. Built on the public C ABI only.
+#include "../../include/rpd_engine.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Request {
+    uint64_t id = 0;
+    rpd_scene* scene = nullptr;
+    rpd_render_config cfg{};
+    rpd_camera cam{};
+    uint32_t seed = 0;
+    uint64_t bytes = 0;
+    Clock::time_point t_submit, t_admit, t_done;
+    rpd_job* job = nullptr;
+    int status = 0;
+    enum State { QUEUED, RUNNING, DONE, CLAIMED } state = QUEUED;
+};
+
+constexpr int kLookAhead = 8;
+
+double percentile(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double pos = q * double(v.size() - 1);
+    const size_t lo = size_t(pos), hi = std::min(lo + 1, v.size() - 1);
+    return v[lo] + (v[hi] - v[lo]) * (pos - double(lo));
+}
+
+}  // namespace
+
+struct rpd_server {
+    rpd_context* ctx = nullptr;
+    rpd_server_config cfg{};
+    uint64_t budget = 0;
+    std::mutex mu;
+    std::condition_variable cv_sched, cv_done;
+    std::map<uint64_t, std::unique_ptr<Request>> all;
+    std::deque<Request*> queue;
+    std::vector<Request*> running;
+    uint64_t next_id = 1, in_use = 0, peak = 0;
+    uint64_t submitted = 0, completed = 0, failed = 0;
+    int max_seen = 0;
+    std::vector<double> lat, qwait;
+    bool stop = false;
+    std::thread sched;
+
+    void loop() {
+        std::unique_lock<std::mutex> lk(mu);
+        while (true) {
+            // completions
+            for (size_t k = 0; k < running.size();) {
+                Request* r = running[k];
+                int done = 0;
+                rpd_job_poll(r->job, &done);
+                if (!done) { ++k; continue; }
+                r->status = rpd_job_wait(r->job);  // joins the job's worker; already finished
+                r->t_done = Clock::now();
+                r->state = Request::DONE;
+                in_use -= r->bytes;
+                lat.push_back(ms_between(r->t_submit, r->t_done));
+                if (r->status) ++failed; else ++completed;
+                running.erase(running.begin() + long(k));
+                cv_done.notify_all();
+            }
+            // admission
+            int looked = 0;
+            for (auto it = queue.begin(); it != queue.end() && looked < kLookAhead &&
+                                          int(running.size()) < cfg.max_in_flight;) {
+                Request* r = *it;
+                ++looked;
+                if (in_use + r->bytes > budget && !running.empty()) { ++it; continue; }
+                it = queue.erase(it);
+                in_use += r->bytes;
+                r->t_admit = Clock::now();
+                qwait.push_back(ms_between(r->t_submit, r->t_admit));
+                lk.unlock();
+                const int st = rpd_job_submit(ctx, r->scene, &r->cfg, &r->cam, r->seed, &r->job);
+                lk.lock();
+                if (st) {
+                    in_use -= r->bytes;
+                    r->status = st;
+                    r->job = nullptr;
+                    r->t_done = Clock::now();
+                    r->state = Request::DONE;
+                    ++failed;
+                    cv_done.notify_all();
+                } else {
+                    r->state = Request::RUNNING;
+                    running.push_back(r);
+                    max_seen = std::max(max_seen, int(running.size()));
+                    peak = std::max(peak, in_use);
+                }
+                it = queue.begin();  // the queue changed while unlocked
+                looked = 0;
+            }
+            if (stop && queue.empty() && running.empty()) break;
+            cv_sched.wait_for(lk, std::chrono::microseconds(running.empty() ? 2000 : 100));
+        }
+    }
+};
+
+extern "C" {
+
+int rpd_server_create(rpd_context* ctx, const rpd_server_config* cfg, rpd_server** out) {
+    if (!ctx || !cfg || !out) return RPD_ERR_INVALID_ARG;
+    if (cfg->max_in_flight < 1) return RPD_ERR_INVALID_ARG;
+    auto s = std::make_unique<rpd_server>();
+    s->ctx = ctx;
+    s->cfg = *cfg;
+    s->budget = cfg->memory_budget;
+    if (!s->budget) {
+        uint64_t fr = 0, tot = 0;
+        const int st = rpd_ctx_memory(ctx, nullptr, nullptr, &fr, &tot);
+        if (st) return st;
+        s->budget = fr;
+    }
+    int st = rpd_ctx_set_memory_budget(ctx, 0);  // admission is the server's; the context does not refuse
+    if (!st) st = rpd_ctx_set_graph_passes(ctx, cfg->graph_passes);
+    if (st) return st;
+    rpd_server* raw = s.get();
+    s->sched = std::thread([raw]() { raw->loop(); });
+    *out = s.release();
+    return RPD_OK;
+}
+
+int rpd_server_submit(rpd_server* s, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam,
+                      uint32_t seed, uint64_t* request_id) {
+    if (!s || !scene || !cfg || !cam || !request_id) return RPD_ERR_INVALID_ARG;
+    uint64_t bytes = 0;
+    const int st = rpd_job_bytes(cfg, &bytes);  // validates the configuration too
+    if (st) return st;
+    if (bytes > s->budget) return RPD_ERR_OUT_OF_MEMORY;  // could never be admitted
+    auto r = std::make_unique<Request>();
+    r->scene = scene; r->cfg = *cfg; r->cam = *cam; r->seed = seed; r->bytes = bytes;
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) return RPD_ERR_STATE;
+        r->id = s->next_id++;
+        *request_id = r->id;
+        s->queue.push_back(r.get());
+        s->all[r->id] = std::move(r);
+        ++s->submitted;
+    }
+    s->cv_sched.notify_one();
+    return RPD_OK;
+}
+
+int rpd_server_wait(rpd_server* s, uint64_t request_id, rpd_job** job, double* latency_ms) {
+    if (!s) return RPD_ERR_INVALID_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(request_id);
+    if (it == s->all.end()) return RPD_ERR_INVALID_ARG;
+    Request* r = it->second.get();
+    if (r->state == Request::CLAIMED) return RPD_ERR_STATE;
+    s->cv_done.wait(lk, [r]() { return r->state == Request::DONE; });
+    if (latency_ms) *latency_ms = ms_between(r->t_submit, r->t_done);
+    const int status = r->status;
+    if (job) {
+        *job = r->job;
+        r->job = nullptr;
+    } else if (r->job) {
+        rpd_job_release(r->job);
+        r->job = nullptr;
+    }
+    r->state = Request::CLAIMED;
+    return status;
+}
+
+int rpd_server_stats_get(rpd_server* s, rpd_server_stats* out) {
+    if (!s || !out) return RPD_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->failed = s->failed;
+    out->max_in_flight_seen = s->max_seen;
+    out->peak_bytes = s->peak;
+    out->latency_p50_ms = percentile(s->lat, 0.50);
+    out->latency_p99_ms = percentile(s->lat, 0.99);
+    double sum = 0.0, mx = 0.0;
+    for (double v : s->lat) { sum += v; mx = std::max(mx, v); }
+    out->latency_mean_ms = s->lat.empty() ? 0.0 : sum / double(s->lat.size());
+    out->latency_max_ms = mx;
+    out->queue_p50_ms = percentile(s->qwait, 0.50);
+    out->queue_p99_ms = percentile(s->qwait, 0.99);
+    return RPD_OK;
+}
+
+int rpd_server_destroy(rpd_server* s) {
+    if (!s) return RPD_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_sched.notify_all();
+    if (s->sched.joinable()) s->sched.join();
+    for (auto& kv : s->all)
+        if (kv.second->job) rpd_job_release(kv.second->job);
+    delete s;
+    return RPD_OK;
+}
+
+}  // extern "C"

@@ -57,6 +57,17 @@ class SynthConfig(C.Structure):
                 ("firefly_scale", C.c_float), ("correlation", C.c_float)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("max_in_flight", C.c_int), ("memory_budget", C.c_uint64), ("graph_passes", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("max_in_flight_seen", C.c_int), ("peak_bytes", C.c_uint64),
+                ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double), ("latency_mean_ms", C.c_double),
+                ("latency_max_ms", C.c_double), ("queue_p50_ms", C.c_double), ("queue_p99_ms", C.c_double)]
+
+
 # rpd_sample_record as a numpy structured type (100 bytes)
 SAMPLE_RECORD = np.dtype([
     ("radiance", np.float32, 3), ("flags", np.uint32), ("normal", np.float32, 3), ("depth", np.float32),
@@ -139,10 +150,15 @@ def lib():
             ("rpd_job_synthesize_records", [vp, C.POINTER(SynthConfig)]),
             ("rpd_job_denoise_records", [vp]),
             ("rpd_denoise_batch", [vp, vp, C.c_int, C.POINTER(C.c_double)]),
-            ("rpd_job_reset", [vp]),
-            ("rpd_ctx_autotune_filter", [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]),
-            ("rpd_ctx_filter_tuning", [vp, C.c_int, vp, vp, vp, vp]),
-            ("rpd_ctx_set_filter_shape", [vp, C.c_int, C.c_int]),
+            ("rpd_ctx_set_graph_passes", [vp, C.c_int]),
+            ("rpd_job_bytes", [C.POINTER(RenderConfig), C.POINTER(C.c_uint64)]),
+            ("rpd_ctx_memory", [vp, vp, vp, vp, vp]),
+            ("rpd_server_create", [vp, C.POINTER(ServerConfig), C.POINTER(vp)]),
+            ("rpd_server_submit", [vp, vp, C.POINTER(RenderConfig), C.POINTER(Camera), C.c_uint32,
+                                   C.POINTER(C.c_uint64)]),
+            ("rpd_server_wait", [vp, C.c_uint64, C.POINTER(vp), C.POINTER(C.c_double)]),
+            ("rpd_server_stats_get", [vp, C.POINTER(ServerStats)]),
+            ("rpd_server_destroy", [vp]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -196,20 +212,13 @@ class Context:
             self._scenes[name] = s
         return self._scenes[name]
 
-    def autotune_filter(self, radii, width=512, height=512, spp=16):
-        """Time every StatMC tile shape per radius; the context keeps the fastest."""
-        r = np.ascontiguousarray(radii, np.int32)
-        check(self.L.rpd_ctx_autotune_filter(self.h, _ptr(r), r.size, width, height, spp))
-        return {int(k): self.filter_tuning(int(k)) for k in r}
+    def set_graph_passes(self, on):
+        check(self.L.rpd_ctx_set_graph_passes(self.h, int(on)))
 
-    def filter_tuning(self, radius):
-        shape, tw, th = C.c_int(), C.c_int(), C.c_int()
-        ms = np.zeros(4, np.float32)
-        check(self.L.rpd_ctx_filter_tuning(self.h, radius, C.byref(shape), C.byref(tw), C.byref(th), _ptr(ms)))
-        return {"shape": shape.value, "tile": [tw.value, th.value], "ms_per_shape": [float(x) for x in ms]}
-
-    def set_filter_shape(self, radius, shape):
-        check(self.L.rpd_ctx_set_filter_shape(self.h, radius, shape))
+    def memory(self):
+        v = [C.c_uint64() for _ in range(4)]
+        check(self.L.rpd_ctx_memory(self.h, *[C.byref(x) for x in v]))
+        return dict(in_use=v[0].value, budget=v[1].value, device_free=v[2].value, device_total=v[3].value)
 
     def set_memory_budget(self, nbytes):
         check(self.L.rpd_ctx_set_memory_budget(self.h, int(nbytes)))
@@ -239,6 +248,47 @@ def denoise_batch(ctx, jobs):
     return ms.value
 
 
+def job_bytes(cfg):
+    b = C.c_uint64()
+    check(lib().rpd_job_bytes(C.byref(cfg), C.byref(b)))
+    return b.value
+
+
+class Server:
+    """Request loop: continuous admission by slots and HBM footprint (rpd_server_*)."""
+
+    def __init__(self, ctx, max_in_flight=4, memory_budget=0, graph_passes=True):
+        self.ctx = ctx
+        self.L = ctx.L
+        c = ServerConfig(max_in_flight, int(memory_budget), int(graph_passes))
+        h = C.c_void_p()
+        check(self.L.rpd_server_create(ctx.h, C.byref(c), C.byref(h)))
+        self.h = h
+
+    def submit(self, cfg, cam, seed, scene="cornell_box"):
+        rid = C.c_uint64()
+        check(self.L.rpd_server_submit(self.h, self.ctx.scene(scene), C.byref(cfg), C.byref(cam),
+                                       seed & 0xFFFFFFFF, C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid):
+        """Returns (Job, latency ms); the Job owns the finished job."""
+        jh = C.c_void_p()
+        ms = C.c_double()
+        check(self.L.rpd_server_wait(self.h, rid, C.byref(jh), C.byref(ms)))
+        return Job.adopt(self.ctx, jh), ms.value
+
+    def stats(self):
+        st = ServerStats()
+        check(self.L.rpd_server_stats_get(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            self.L.rpd_server_destroy(self.h)
+            self.h = None
+
+
 class Job:
     """One render job's device state; stage methods mirror the reference's functions.
     denoiser_spp > 0 creates a job that denoises given records instead (no scene)."""
@@ -257,6 +307,22 @@ class Job:
         d = np.zeros(4, np.int32)
         check(self.L.rpd_job_grid_dims(self.h, _ptr(d)))
         self.nodes_x, self.nodes_y, self.step, self.support = (int(v) for v in d)
+
+    @classmethod
+    def adopt(cls, ctx, handle):
+        """Wraps a job handle the engine handed over (e.g. by rpd_server_wait)."""
+        j = cls.__new__(cls)
+        j.L = ctx.L
+        j.h = handle
+        d = np.zeros(4, np.int32)
+        check(j.L.rpd_job_grid_dims(j.h, _ptr(d)))
+        j.nodes_x, j.nodes_y, j.step, j.support = (int(v) for v in d)
+        n = C.c_size_t()
+        check(j.L.rpd_job_buffer_size(j.h, BUF["depth"], C.byref(n)))
+        j.cfg = None
+        j.w, j.h_px = None, None
+        j.n_px = n.value // 4
+        return j
 
     def release(self):
         if self.h:
@@ -297,8 +363,6 @@ class Job:
     def denoise_records(self):
         check(self.L.rpd_job_denoise_records(self.h))
 
-    def reset(self):
-        check(self.L.rpd_job_reset(self.h))
 
     def run_adaptive_pass(self):
         check(self.L.rpd_job_run_adaptive_pass(self.h))

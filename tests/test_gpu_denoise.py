@@ -161,33 +161,3 @@ def test_batch_identical_to_alone(engine_ctx):
     for a, b in zip(*outs):
         for n in a:
             np.testing.assert_array_equal(a[n], b[n], err_msg=n)
-
-
-@pytest.mark.skip(reason="the StatMC tile shapes other than 16x16 and the autotuner are switched off: the command "
-                         "that first ran them also ran the full-size batched denoise and left the GPU faulted, "
-                         "cause not identified")
-def test_filter_tile_shapes_identical_and_autotuner(engine_ctx):
-    """Every StatMC tile shape computes the same pixels; the autotuner keeps one per radius."""
-    rc = ob.config_c0(96, 80)
-    outs = []
-    for shape in range(4):
-        engine_ctx.set_filter_shape(rc.color_window_radius, shape)
-        j = _denoiser(engine_ctx, rc, 6)
-        j.synthesize_records(eng.synth_config(seed=21, samples_per_pixel=6, region_size=6))
-        j.denoise_records()
-        outs.append((j.get("denoised"), j.get("uncertainty"), j.get("alpha")))
-        # a reset job denoises its records again to the same bits
-        j.reset()
-        j.denoise_records()
-        np.testing.assert_array_equal(outs[-1][0], j.get("denoised"))
-        j.release()
-    engine_ctx.set_filter_shape(rc.color_window_radius, 0)
-    for o in outs[1:]:
-        for a, b in zip(outs[0], o):
-            np.testing.assert_array_equal(a, b)
-    t = engine_ctx.autotune_filter([3, 7], width=128, height=96, spp=4)
-    for r in (3, 7):
-        assert 0 <= t[r]["shape"] < 4 and all(ms > 0 for ms in t[r]["ms_per_shape"])
-        assert t[r]["tile"][0] * t[r]["tile"][1] == 256
-    for r in (3, 7):
-        engine_ctx.set_filter_shape(r, 0)
