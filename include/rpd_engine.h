@@ -89,6 +89,10 @@ typedef struct rpd_render_config {
     float adaptive_sigma_max;
     int adaptive_passes, adaptive_importance_smoothing_radius;
     float rpf_shrinkage_scale, rpf_confidence_samples;
+    /* Extension (no counterpart in the reference, whose paths end only by
+       Russian roulette): at most max_depth shading points per path; the last
+       one draws its BRDF sample but traces no BRDF ray. 0 = unbounded. */
+    int max_depth;
 } rpd_render_config;
 
 /* Fills `cfg` with the reference defaults (core/config.h:27-48). */
@@ -256,6 +260,44 @@ int rpd_job_denoise_records(rpd_job* job);
    StatMC window). Results are identical to denoising each job alone.
    gpu_ms (optional) receives the device time of the whole batch. */
 int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu_ms);
+
+/* ---- procedural benchmark scene (no counterpart in the reference) ---------
+   An icosphere subdivided subdivision_level times (20 * 4^level flat
+   triangles; level 8 gives 1,310,720) with its vertices displaced along the
+   radius by noise_octaves octaves of value noise, n_spheres spheres with GGX
+   microfacet materials (roughness uniform in [0.05, 0.8], albedo uniform in
+   [0.1, 0.9]) in a shell around it, four rectangular area lights facing it and
+   a constant environment. The same configuration always gives the same bits. */
+typedef struct rpd_procedural_config {
+    uint32_t seed;
+    int subdivision_level;   /* 0..9 */
+    int n_spheres;
+    float displacement;      /* radial displacement amplitude (unit sphere) */
+    float noise_frequency;
+    int noise_octaves;
+    float environment[3];    /* constant environment radiance */
+    float light_emission;
+} rpd_procedural_config;
+void rpd_procedural_config_defaults(rpd_procedural_config* c); /* seed 7, level 8, 256 spheres */
+/* The scene as arrays for rpd_scene_create. Call with null arrays to get the
+   counts; then with arrays of at least those sizes. */
+int rpd_procedural_scene_arrays(const rpd_procedural_config* c, rpd_triangle* tris, int* n_tris,
+                                rpd_sphere* spheres, int* n_spheres, rpd_material* mats, int* n_mats,
+                                rpd_light* lights, int* n_lights, float env[3]);
+/* ctx == NULL (here and in rpd_scene_create) keeps the scene on the host, for
+   rpd_scene_query only. */
+int rpd_scene_procedural(rpd_context* ctx, const rpd_procedural_config* c, rpd_scene** out);
+
+/* Ray queries against a scene on the host, with the same intersection code
+   the kernels run: use_bvh != 0 walks the scene's BVH (scenes that have one),
+   0 loops over every primitive. occlusion[i] != 0 asks whether anything lies
+   before t_max[i] (t_max already reduced by 5e-4 as in any_hit); otherwise the
+   closest hit below t_max[i]. Closest-hit and any-hit semantics are those of
+   core/ray_tracer.h:233-344. For checking the BVH against the exhaustive loop. */
+int rpd_scene_query(const rpd_scene* s, int n, const float* origins, const float* dirs, const float* t_max,
+                    const int* occlusion, int use_bvh, int* hit, float* t, int* index, int* is_sphere);
+/* Nodes and 4-wide depth of the scene's BVH (0, 0 for scenes looped over). */
+int rpd_scene_bvh_info(const rpd_scene* s, int* nodes, int* depth);
 
 /* Capture each job's adaptive-pass sequence (variance smoothing, allocation,
    sampling pass) as a hipGraph on its first pass and replay it for every

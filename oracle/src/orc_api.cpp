@@ -22,6 +22,7 @@ struct orc_config {
     float adaptive_sigma_max;
     int adaptive_passes, adaptive_importance_smoothing_radius;
     float rpf_shrinkage_scale, rpf_confidence_samples;
+    int max_depth;
 };
 
 void* orc_job_create(const orc_config* c, uint32_t seed, int nthreads) {
@@ -42,6 +43,7 @@ void* orc_job_create(const orc_config* c, uint32_t seed, int nthreads) {
     k.adaptive_sigma_max = c->adaptive_sigma_max; k.adaptive_passes = c->adaptive_passes;
     k.adaptive_importance_smoothing_radius = c->adaptive_importance_smoothing_radius;
     k.rpf_shrinkage_scale = c->rpf_shrinkage_scale; k.rpf_confidence_samples = c->rpf_confidence_samples;
+    k.max_depth = c->max_depth;
     j->seed = seed;
     j->nthreads = nthreads;
     j->w = k.width; j->h = k.height;

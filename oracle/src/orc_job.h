@@ -28,6 +28,8 @@ struct Config {
     float adaptive_sigma_max = 3.0f;
     int adaptive_passes = 1, adaptive_importance_smoothing_radius = 1;
     float rpf_shrinkage_scale = 1.0f, rpf_confidence_samples = 128.0f;
+    // extension of the engine (not in the reference): shading points per path, 0 = unbounded
+    int max_depth = 0;
 };
 
 // core/camera.h:9-88

@@ -87,9 +87,10 @@ struct Tracer {
     const Scene& sc;
     Sampler& smp;
     RpfSample* rpf;
+    int max_depth = 0;  // engine extension: shading points per path, 0 = unbounded
 
     // main.cpp:394-533
-    F3 shade(int is_sphere, uint32_t index, F3 p, F3 n, F3 wo) {
+    F3 shade(int is_sphere, uint32_t index, F3 p, F3 n, F3 wo, int depth = 1) {
         const int mid = is_sphere ? sc.spheres[index].material_id : sc.triangles[index].material_id;
         const Material& m = sc.materials[mid];
         const bool delta_brdf = brdf_is_delta(m);
@@ -145,7 +146,7 @@ struct Tracer {
         F3 dir; float pdf;
         brdf_sample(m, wo, n, bu, dir, pdf);
         F3 L_ind{0, 0, 0};
-        if (pdf > ORC_EPS_SMALL && sqnorm(dir) > 0.0f) {
+        if (pdf > ORC_EPS_SMALL && sqnorm(dir) > 0.0f && (max_depth <= 0 || depth < max_depth)) {
             const Ray br{offset_origin(p, n), dir};
             const Hit h = closest_hit(sc, br);
             bool emitter = false; int ei = -1;
@@ -172,7 +173,7 @@ struct Tracer {
                 if (ksi < 0.8f) {
                     const F3 np = ray_at(br, h.t);
                     const F3 nn = h.is_sphere ? h.sphere_normal : sc.triangles[h.index].normal;
-                    const F3 R = shade(h.is_sphere, h.index, np, nn, -dir);
+                    const F3 R = shade(h.is_sphere, h.index, np, nn, -dir, depth + 1);
                     L_ind = mul(R, brdf_sample_weight(m, wo, dir, n, pdf)) / 0.8f;
                 }
             }
@@ -277,7 +278,7 @@ void render_pass(Job& j, const int* counts, bool statmc, bool seed_with_n) {
                     const Ray ray = j.cam.generate(u, 1.0f - v, ls);
                     SampleRecord rec;
                     rec.pixel = idx;
-                    Tracer tr{j.scene, smp, capture ? &rs : nullptr};
+                    Tracer tr{j.scene, smp, capture ? &rs : nullptr, j.cfg.max_depth};
                     const F3 L = tr.trace(ray, rec);
                     if (rec.hit) {
                         j.hit_count[idx]++;

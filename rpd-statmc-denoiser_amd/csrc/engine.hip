@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/rpd_engine.h"
+#include "bvh_build.h"
 #include "kernels.h"
 
 using namespace rpd;
@@ -89,6 +90,11 @@ struct rpd_scene {
     DevScene dev{};
     int n_tris = 0, n_spheres = 0, n_materials = 0, n_lights = 0;
     int trace_per_cu = 0;  // resident trace blocks per CU, set on first use
+    // host copies for host-side queries (rpd_scene_query)
+    std::vector<DevTriangle> h_tris;
+    std::vector<DevSphere> h_spheres;
+    std::vector<float4> h_perm;
+    BvhHost h_bvh;
 };
 
 struct rpd_job {
@@ -201,6 +207,7 @@ int validate(const rpd_render_config& c) {
     if (c.var_iterations <= 0) return bad("var_iterations must be positive");
     if (!std::isfinite(c.rpf_shrinkage_scale) || c.rpf_shrinkage_scale < 0.0f) return bad("rpf_shrinkage_scale must be finite and non-negative");
     if (!std::isfinite(c.rpf_confidence_samples) || c.rpf_confidence_samples <= 0.0f) return bad("rpf_confidence_samples must be finite and positive");
+    if (c.max_depth < 0) return bad("max_depth must be non-negative");
     return RPD_OK;
 }
 
@@ -331,6 +338,7 @@ TraceParams trace_params(rpd_job* j, int statmc) {
     p.seed_n = statmc ? j->pb.n : nullptr;
     p.base_seed = j->seed;
     p.tape = j->tape;
+    p.max_depth = j->cfg.max_depth;
     p.work_counter = j->work;
     p.query_count = j->query_count;
     p.rec = j->rec;
@@ -338,7 +346,7 @@ TraceParams trace_params(rpd_job* j, int statmc) {
 }
 
 int trace_blocks(rpd_job* j) {
-    if (j->scene->trace_per_cu <= 0) j->scene->trace_per_cu = trace_blocks_per_cu(j->scene->n_tris);
+    if (j->scene->trace_per_cu <= 0) j->scene->trace_per_cu = trace_blocks_per_cu(j->scene->dev);
     const int per_cu = j->scene->trace_per_cu;
     const int by_work = (j->N + trace_block_size() - 1) / trace_block_size();
     return std::max(1, std::min(by_work, per_cu * std::max(1, j->ctx->num_cus)));
@@ -658,6 +666,7 @@ int job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg,
                rpd_job** out) {
     if (!ctx || !scene || !cfg || !cam || !out) return fail(RPD_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
+    if (scene->ctx != ctx) return fail(RPD_ERR_INVALID_ARG, "scene belongs to another context or to none (host-only)");
     int st = validate(*cfg);
     if (st) return st;
     HIP_TRY(hipSetDevice(ctx->device));
@@ -821,12 +830,12 @@ int rpd_ctx_set_memory_budget(rpd_context* ctx, uint64_t bytes) {
 int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, const rpd_sphere* spheres, int n_spheres,
                      const rpd_material* mats, int n_mats, const rpd_light* lights, int n_lights, const float env[3],
                      rpd_scene** out) {
-    if (!ctx || !out || n_tris < 0 || n_spheres < 0 || n_mats <= 0 || n_lights < 0 || !mats)
+    if (!out || n_tris < 0 || n_spheres < 0 || n_mats <= 0 || n_lights < 0 || !mats)
         return fail(RPD_ERR_INVALID_ARG, "invalid scene arguments");
     if ((n_tris && !tris) || (n_spheres && !spheres) || (n_lights && !lights))
         return fail(RPD_ERR_INVALID_ARG, "null primitive array");
     *out = nullptr;
-    HIP_TRY(hipSetDevice(ctx->device));
+    if (ctx) HIP_TRY(hipSetDevice(ctx->device));
     std::vector<DevTriangle> dt(n_tris);
     for (int i = 0; i < n_tris; ++i) {
         const rpd_triangle& t = tris[i];
@@ -904,26 +913,50 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
             for (int q = 0; q < 3; ++q) e[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
         }
     }
-    DevTriangle* pt; DevSphere* ps; DevMaterial* pm; DevLight* pl; float* pc; float4* pp;
+    // scenes too large for the flat LDS loop get a BVH over triangles and spheres
+    BvhHost bvh;
+    const bool use_bvh = trace_needs_bvh(n_tris, n_spheres);
+    if (use_bvh) {
+        const int bst = build_bvh4(dt.data(), n_tris, ds.data(), n_spheres, bvh);
+        if (bst == -2) return fail(RPD_ERR_UNSUPPORTED, "BVH deeper than the traversal stack allows");
+        if (bst) return fail(RPD_ERR_INVALID_ARG, "scene has no primitives");
+    }
+    DevTriangle* pt = nullptr; DevSphere* ps = nullptr; DevMaterial* pm = nullptr; DevLight* pl = nullptr;
+    float* pc = nullptr; float4* pp = nullptr;
+    Bvh4Node* pn = nullptr; float4* pslot = nullptr; int* pprim = nullptr;
     int st;
-    if ((st = dalloc(&pt, dt.size(), sc->owned)) || (st = dalloc(&pp, perm.size(), sc->owned)) ||
-        (st = dalloc(&ps, ds.size(), sc->owned)) ||
-        (st = dalloc(&pm, dm.size(), sc->owned)) || (st = dalloc(&pl, dl.size(), sc->owned)) ||
-        (st = dalloc(&pc, cdf.size(), sc->owned))) {
-        for (void* p : sc->owned) hipFree(p);
-        return st;
-    }
-    if (n_tris) {
-        HIP_TRY(hipMemcpy(pt, dt.data(), dt.size() * sizeof(DevTriangle), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(pp, perm.data(), perm.size() * sizeof(float4), hipMemcpyHostToDevice));
-    }
-    if (n_spheres) HIP_TRY(hipMemcpy(ps, ds.data(), ds.size() * sizeof(DevSphere), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(pm, dm.data(), dm.size() * sizeof(DevMaterial), hipMemcpyHostToDevice));
-    if (n_lights) {
-        HIP_TRY(hipMemcpy(pl, dl.data(), dl.size() * sizeof(DevLight), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(pc, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (ctx) {  // a null context keeps the scene on the host (rpd_scene_query only)
+        if ((st = dalloc(&pt, dt.size(), sc->owned)) || (st = dalloc(&pp, perm.size(), sc->owned)) ||
+            (st = dalloc(&ps, ds.size(), sc->owned)) ||
+            (st = dalloc(&pm, dm.size(), sc->owned)) || (st = dalloc(&pl, dl.size(), sc->owned)) ||
+            (st = dalloc(&pc, cdf.size(), sc->owned))) {
+            for (void* p : sc->owned) hipFree(p);
+            return st;
+        }
+        if (n_tris) {
+            HIP_TRY(hipMemcpy(pt, dt.data(), dt.size() * sizeof(DevTriangle), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(pp, perm.data(), perm.size() * sizeof(float4), hipMemcpyHostToDevice));
+        }
+        if (use_bvh) {
+            if ((st = dalloc(&pn, bvh.nodes.size(), sc->owned)) || (st = dalloc(&pslot, bvh.slots.size(), sc->owned)) ||
+                (st = dalloc(&pprim, bvh.slot_prim.size(), sc->owned))) {
+                for (void* p : sc->owned) hipFree(p);
+                return st;
+            }
+            HIP_TRY(hipMemcpy(pn, bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(pslot, bvh.slots.data(), bvh.slots.size() * sizeof(float4), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(pprim, bvh.slot_prim.data(), bvh.slot_prim.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        if (n_spheres) HIP_TRY(hipMemcpy(ps, ds.data(), ds.size() * sizeof(DevSphere), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(pm, dm.data(), dm.size() * sizeof(DevMaterial), hipMemcpyHostToDevice));
+        if (n_lights) {
+            HIP_TRY(hipMemcpy(pl, dl.data(), dl.size() * sizeof(DevLight), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(pc, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
     }
     sc->dev.tris = pt; sc->dev.n_tris = n_tris; sc->dev.tri_perm = pp;
+    sc->dev.bvh = pn; sc->dev.slots = pslot; sc->dev.slot_prim = pprim;
+    sc->h_tris = dt; sc->h_spheres = ds; sc->h_perm = std::move(perm); sc->h_bvh = std::move(bvh);
     sc->dev.spheres = ps; sc->dev.n_spheres = n_spheres;
     sc->dev.materials = pm;
     sc->dev.lights = pl; sc->dev.n_lights = n_lights;
@@ -943,6 +976,36 @@ int rpd_scene_builtin(rpd_context* ctx, const char* name, rpd_scene** out) {
     build_cornell(s);
     return rpd_scene_create(ctx, s.tris.data(), int(s.tris.size()), nullptr, 0, s.mats.data(), int(s.mats.size()),
                             s.lights.data(), int(s.lights.size()), s.env, out);
+}
+
+int rpd_scene_query(const rpd_scene* s, int n, const float* origins, const float* dirs, const float* t_max,
+                    const int* occlusion, int use_bvh, int* hit, float* t, int* index, int* is_sphere) {
+    if (!s || n < 0 || (n > 0 && (!origins || !dirs || !t_max || !occlusion || !hit || !t || !index || !is_sphere)))
+        return fail(RPD_ERR_INVALID_ARG, "null argument");
+    if (use_bvh && s->h_bvh.nodes.empty()) return fail(RPD_ERR_STATE, "scene has no BVH (small scenes are looped over)");
+    DevScene hs = s->dev;
+    hs.tris = s->h_tris.data();
+    hs.spheres = s->h_spheres.data();
+    hs.tri_perm = s->h_perm.data();
+    hs.bvh = s->h_bvh.nodes.empty() ? nullptr : s->h_bvh.nodes.data();
+    hs.slots = s->h_bvh.slots.empty() ? nullptr : s->h_bvh.slots.data();
+    hs.slot_prim = s->h_bvh.slot_prim.empty() ? nullptr : s->h_bvh.slot_prim.data();
+    int stack[BVH_STACK];
+    for (int i = 0; i < n; ++i) {
+        const v3 o = mk3(origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]);
+        const v3 d = mk3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
+        const HitRec h = use_bvh ? bvh_query(hs, o, d, t_max[i], occlusion[i] != 0, stack, 1)
+                                 : trace_query(hs, hs.tri_perm, o, d, t_max[i], occlusion[i] != 0);
+        hit[i] = h.hit; t[i] = h.t; index[i] = h.index; is_sphere[i] = h.sphere;
+    }
+    return RPD_OK;
+}
+
+int rpd_scene_bvh_info(const rpd_scene* s, int* nodes, int* depth) {
+    if (!s) return fail(RPD_ERR_INVALID_ARG, "null scene");
+    if (nodes) *nodes = int(s->h_bvh.nodes.size());
+    if (depth) *depth = s->h_bvh.depth;
+    return RPD_OK;
 }
 
 int rpd_scene_counts(const rpd_scene* s, int* nt, int* ns, int* nm, int* nl) {

@@ -6,6 +6,8 @@
 // core/stats.h:25-41 (Welford fold).
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace rpd {
 
 // ---------------------------------------------------------------------------
@@ -83,18 +85,23 @@ constexpr int TRACE_CHUNK = 64;
 __device__ __forceinline__ uint64_t ballot64(bool b) { return __ballot(b); }
 __device__ __forceinline__ bool is_zero3(v3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
 
-// Triangle tables up to this many triangles are staged in LDS.
+// Triangle tables up to this many triangles are staged in LDS and looped over
+// (with up to TRACE_FLAT_SPHERES spheres); larger scenes go through the BVH.
 constexpr int TRACE_LDS_TRIS = 256;
+constexpr int TRACE_FLAT_SPHERES = 32;
 
-template <bool kLds>
+// kBvh: the scene's BVH with a per-lane traversal stack in LDS; otherwise the
+// triangle table staged in LDS and looped over.
+template <bool kBvh>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
     const DevScene& S = p.scene;
-    extern __shared__ float4 s_tri[];
-    if (kLds) {
-        for (int k = threadIdx.x; k < S.n_tris * 9; k += TRACE_BLOCK) s_tri[k] = S.tri_perm[k];
+    extern __shared__ float4 s_dyn[];
+    if (!kBvh) {
+        for (int k = threadIdx.x; k < S.n_tris * 9; k += TRACE_BLOCK) s_dyn[k] = S.tri_perm[k];
         __syncthreads();
     }
-    const float4* tp = kLds ? s_tri : S.tri_perm;
+    const float4* tp = s_dyn;
+    int* stack = reinterpret_cast<int*>(s_dyn) + threadIdx.x;  // entry k at stack[k * TRACE_BLOCK]
     if (p.query_count && blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(p.query_count + 1, (unsigned long long)p.plan.totals[0]);  // samples of this pass
     const int lane = threadIdx.x & 63;
@@ -118,7 +125,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
     v3 sp = mk3(0, 0, 0), sn = mk3(0, 0, 0), swo = mk3(0, 0, 0);  // shading point
     v3 cw = mk3(0, 0, 0);                       // shadow-ray contribution, or BRDF weight of the extension ray
     float bpdf = 0.0f;
-    int smat = 0;
+    int smat = 0, depth = 0;
     uint32_t flags = 0, light_bits = 255u;
     HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
     uint32_t n_queries = 0;
@@ -158,6 +165,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
                         st = ST_DONE;
                     } else {
                         sp = hp; sn = hn; swo = -normalized3(rd); smat = hm;
+                        depth = 1;
                         st = ST_LIGHT;
                     }
                 }
@@ -188,6 +196,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
                         sn = hn;
                         swo = -rd;
                         smat = hm;
+                        ++depth;
                         st = ST_LIGHT;
                     }
                 }
@@ -344,7 +353,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
             }
             v3 dir; float pdf;
             brdf_sample(m, swo, sn, bu, dir, pdf);
-            if (pdf > RPD_EPS_SMALL && sqnorm3(dir) > 0.0f) {
+            if (pdf > RPD_EPS_SMALL && sqnorm3(dir) > 0.0f && (p.max_depth <= 0 || depth < p.max_depth)) {
                 bpdf = pdf;
                 cw = brdf_sample_weight(m, swo, dir, sn, pdf);
                 rd = dir; qt = inf; q = Q_EXT;
@@ -356,31 +365,36 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
 
         // -- one intersection query per lane ----------------------------------
         if (q != Q_NONE) {
-            h = trace_query(S, tp, ro, rd, qt, q >= Q_SHADOW_L);
+            h = kBvh ? bvh_query(S, ro, rd, qt, q >= Q_SHADOW_L, stack, TRACE_BLOCK)
+                     : trace_query(S, tp, ro, rd, qt, q >= Q_SHADOW_L);
             ++n_queries;
         }
     }
 }
 int trace_block_size() { return TRACE_BLOCK; }
 
-static size_t trace_lds_bytes(int n_tris) {
-    return n_tris <= TRACE_LDS_TRIS ? size_t(n_tris) * 9 * sizeof(float4) : 0;
+bool trace_needs_bvh(int n_tris, int n_spheres) { return n_tris > TRACE_LDS_TRIS || n_spheres > TRACE_FLAT_SPHERES; }
+
+static size_t trace_lds_bytes(const DevScene& s) {
+    return s.bvh ? size_t(BVH_STACK) * TRACE_BLOCK * sizeof(int)
+                 : std::max<size_t>(16, size_t(s.n_tris) * 9 * sizeof(float4));
 }
 
-int trace_blocks_per_cu(int n_tris) {
-    const size_t lds = trace_lds_bytes(n_tris);
-    const void* fn = lds ? reinterpret_cast<const void*>(&k_trace<true>) : reinterpret_cast<const void*>(&k_trace<false>);
+int trace_blocks_per_cu(const DevScene& s) {
+    const size_t lds = trace_lds_bytes(s);
+    const void* fn = s.bvh ? reinterpret_cast<const void*>(&k_trace<true>)
+                           : reinterpret_cast<const void*>(&k_trace<false>);
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, TRACE_BLOCK, lds) != hipSuccess || nb <= 0) nb = 1;
     return nb;
 }
 
 void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st) {
-    const size_t lds = trace_lds_bytes(p.scene.n_tris);
-    if (lds)
+    const size_t lds = trace_lds_bytes(p.scene);
+    if (p.scene.bvh)
         hipLaunchKernelGGL(k_trace<true>, dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
     else
-        hipLaunchKernelGGL(k_trace<false>, dim3(n_blocks), dim3(TRACE_BLOCK), 0, st, p);
+        hipLaunchKernelGGL(k_trace<false>, dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
 }
 
 // ---------------------------------------------------------------------------

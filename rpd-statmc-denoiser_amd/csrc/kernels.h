@@ -56,6 +56,7 @@ struct TraceParams {
     const int* seed_n;       // per-pixel sample offset used for seeding (finite samples so far), or null for 0
     uint32_t base_seed;
     uint32_t* tape;          // ceil(n/64)*64 tapes of 624 words, layout of tape_base/tape_word
+    int max_depth;           // shading points per path, 0 = unbounded (extension)
     int* work_counter;       // zero before launch
     unsigned long long* query_count;  // ray queries issued (closest + occlusion), accumulated; may be null
     Records rec;
@@ -184,7 +185,8 @@ __device__ inline int64_t batch_take(const Batch& b, int kind, int& cursor) {
 void launch_tape(const TraceParams& p, hipStream_t st);
 void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st);
 int trace_block_size();
-int trace_blocks_per_cu(int n_tris);  // resident trace blocks per CU for a scene of n_tris triangles
+int trace_blocks_per_cu(const DevScene& s);   // resident trace blocks per CU for this scene
+bool trace_needs_bvh(int n_tris, int n_spheres);  // too large for the flat LDS loop: traced through a BVH
 // per-pixel fold of the current pass's records of every image (Welford statistics when statmc)
 void launch_fold(const Batch& b, int statmc, hipStream_t st);
 void launch_plan_uniform(int n_pixels, int count, int* counts, int* offsets, int* active, int* totals,

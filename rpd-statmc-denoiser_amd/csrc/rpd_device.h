@@ -16,57 +16,61 @@
 
 namespace rpd {
 
+// Plain float helpers used by both the kernels and host-side checks of the
+// intersection code (same flags on both sides: no contraction, IEEE sqrt/div).
+#define RPD_HD __host__ __device__ __forceinline__
+
 struct v2 { float x, y; };
 struct v3 { float x, y, z; };
 
-__device__ __forceinline__ v3 mk3(float a, float b, float c) { v3 r; r.x = a; r.y = b; r.z = c; return r; }
-__device__ __forceinline__ v3 operator+(v3 a, v3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ v3 operator-(v3 a, v3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ v3 operator-(v3 a) { return mk3(-a.x, -a.y, -a.z); }
-__device__ __forceinline__ v3 operator*(v3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ v3 operator*(float s, v3 a) { return mk3(s * a.x, s * a.y, s * a.z); }
-__device__ __forceinline__ v3 operator/(v3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
-__device__ __forceinline__ v3 vmul(v3 a, v3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
-__device__ __forceinline__ v3 vdiv(v3 a, v3 b) { return mk3(a.x / b.x, a.y / b.y, a.z / b.z); }
-__device__ __forceinline__ float dot3(v3 a, v3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }
-__device__ __forceinline__ float sqnorm3(v3 a) { return a.x * a.x + (a.y * a.y + a.z * a.z); }
-__device__ __forceinline__ float norm3(v3 a) { return sqrtf(sqnorm3(a)); }
-__device__ __forceinline__ v3 normalized3(v3 a) {
+RPD_HD v3 mk3(float a, float b, float c) { v3 r; r.x = a; r.y = b; r.z = c; return r; }
+RPD_HD v3 operator+(v3 a, v3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+RPD_HD v3 operator-(v3 a, v3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+RPD_HD v3 operator-(v3 a) { return mk3(-a.x, -a.y, -a.z); }
+RPD_HD v3 operator*(v3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+RPD_HD v3 operator*(float s, v3 a) { return mk3(s * a.x, s * a.y, s * a.z); }
+RPD_HD v3 operator/(v3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
+RPD_HD v3 vmul(v3 a, v3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+RPD_HD v3 vdiv(v3 a, v3 b) { return mk3(a.x / b.x, a.y / b.y, a.z / b.z); }
+RPD_HD float dot3(v3 a, v3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }
+RPD_HD float sqnorm3(v3 a) { return a.x * a.x + (a.y * a.y + a.z * a.z); }
+RPD_HD float norm3(v3 a) { return sqrtf(sqnorm3(a)); }
+RPD_HD v3 normalized3(v3 a) {
     const float z = sqnorm3(a);
     if (z > 0.0f) { const float s = sqrtf(z); return mk3(a.x / s, a.y / s, a.z / s); }
     return a;
 }
-__device__ __forceinline__ v3 cross3(v3 a, v3 b) {
+RPD_HD v3 cross3(v3 a, v3 b) {
     return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-__device__ __forceinline__ float maxc3(v3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
-__device__ __forceinline__ bool finite3(v3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
-__device__ __forceinline__ float comp3(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
-__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
-__device__ __forceinline__ float lerpf_(float a, float b, float t) { return a + (b - a) * t; }
+RPD_HD float maxc3(v3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+RPD_HD bool finite3(v3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+RPD_HD float comp3(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+RPD_HD float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+RPD_HD float lerpf_(float a, float b, float t) { return a + (b - a) * t; }
 // Correctly rounded expf: exp in double, rounded once to float. The
 // reference's std::exp(float) is glibc's expf, which agrees with the correctly
 // rounded value except on roughly one argument in 10^4.
-__device__ __forceinline__ float expf_cr(float x) { return float(exp(double(x))); }
-__device__ __forceinline__ float luminance3(v3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+RPD_HD float expf_cr(float x) { return float(exp(double(x))); }
+RPD_HD float luminance3(v3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
 // core/common.h:14-35: Pixar revised orthonormal basis
-__device__ __forceinline__ void onb(v3 n, v3& bx, v3& by) {
+RPD_HD void onb(v3 n, v3& bx, v3& by) {
     const float sign = copysignf(1.0f, n.z);
     const float a = -1.0f / (sign + n.z);
     const float b = n.x * n.y * a;
     bx = mk3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
     by = mk3(b, sign + n.y * n.y * a, -n.y);
 }
-__device__ __forceinline__ v3 local_to_world(v3 v, v3 n) {
+RPD_HD v3 local_to_world(v3 v, v3 n) {
     v3 bx, by; onb(n, bx, by);
     return v.x * bx + v.y * by + v.z * n;
 }
-__device__ __forceinline__ v3 world_to_local(v3 v, v3 n) {
+RPD_HD v3 world_to_local(v3 v, v3 n) {
     v3 bx, by; onb(n, bx, by);
     return mk3(dot3(v, bx), dot3(v, by), dot3(v, n));
 }
-__device__ __forceinline__ v3 offset_origin(v3 p, v3 n) { return p + RPD_EPS * n; }
+RPD_HD v3 offset_origin(v3 p, v3 n) { return p + RPD_EPS * n; }
 
 // path_tracer/main.cpp:266-286
 __host__ __device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
@@ -203,9 +207,26 @@ struct DevLight {  // 80 B
 struct DevTriangle { float v0[3], v1[3], v2[3], n[3]; int material, light, emitter, pad; };
 struct DevSphere { float c[3], r; int material, light, emitter, pad; };
 
+// 4-wide BVH node: bounds of the four children as structure of arrays (one
+// 16-byte load per axis and side), then the links. count > 0: a leaf of that
+// many primitive slots starting at child; count == 0: inner node child;
+// count < 0: no child.
+struct Bvh4Node {
+    float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+    int child[4];
+    int count[4];
+};
+constexpr int BVH_STACK = 48;        // traversal stack entries per lane
+constexpr int BVH_MAX_DEPTH4 = 16;   // the builder keeps 4-wide depth within this
+constexpr int BVH_SLOT_FLOAT4 = 9;   // per primitive slot: triangle (3 axis orders x 3 float4) or sphere (1 float4)
+
 struct DevScene {
     const DevTriangle* tris; int n_tris;
     const float4* tri_perm;  // n_tris * 3 entries of 3 float4, see tri_perm_entry
+    // BVH over triangles and spheres (scenes too large for the flat loop), or null
+    const Bvh4Node* bvh;
+    const float4* slots;     // BVH_SLOT_FLOAT4 float4 per primitive slot, in leaf order
+    const int* slot_prim;    // triangle index, or sphere index | 0x80000000
     const DevSphere* spheres; int n_spheres;
     const DevMaterial* materials;
     const DevLight* lights; int n_lights;
@@ -214,7 +235,7 @@ struct DevScene {
     int env_on;
 };
 
-__device__ __forceinline__ v3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
+RPD_HD v3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
 
 // Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), as in
 // path_tracer/core/triangle.h:35-104.
@@ -232,7 +253,7 @@ struct RayPre {
     float Sx, Sy, Sz;
     float okx, oky, okz;
 };
-__device__ __forceinline__ RayPre ray_pre(v3 o, v3 d) {
+RPD_HD RayPre ray_pre(v3 o, v3 d) {
     RayPre r;
     const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = 0;
@@ -252,7 +273,7 @@ __device__ __forceinline__ RayPre ray_pre(v3 o, v3 d) {
 // (kz+1, kz+2, kz) mod 3, packed as 9 floats in 3 float4 (3 floats padding).
 __host__ __device__ __forceinline__ int tri_perm_entry(int tri, int kz) { return (tri * 3 + kz) * 3; }
 
-__device__ __forceinline__ bool tri_hit(float4 q0, float4 q1, float4 q2, const RayPre& r, float t_min, float t_max,
+RPD_HD bool tri_hit(float4 q0, float4 q1, float4 q2, const RayPre& r, float t_min, float t_max,
                                         float& t_out) {
     const float Akx = q0.x - r.okx, Aky = q0.y - r.oky, Akz = q0.z - r.okz;
     const float Bkx = q0.w - r.okx, Bky = q1.x - r.oky, Bkz = q1.y - r.okz;
@@ -279,7 +300,7 @@ __device__ __forceinline__ bool tri_hit(float4 q0, float4 q1, float4 q2, const R
 }
 
 // path_tracer/core/sphere.h:19-54
-__device__ __forceinline__ bool sphere_hit(const DevSphere& s, v3 o, v3 d, float t_min, float t_max, float& t_out, v3& n_out) {
+RPD_HD bool sphere_hit(const DevSphere& s, v3 o, v3 d, float t_min, float t_max, float& t_out, v3& n_out) {
     const v3 oc = o - ld3(s.c);
     const float a = sqnorm3(d);
     const float hb = dot3(oc, d);
@@ -315,7 +336,7 @@ struct HitRec {
 // a triangle blocks when 0 <= t < t_max, a sphere when its root is in
 // [1e-8, t_max]. In occlusion mode `hit` reports whether the ray is blocked.
 // `tp` is the pre-permuted triangle table (tri_perm_entry), in LDS or HBM.
-__device__ __forceinline__ HitRec trace_query(const DevScene& S, const float4* tp, v3 o, v3 d, float t_max,
+RPD_HD HitRec trace_query(const DevScene& S, const float4* tp, v3 o, v3 d, float t_max,
                                               bool occlusion) {
     HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
     const RayPre rp = ray_pre(o, d);
@@ -334,6 +355,78 @@ __device__ __forceinline__ HitRec trace_query(const DevScene& S, const float4* t
         if (sphere_hit(S.spheres[i], o, d, RPD_EPS_SMALL, sbest, t, n) && (t < sbest || occlusion)) {
             sbest = t; si = i; sn = n;
         }
+    }
+    if (bi >= 0 && (si < 0 || best < sbest)) { h.hit = 1; h.index = bi; h.t = best; }
+    else if (si >= 0) { h.hit = 1; h.sphere = 1; h.index = si; h.t = sbest; h.sn = sn; }
+    return h;
+}
+
+// The same query through the BVH. Nodes are visited near child first; the
+// stack holds node indices at stack[k * stride]. Triangle ties go to the lower
+// index and sphere ties to the lower index, as in the index-ordered loop, and
+// every culling test is widened by 2^-20 so that rounding in the slab test
+// never drops a primitive the exact tests accept.
+RPD_HD HitRec bvh_query(const DevScene& S, v3 o, v3 d, float t_max, bool occlusion, int* stack, int stride) {
+    HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
+    const RayPre rp = ray_pre(o, d);
+    const float ix = 1.0f / (d.x != 0.0f ? d.x : copysignf(1e-30f, d.x));
+    const float iy = 1.0f / (d.y != 0.0f ? d.y : copysignf(1e-30f, d.y));
+    const float iz = 1.0f / (d.z != 0.0f ? d.z : copysignf(1e-30f, d.z));
+    const float widen = 1.0f + 0x1p-20f;
+    float best = t_max, sbest = t_max;
+    int bi = -1, si = -1;
+    v3 sn = mk3(0, 0, 0);
+    int sp = 0, node = 0;
+    while (true) {
+        const Bvh4Node& N = S.bvh[node];
+        const float cull = fminf(best, sbest) * widen;
+        int cn[4];
+        float ct[4];
+        int nc = 0;
+        for (int k = 0; k < 4; ++k) {
+            const int cnt = N.count[k];
+            if (cnt < 0) continue;
+            const float ax = (N.lo_x[k] - o.x) * ix, bx = (N.hi_x[k] - o.x) * ix;
+            const float ay = (N.lo_y[k] - o.y) * iy, by = (N.hi_y[k] - o.y) * iy;
+            const float az = (N.lo_z[k] - o.z) * iz, bz = (N.hi_z[k] - o.z) * iz;
+            const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
+            const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+            if (!(tn <= tf * widen) || !(tn <= cull)) continue;
+            if (cnt == 0) { cn[nc] = N.child[k]; ct[nc] = tn; ++nc; continue; }
+            for (int q = 0; q < cnt; ++q) {
+                const int slot = N.child[k] + q;
+                const int prim = S.slot_prim[slot];
+                const float4* e = S.slots + size_t(slot) * BVH_SLOT_FLOAT4;
+                if (prim >= 0) {
+                    const float4* pe = e + rp.kz * 3;
+                    float t;
+                    if (tri_hit(pe[0], pe[1], pe[2], rp, 0.0f, best, t) && (t < best || (bi >= 0 && prim < bi))) {
+                        best = t; bi = prim;
+                    }
+                } else {
+                    const int sidx = prim & 0x7fffffff;
+                    DevSphere sph;
+                    sph.c[0] = e[0].x; sph.c[1] = e[0].y; sph.c[2] = e[0].z; sph.r = e[0].w;
+                    float t; v3 n;
+                    if (sphere_hit(sph, o, d, RPD_EPS_SMALL, sbest, t, n) &&
+                        (occlusion || t < sbest || (si >= 0 && sidx < si))) {
+                        sbest = t; si = sidx; sn = n;
+                    }
+                }
+                if (occlusion && (bi >= 0 || si >= 0)) { h.hit = 1; return h; }
+            }
+        }
+        // nearest inner child next, the others on the stack farthest first
+        for (int a = 1; a < nc; ++a)
+            for (int b = a; b > 0 && ct[b] < ct[b - 1]; --b) {
+                const float tt = ct[b]; ct[b] = ct[b - 1]; ct[b - 1] = tt;
+                const int tc = cn[b]; cn[b] = cn[b - 1]; cn[b - 1] = tc;
+            }
+        for (int a = nc - 1; a >= 1; --a)
+            if (sp < BVH_STACK) stack[(sp++) * stride] = cn[a];
+        if (nc > 0) { node = cn[0]; continue; }
+        if (sp == 0) break;
+        node = stack[(--sp) * stride];
     }
     if (bi >= 0 && (si < 0 || best < sbest)) { h.hit = 1; h.index = bi; h.t = best; }
     else if (si >= 0) { h.hit = 1; h.sphere = 1; h.index = si; h.t = sbest; h.sn = sn; }

@@ -42,6 +42,7 @@ class RenderConfig(C.Structure):
         ("adaptive_sigma_max", C.c_float),
         ("adaptive_passes", C.c_int), ("adaptive_importance_smoothing_radius", C.c_int),
         ("rpf_shrinkage_scale", C.c_float), ("rpf_confidence_samples", C.c_float),
+        ("max_depth", C.c_int),
     ]
 
 
@@ -55,6 +56,33 @@ class SynthConfig(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("samples_per_pixel", C.c_int), ("region_size", C.c_int),
                 ("log_mean", C.c_float), ("log_sigma", C.c_float), ("firefly_fraction", C.c_float),
                 ("firefly_scale", C.c_float), ("correlation", C.c_float)]
+
+
+class ProceduralConfig(C.Structure):
+    """rpd_procedural_config: the icosphere + spheres benchmark scene."""
+    _fields_ = [("seed", C.c_uint32), ("subdivision_level", C.c_int), ("n_spheres", C.c_int),
+                ("displacement", C.c_float), ("noise_frequency", C.c_float), ("noise_octaves", C.c_int),
+                ("environment", C.c_float * 3), ("light_emission", C.c_float)]
+
+
+class Triangle(C.Structure):
+    _fields_ = [("v0", C.c_float * 3), ("v1", C.c_float * 3), ("v2", C.c_float * 3), ("normal", C.c_float * 3),
+                ("emission", C.c_float * 3), ("material_id", C.c_int), ("light_id", C.c_int)]
+
+
+class Sphere(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("radius", C.c_float), ("emission", C.c_float * 3),
+                ("material_id", C.c_int), ("light_id", C.c_int)]
+
+
+class Material(C.Structure):
+    _fields_ = [("kind", C.c_int), ("albedo", C.c_float * 3), ("roughness", C.c_float), ("n1", C.c_float * 3),
+                ("n2", C.c_float * 3), ("distribution", C.c_int)]
+
+
+class Light(C.Structure):
+    _fields_ = [("kind", C.c_int), ("p0", C.c_float * 3), ("p1", C.c_float * 3), ("p2", C.c_float * 3),
+                ("normal", C.c_float * 3), ("emission", C.c_float * 3)]
 
 
 class ServerConfig(C.Structure):
@@ -124,6 +152,8 @@ def lib():
         L.rpd_last_error.restype = C.c_char_p
         L.rpd_render_config_defaults.argtypes = [C.POINTER(RenderConfig)]
         L.rpd_render_config_defaults.restype = None
+        L.rpd_procedural_config_defaults.argtypes = [C.POINTER(ProceduralConfig)]
+        L.rpd_procedural_config_defaults.restype = None
         L.rpd_synth_config_defaults.argtypes = [C.POINTER(SynthConfig)]
         L.rpd_synth_config_defaults.restype = None
         for name, args in [
@@ -151,6 +181,10 @@ def lib():
             ("rpd_job_denoise_records", [vp]),
             ("rpd_denoise_batch", [vp, vp, C.c_int, C.POINTER(C.c_double)]),
             ("rpd_ctx_set_graph_passes", [vp, C.c_int]),
+            ("rpd_procedural_scene_arrays", [C.POINTER(ProceduralConfig), vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+            ("rpd_scene_procedural", [vp, C.POINTER(ProceduralConfig), C.POINTER(vp)]),
+            ("rpd_scene_query", [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]),
+            ("rpd_scene_bvh_info", [vp, vp, vp]),
             ("rpd_job_bytes", [C.POINTER(RenderConfig), C.POINTER(C.c_uint64)]),
             ("rpd_ctx_memory", [vp, vp, vp, vp, vp]),
             ("rpd_server_create", [vp, C.POINTER(ServerConfig), C.POINTER(vp)]),
@@ -212,6 +246,14 @@ class Context:
             self._scenes[name] = s
         return self._scenes[name]
 
+    def procedural_scene(self, name, pc):
+        """Creates (once) the procedural scene `pc` on the device under `name`."""
+        if name not in self._scenes:
+            s = C.c_void_p()
+            check(self.L.rpd_scene_procedural(self.h, C.byref(pc), C.byref(s)))
+            self._scenes[name] = s
+        return self._scenes[name]
+
     def set_graph_passes(self, on):
         check(self.L.rpd_ctx_set_graph_passes(self.h, int(on)))
 
@@ -229,6 +271,64 @@ class Context:
         self._scenes = {}
         if self.h:
             self.L.rpd_ctx_destroy(self.h)
+            self.h = None
+
+
+def procedural_config(**kw):
+    c = ProceduralConfig()
+    lib().rpd_procedural_config_defaults(C.byref(c))
+    for k, v in kw.items():
+        if k == "environment":
+            for i in range(3):
+                c.environment[i] = v[i]
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def procedural_arrays(pc):
+    """(triangles, spheres, materials, lights, env) ctypes arrays of the procedural scene."""
+    L = lib()
+    n = [C.c_int() for _ in range(4)]
+    check(L.rpd_procedural_scene_arrays(C.byref(pc), None, C.byref(n[0]), None, C.byref(n[1]), None,
+                                        C.byref(n[2]), None, C.byref(n[3]), None))
+    tris, sph = (Triangle * max(1, n[0].value))(), (Sphere * max(1, n[1].value))()
+    mats, lights = (Material * max(1, n[2].value))(), (Light * max(1, n[3].value))()
+    env = (C.c_float * 3)()
+    check(L.rpd_procedural_scene_arrays(C.byref(pc), tris, C.byref(n[0]), sph, C.byref(n[1]), mats, C.byref(n[2]),
+                                        lights, C.byref(n[3]), env))
+    return tris, sph, mats, lights, env, [x.value for x in n]
+
+
+class HostScene:
+    """A scene kept on the host (no device) for rpd_scene_query checks."""
+
+    def __init__(self, pc):
+        self.L = lib()
+        h = C.c_void_p()
+        check(self.L.rpd_scene_procedural(None, C.byref(pc), C.byref(h)))
+        self.h = h
+
+    def query(self, origins, dirs, t_max, occlusion, use_bvh):
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        tm = np.ascontiguousarray(t_max, np.float32)
+        oc = np.ascontiguousarray(occlusion, np.int32)
+        n = tm.size
+        hit, idx, sph = (np.zeros(n, np.int32) for _ in range(3))
+        t = np.zeros(n, np.float32)
+        check(self.L.rpd_scene_query(self.h, n, _ptr(o), _ptr(d), _ptr(tm), _ptr(oc), int(use_bvh), _ptr(hit),
+                                     _ptr(t), _ptr(idx), _ptr(sph)))
+        return hit, t, idx, sph
+
+    def bvh_info(self):
+        a, b = C.c_int(), C.c_int()
+        check(self.L.rpd_scene_bvh_info(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self.h:
+            self.L.rpd_scene_destroy(self.h)
             self.h = None
 
 

@@ -29,6 +29,7 @@ class OrcConfig(C.Structure):
         ("adaptive_sigma_max", C.c_float),
         ("adaptive_passes", C.c_int), ("adaptive_importance_smoothing_radius", C.c_int),
         ("rpf_shrinkage_scale", C.c_float), ("rpf_confidence_samples", C.c_float),
+        ("max_depth", C.c_int),
     ]
 
 
@@ -64,6 +65,7 @@ class RenderConfig:
     adaptive_importance_smoothing_radius: int = 1
     rpf_shrinkage_scale: float = 1.0
     rpf_confidence_samples: float = 128.0
+    max_depth: int = 0
 
     def to_c(self):
         c = OrcConfig()

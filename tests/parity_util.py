@@ -18,7 +18,8 @@ def engine_config(rc: ob.RenderConfig):
         adaptive_base_samples=rc.adaptive_base_samples, adaptive_spp=rc.adaptive_spp,
         adaptive_sigma_max=rc.adaptive_sigma_max, adaptive_passes=rc.adaptive_passes,
         adaptive_importance_smoothing_radius=rc.adaptive_importance_smoothing_radius,
-        rpf_shrinkage_scale=rc.rpf_shrinkage_scale, rpf_confidence_samples=rc.rpf_confidence_samples)
+        rpf_shrinkage_scale=rc.rpf_shrinkage_scale, rpf_confidence_samples=rc.rpf_confidence_samples,
+        max_depth=rc.max_depth)
 
 
 def engine_camera(rc: ob.RenderConfig):
