@@ -341,6 +341,11 @@ int rpd_server_stats_get(rpd_server* s, rpd_server_stats* out);
 /* Finishes queued and running requests, releases jobs nobody claimed. */
 int rpd_server_destroy(rpd_server* s);
 
+/* sizeof of the ABI structs, for bindings to check their mirrors: render
+   config, camera, triangle, sphere, material, light, sample record, synth
+   config, procedural config, server config, server stats (in that order). */
+int rpd_abi_struct_sizes(int* out, int n);
+
 /* Work counters of the job so far: [0] ray queries traced (camera, extension
    and shadow rays), [1] path samples traced, [2] scene triangles, [3] scene
    spheres. Used to express the trace kernel's work in intersection tests.

@@ -25,6 +25,57 @@ struct orc_config {
     int max_depth;
 };
 
+// Scene arrays in the memory layout of the engine's scene PODs
+// (include/rpd_engine.h): triangles, spheres, materials and lights.
+struct orc_tri { float v0[3], v1[3], v2[3], normal[3], emission[3]; int material_id, light_id; };
+struct orc_sphere { float center[3], radius, emission[3]; int material_id, light_id; };
+struct orc_mat { int kind; float albedo[3], roughness, n1[3], n2[3]; int distribution; };
+struct orc_light { int kind; float p0[3], p1[3], p2[3], normal[3], emission[3]; };
+
+static F3 f3(const float* p) { return {p[0], p[1], p[2]}; }
+
+// The same Scene the reference builds from its scene files (core/scene.h:31-65):
+// primitives, materials, lights with their power CDF, emitter links, BVH.
+static Scene scene_from_arrays(const orc_tri* tris, int nt, const orc_sphere* sph, int ns, const orc_mat* mats,
+                               int nm, const orc_light* lights, int nl, const float* env) {
+    Scene s;
+    for (int i = 0; i < nm; ++i) {
+        Material m;
+        m.kind = mats[i].kind; m.albedo = f3(mats[i].albedo); m.roughness = mats[i].roughness;
+        m.n1 = f3(mats[i].n1); m.n2 = f3(mats[i].n2); m.beckmann = mats[i].distribution == 1 ? 1 : 0;
+        s.add_material(m);
+    }
+    for (int i = 0; i < nl; ++i) {
+        const orc_light& L = lights[i];
+        Light l;
+        if (L.kind == LIGHT_RECT) {
+            l = make_rect_light(f3(L.p0), f3(L.p1), f3(L.p2), f3(L.emission));
+        } else {
+            l.kind = L.kind; l.p0 = f3(L.p0); l.p1 = f3(L.p1); l.p2 = f3(L.p2); l.normal = f3(L.normal);
+            l.emission = f3(L.emission);
+            if (L.kind == LIGHT_AREA) l.area = 0.5f * norm(cross(l.p1 - l.p0, l.p2 - l.p0));
+        }
+        s.add_light(l);
+    }
+    for (int i = 0; i < nt; ++i) {
+        Triangle t;
+        t.v0 = f3(tris[i].v0); t.v1 = f3(tris[i].v1); t.v2 = f3(tris[i].v2);
+        t.normal = f3(tris[i].normal); t.emission = f3(tris[i].emission); t.material_id = tris[i].material_id;
+        const int k = s.add_triangle(t);
+        if (tri_is_emitter(t)) s.triangle_to_light[k] = tris[i].light_id;
+    }
+    for (int i = 0; i < ns; ++i) {
+        Sphere sp;
+        sp.center = f3(sph[i].center); sp.radius = sph[i].radius; sp.emission = f3(sph[i].emission);
+        sp.material_id = sph[i].material_id;
+        const int k = s.add_sphere(sp);
+        if (sphere_is_emitter(sp)) s.sphere_to_light[k] = sph[i].light_id;
+    }
+    s.environment = f3(env);
+    s.build_bvh();
+    return s;
+}
+
 void* orc_job_create(const orc_config* c, uint32_t seed, int nthreads) {
     Job* j = new Job();
     Config& k = j->cfg;
@@ -55,6 +106,14 @@ void* orc_job_create(const orc_config* c, uint32_t seed, int nthreads) {
     return j;
 }
 void orc_job_destroy(void* p) { delete static_cast<Job*>(p); }
+// A job on a scene given as arrays instead of the built-in cornell_box.
+void* orc_job_create_scene(const orc_config* c, uint32_t seed, int nthreads, const orc_tri* tris, int nt,
+                           const orc_sphere* sph, int ns, const orc_mat* mats, int nm, const orc_light* lights,
+                           int nl, const float* env) {
+    Job* j = static_cast<Job*>(orc_job_create(c, seed, nthreads));
+    j->scene = scene_from_arrays(tris, nt, sph, ns, mats, nm, lights, nl, env);
+    return j;
+}
 void orc_job_flags(void* p, int skip_dead, int keep_records, int verbose) {
     Job* j = static_cast<Job*>(p);
     j->skip_dead_work = skip_dead != 0; j->keep_records = keep_records != 0; j->verbose = verbose != 0;

@@ -1313,6 +1313,17 @@ int rpd_ctx_set_graph_passes(rpd_context* ctx, int on) {
     return RPD_OK;
 }
 
+int rpd_abi_struct_sizes(int* out, int n) {
+    if (!out) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    const int s[11] = {int(sizeof(rpd_render_config)), int(sizeof(rpd_camera)), int(sizeof(rpd_triangle)),
+                       int(sizeof(rpd_sphere)), int(sizeof(rpd_material)), int(sizeof(rpd_light)),
+                       int(sizeof(rpd_sample_record)), int(sizeof(rpd_synth_config)),
+                       int(sizeof(rpd_procedural_config)), int(sizeof(rpd_server_config)),
+                       int(sizeof(rpd_server_stats))};
+    for (int k = 0; k < n && k < 11; ++k) out[k] = s[k];
+    return RPD_OK;
+}
+
 int rpd_job_bytes(const rpd_render_config* cfg, uint64_t* bytes) {
     if (!cfg || !bytes) return fail(RPD_ERR_INVALID_ARG, "null argument");
     const int st = validate(*cfg);

@@ -181,6 +181,7 @@ def lib():
             ("rpd_job_denoise_records", [vp]),
             ("rpd_denoise_batch", [vp, vp, C.c_int, C.POINTER(C.c_double)]),
             ("rpd_ctx_set_graph_passes", [vp, C.c_int]),
+            ("rpd_abi_struct_sizes", [vp, C.c_int]),
             ("rpd_procedural_scene_arrays", [C.POINTER(ProceduralConfig), vp, vp, vp, vp, vp, vp, vp, vp, vp]),
             ("rpd_scene_procedural", [vp, C.POINTER(ProceduralConfig), C.POINTER(vp)]),
             ("rpd_scene_query", [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]),

@@ -104,6 +104,10 @@ def lib():
         vp = C.c_void_p
         L.orc_job_create.restype = vp
         L.orc_job_create.argtypes = [C.POINTER(OrcConfig), C.c_uint32, C.c_int]
+        L.orc_job_create_scene.restype = vp
+        L.orc_job_create_scene.argtypes = [C.POINTER(OrcConfig), C.c_uint32, C.c_int, C.c_void_p, C.c_int,
+                                           C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                           C.c_void_p]
         for name in ("orc_job_destroy", "orc_render", "orc_recompute_sensitivity", "orc_statmc",
                      "orc_variance", "orc_finalize"):
             getattr(L, name).argtypes = [vp]
@@ -182,11 +186,18 @@ class OracleJob:
     """One render job of the CPU oracle (mirrors the reference's process state)."""
 
     def __init__(self, cfg: RenderConfig, seed: int, nthreads: int = 1, skip_dead=False,
-                 keep_records=False, verbose=False):
+                 keep_records=False, verbose=False, scene=None):
+        """scene: None for the built-in cornell_box, or (triangles, spheres, materials,
+        lights, env, counts) ctypes arrays in the engine's scene layout."""
         self.L = lib()
         self.cfg = cfg
         self._c = cfg.to_c()
-        self.h = self.L.orc_job_create(C.byref(self._c), seed & 0xFFFFFFFF, nthreads)
+        if scene is None:
+            self.h = self.L.orc_job_create(C.byref(self._c), seed & 0xFFFFFFFF, nthreads)
+        else:
+            tris, sph, mats, lights, env, n = scene
+            self.h = self.L.orc_job_create_scene(C.byref(self._c), seed & 0xFFFFFFFF, nthreads, tris, n[0], sph,
+                                                 n[1], mats, n[2], lights, n[3], env)
         self.L.orc_job_flags(self.h, int(skip_dead), int(keep_records), int(verbose))
         d = np.zeros(6, np.int32)
         self.L.orc_dims(self.h, ptr(d))
