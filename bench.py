@@ -98,6 +98,61 @@ def cpu_baseline(seconds_hint=20.0):
                       f"{threads} OpenMP threads)"}
 
 
+C2 = dict(width=1920, height=1080, spp=8, adaptive_spp=8, adaptive_passes=4, max_depth=8, aperture=0.06,
+          focus_distance=3.4, fov=40.0, cam_pos=(0.0, 0.6, 3.4), cam_dir=(0.0, -0.17, -1.0), seed=1234)
+
+
+def run_c2(args):
+    """The procedural scene (seed 7: 1,310,720-triangle displaced icosphere, 256 GGX
+    spheres, 4 area lights, constant environment) at 1920x1080, 8 spp + 4 adaptive
+    passes of 8 spp, max depth 8 with Russian roulette, thin lens: one job per step."""
+    ctx = eng.Context(0)
+    pc = eng.procedural_config()
+    t0 = time.perf_counter()
+    ctx.procedural_scene("c2", pc)
+    build_s = time.perf_counter() - t0
+    cfg = eng.default_config(image_width=C2["width"], image_height=C2["height"], samples_per_pixel=C2["spp"],
+                             use_statmc=1, adaptive_spp=C2["adaptive_spp"], adaptive_passes=C2["adaptive_passes"],
+                             max_depth=C2["max_depth"])
+    cam = eng.make_camera(C2["cam_pos"], C2["cam_dir"], (0.0, 1.0, 0.0), C2["fov"], C2["focus_distance"],
+                          C2["aperture"])
+    for _ in range(args.warmup):
+        j = eng.Job(ctx, cfg, cam, C2["seed"], scene="c2", submit=True)
+        j.wait()
+        j.release()
+    samples = queries = 0
+    stage = np.zeros(11)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        j = eng.Job(ctx, cfg, cam, C2["seed"], scene="c2", submit=True)
+        j.wait()
+        samples += int(j.get("sample_count").sum())
+        stage += j.stage_ms()
+        queries += j.counters()["queries"]
+        j.release()
+    wall = time.perf_counter() - t0
+    n_px = C2["width"] * C2["height"]
+    denoise_ms = (stage[2] + stage[3] + stage[4] + stage[5]) / args.steps
+    result = {
+        "metric": "Msamples/s & denoised MP/s; p50/p99 job latency vs plain at equal relMSE",
+        "value": samples / wall / 1e6, "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic: procedural scene generated on the host, rendered on device",
+        "config": {"workload": "procedural scene seed 7 (1,310,720 triangles, 256 GGX spheres, 4 area lights, "
+                               "environment), 1920x1080, 8 spp + 4 adaptive passes x 8 spp, max depth 8, thin lens",
+                   "parallelism": "1 GPU"},
+        "denoised_mp_per_s": n_px / (denoise_ms / 1e3) / 1e6 if denoise_ms > 0 else None,
+        "samples_per_job": samples // args.steps,
+        "ray_queries_per_s": queries / (stage[7] / 1e3) if stage[7] > 0 else None,
+        "stage_ms": {"base_render": stage[0] / args.steps, "adaptive_render": stage[1] / args.steps,
+                     "statmc": stage[3] / args.steps, "trace_kernel": stage[7] / args.steps,
+                     "grid_gather": stage[10] / args.steps},
+        "scene_build_s": build_s,
+    }
+    print(json.dumps(result))
+    ctx.close()
+
+
 C3 = dict(images=32, side=(256, 2048), spp=(4, 64), seed=7, budget_gb=240.0)
 
 
@@ -193,12 +248,18 @@ def main():
     ap.add_argument("--size", type=int, default=WORKLOAD["width"])
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy (counter calibration) and exit")
-    ap.add_argument("--workload", default="c1", choices=["c1", "c3"],
-                    help="c1: full-size cornell_box render+denoise job (default); "
-                         "c3: batched denoise of 32 synthetic per-sample buffers")
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3"],
+                    help="c1: full-size cornell_box render+denoise job (default); c2: the 1.3M-triangle "
+                         "procedural scene at 1920x1080; c3: batched denoise of 32 synthetic per-sample buffers")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload == "c2":
+        if args.gpus != 1:
+            print("bench.py measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        run_c2(args)
         return
     if args.workload == "c3":
         print("the full-size batched-denoise workload is switched off: its first run left the GPU faulted and "
