@@ -88,8 +88,9 @@ struct rpd_scene {
     rpd_context* ctx = nullptr;
     std::vector<void*> owned;
     DevScene dev{};
+    DevBvh bvh{};          // device BVH, or null nodes for the flat loop
     int n_tris = 0, n_spheres = 0, n_materials = 0, n_lights = 0;
-    int trace_per_cu = 0;  // resident trace blocks per CU, set on first use
+    int trace_per_cu[2] = {0, 0};  // resident trace blocks per CU without / with a depth limit, set on first use
     // host copies for host-side queries (rpd_scene_query)
     std::vector<DevTriangle> h_tris;
     std::vector<DevSphere> h_spheres;
@@ -339,6 +340,7 @@ TraceParams trace_params(rpd_job* j, int statmc) {
     p.base_seed = j->seed;
     p.tape = j->tape;
     p.max_depth = j->cfg.max_depth;
+    p.bvh = j->scene->bvh;
     p.work_counter = j->work;
     p.query_count = j->query_count;
     p.rec = j->rec;
@@ -346,8 +348,9 @@ TraceParams trace_params(rpd_job* j, int statmc) {
 }
 
 int trace_blocks(rpd_job* j) {
-    if (j->scene->trace_per_cu <= 0) j->scene->trace_per_cu = trace_blocks_per_cu(j->scene->dev);
-    const int per_cu = j->scene->trace_per_cu;
+    const int limited = j->cfg.max_depth > 0 ? 1 : 0;
+    int& per_cu = j->scene->trace_per_cu[limited];
+    if (per_cu <= 0) per_cu = trace_blocks_per_cu(j->scene->dev, j->scene->bvh, j->cfg.max_depth);
     const int by_work = (j->N + trace_block_size() - 1) / trace_block_size();
     return std::max(1, std::min(by_work, per_cu * std::max(1, j->ctx->num_cus)));
 }
@@ -955,7 +958,7 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
         }
     }
     sc->dev.tris = pt; sc->dev.n_tris = n_tris; sc->dev.tri_perm = pp;
-    sc->dev.bvh = pn; sc->dev.slots = pslot; sc->dev.slot_prim = pprim;
+    sc->bvh.nodes = pn; sc->bvh.slots = pslot; sc->bvh.slot_prim = pprim;
     sc->h_tris = dt; sc->h_spheres = ds; sc->h_perm = std::move(perm); sc->h_bvh = std::move(bvh);
     sc->dev.spheres = ps; sc->dev.n_spheres = n_spheres;
     sc->dev.materials = pm;
@@ -987,14 +990,15 @@ int rpd_scene_query(const rpd_scene* s, int n, const float* origins, const float
     hs.tris = s->h_tris.data();
     hs.spheres = s->h_spheres.data();
     hs.tri_perm = s->h_perm.data();
-    hs.bvh = s->h_bvh.nodes.empty() ? nullptr : s->h_bvh.nodes.data();
-    hs.slots = s->h_bvh.slots.empty() ? nullptr : s->h_bvh.slots.data();
-    hs.slot_prim = s->h_bvh.slot_prim.empty() ? nullptr : s->h_bvh.slot_prim.data();
+    DevBvh hb;
+    hb.nodes = s->h_bvh.nodes.empty() ? nullptr : s->h_bvh.nodes.data();
+    hb.slots = s->h_bvh.slots.empty() ? nullptr : s->h_bvh.slots.data();
+    hb.slot_prim = s->h_bvh.slot_prim.empty() ? nullptr : s->h_bvh.slot_prim.data();
     int stack[BVH_STACK];
     for (int i = 0; i < n; ++i) {
         const v3 o = mk3(origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]);
         const v3 d = mk3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
-        const HitRec h = use_bvh ? bvh_query(hs, o, d, t_max[i], occlusion[i] != 0, stack, 1)
+        const HitRec h = use_bvh ? bvh_query(hb, o, d, t_max[i], occlusion[i] != 0, stack, 1)
                                  : trace_query(hs, hs.tri_perm, o, d, t_max[i], occlusion[i] != 0);
         hit[i] = h.hit; t[i] = h.t; index[i] = h.index; is_sphere[i] = h.sphere;
     }

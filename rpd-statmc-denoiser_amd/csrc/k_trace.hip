@@ -91,8 +91,9 @@ constexpr int TRACE_LDS_TRIS = 256;
 constexpr int TRACE_FLAT_SPHERES = 32;
 
 // kBvh: the scene's BVH with a per-lane traversal stack in LDS; otherwise the
-// triangle table staged in LDS and looped over.
-template <bool kBvh>
+// triangle table staged in LDS and looped over. kDepth: paths stop after
+// p.max_depth shading points; without it the loop carries no depth count.
+template <bool kBvh, bool kDepth>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
     const DevScene& S = p.scene;
     extern __shared__ float4 s_dyn[];
@@ -353,7 +354,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
             }
             v3 dir; float pdf;
             brdf_sample(m, swo, sn, bu, dir, pdf);
-            if (pdf > RPD_EPS_SMALL && sqnorm3(dir) > 0.0f && (p.max_depth <= 0 || depth < p.max_depth)) {
+            if (pdf > RPD_EPS_SMALL && sqnorm3(dir) > 0.0f && (!kDepth || p.max_depth <= 0 || depth < p.max_depth)) {
                 bpdf = pdf;
                 cw = brdf_sample_weight(m, swo, dir, sn, pdf);
                 rd = dir; qt = inf; q = Q_EXT;
@@ -365,7 +366,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
 
         // -- one intersection query per lane ----------------------------------
         if (q != Q_NONE) {
-            h = kBvh ? bvh_query(S, ro, rd, qt, q >= Q_SHADOW_L, stack, TRACE_BLOCK)
+            h = kBvh ? bvh_query(p.bvh, ro, rd, qt, q >= Q_SHADOW_L, stack, TRACE_BLOCK)
                      : trace_query(S, tp, ro, rd, qt, q >= Q_SHADOW_L);
             ++n_queries;
         }
@@ -375,26 +376,35 @@ int trace_block_size() { return TRACE_BLOCK; }
 
 bool trace_needs_bvh(int n_tris, int n_spheres) { return n_tris > TRACE_LDS_TRIS || n_spheres > TRACE_FLAT_SPHERES; }
 
-static size_t trace_lds_bytes(const DevScene& s) {
-    return s.bvh ? size_t(BVH_STACK) * TRACE_BLOCK * sizeof(int)
+static size_t trace_lds_bytes(const DevScene& s, const DevBvh& b) {
+    return b.nodes ? size_t(BVH_STACK) * TRACE_BLOCK * sizeof(int)
                  : std::max<size_t>(16, size_t(s.n_tris) * 9 * sizeof(float4));
 }
 
-int trace_blocks_per_cu(const DevScene& s) {
-    const size_t lds = trace_lds_bytes(s);
-    const void* fn = s.bvh ? reinterpret_cast<const void*>(&k_trace<true>)
-                           : reinterpret_cast<const void*>(&k_trace<false>);
+// BVH scenes always take the depth-limited variant (max_depth 0 still means
+// unbounded there); flat scenes take it only when a limit is set.
+static const void* trace_kernel(const DevBvh& b, int max_depth) {
+    if (b.nodes) return reinterpret_cast<const void*>(&k_trace<true, true>);
+    return max_depth > 0 ? reinterpret_cast<const void*>(&k_trace<false, true>)
+                         : reinterpret_cast<const void*>(&k_trace<false, false>);
+}
+
+int trace_blocks_per_cu(const DevScene& s, const DevBvh& b, int max_depth) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, TRACE_BLOCK, lds) != hipSuccess || nb <= 0) nb = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, trace_kernel(b, max_depth), TRACE_BLOCK,
+                                                     trace_lds_bytes(s, b)) != hipSuccess || nb <= 0)
+        nb = 1;
     return nb;
 }
 
 void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st) {
-    const size_t lds = trace_lds_bytes(p.scene);
-    if (p.scene.bvh)
-        hipLaunchKernelGGL(k_trace<true>, dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
+    const size_t lds = trace_lds_bytes(p.scene, p.bvh);
+    if (p.bvh.nodes)
+        hipLaunchKernelGGL((k_trace<true, true>), dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
+    else if (p.max_depth > 0)
+        hipLaunchKernelGGL((k_trace<false, true>), dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
     else
-        hipLaunchKernelGGL(k_trace<false>, dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
+        hipLaunchKernelGGL((k_trace<false, false>), dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
 }
 
 // ---------------------------------------------------------------------------

@@ -56,10 +56,12 @@ struct TraceParams {
     const int* seed_n;       // per-pixel sample offset used for seeding (finite samples so far), or null for 0
     uint32_t base_seed;
     uint32_t* tape;          // ceil(n/64)*64 tapes of 624 words, layout of tape_base/tape_word
-    int max_depth;           // shading points per path, 0 = unbounded (extension)
     int* work_counter;       // zero before launch
     unsigned long long* query_count;  // ray queries issued (closest + occlusion), accumulated; may be null
     Records rec;
+    // extensions, kept last so that the fields above keep their offsets
+    int max_depth;           // shading points per path, 0 = unbounded
+    DevBvh bvh;              // nodes null: the flat loop over scene.tri_perm
 };
 
 // Per-pixel accumulators (structure of arrays, n_pixels each unless noted)
@@ -185,7 +187,7 @@ __device__ inline int64_t batch_take(const Batch& b, int kind, int& cursor) {
 void launch_tape(const TraceParams& p, hipStream_t st);
 void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st);
 int trace_block_size();
-int trace_blocks_per_cu(const DevScene& s);   // resident trace blocks per CU for this scene
+int trace_blocks_per_cu(const DevScene& s, const DevBvh& b, int max_depth);  // resident trace blocks per CU
 bool trace_needs_bvh(int n_tris, int n_spheres);  // too large for the flat LDS loop: traced through a BVH
 // per-pixel fold of the current pass's records of every image (Welford statistics when statmc)
 void launch_fold(const Batch& b, int statmc, hipStream_t st);

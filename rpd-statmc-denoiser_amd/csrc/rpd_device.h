@@ -223,16 +223,21 @@ constexpr int BVH_SLOT_FLOAT4 = 9;   // per primitive slot: triangle (3 axis ord
 struct DevScene {
     const DevTriangle* tris; int n_tris;
     const float4* tri_perm;  // n_tris * 3 entries of 3 float4, see tri_perm_entry
-    // BVH over triangles and spheres (scenes too large for the flat loop), or null
-    const Bvh4Node* bvh;
-    const float4* slots;     // BVH_SLOT_FLOAT4 float4 per primitive slot, in leaf order
-    const int* slot_prim;    // triangle index, or sphere index | 0x80000000
     const DevSphere* spheres; int n_spheres;
     const DevMaterial* materials;
     const DevLight* lights; int n_lights;
     const float* light_cdf;  // inclusive cumulative power
     float env[3];
     int env_on;
+};
+
+// BVH over a scene's triangles and spheres, for scenes too large for the flat
+// loop. A separate block so that the flat-loop kernel's arguments keep their
+// layout whether or not a scene has one.
+struct DevBvh {
+    const Bvh4Node* nodes;   // null: the scene has no BVH
+    const float4* slots;     // BVH_SLOT_FLOAT4 float4 per primitive slot, in leaf order
+    const int* slot_prim;    // triangle index, or sphere index | 0x80000000
 };
 
 RPD_HD v3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
@@ -366,7 +371,7 @@ RPD_HD HitRec trace_query(const DevScene& S, const float4* tp, v3 o, v3 d, float
 // index and sphere ties to the lower index, as in the index-ordered loop, and
 // every culling test is widened by 2^-20 so that rounding in the slab test
 // never drops a primitive the exact tests accept.
-RPD_HD HitRec bvh_query(const DevScene& S, v3 o, v3 d, float t_max, bool occlusion, int* stack, int stride) {
+RPD_HD HitRec bvh_query(const DevBvh& B, v3 o, v3 d, float t_max, bool occlusion, int* stack, int stride) {
     HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
     const RayPre rp = ray_pre(o, d);
     const float ix = 1.0f / (d.x != 0.0f ? d.x : copysignf(1e-30f, d.x));
@@ -378,7 +383,7 @@ RPD_HD HitRec bvh_query(const DevScene& S, v3 o, v3 d, float t_max, bool occlusi
     v3 sn = mk3(0, 0, 0);
     int sp = 0, node = 0;
     while (true) {
-        const Bvh4Node& N = S.bvh[node];
+        const Bvh4Node& N = B.nodes[node];
         const float cull = fminf(best, sbest) * widen;
         int cn[4];
         float ct[4];
@@ -395,8 +400,8 @@ RPD_HD HitRec bvh_query(const DevScene& S, v3 o, v3 d, float t_max, bool occlusi
             if (cnt == 0) { cn[nc] = N.child[k]; ct[nc] = tn; ++nc; continue; }
             for (int q = 0; q < cnt; ++q) {
                 const int slot = N.child[k] + q;
-                const int prim = S.slot_prim[slot];
-                const float4* e = S.slots + size_t(slot) * BVH_SLOT_FLOAT4;
+                const int prim = B.slot_prim[slot];
+                const float4* e = B.slots + size_t(slot) * BVH_SLOT_FLOAT4;
                 if (prim >= 0) {
                     const float4* pe = e + rp.kz * 3;
                     float t;

@@ -9,13 +9,10 @@ import pytest
 
 import oracle_bind as ob
 import rpd_engine as eng
+from fault_record import UNEXPLAINED_FAULT
 from parity_util import engine_camera, engine_config
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="these tests ran in a command that left the GPU faulted; the cause is not "
-                            "identified, so the serving loop and graph replay stay off the GPU for now"),
-]
+pytestmark = [pytest.mark.gpu, UNEXPLAINED_FAULT]
 
 OUTPUTS = ("radiance", "raw_radiance", "denoised", "variance_denoised", "uncertainty", "sample_count")
 
