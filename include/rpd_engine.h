@@ -341,9 +341,32 @@ int rpd_server_stats_get(rpd_server* s, rpd_server_stats* out);
 /* Finishes queued and running requests, releases jobs nobody claimed. */
 int rpd_server_destroy(rpd_server* s);
 
+/* ---- image-quality metrics (shared/tools/eval_tools.cpp) -------------------
+   The reference computes these on the CPU from .hdr files: hdr_metrics
+   (eval_tools.cpp:215-285: MAE, RMSE, PSNR against the reference's peak,
+   global per-channel SSIM) and compute_extended_metrics (120-213: NRMSE,
+   LogSSIM11 over 11x11 windows of log1p luminance, LogGradNRMSE). relMSE,
+   mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2), is added for
+   equal-quality comparisons. Images are float RGB, interleaved, row-major;
+   all arithmetic is double. */
+typedef struct rpd_image_metrics {
+    double mae, rmse, psnr, peak;
+    double ssim[3], ssim_mean;
+    double nrmse, log_ssim11, log_gradient_nrmse;
+    double relmse;
+} rpd_image_metrics;
+/* `test` against `ref`, both width x height; on_device: the pointers are
+   device memory of the context's device (else host memory, copied in). */
+int rpd_image_metrics_compute(rpd_context* ctx, const float* ref, const float* test, int width, int height,
+                              int on_device, rpd_image_metrics* out);
+/* A job's RPD_BUF_RADIANCE / RPD_BUF_RAW_RADIANCE (finalized jobs) or
+   RPD_BUF_DENOISED buffer, read in place on the device, against `ref`. */
+int rpd_job_image_metrics(rpd_job* job, int buffer_id, const float* ref, int ref_on_device, rpd_image_metrics* out);
+
 /* sizeof of the ABI structs, for bindings to check their mirrors: render
    config, camera, triangle, sphere, material, light, sample record, synth
-   config, procedural config, server config, server stats (in that order). */
+   config, procedural config, server config, server stats, image metrics
+   (in that order). */
 int rpd_abi_struct_sizes(int* out, int n);
 
 /* Work counters of the job so far: [0] ray queries traced (camera, extension

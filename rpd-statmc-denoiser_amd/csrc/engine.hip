@@ -82,6 +82,14 @@ struct rpd_context {
     hipStream_t b_stream = nullptr;
     hipEvent_t b_ev[2] = {nullptr, nullptr};
     bool graph_passes = false;  // capture each job's adaptive-pass sequence as a hipGraph and replay it
+    // image metrics: partials, reduced fields and (host-image calls) staged images, grown on demand
+    std::mutex metrics_mu;
+    double* m_part = nullptr;
+    size_t m_part_cap = 0;
+    double* m_out = nullptr;
+    float* m_img = nullptr;
+    size_t m_img_cap = 0;
+    hipStream_t m_stream = nullptr;
 };
 
 struct rpd_scene {
@@ -821,6 +829,11 @@ int rpd_ctx_destroy(rpd_context* ctx) {
     if (ctx->b_queues) hipFree(ctx->b_queues);
     for (hipEvent_t e : ctx->b_ev) if (e) hipEventDestroy(e);
     if (ctx->b_stream) hipStreamDestroy(ctx->b_stream);
+    if (ctx->m_stream) hipStreamSynchronize(ctx->m_stream);
+    if (ctx->m_part) hipFree(ctx->m_part);
+    if (ctx->m_out) hipFree(ctx->m_out);
+    if (ctx->m_img) hipFree(ctx->m_img);
+    if (ctx->m_stream) hipStreamDestroy(ctx->m_stream);
     delete ctx;
     return RPD_OK;
 }
@@ -1319,12 +1332,12 @@ int rpd_ctx_set_graph_passes(rpd_context* ctx, int on) {
 
 int rpd_abi_struct_sizes(int* out, int n) {
     if (!out) return fail(RPD_ERR_INVALID_ARG, "null argument");
-    const int s[11] = {int(sizeof(rpd_render_config)), int(sizeof(rpd_camera)), int(sizeof(rpd_triangle)),
+    const int s[12] = {int(sizeof(rpd_render_config)), int(sizeof(rpd_camera)), int(sizeof(rpd_triangle)),
                        int(sizeof(rpd_sphere)), int(sizeof(rpd_material)), int(sizeof(rpd_light)),
                        int(sizeof(rpd_sample_record)), int(sizeof(rpd_synth_config)),
                        int(sizeof(rpd_procedural_config)), int(sizeof(rpd_server_config)),
-                       int(sizeof(rpd_server_stats))};
-    for (int k = 0; k < n && k < 11; ++k) out[k] = s[k];
+                       int(sizeof(rpd_server_stats)), int(sizeof(rpd_image_metrics))};
+    for (int k = 0; k < n && k < 12; ++k) out[k] = s[k];
     return RPD_OK;
 }
 
@@ -1367,6 +1380,133 @@ int rpd_job_stage_ms(rpd_job* j, double* ms, int n) {
     j->stage_ms[6] = total;
     for (int k = 0; k < n && k < 11; ++k) ms[k] = j->stage_ms[k];
     return RPD_OK;
+}
+
+// ---- image-quality metrics ---------------------------------------------------
+// Pass A on the device, its fields to the host (means and log-luminance peak
+// set the constants of pass B), pass B, then the closing formulas of
+// hdr_metrics (shared/tools/eval_tools.cpp:236-279) and
+// compute_extended_metrics (eval_tools.cpp:145, 172-174, 191, 212).
+// The caller holds ctx->metrics_mu with the context's device current.
+static int metrics_run(rpd_context* ctx, const MetricImage& ref, const MetricImage& tst, int W, int H,
+                       hipStream_t st, rpd_image_metrics* out) {
+    const size_t need = size_t(std::max(metrics_a_blocks(W, H) * MA_FIELDS, metrics_b_blocks(W, H) * MB_FIELDS));
+    if (need > ctx->m_part_cap) {
+        if (ctx->m_part) HIP_TRY(hipFree(ctx->m_part));
+        ctx->m_part = nullptr;
+        ctx->m_part_cap = 0;
+        HIP_TRY(hipMalloc(&ctx->m_part, need * sizeof(double)));
+        ctx->m_part_cap = need;
+    }
+    if (!ctx->m_out) HIP_TRY(hipMalloc(&ctx->m_out, 32 * sizeof(double)));
+    double a[MA_FIELDS], b[MB_FIELDS];
+    launch_metrics_a(ref, tst, W, H, ctx->m_part, ctx->m_out, st);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(a, ctx->m_out, sizeof(a), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const double pixels = double(W) * double(H);
+    MetricsPassB pb;
+    for (int c = 0; c < 3; ++c) { pb.mean_ref[c] = a[MA_MEAN_REF + c] / pixels; pb.mean_tst[c] = a[MA_MEAN_TST + c] / pixels; }
+    const double log_peak = std::max(1e-6, a[MA_LOG_PEAK]);
+    pb.c1 = std::pow(0.01 * log_peak, 2.0);
+    pb.c2 = std::pow(0.03 * log_peak, 2.0);
+    launch_metrics_b(ref, tst, W, H, pb, ctx->m_part, ctx->m_out, st);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(b, ctx->m_out, sizeof(b), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+
+    const double peak = a[MA_PEAK] > 0.0 ? a[MA_PEAK] : 1.0;
+    const double C1 = (0.01 * peak) * (0.01 * peak), C2 = (0.03 * peak) * (0.03 * peak);
+    for (int c = 0; c < 3; ++c) {
+        const double vr = b[MB_VAR_REF + c] / pixels, vt = b[MB_VAR_TST + c] / pixels, cov = b[MB_COV + c] / pixels;
+        const double mr = pb.mean_ref[c], mt = pb.mean_tst[c];
+        const double num = (2.0 * mr * mt + C1) * (2.0 * cov + C2);
+        const double den = (mr * mr + mt * mt + C1) * (vr + vt + C2);
+        out->ssim[c] = den > 0.0 ? num / den : 0.0;
+    }
+    out->ssim_mean = (out->ssim[0] + out->ssim[1] + out->ssim[2]) / 3.0;
+    const double rmse = std::sqrt(a[MA_SUM_SQ] / (pixels * 3.0));
+    out->mae = a[MA_SUM_ABS] / (pixels * 3.0);
+    out->rmse = rmse;
+    out->psnr = rmse > 0.0 ? 20.0 * std::log10(peak / rmse) : INFINITY;
+    out->peak = peak;
+    out->nrmse = std::sqrt(a[MA_SUM_SQ] / std::max(1e-18, a[MA_REF_ENERGY]));
+    out->log_ssim11 = b[MB_SSIM] / pixels;
+    out->log_gradient_nrmse = std::sqrt(a[MA_GRAD_ERR] / std::max(1e-18, a[MA_GRAD_REF]));
+    out->relmse = a[MA_RELMSE] / (pixels * 3.0);
+    return RPD_OK;
+}
+
+static int metrics_check_dims(int width, int height) {
+    if (width <= 0 || height <= 0) return fail(RPD_ERR_INVALID_ARG, "image dimensions must be positive");
+    if ((height + 15) / 16 > 65535) return fail(RPD_ERR_INVALID_ARG, "image taller than 1048560 rows");
+    return RPD_OK;
+}
+
+// Grows the context's image staging area to `floats` and returns it.
+static int metrics_stage(rpd_context* ctx, size_t floats, float** p) {
+    if (floats > ctx->m_img_cap) {
+        if (ctx->m_img) HIP_TRY(hipFree(ctx->m_img));
+        ctx->m_img = nullptr;
+        ctx->m_img_cap = 0;
+        HIP_TRY(hipMalloc(&ctx->m_img, floats * sizeof(float)));
+        ctx->m_img_cap = floats;
+    }
+    *p = ctx->m_img;
+    return RPD_OK;
+}
+
+int rpd_image_metrics_compute(rpd_context* ctx, const float* ref, const float* test, int width, int height,
+                              int on_device, rpd_image_metrics* out) {
+    if (!ctx || !ref || !test || !out) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    int st = metrics_check_dims(width, height);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(ctx->metrics_mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (!ctx->m_stream) HIP_TRY(hipStreamCreateWithFlags(&ctx->m_stream, hipStreamNonBlocking));
+    const size_t n3 = size_t(width) * size_t(height) * 3;
+    const float* dr = ref;
+    const float* dt = test;
+    if (!on_device) {
+        float* stage = nullptr;
+        if ((st = metrics_stage(ctx, 2 * n3, &stage))) return st;
+        HIP_TRY(hipMemcpyAsync(stage, ref, n3 * sizeof(float), hipMemcpyHostToDevice, ctx->m_stream));
+        HIP_TRY(hipMemcpyAsync(stage + n3, test, n3 * sizeof(float), hipMemcpyHostToDevice, ctx->m_stream));
+        dr = stage;
+        dt = stage + n3;
+    }
+    return metrics_run(ctx, MetricImage{dr, 1, 3}, MetricImage{dt, 1, 3}, width, height, ctx->m_stream, out);
+}
+
+int rpd_job_image_metrics(rpd_job* j, int buffer_id, const float* ref, int ref_on_device, rpd_image_metrics* out) {
+    if (!j || !ref || !out) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    const float* planes = nullptr;
+    switch (buffer_id) {
+        case RPD_BUF_RADIANCE:
+            if (!j->finalized) return fail(RPD_ERR_STATE, "the radiance buffer holds sums until the job is finalized");
+            planes = j->out_rad;
+            break;
+        case RPD_BUF_RAW_RADIANCE:
+            if (!j->finalized) return fail(RPD_ERR_STATE, "the raw radiance is written when the job is finalized");
+            planes = j->out_raw;
+            break;
+        case RPD_BUF_DENOISED: planes = j->den; break;
+        default: return fail(RPD_ERR_INVALID_ARG, "metrics are defined for the radiance, raw radiance and denoised buffers");
+    }
+    if (j->worker.joinable()) j->worker.join();
+    rpd_context* ctx = j->ctx;
+    std::lock_guard<std::mutex> lk(ctx->metrics_mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t n3 = size_t(j->N) * 3;
+    const float* dr = ref;
+    if (!ref_on_device) {
+        float* stage = nullptr;
+        const int st = metrics_stage(ctx, n3, &stage);
+        if (st) return st;
+        HIP_TRY(hipMemcpyAsync(stage, ref, n3 * sizeof(float), hipMemcpyHostToDevice, j->stream));
+        dr = stage;
+    }
+    return metrics_run(ctx, MetricImage{dr, 1, 3}, MetricImage{planes, int64_t(j->N), 1}, j->W, j->H, j->stream, out);
 }
 
 int rpd_job_counters(rpd_job* j, uint64_t* out, int n) {

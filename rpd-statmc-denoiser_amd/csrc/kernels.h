@@ -235,4 +235,25 @@ size_t alloc_scratch_bytes(int n_pixels);
 void launch_finalize(int n, const PixelBuffers& pb, float* out_radiance, float* out_albedo, float* out_normal,
                      float* out_pos, float* out_depth, hipStream_t st);
 
+// ---- image-quality metrics (shared/tools/eval_tools.cpp:120-285) ----------
+// An RGB float image: channel c of pixel i at p[c * cs + i * ps]
+// (interleaved: cs 1, ps 3; planar: cs N, ps 1).
+struct MetricImage { const float* p; int64_t cs, ps; };
+// pass A partial fields; the last two are maxima, the others sums
+enum : int {
+    MA_SUM_SQ = 0, MA_SUM_ABS = 1, MA_MEAN_REF = 2, MA_MEAN_TST = 5, MA_REF_ENERGY = 8, MA_GRAD_ERR = 9,
+    MA_GRAD_REF = 10, MA_RELMSE = 11, MA_PEAK = 12, MA_LOG_PEAK = 13, MA_FIELDS = 14
+};
+// pass B partial fields (sums)
+enum : int { MB_VAR_REF = 0, MB_VAR_TST = 3, MB_COV = 6, MB_SSIM = 9, MB_FIELDS = 10 };
+struct MetricsPassB { double mean_ref[3], mean_tst[3], c1, c2; };
+int64_t metrics_a_blocks(int W, int H);
+int64_t metrics_b_blocks(int W, int H);
+// each writes its reduced fields (MA_FIELDS or MB_FIELDS doubles) to `out`;
+// `part` holds blocks x fields doubles
+void launch_metrics_a(const MetricImage& ref, const MetricImage& tst, int W, int H, double* part, double* out,
+                      hipStream_t st);
+void launch_metrics_b(const MetricImage& ref, const MetricImage& tst, int W, int H, const MetricsPassB& pb,
+                      double* part, double* out, hipStream_t st);
+
 }  // namespace rpd

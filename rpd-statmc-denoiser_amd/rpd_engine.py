@@ -96,6 +96,18 @@ class ServerStats(C.Structure):
                 ("latency_max_ms", C.c_double), ("queue_p50_ms", C.c_double), ("queue_p99_ms", C.c_double)]
 
 
+class ImageMetrics(C.Structure):
+    """rpd_image_metrics (shared/tools/eval_tools.cpp metrics + relMSE)."""
+    _fields_ = [("mae", C.c_double), ("rmse", C.c_double), ("psnr", C.c_double), ("peak", C.c_double),
+                ("ssim", C.c_double * 3), ("ssim_mean", C.c_double), ("nrmse", C.c_double),
+                ("log_ssim11", C.c_double), ("log_gradient_nrmse", C.c_double), ("relmse", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "ssim"}
+        d.update(ssim_r=self.ssim[0], ssim_g=self.ssim[1], ssim_b=self.ssim[2])
+        return d
+
+
 # rpd_sample_record as a numpy structured type (100 bytes)
 SAMPLE_RECORD = np.dtype([
     ("radiance", np.float32, 3), ("flags", np.uint32), ("normal", np.float32, 3), ("depth", np.float32),
@@ -182,6 +194,8 @@ def lib():
             ("rpd_denoise_batch", [vp, vp, C.c_int, C.POINTER(C.c_double)]),
             ("rpd_ctx_set_graph_passes", [vp, C.c_int]),
             ("rpd_abi_struct_sizes", [vp, C.c_int]),
+            ("rpd_image_metrics_compute", [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.POINTER(ImageMetrics)]),
+            ("rpd_job_image_metrics", [vp, C.c_int, vp, C.c_int, C.POINTER(ImageMetrics)]),
             ("rpd_procedural_scene_arrays", [C.POINTER(ProceduralConfig), vp, vp, vp, vp, vp, vp, vp, vp, vp]),
             ("rpd_scene_procedural", [vp, C.POINTER(ProceduralConfig), C.POINTER(vp)]),
             ("rpd_scene_query", [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]),
@@ -273,6 +287,17 @@ class Context:
         if self.h:
             self.L.rpd_ctx_destroy(self.h)
             self.h = None
+
+
+def image_metrics(ctx, ref, test):
+    """Device metrics of host image `test` against `ref`, both (H, W, 3) float32."""
+    ref = np.ascontiguousarray(ref, np.float32)
+    test = np.ascontiguousarray(test, np.float32)
+    if ref.shape != test.shape or ref.ndim != 3 or ref.shape[2] != 3:
+        raise ValueError("images must both be (H, W, 3)")
+    m = ImageMetrics()
+    check(lib().rpd_image_metrics_compute(ctx.h, _ptr(ref), _ptr(test), ref.shape[1], ref.shape[0], 0, C.byref(m)))
+    return m.as_dict()
 
 
 def procedural_config(**kw):
@@ -507,6 +532,14 @@ class Job:
         n = C.c_size_t()
         check(self.L.rpd_job_buffer_size(self.h, BUF[name], C.byref(n)))
         return n.value
+
+    def metrics(self, name, ref):
+        """Quality metrics of this job's `name` buffer (radiance, raw_radiance or
+        denoised) against the host image `ref` ((H, W, 3) or (N, 3) float32)."""
+        ref = np.ascontiguousarray(ref, np.float32)
+        m = ImageMetrics()
+        check(self.L.rpd_job_image_metrics(self.h, BUF[name], _ptr(ref), 0, C.byref(m)))
+        return m.as_dict()
 
     def get(self, name):
         nb = self.nbytes(name)

@@ -161,12 +161,29 @@ def lib():
         L.orc_cornell_triangles.argtypes = [C.c_void_p]
         L.orc_cornell_triangles.restype = C.c_long
         L.orc_cornell_hits.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_image_metrics.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         _lib = L
     return _lib
 
 
 def ptr(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+METRIC_NAMES = ("mae", "rmse", "psnr", "peak", "ssim_r", "ssim_g", "ssim_b", "ssim_mean", "nrmse", "log_ssim11",
+                "log_gradient_nrmse", "relmse")
+
+
+def image_metrics(ref, test):
+    """eval_tools.cpp hdr_metrics + compute_extended_metrics (+ relMSE) on
+    (H, W, 3) or (N, 3) float images of equal shape, given the width."""
+    ref = np.ascontiguousarray(ref, np.float32)
+    test = np.ascontiguousarray(test, np.float32)
+    assert ref.shape == test.shape and ref.ndim == 3 and ref.shape[2] == 3
+    h, w = ref.shape[:2]
+    out = np.zeros(len(METRIC_NAMES), np.float64)
+    lib().orc_image_metrics(ptr(ref), ptr(test), w, h, ptr(out))
+    return dict(zip(METRIC_NAMES, out.tolist()))
 
 
 # shapes of the per-job buffers: name -> (dtype, per-pixel width)

@@ -26,10 +26,10 @@ def test_library_exports_every_declared_function():
 
 
 def test_struct_sizes_match_python_mirrors():
-    out = np.zeros(11, np.int32)
-    eng.check(eng.lib().rpd_abi_struct_sizes(out.ctypes.data_as(C.c_void_p), 11))
+    out = np.zeros(12, np.int32)
+    eng.check(eng.lib().rpd_abi_struct_sizes(out.ctypes.data_as(C.c_void_p), 12))
     mirrors = [eng.RenderConfig, eng.Camera, eng.Triangle, eng.Sphere, eng.Material, eng.Light, None,
-               eng.SynthConfig, eng.ProceduralConfig, eng.ServerConfig, eng.ServerStats]
+               eng.SynthConfig, eng.ProceduralConfig, eng.ServerConfig, eng.ServerStats, eng.ImageMetrics]
     for k, m in enumerate(mirrors):
         want = eng.SAMPLE_RECORD.itemsize if m is None else C.sizeof(m)
         assert out[k] == want, (k, m, out[k], want)
