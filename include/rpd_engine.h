@@ -363,10 +363,37 @@ int rpd_image_metrics_compute(rpd_context* ctx, const float* ref, const float* t
    RPD_BUF_DENOISED buffer, read in place on the device, against `ref`. */
 int rpd_job_image_metrics(rpd_job* job, int buffer_id, const float* ref, int ref_on_device, rpd_image_metrics* out);
 
+/* ---- comparison denoisers (denoising/joint_bilateral.cpp, atrous_wavelet.cpp)
+   The reference's two baseline filters, which it runs as separate tools on
+   .hdr files. Images are float RGB, interleaved, row-major, as those tools
+   read them; outputs must not alias inputs. on_device: every pointer is device
+   memory of the context's device (else host memory). Float arithmetic in the
+   reference's order. */
+typedef struct rpd_bilateral_config {
+    float sigma_c, sigma_n, sigma_p, sigma_a;  /* colour, normal angle (radians), pixel distance, albedo */
+} rpd_bilateral_config;
+void rpd_bilateral_config_defaults(rpd_bilateral_config* c);  /* all 1 (joint_bilateral.cpp:172-181) */
+/* joint_bilateral.cpp:15-128: Gaussian, colour bilateral and joint bilateral
+   images from one sweep over windows of radius max(1, ceil(3 sigma_p)) with
+   clamped coordinates; `normal` is encoded as (n + 1) / 2 and decoded as
+   there. Any of the three outputs may be NULL; sigma_p at most 32. */
+int rpd_joint_bilateral(rpd_context* ctx, const float* raw, const float* normal, const float* albedo, int width,
+                        int height, const rpd_bilateral_config* cfg, float* gaussian, float* bilateral, float* joint,
+                        int on_device);
+typedef struct rpd_atrous_config {
+    float sigma_c, sigma_n, sigma_a;  /* luminance, normal angle (radians), albedo */
+    int iterations;                   /* passes at step 1, 2, 4, ... (1..30) */
+} rpd_atrous_config;
+void rpd_atrous_config_defaults(rpd_atrous_config* c);  /* 0.5, 0.5, 0.1, 5 iterations (atrous_wavelet.cpp:220-230) */
+/* atrous_wavelet.cpp:57-183: edge-avoiding a-trous passes of the 5x5
+   B3-spline kernel; normals are normalised as stored. */
+int rpd_atrous(rpd_context* ctx, const float* raw, const float* normal, const float* albedo, int width, int height,
+               const rpd_atrous_config* cfg, float* out, int on_device);
+
 /* sizeof of the ABI structs, for bindings to check their mirrors: render
    config, camera, triangle, sphere, material, light, sample record, synth
-   config, procedural config, server config, server stats, image metrics
-   (in that order). */
+   config, procedural config, server config, server stats, image metrics,
+   bilateral config, a-trous config (in that order). */
 int rpd_abi_struct_sizes(int* out, int n);
 
 /* Work counters of the job so far: [0] ray queries traced (camera, extension

@@ -90,6 +90,11 @@ struct rpd_context {
     float* m_img = nullptr;
     size_t m_img_cap = 0;
     hipStream_t m_stream = nullptr;
+    // comparison denoisers: staging and scratch floats, grown on demand
+    std::mutex filters_mu;
+    float* f_buf = nullptr;
+    size_t f_cap = 0;
+    hipStream_t f_stream = nullptr;
 };
 
 struct rpd_scene {
@@ -834,6 +839,9 @@ int rpd_ctx_destroy(rpd_context* ctx) {
     if (ctx->m_out) hipFree(ctx->m_out);
     if (ctx->m_img) hipFree(ctx->m_img);
     if (ctx->m_stream) hipStreamDestroy(ctx->m_stream);
+    if (ctx->f_stream) hipStreamSynchronize(ctx->f_stream);
+    if (ctx->f_buf) hipFree(ctx->f_buf);
+    if (ctx->f_stream) hipStreamDestroy(ctx->f_stream);
     delete ctx;
     return RPD_OK;
 }
@@ -1332,12 +1340,13 @@ int rpd_ctx_set_graph_passes(rpd_context* ctx, int on) {
 
 int rpd_abi_struct_sizes(int* out, int n) {
     if (!out) return fail(RPD_ERR_INVALID_ARG, "null argument");
-    const int s[12] = {int(sizeof(rpd_render_config)), int(sizeof(rpd_camera)), int(sizeof(rpd_triangle)),
+    const int s[14] = {int(sizeof(rpd_render_config)), int(sizeof(rpd_camera)), int(sizeof(rpd_triangle)),
                        int(sizeof(rpd_sphere)), int(sizeof(rpd_material)), int(sizeof(rpd_light)),
                        int(sizeof(rpd_sample_record)), int(sizeof(rpd_synth_config)),
                        int(sizeof(rpd_procedural_config)), int(sizeof(rpd_server_config)),
-                       int(sizeof(rpd_server_stats)), int(sizeof(rpd_image_metrics))};
-    for (int k = 0; k < n && k < 12; ++k) out[k] = s[k];
+                       int(sizeof(rpd_server_stats)), int(sizeof(rpd_image_metrics)),
+                       int(sizeof(rpd_bilateral_config)), int(sizeof(rpd_atrous_config))};
+    for (int k = 0; k < n && k < 14; ++k) out[k] = s[k];
     return RPD_OK;
 }
 
@@ -1507,6 +1516,131 @@ int rpd_job_image_metrics(rpd_job* j, int buffer_id, const float* ref, int ref_o
         dr = stage;
     }
     return metrics_run(ctx, MetricImage{dr, 1, 3}, MetricImage{planes, int64_t(j->N), 1}, j->W, j->H, j->stream, out);
+}
+
+// ---- comparison denoisers ---------------------------------------------------
+void rpd_bilateral_config_defaults(rpd_bilateral_config* c) {
+    if (!c) return;
+    c->sigma_c = 1.0f; c->sigma_n = 1.0f; c->sigma_p = 1.0f; c->sigma_a = 1.0f;  // joint_bilateral.cpp:172-181
+}
+void rpd_atrous_config_defaults(rpd_atrous_config* c) {
+    if (!c) return;
+    c->sigma_c = 0.5f; c->sigma_n = 0.5f; c->sigma_a = 0.1f; c->iterations = 5;  // atrous_wavelet.cpp:220-230
+}
+
+// The context's filter scratch of at least `floats` floats, on its stream.
+static int filters_scratch(rpd_context* ctx, size_t floats, float** p) {
+    if (!ctx->f_stream) HIP_TRY(hipStreamCreateWithFlags(&ctx->f_stream, hipStreamNonBlocking));
+    if (floats > ctx->f_cap) {
+        if (ctx->f_buf) HIP_TRY(hipFree(ctx->f_buf));
+        ctx->f_buf = nullptr;
+        ctx->f_cap = 0;
+        HIP_TRY(hipMalloc(&ctx->f_buf, floats * sizeof(float)));
+        ctx->f_cap = floats;
+    }
+    *p = ctx->f_buf;
+    return RPD_OK;
+}
+
+static int filter_dims(int width, int height) {
+    if (width <= 0 || height <= 0) return fail(RPD_ERR_INVALID_ARG, "image dimensions must be positive");
+    if ((height + 15) / 16 > 65535) return fail(RPD_ERR_INVALID_ARG, "image taller than 1048560 rows");
+    return RPD_OK;
+}
+
+int rpd_joint_bilateral(rpd_context* ctx, const float* raw, const float* normal, const float* albedo, int width,
+                        int height, const rpd_bilateral_config* cfg, float* gaussian, float* bilateral, float* joint,
+                        int on_device) {
+    if (!ctx || !raw || !normal || !albedo || !cfg) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    if (!gaussian && !bilateral && !joint) return fail(RPD_ERR_INVALID_ARG, "no output requested");
+    int st = filter_dims(width, height);
+    if (st) return st;
+    if (!(cfg->sigma_c > 0.0f) || !(cfg->sigma_n > 0.0f) || !(cfg->sigma_p > 0.0f) || !(cfg->sigma_a > 0.0f))
+        return fail(RPD_ERR_INVALID_ARG, "All sigma values must be positive");  // joint_bilateral.cpp:283-287
+    if (bilateral_radius(cfg->sigma_p) > 96) return fail(RPD_ERR_INVALID_ARG, "sigma_p above 32 (window radius beyond 96)");
+    for (const float* o : {static_cast<const float*>(gaussian), static_cast<const float*>(bilateral),
+                           static_cast<const float*>(joint)})
+        if (o && (o == raw || o == normal || o == albedo)) return fail(RPD_ERR_INVALID_ARG, "outputs must not alias an input");
+    std::lock_guard<std::mutex> lk(ctx->filters_mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t n3 = size_t(width) * size_t(height) * 3;
+    float* buf = nullptr;
+    if ((st = filters_scratch(ctx, on_device ? 0 : 6 * n3, &buf))) return st;
+    hipStream_t s = ctx->f_stream;
+    BilateralParams bp;
+    bp.width = width; bp.height = height;
+    bp.sigma_c = cfg->sigma_c; bp.sigma_n = cfg->sigma_n; bp.sigma_p = cfg->sigma_p; bp.sigma_a = cfg->sigma_a;
+    bp.radius = bilateral_radius(cfg->sigma_p);
+    if (on_device) {
+        bp.raw = raw; bp.normal = normal; bp.albedo = albedo;
+        bp.gaussian = gaussian; bp.bilateral = bilateral; bp.joint = joint;
+    } else {
+        HIP_TRY(hipMemcpyAsync(buf, raw, n3 * sizeof(float), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(buf + n3, normal, n3 * sizeof(float), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(buf + 2 * n3, albedo, n3 * sizeof(float), hipMemcpyHostToDevice, s));
+        bp.raw = buf; bp.normal = buf + n3; bp.albedo = buf + 2 * n3;
+        bp.gaussian = gaussian ? buf + 3 * n3 : nullptr;
+        bp.bilateral = bilateral ? buf + 4 * n3 : nullptr;
+        bp.joint = joint ? buf + 5 * n3 : nullptr;
+    }
+    launch_joint_bilateral(bp, s);
+    HIP_TRY(hipGetLastError());
+    if (!on_device) {
+        if (gaussian) HIP_TRY(hipMemcpyAsync(gaussian, buf + 3 * n3, n3 * sizeof(float), hipMemcpyDeviceToHost, s));
+        if (bilateral) HIP_TRY(hipMemcpyAsync(bilateral, buf + 4 * n3, n3 * sizeof(float), hipMemcpyDeviceToHost, s));
+        if (joint) HIP_TRY(hipMemcpyAsync(joint, buf + 5 * n3, n3 * sizeof(float), hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return RPD_OK;
+}
+
+int rpd_atrous(rpd_context* ctx, const float* raw, const float* normal, const float* albedo, int width, int height,
+               const rpd_atrous_config* cfg, float* out, int on_device) {
+    if (!ctx || !raw || !normal || !albedo || !cfg || !out) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    int st = filter_dims(width, height);
+    if (st) return st;
+    if (!(cfg->sigma_c > 0.0f) || !(cfg->sigma_n > 0.0f) || !(cfg->sigma_a > 0.0f))
+        return fail(RPD_ERR_INVALID_ARG, "All sigma values must be positive");  // atrous_wavelet.cpp:363-369
+    if (cfg->iterations < 1) return fail(RPD_ERR_INVALID_ARG, "Number of iterations must be at least 1");
+    if (cfg->iterations > 30) return fail(RPD_ERR_INVALID_ARG, "more than 30 iterations (step 2^iteration overflows)");
+    if (out == raw || out == normal || out == albedo) return fail(RPD_ERR_INVALID_ARG, "out must not alias an input");
+    std::lock_guard<std::mutex> lk(ctx->filters_mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t n = size_t(width) * size_t(height), n3 = 3 * n;
+    // scratch: ping-pong image, unit normals, luminance [, staged raw, normal, albedo, out]
+    float* buf = nullptr;
+    if ((st = filters_scratch(ctx, 7 * n + (on_device ? 0 : 4 * n3), &buf))) return st;
+    hipStream_t s = ctx->f_stream;
+    float* tmp = buf;
+    float* unit = buf + n3;
+    float* lum = buf + 2 * n3;
+    const float* d_raw = raw;
+    const float* d_nrm = normal;
+    const float* d_alb = albedo;
+    float* d_out = out;
+    if (!on_device) {
+        float* stage = buf + 7 * n;
+        HIP_TRY(hipMemcpyAsync(stage, raw, n3 * sizeof(float), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(stage + n3, normal, n3 * sizeof(float), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(stage + 2 * n3, albedo, n3 * sizeof(float), hipMemcpyHostToDevice, s));
+        d_raw = stage; d_nrm = stage + n3; d_alb = stage + 2 * n3; d_out = stage + 3 * n3;
+    }
+    launch_atrous_prepare(d_nrm, int64_t(n), unit, s);
+    // pass k writes `out` when iterations - 1 - k is even, so the last pass ends there
+    const float* src = d_raw;
+    for (int k = 0; k < cfg->iterations; ++k) {
+        float* dst = ((cfg->iterations - 1 - k) % 2 == 0) ? d_out : tmp;
+        AtrousPass ap;
+        ap.in = src; ap.out = dst; ap.unit = unit; ap.albedo = d_alb; ap.lum = lum;
+        ap.width = width; ap.height = height; ap.step = 1 << k;
+        ap.sigma_c = cfg->sigma_c; ap.sigma_n = cfg->sigma_n; ap.sigma_a = cfg->sigma_a;
+        launch_atrous_pass(ap, s);
+        src = dst;
+    }
+    HIP_TRY(hipGetLastError());
+    if (!on_device) HIP_TRY(hipMemcpyAsync(out, d_out, n3 * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return RPD_OK;
 }
 
 int rpd_job_counters(rpd_job* j, uint64_t* out, int n) {

@@ -256,4 +256,25 @@ void launch_metrics_a(const MetricImage& ref, const MetricImage& tst, int W, int
 void launch_metrics_b(const MetricImage& ref, const MetricImage& tst, int W, int H, const MetricsPassB& pb,
                       double* part, double* out, hipStream_t st);
 
+// ---- comparison denoisers (denoising/joint_bilateral.cpp, atrous_wavelet.cpp)
+// Images are interleaved float RGB, row-major, as the reference's tools read them.
+struct BilateralParams {
+    const float* raw; const float* normal; const float* albedo;   // normal encoded as (n + 1) / 2
+    float* gaussian; float* bilateral; float* joint;             // any may be null
+    int width, height, radius;
+    float sigma_c, sigma_n, sigma_p, sigma_a;
+};
+int bilateral_radius(float sigma_p);                              // max(1, ceil(3 sigma_p))
+void launch_joint_bilateral(const BilateralParams& bp, hipStream_t st);
+struct AtrousPass {
+    const float* in; float* out;
+    const float* unit;      // unit normals (3 per pixel)
+    const float* albedo;
+    float* lum;             // scratch: luminance of `in`, one per pixel
+    int width, height, step;
+    float sigma_c, sigma_n, sigma_a;
+};
+void launch_atrous_prepare(const float* normal, int64_t n, float* unit, hipStream_t st);
+void launch_atrous_pass(const AtrousPass& ap, hipStream_t st);
+
 }  // namespace rpd

@@ -108,6 +108,14 @@ class ImageMetrics(C.Structure):
         return d
 
 
+class BilateralConfig(C.Structure):
+    _fields_ = [("sigma_c", C.c_float), ("sigma_n", C.c_float), ("sigma_p", C.c_float), ("sigma_a", C.c_float)]
+
+
+class AtrousConfig(C.Structure):
+    _fields_ = [("sigma_c", C.c_float), ("sigma_n", C.c_float), ("sigma_a", C.c_float), ("iterations", C.c_int)]
+
+
 # rpd_sample_record as a numpy structured type (100 bytes)
 SAMPLE_RECORD = np.dtype([
     ("radiance", np.float32, 3), ("flags", np.uint32), ("normal", np.float32, 3), ("depth", np.float32),
@@ -166,6 +174,10 @@ def lib():
         L.rpd_render_config_defaults.restype = None
         L.rpd_procedural_config_defaults.argtypes = [C.POINTER(ProceduralConfig)]
         L.rpd_procedural_config_defaults.restype = None
+        L.rpd_bilateral_config_defaults.argtypes = [C.POINTER(BilateralConfig)]
+        L.rpd_bilateral_config_defaults.restype = None
+        L.rpd_atrous_config_defaults.argtypes = [C.POINTER(AtrousConfig)]
+        L.rpd_atrous_config_defaults.restype = None
         L.rpd_synth_config_defaults.argtypes = [C.POINTER(SynthConfig)]
         L.rpd_synth_config_defaults.restype = None
         for name, args in [
@@ -196,6 +208,8 @@ def lib():
             ("rpd_abi_struct_sizes", [vp, C.c_int]),
             ("rpd_image_metrics_compute", [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.POINTER(ImageMetrics)]),
             ("rpd_job_image_metrics", [vp, C.c_int, vp, C.c_int, C.POINTER(ImageMetrics)]),
+            ("rpd_joint_bilateral", [vp, vp, vp, vp, C.c_int, C.c_int, C.POINTER(BilateralConfig), vp, vp, vp, C.c_int]),
+            ("rpd_atrous", [vp, vp, vp, vp, C.c_int, C.c_int, C.POINTER(AtrousConfig), vp, C.c_int]),
             ("rpd_procedural_scene_arrays", [C.POINTER(ProceduralConfig), vp, vp, vp, vp, vp, vp, vp, vp, vp]),
             ("rpd_scene_procedural", [vp, C.POINTER(ProceduralConfig), C.POINTER(vp)]),
             ("rpd_scene_query", [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]),
@@ -298,6 +312,41 @@ def image_metrics(ctx, ref, test):
     m = ImageMetrics()
     check(lib().rpd_image_metrics_compute(ctx.h, _ptr(ref), _ptr(test), ref.shape[1], ref.shape[0], 0, C.byref(m)))
     return m.as_dict()
+
+
+def _rgb_image(a):
+    a = np.ascontiguousarray(a, np.float32)
+    if a.ndim != 3 or a.shape[2] != 3:
+        raise ValueError("images must be (H, W, 3)")
+    return a
+
+
+def joint_bilateral(ctx, raw, normal, albedo, **sigmas):
+    """denoising/joint_bilateral.cpp on the device: (gaussian, bilateral, joint)
+    images from host images (normal encoded as (n + 1) / 2)."""
+    raw, normal, albedo = _rgb_image(raw), _rgb_image(normal), _rgb_image(albedo)
+    cfg = BilateralConfig()
+    lib().rpd_bilateral_config_defaults(C.byref(cfg))
+    for k, v in sigmas.items():
+        setattr(cfg, k, v)
+    outs = [np.zeros_like(raw) for _ in range(3)]
+    h, w = raw.shape[:2]
+    check(lib().rpd_joint_bilateral(ctx.h, _ptr(raw), _ptr(normal), _ptr(albedo), w, h, C.byref(cfg),
+                                    *[_ptr(o) for o in outs], 0))
+    return tuple(outs)
+
+
+def atrous(ctx, raw, normal, albedo, **kw):
+    """denoising/atrous_wavelet.cpp on the device, on host images."""
+    raw, normal, albedo = _rgb_image(raw), _rgb_image(normal), _rgb_image(albedo)
+    cfg = AtrousConfig()
+    lib().rpd_atrous_config_defaults(C.byref(cfg))
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    out = np.zeros_like(raw)
+    h, w = raw.shape[:2]
+    check(lib().rpd_atrous(ctx.h, _ptr(raw), _ptr(normal), _ptr(albedo), w, h, C.byref(cfg), _ptr(out), 0))
+    return out
 
 
 def procedural_config(**kw):

@@ -162,6 +162,9 @@ def lib():
         L.orc_cornell_triangles.restype = C.c_long
         L.orc_cornell_hits.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.orc_image_metrics.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        L.orc_joint_bilateral.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                                          vp, vp, vp]
+        L.orc_atrous.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, vp]
         _lib = L
     return _lib
 
@@ -172,6 +175,31 @@ def ptr(a):
 
 METRIC_NAMES = ("mae", "rmse", "psnr", "peak", "ssim_r", "ssim_g", "ssim_b", "ssim_mean", "nrmse", "log_ssim11",
                 "log_gradient_nrmse", "relmse")
+
+
+def _rgb(a):
+    a = np.ascontiguousarray(a, np.float32)
+    assert a.ndim == 3 and a.shape[2] == 3
+    return a
+
+
+def joint_bilateral(raw, normal, albedo, sigma_c=1.0, sigma_n=1.0, sigma_p=1.0, sigma_a=1.0):
+    """denoising/joint_bilateral.cpp filter(): (gaussian, bilateral, joint), each (H, W, 3)."""
+    raw, normal, albedo = _rgb(raw), _rgb(normal), _rgb(albedo)
+    h, w = raw.shape[:2]
+    outs = [np.zeros_like(raw) for _ in range(3)]
+    lib().orc_joint_bilateral(ptr(raw), ptr(normal), ptr(albedo), w, h, sigma_c, sigma_n, sigma_p, sigma_a,
+                              *[ptr(o) for o in outs])
+    return tuple(outs)
+
+
+def atrous(raw, normal, albedo, sigma_c=0.5, sigma_n=0.5, sigma_a=0.1, iterations=5):
+    """denoising/atrous_wavelet.cpp filter(): the filtered (H, W, 3) image."""
+    raw, normal, albedo = _rgb(raw), _rgb(normal), _rgb(albedo)
+    h, w = raw.shape[:2]
+    out = np.zeros_like(raw)
+    lib().orc_atrous(ptr(raw), ptr(normal), ptr(albedo), w, h, sigma_c, sigma_n, sigma_a, iterations, ptr(out))
+    return out
 
 
 def image_metrics(ref, test):

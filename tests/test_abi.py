@@ -26,10 +26,11 @@ def test_library_exports_every_declared_function():
 
 
 def test_struct_sizes_match_python_mirrors():
-    out = np.zeros(12, np.int32)
-    eng.check(eng.lib().rpd_abi_struct_sizes(out.ctypes.data_as(C.c_void_p), 12))
+    out = np.zeros(14, np.int32)
+    eng.check(eng.lib().rpd_abi_struct_sizes(out.ctypes.data_as(C.c_void_p), 14))
     mirrors = [eng.RenderConfig, eng.Camera, eng.Triangle, eng.Sphere, eng.Material, eng.Light, None,
-               eng.SynthConfig, eng.ProceduralConfig, eng.ServerConfig, eng.ServerStats, eng.ImageMetrics]
+               eng.SynthConfig, eng.ProceduralConfig, eng.ServerConfig, eng.ServerStats, eng.ImageMetrics,
+               eng.BilateralConfig, eng.AtrousConfig]
     for k, m in enumerate(mirrors):
         want = eng.SAMPLE_RECORD.itemsize if m is None else C.sizeof(m)
         assert out[k] == want, (k, m, out[k], want)
