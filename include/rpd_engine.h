@@ -117,8 +117,21 @@ int rpd_ctx_set_memory_budget(rpd_context* ctx, uint64_t bytes);
 int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, const rpd_sphere* spheres,
                      int n_spheres, const rpd_material* materials, int n_materials, const rpd_light* lights,
                      int n_lights, const float environment[3], rpd_scene** out);
-/* Built-in scenes by name (scenes/scene_registry.h:18): "cornell_box". */
+/* The reference's built-in scenes by name (scenes/scene_registry.h:17-53):
+   cornell_box, cornell_spheres, veach_mis, material_showcase, caustics,
+   debug, and the mesh scenes bunny, bunny_dof and dragon, which read
+   resources/models/{bunny,dragon}.obj relative to the working directory as
+   the reference does. ctx NULL keeps the scene on the host (rpd_scene_query). */
 int rpd_scene_builtin(rpd_context* ctx, const char* name, rpd_scene** out);
+/* The same scene as arrays. Pass NULL arrays to get the counts; otherwise
+   each n_* holds the array's capacity on entry and the count on return. */
+int rpd_builtin_scene_arrays(const char* name, rpd_triangle* tris, int* n_tris, rpd_sphere* spheres, int* n_spheres,
+                             rpd_material* materials, int* n_materials, rpd_light* lights, int* n_lights,
+                             float environment[3]);
+/* Triangles of an OBJ mesh (core/obj_loader.h:19-65): every face's vertex
+   positions (polygons split into a fan from their first vertex), normal from
+   the winding, no emission, the given material. tris NULL: count only. */
+int rpd_obj_load(const char* path, int material_id, rpd_triangle* tris, int* n_tris);
 int rpd_scene_counts(const rpd_scene* scene, int* n_tris, int* n_spheres, int* n_materials, int* n_lights);
 int rpd_scene_destroy(rpd_scene* scene);
 

@@ -13,9 +13,11 @@
 
 #include "../../include/rpd_engine.h"
 #include "bvh_build.h"
+#include "host_scene.h"
 #include "kernels.h"
 
 using namespace rpd;
+using namespace rpd_host;
 
 namespace {
 
@@ -32,25 +34,6 @@ int fail(int code, const std::string& msg) {
         if (e_ != hipSuccess) return fail(RPD_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// ---- host float3 with the reference's evaluation order ----------------------
-struct h3 { float x, y, z; };
-h3 H3(float a, float b, float c) { return {a, b, c}; }
-h3 operator+(h3 a, h3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-h3 operator-(h3 a, h3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-h3 operator-(h3 a) { return {-a.x, -a.y, -a.z}; }
-h3 operator*(float s, h3 a) { return {s * a.x, s * a.y, s * a.z}; }
-h3 operator*(h3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
-h3 operator/(h3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
-float hdot(h3 a, h3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }
-float hsq(h3 a) { return hdot(a, a); }
-h3 hnorm(h3 a) {
-    const float z = hsq(a);
-    if (z > 0.0f) { const float s = std::sqrt(z); return {a.x / s, a.y / s, a.z / s}; }
-    return a;
-}
-h3 hcross(h3 a, h3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-void put3(float* d, h3 v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; }
-h3 get3(const float* s) { return {s[0], s[1], s[2]}; }
 float lum(h3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
 template <class T>
@@ -260,78 +243,6 @@ int make_camera(const rpd_camera& in, float aspect, rpd::Camera& out) {
     put3(out.llc, llc); put3(out.horiz, horiz); put3(out.vert, vert);
     out.lens_radius = 0.5f * in.aperture;
     return RPD_OK;
-}
-
-// ---- built-in cornell_box: scenes/cornell_box.h:9-109 --------------------------
-struct HostScene {
-    std::vector<rpd_triangle> tris;
-    std::vector<rpd_material> mats;
-    std::vector<rpd_light> lights;
-    float env[3] = {0, 0, 0};
-};
-void build_cornell(HostScene& s) {
-    auto lam = [&](float r, float g, float b) {
-        rpd_material m{};
-        m.kind = RPD_MAT_LAMBERTIAN; m.albedo[0] = r; m.albedo[1] = g; m.albedo[2] = b;
-        m.roughness = 0.5f;
-        s.mats.push_back(m);
-        return int(s.mats.size()) - 1;
-    };
-    const int white = lam(0.78f, 0.78f, 0.78f), red = lam(0.63f, 0.065f, 0.05f), green = lam(0.14f, 0.45f, 0.091f);
-    const int light_mat = lam(1.0f, 1.0f, 1.0f), box_white = lam(0.73f, 0.73f, 0.73f);
-    const float sc = 0.001f;
-    auto P = [&](float x, float y, float z) { return H3(x * sc, y * sc, z * sc); };
-    const h3 center = P(278.0f, 274.4f, 279.6f);
-    const h3 zero = H3(0, 0, 0);
-    auto add_tri = [&](h3 a, h3 b, h3 c, int mat, h3 em, h3 ref, bool toward, int light) {
-        rpd_triangle t{};
-        put3(t.v0, a); put3(t.v1, b); put3(t.v2, c); put3(t.emission, em);
-        const h3 n = hcross(b - a, c - a);
-        h3 nn = hsq(n) > 0.0f ? hnorm(n) : H3(0, 1, 0);
-        const h3 mid = (a + b + c) / 3.0f;
-        const float orient = hdot(nn, ref - mid);
-        if (toward ? (orient < 0.0f) : (orient > 0.0f)) nn = -nn;
-        put3(t.normal, nn);
-        t.material_id = mat;
-        t.light_id = light;
-        s.tris.push_back(t);
-    };
-    auto quad = [&](h3 a, h3 b, h3 c, h3 d, int mat, h3 ref, bool toward) {
-        add_tri(a, b, c, mat, zero, ref, toward, -1);
-        add_tri(a, c, d, mat, zero, ref, toward, -1);
-    };
-    const h3 em = H3(70.0f, 70.0f, 70.0f);
-    const h3 lo = P(213.0f, 548.0f, 227.0f);
-    rpd_light L{};
-    L.kind = RPD_LIGHT_RECT;
-    put3(L.p0, lo); put3(L.p1, P(343.0f, 548.0f, 227.0f) - lo); put3(L.p2, P(213.0f, 548.0f, 332.0f) - lo);
-    put3(L.emission, em);
-    s.lights.push_back(L);
-    const h3 l0 = P(213.0f, 548.0f, 227.0f), l1 = P(343.0f, 548.0f, 227.0f), l2 = P(343.0f, 548.0f, 332.0f),
-             l3 = P(213.0f, 548.0f, 332.0f);
-    add_tri(l0, l2, l1, light_mat, em, center, true, 0);
-    add_tri(l0, l3, l2, light_mat, em, center, true, 0);
-    quad(P(552.8f, 0, 0), P(0, 0, 0), P(0, 0, 559.2f), P(549.6f, 0, 559.2f), white, center, true);
-    quad(P(556.0f, 548.8f, 0), P(556.0f, 548.8f, 559.2f), P(0, 548.8f, 559.2f), P(0, 548.8f, 0), white, center, true);
-    quad(P(549.6f, 0, 559.2f), P(0, 0, 559.2f), P(0, 548.8f, 559.2f), P(556.0f, 548.8f, 559.2f), white, center, true);
-    quad(P(0, 0, 559.2f), P(0, 0, 0), P(0, 548.8f, 0), P(0, 548.8f, 559.2f), green, center, true);
-    quad(P(552.8f, 0, 0), P(549.6f, 0, 559.2f), P(556.0f, 548.8f, 559.2f), P(556.0f, 548.8f, 0), red, center, true);
-    const h3 sb[8] = {P(130, 0, 65), P(82, 0, 225), P(240, 0, 272), P(290, 0, 114),
-                      P(130, 165, 65), P(82, 165, 225), P(240, 165, 272), P(290, 165, 114)};
-    const h3 sbc = (sb[0] + sb[2] + sb[5] + sb[7]) * 0.25f;
-    quad(sb[4], sb[5], sb[6], sb[7], box_white, sbc, false);
-    quad(sb[0], sb[4], sb[7], sb[3], box_white, sbc, false);
-    quad(sb[3], sb[7], sb[6], sb[2], box_white, sbc, false);
-    quad(sb[2], sb[6], sb[5], sb[1], box_white, sbc, false);
-    quad(sb[1], sb[5], sb[4], sb[0], box_white, sbc, false);
-    const h3 tb[8] = {P(423, 0, 247), P(265, 0, 296), P(314, 0, 456), P(472, 0, 406),
-                      P(423, 330, 247), P(265, 330, 296), P(314, 330, 456), P(472, 330, 406)};
-    const h3 tbc = (tb[0] + tb[2] + tb[5] + tb[7]) * 0.25f;
-    quad(tb[4], tb[5], tb[6], tb[7], box_white, tbc, false);
-    quad(tb[0], tb[4], tb[7], tb[3], box_white, tbc, false);
-    quad(tb[3], tb[7], tb[6], tb[2], box_white, tbc, false);
-    quad(tb[2], tb[6], tb[5], tb[1], box_white, tbc, false);
-    quad(tb[1], tb[5], tb[4], tb[0], box_white, tbc, false);
 }
 
 // ---- job pipeline -----------------------------------------------------------------
@@ -993,13 +904,44 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
 }
 
 int rpd_scene_builtin(rpd_context* ctx, const char* name, rpd_scene** out) {
-    if (!ctx || !name || !out) return fail(RPD_ERR_INVALID_ARG, "null argument");
-    if (std::string(name) != "cornell_box")
-        return fail(RPD_ERR_UNSUPPORTED, std::string("Unknown scene '") + name + "'. Available scenes: cornell_box");
+    if (!name || !out) return fail(RPD_ERR_INVALID_ARG, "null argument");
     HostScene s;
-    build_cornell(s);
-    return rpd_scene_create(ctx, s.tris.data(), int(s.tris.size()), nullptr, 0, s.mats.data(), int(s.mats.size()),
-                            s.lights.data(), int(s.lights.size()), s.env, out);
+    std::string err;
+    if (int st = build_builtin_scene(name, s, err)) return fail(st, err);
+    return rpd_scene_create(ctx, s.tris.data(), int(s.tris.size()), s.spheres.data(), int(s.spheres.size()),
+                            s.mats.data(), int(s.mats.size()), s.lights.data(), int(s.lights.size()), s.env, out);
+}
+
+int rpd_builtin_scene_arrays(const char* name, rpd_triangle* tris, int* n_tris, rpd_sphere* spheres, int* n_spheres,
+                             rpd_material* mats, int* n_mats, rpd_light* lights, int* n_lights, float env[3]) {
+    if (!name || !n_tris || !n_spheres || !n_mats || !n_lights) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    HostScene s;
+    std::string err;
+    if (int st = build_builtin_scene(name, s, err)) return fail(st, err);
+    const bool fits = (!tris || *n_tris >= int(s.tris.size())) && (!spheres || *n_spheres >= int(s.spheres.size())) &&
+                      (!mats || *n_mats >= int(s.mats.size())) && (!lights || *n_lights >= int(s.lights.size()));
+    if (!fits) return fail(RPD_ERR_INVALID_ARG, "an output array is smaller than the scene's");
+    if (tris) std::copy(s.tris.begin(), s.tris.end(), tris);
+    if (spheres) std::copy(s.spheres.begin(), s.spheres.end(), spheres);
+    if (mats) std::copy(s.mats.begin(), s.mats.end(), mats);
+    if (lights) std::copy(s.lights.begin(), s.lights.end(), lights);
+    if (env) std::copy(s.env, s.env + 3, env);
+    *n_tris = int(s.tris.size()); *n_spheres = int(s.spheres.size());
+    *n_mats = int(s.mats.size()); *n_lights = int(s.lights.size());
+    return RPD_OK;
+}
+
+int rpd_obj_load(const char* path, int material_id, rpd_triangle* tris, int* n_tris) {
+    if (!path || !n_tris) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    std::vector<rpd_triangle> t;
+    std::string err;
+    if (int st = load_obj_triangles(path, material_id, t, err)) return fail(st, err);
+    if (tris) {
+        if (*n_tris < int(t.size())) return fail(RPD_ERR_INVALID_ARG, "output array smaller than the mesh");
+        std::copy(t.begin(), t.end(), tris);
+    }
+    *n_tris = int(t.size());
+    return RPD_OK;
 }
 
 int rpd_scene_query(const rpd_scene* s, int n, const float* origins, const float* dirs, const float* t_max,

@@ -155,6 +155,11 @@ def records_from_rows(rows):
     return r
 
 
+# scene POD enums (include/rpd_engine.h)
+RPD_MAT_LAMBERTIAN, RPD_MAT_MICROFACET, RPD_MAT_MIRROR = 0, 1, 2
+RPD_LIGHT_RECT, RPD_LIGHT_AREA, RPD_LIGHT_POINT, RPD_LIGHT_DIRECTIONAL = 0, 1, 2, 3
+
+
 class EngineError(RuntimeError):
     pass
 
@@ -208,6 +213,8 @@ def lib():
             ("rpd_abi_struct_sizes", [vp, C.c_int]),
             ("rpd_image_metrics_compute", [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.POINTER(ImageMetrics)]),
             ("rpd_job_image_metrics", [vp, C.c_int, vp, C.c_int, C.POINTER(ImageMetrics)]),
+            ("rpd_builtin_scene_arrays", [C.c_char_p, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+            ("rpd_obj_load", [C.c_char_p, C.c_int, vp, vp]),
             ("rpd_joint_bilateral", [vp, vp, vp, vp, C.c_int, C.c_int, C.POINTER(BilateralConfig), vp, vp, vp, C.c_int]),
             ("rpd_atrous", [vp, vp, vp, vp, C.c_int, C.c_int, C.POINTER(AtrousConfig), vp, C.c_int]),
             ("rpd_procedural_scene_arrays", [C.POINTER(ProceduralConfig), vp, vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -359,6 +366,32 @@ def procedural_config(**kw):
         else:
             setattr(c, k, v)
     return c
+
+
+def builtin_scene_arrays(name):
+    """(triangles, spheres, materials, lights, env, counts) of a reference scene
+    (scenes/scene_registry.h); mesh scenes read resources/models/*.obj from the
+    working directory."""
+    L = lib()
+    n = [C.c_int() for _ in range(4)]
+    check(L.rpd_builtin_scene_arrays(name.encode(), None, C.byref(n[0]), None, C.byref(n[1]), None, C.byref(n[2]),
+                                     None, C.byref(n[3]), None))
+    tris, sph = (Triangle * max(1, n[0].value))(), (Sphere * max(1, n[1].value))()
+    mats, lights = (Material * max(1, n[2].value))(), (Light * max(1, n[3].value))()
+    env = (C.c_float * 3)()
+    check(L.rpd_builtin_scene_arrays(name.encode(), tris, C.byref(n[0]), sph, C.byref(n[1]), mats, C.byref(n[2]),
+                                     lights, C.byref(n[3]), env))
+    return tris, sph, mats, lights, env, [x.value for x in n]
+
+
+def obj_triangles(path, material_id=0):
+    """Triangles of an OBJ mesh as the engine reads it (core/obj_loader.h)."""
+    L = lib()
+    n = C.c_int()
+    check(L.rpd_obj_load(str(path).encode(), material_id, None, C.byref(n)))
+    tris = (Triangle * max(1, n.value))()
+    check(L.rpd_obj_load(str(path).encode(), material_id, tris, C.byref(n)))
+    return tris, n.value
 
 
 def procedural_arrays(pc):
