@@ -521,7 +521,10 @@ int job_alloc(rpd_job* j) {
     const size_t nn = size_t(j->geom.nx) * j->geom.ny;
     auto& o = j->owned;
     int st;
-#define A(ptr, n) if ((st = dalloc(&(ptr), (n), o))) return st
+#define A(ptr, n)                                        \
+    do {                                                 \
+        if ((st = dalloc(&(ptr), (n), o))) return st;    \
+    } while (0)
     float* sums; A(sums, 13 * N);
     j->pb.radiance = sums; j->pb.albedo = sums + 3 * N; j->pb.normal = sums + 6 * N; j->pb.world_pos = sums + 9 * N;
     j->pb.depth = sums + 12 * N;
@@ -544,8 +547,9 @@ int job_alloc(rpd_job* j) {
     j->work = plan + 3 * N + 4;
     A(j->scan_scratch, (size_t)plan_scan_scratch_elems(j->N));
     A(j->query_count, 2);
-    if (j->denoiser) A(j->tape, 1);
-    else A(j->tape, ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_N);
+    // a denoiser job draws no random numbers: one placeholder word instead of the tapes
+    const size_t tape_words = j->denoiser ? 1 : ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_N;
+    A(j->tape, tape_words);
     const int per_px = j->denoiser ? j->max_spp : std::max(j->cfg.samples_per_pixel, j->cfg.adaptive_spp + 1);
     const int64_t cap = int64_t(per_px) * int64_t(N);
     j->rec.capacity = cap;

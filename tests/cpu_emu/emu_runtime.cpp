@@ -1,0 +1,396 @@
+// Runtime of the CPU emulation declared in include/hip/hip_runtime.h (tests
+// only). Each block runs its threads as fibers on the launching OS thread;
+// barriers and 64-lane wave collectives are released once every live thread
+// (every live lane of the wave) has arrived, and a state in which nothing can
+// run or be released is reported as a deadlock and aborts.
+#include <hip/hip_runtime.h>
+
+#include <dlfcn.h>
+#include <ucontext.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#if defined(__SANITIZE_ADDRESS__)
+#include <sanitizer/common_interface_defs.h>
+#define EMU_ASAN 1
+#else
+#define EMU_ASAN 0
+#endif
+
+struct ihipStream_t {
+    bool capturing = false;
+    ihipGraph* graph = nullptr;
+};
+struct ihipEvent_t {
+    std::chrono::steady_clock::time_point t;
+};
+struct ihipGraph {
+    std::vector<std::function<void()>> nodes;
+};
+struct hipGraphExec {
+    std::vector<std::function<void()>> nodes;
+};
+
+namespace rpd_emu {
+namespace {
+
+constexpr size_t kStack = 512 * 1024;
+constexpr int kWave = 64;
+constexpr size_t kMaxDynLds = 160 * 1024;  // LDS of one CU on gfx950
+
+enum FiberState { RUNNABLE, AT_BARRIER, AT_WAVE, DONE };
+
+struct Fiber {
+    ucontext_t ctx;
+    char* stack = nullptr;
+    FiberState state = RUNNABLE;
+    unsigned tid = 0;
+};
+
+struct Block {
+    dim3 grid, dim;
+    uint3 idx{0, 0, 0};
+    std::vector<Fiber> fibers;
+    ucontext_t sched;
+    int cur = -1;
+    void* dyn = nullptr;
+    std::vector<uint64_t> slot;     // per thread: wave-collective input
+    std::vector<uint64_t> result;   // per thread: wave-collective output
+    std::vector<int> mode, arg;
+    std::vector<void*> site;        // per thread: return address of the collective's caller
+    std::vector<long> count;        // per thread: wave collectives completed
+    const std::function<void()>* body = nullptr;
+    const void* sched_stack_bottom = nullptr;
+    size_t sched_stack_size = 0;
+};
+
+thread_local Block* g_block = nullptr;
+thread_local hipError_t g_last = hipSuccess;
+
+void fail_launch(const char* what) {
+    std::fprintf(stderr, "[cpu-emu] %s\n", what);
+    std::fflush(stderr);
+    std::abort();
+}
+
+Fiber& me() { return g_block->fibers[size_t(g_block->cur)]; }
+
+// fiber -> scheduler
+void yield_to_scheduler() {
+    Block* b = g_block;
+    Fiber& f = me();
+#if EMU_ASAN
+    void* fake = nullptr;
+    __sanitizer_start_switch_fiber(&fake, b->sched_stack_bottom, b->sched_stack_size);
+#endif
+    swapcontext(&f.ctx, &b->sched);
+#if EMU_ASAN
+    __sanitizer_finish_switch_fiber(fake, &b->sched_stack_bottom, &b->sched_stack_size);
+#endif
+}
+
+void fiber_entry() {
+    Block* b = g_block;
+#if EMU_ASAN
+    __sanitizer_finish_switch_fiber(nullptr, &b->sched_stack_bottom, &b->sched_stack_size);
+#endif
+    (*b->body)();
+    me().state = DONE;
+#if EMU_ASAN
+    __sanitizer_start_switch_fiber(nullptr, b->sched_stack_bottom, b->sched_stack_size);
+#endif
+    swapcontext(&me().ctx, &b->sched);
+}
+
+// scheduler -> fiber k
+void resume(Block& b, int k) {
+    b.cur = k;
+    Fiber& f = b.fibers[size_t(k)];
+#if EMU_ASAN
+    void* fake = nullptr;
+    __sanitizer_start_switch_fiber(&fake, f.stack, kStack);
+#endif
+    swapcontext(&b.sched, &f.ctx);
+#if EMU_ASAN
+    __sanitizer_finish_switch_fiber(fake, nullptr, nullptr);
+#endif
+    b.cur = -1;
+}
+
+// the collective of wave w, once all its live lanes have deposited their input
+void complete_wave(Block& b, int w, int n) {
+    const int lo = w * kWave, hi = std::min(n, lo + kWave);
+    uint64_t ballot = 0;
+    int md = -1;
+    void* site0 = nullptr;
+    for (int t = lo; t < hi; ++t) {
+        const Fiber& f = b.fibers[size_t(t)];
+        if (f.state != AT_WAVE) continue;
+        if (md < 0) { md = b.mode[size_t(t)]; site0 = b.site[size_t(t)]; }
+        else if (md != b.mode[size_t(t)] || site0 != b.site[size_t(t)]) {
+            for (int u = lo; u < hi; ++u)
+                if (b.fibers[size_t(u)].state == AT_WAVE)
+                {
+                    Dl_info di{};
+                    dladdr(b.site[size_t(u)], &di);
+                    std::fprintf(stderr, "[cpu-emu] lane %d (collective #%ld): operation %d called from %s+0x%lx\n",
+                                 u - lo, b.count[size_t(u)], b.mode[size_t(u)], di.dli_fname ? di.dli_fname : "?",
+                                 (unsigned long)((char*)b.site[size_t(u)] - (char*)di.dli_fbase));
+                }
+            fail_launch("divergent wave collectives (lanes of one wave in different collectives)");
+        }
+        if (b.slot[size_t(t)]) ballot |= 1ull << (t - lo);
+    }
+    for (int t = lo; t < hi; ++t) {
+        const Fiber& f = b.fibers[size_t(t)];
+        if (f.state != AT_WAVE) continue;
+        const int lane = t - lo, a = b.arg[size_t(t)];
+        if (md == 0) {
+            b.result[size_t(t)] = ballot;
+        } else {
+            const int src = md == 1 ? a : (lane ^ a);
+            const int st = lo + ((src % kWave) + kWave) % kWave;
+            // a source lane outside the block or already finished reads as its own value
+            b.result[size_t(t)] = (st < hi && b.fibers[size_t(st)].state == AT_WAVE) ? b.slot[size_t(st)] : b.slot[size_t(t)];
+        }
+    }
+    // release only after every result is computed: sources are read while still waiting
+    for (int t = lo; t < hi; ++t) {
+        Fiber& f = b.fibers[size_t(t)];
+        if (f.state != AT_WAVE) continue;
+        f.state = RUNNABLE;
+        ++b.count[size_t(t)];
+    }
+}
+
+void run_block(Block& b) {
+    const int n = int(b.fibers.size());
+    for (int t = 0; t < n; ++t) {
+        Fiber& f = b.fibers[size_t(t)];
+        f.state = RUNNABLE;
+        f.tid = unsigned(t);
+        b.count[size_t(t)] = 0;
+        getcontext(&f.ctx);
+        f.ctx.uc_stack.ss_sp = f.stack;
+        f.ctx.uc_stack.ss_size = kStack;
+        f.ctx.uc_link = nullptr;
+        makecontext(&f.ctx, fiber_entry, 0);
+    }
+    while (true) {
+        for (int t = 0; t < n; ++t)
+            while (b.fibers[size_t(t)].state == RUNNABLE) resume(b, t);
+        int live = 0, at_barrier = 0;
+        bool released = false;
+        for (int w = 0; w * kWave < n; ++w) {
+            int wl = 0, ww = 0;
+            for (int t = w * kWave; t < std::min(n, (w + 1) * kWave); ++t) {
+                const FiberState s = b.fibers[size_t(t)].state;
+                if (s != DONE) ++wl;
+                if (s == AT_WAVE) ++ww;
+            }
+            if (ww > 0 && ww == wl) { complete_wave(b, w, n); released = true; }
+        }
+        if (released) continue;
+        for (const Fiber& f : b.fibers) {
+            if (f.state != DONE) ++live;
+            if (f.state == AT_BARRIER) ++at_barrier;
+        }
+        if (live == 0) break;
+        if (at_barrier == live) {
+            for (Fiber& f : b.fibers) if (f.state == AT_BARRIER) f.state = RUNNABLE;
+            continue;
+        }
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "deadlock in block (%u,%u,%u): %d live threads, %d at __syncthreads",
+                      b.idx.x, b.idx.y, b.idx.z, live, at_barrier);
+        fail_launch(msg);
+    }
+}
+
+void execute(dim3 grid, dim3 block, size_t dyn_lds, const std::function<void()>& body) {
+    const unsigned nthreads = block.x * block.y * block.z;
+    Block b;
+    b.grid = grid;
+    b.dim = block;
+    b.body = &body;
+    b.fibers.resize(nthreads);
+    for (Fiber& f : b.fibers) f.stack = static_cast<char*>(std::malloc(kStack));
+    b.slot.assign(nthreads, 0);
+    b.result.assign(nthreads, 0);
+    b.mode.assign(nthreads, 0);
+    b.arg.assign(nthreads, 0);
+    b.site.assign(nthreads, nullptr);
+    b.count.assign(nthreads, 0);
+    Block* outer = g_block;
+    g_block = &b;
+    for (unsigned z = 0; z < grid.z; ++z)
+        for (unsigned y = 0; y < grid.y; ++y)
+            for (unsigned x = 0; x < grid.x; ++x) {
+                b.idx = {x, y, z};
+                // fresh per block, so overruns of the declared size are caught; filled
+                // with a pattern because LDS starts with undefined contents
+                b.dyn = std::malloc(dyn_lds ? dyn_lds : 1);
+                std::memset(b.dyn, 0xA5, dyn_lds ? dyn_lds : 1);
+                run_block(b);
+                std::free(b.dyn);
+                b.dyn = nullptr;
+            }
+    g_block = outer;
+    for (Fiber& f : b.fibers) std::free(f.stack);
+}
+
+}  // namespace
+
+hipError_t launch(dim3 grid, dim3 block, size_t dyn_lds, hipStream_t s, std::function<void()> body) {
+    const unsigned long long nthreads = 1ull * block.x * block.y * block.z;
+    if (nthreads == 0 || nthreads > 1024 || grid.x == 0 || grid.y == 0 || grid.z == 0 || grid.y > 65535 ||
+        grid.z > 65535 || dyn_lds > kMaxDynLds) {
+        g_last = hipErrorInvalidConfiguration;
+        return g_last;
+    }
+    if (s && s->capturing) {
+        s->graph->nodes.push_back([grid, block, dyn_lds, body]() { execute(grid, block, dyn_lds, body); });
+        return hipSuccess;
+    }
+    execute(grid, block, dyn_lds, body);
+    return hipSuccess;
+}
+
+uint3 thread_idx() {
+    const Block& b = *g_block;
+    const unsigned t = b.fibers[size_t(b.cur)].tid;
+    return {t % b.dim.x, (t / b.dim.x) % b.dim.y, t / (b.dim.x * b.dim.y)};
+}
+uint3 block_idx() { return g_block->idx; }
+dim3 block_dim() { return g_block->dim; }
+dim3 grid_dim() { return g_block->grid; }
+void* dynamic_lds() { return g_block->dyn; }
+
+void block_barrier() {
+    me().state = AT_BARRIER;
+    yield_to_scheduler();
+}
+
+uint64_t wave_exchange_u64(uint64_t v, int mode, int arg) {
+    Block& b = *g_block;
+    const size_t t = size_t(b.cur);
+    b.site[t] = __builtin_return_address(0);
+    b.slot[t] = v;
+    b.mode[t] = mode;
+    b.arg[t] = arg;
+    me().state = AT_WAVE;
+    yield_to_scheduler();
+    return b.result[size_t(b.cur)];
+}
+
+}  // namespace rpd_emu
+
+// ---- runtime API --------------------------------------------------------------------------
+using rpd_emu::g_last;
+
+hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
+hipError_t hipSetDevice(int d) { return d == 0 ? hipSuccess : hipErrorInvalidValue; }
+hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
+    std::memset(p, 0, sizeof(*p));
+    std::strcpy(p->name, "cpu-emulation");
+    p->totalGlobalMem = size_t(16) << 30;
+    p->multiProcessorCount = 4;   // few blocks per persistent launch keeps emulation quick
+    p->warpSize = 64;
+    p->maxThreadsPerBlock = 1024;
+    p->sharedMemPerBlock = 64 * 1024;
+    return hipSuccess;
+}
+hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
+    if (a != hipDeviceAttributeMultiprocessorCount) return hipErrorInvalidValue;
+    *v = 4;
+    return hipSuccess;
+}
+hipError_t hipMemGetInfo(size_t* f, size_t* t) { *f = size_t(12) << 30; *t = size_t(16) << 30; return hipSuccess; }
+hipError_t hipMalloc(void** p, size_t bytes) {
+    *p = std::malloc(bytes ? bytes : 1);
+    if (!*p) return hipErrorOutOfMemory;
+    std::memset(*p, 0xA5, bytes ? bytes : 1);  // device memory is not cleared by the allocator
+    return hipSuccess;
+}
+hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { if (n) std::memmove(d, s, n); return hipSuccess; }
+hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t st) {
+    if (st && st->capturing) {
+        st->graph->nodes.push_back([d, s, n]() { if (n) std::memmove(d, s, n); });
+        return hipSuccess;
+    }
+    return hipMemcpy(d, s, n, k);
+}
+hipError_t hipMemset(void* d, int v, size_t n) { if (n) std::memset(d, v, n); return hipSuccess; }
+hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t st) {
+    if (st && st->capturing) {
+        st->graph->nodes.push_back([d, v, n]() { if (n) std::memset(d, v, n); });
+        return hipSuccess;
+    }
+    return hipMemset(d, v, n);
+}
+hipError_t hipStreamCreate(hipStream_t* s) { *s = new ihipStream_t(); return hipSuccess; }
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { return hipStreamCreate(s); }
+hipError_t hipStreamDestroy(hipStream_t s) { delete s; return hipSuccess; }
+hipError_t hipStreamSynchronize(hipStream_t s) {
+    return (s && s->capturing) ? hipErrorStreamCaptureUnsupported : hipSuccess;
+}
+hipError_t hipDeviceSynchronize() { return hipSuccess; }
+hipError_t hipEventCreate(hipEvent_t* e) { *e = new ihipEvent_t(); (*e)->t = std::chrono::steady_clock::now(); return hipSuccess; }
+hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) {
+    if (s && s->capturing) return hipErrorStreamCaptureUnsupported;
+    e->t = std::chrono::steady_clock::now();
+    return hipSuccess;
+}
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
+    *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
+    return hipSuccess;
+}
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
+    if (!s || s->capturing) return hipErrorInvalidValue;
+    s->capturing = true;
+    s->graph = new ihipGraph();
+    return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g) {
+    if (!s || !s->capturing) return hipErrorInvalidValue;
+    s->capturing = false;
+    *g = s->graph;
+    s->graph = nullptr;
+    return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*, char*, size_t) {
+    *e = new hipGraphExec();
+    (*e)->nodes = g->nodes;
+    return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    for (const auto& n : e->nodes) n();
+    return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }
+hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+hipError_t hipGetLastError() { const hipError_t e = g_last; g_last = hipSuccess; return e; }
+hipError_t hipPeekAtLastError() { return g_last; }
+const char* hipGetErrorString(hipError_t e) {
+    switch (e) {
+        case hipSuccess: return "hipSuccess";
+        case hipErrorInvalidValue: return "hipErrorInvalidValue";
+        case hipErrorOutOfMemory: return "hipErrorOutOfMemory";
+        case hipErrorInvalidConfiguration: return "hipErrorInvalidConfiguration";
+        case hipErrorStreamCaptureUnsupported: return "hipErrorStreamCaptureUnsupported";
+        default: return "hipErrorUnknown";
+    }
+}
+hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t lds) {
+    *n = lds > 160 * 1024 ? 0 : 1;
+    return hipSuccess;
+}

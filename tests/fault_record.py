@@ -8,7 +8,12 @@ to a build on which they passed repeatedly; that removes differences without
 naming the one that mattered, so it does not count as a fix, and the tests
 stay skipped until a cause is shown.
 """
+import os
+
 import pytest
 
-UNEXPLAINED_FAULT = pytest.mark.skip(
+# Under the CPU emulation of the kernels (tests/cpu_emu, RPD_CPU_EMU=1) these
+# tests run: nothing of them reaches a GPU there.
+UNEXPLAINED_FAULT = pytest.mark.skipif(
+    os.environ.get("RPD_CPU_EMU") != "1",
     reason="held off the GPU: a run of these tests ended in a GPU fault whose cause is not identified")
