@@ -14,7 +14,8 @@ sys.path.insert(0, os.path.join(REPO, "rpd-statmc-denoiser_amd"))
 # about the GPU build.
 if os.environ.get("RPD_CPU_EMU") == "1":
     import rpd_engine as _eng
-    _eng.LIB_PATH = os.path.join(REPO, "tests", "cpu_emu", "_build", "librpd_engine_emu.so")
+    _eng.LIB_PATH = os.environ.get("RPD_CPU_EMU_LIB",
+                                   os.path.join(REPO, "tests", "cpu_emu", "_build", "librpd_engine_emu.so"))
 
 
 def pytest_configure(config):

@@ -6,7 +6,6 @@
 #include <hip/hip_runtime.h>
 
 #include <dlfcn.h>
-#include <ucontext.h>
 
 #include <chrono>
 #include <cstdio>
@@ -36,6 +35,41 @@ struct hipGraphExec {
     std::vector<std::function<void()>> nodes;
 };
 
+// Minimal x86-64 context switch: pushes the callee-saved registers, MXCSR and
+// the x87 control word on the current stack, stores the stack pointer in
+// *from and continues on `to` (a stack saved the same way, or a fresh one
+// prepared by make_fiber_stack). No signal-mask system call, unlike
+// swapcontext.
+extern "C" void rpd_emu_switch(void** from, void* to);
+asm(R"(
+    .text
+    .globl rpd_emu_switch
+    .type rpd_emu_switch,@function
+rpd_emu_switch:
+    pushq %rbp
+    pushq %rbx
+    pushq %r12
+    pushq %r13
+    pushq %r14
+    pushq %r15
+    subq $8, %rsp
+    stmxcsr (%rsp)
+    fnstcw 4(%rsp)
+    movq %rsp, (%rdi)
+    movq %rsi, %rsp
+    ldmxcsr (%rsp)
+    fldcw 4(%rsp)
+    addq $8, %rsp
+    popq %r15
+    popq %r14
+    popq %r13
+    popq %r12
+    popq %rbx
+    popq %rbp
+    ret
+    .size rpd_emu_switch, .-rpd_emu_switch
+)");
+
 namespace rpd_emu {
 namespace {
 
@@ -46,7 +80,7 @@ constexpr size_t kMaxDynLds = 160 * 1024;  // LDS of one CU on gfx950
 enum FiberState { RUNNABLE, AT_BARRIER, AT_WAVE, DONE };
 
 struct Fiber {
-    ucontext_t ctx;
+    void* sp = nullptr;       // saved stack pointer while switched out
     char* stack = nullptr;
     FiberState state = RUNNABLE;
     unsigned tid = 0;
@@ -56,7 +90,7 @@ struct Block {
     dim3 grid, dim;
     uint3 idx{0, 0, 0};
     std::vector<Fiber> fibers;
-    ucontext_t sched;
+    void* sched_sp = nullptr;
     int cur = -1;
     void* dyn = nullptr;
     std::vector<uint64_t> slot;     // per thread: wave-collective input
@@ -88,7 +122,7 @@ void yield_to_scheduler() {
     void* fake = nullptr;
     __sanitizer_start_switch_fiber(&fake, b->sched_stack_bottom, b->sched_stack_size);
 #endif
-    swapcontext(&f.ctx, &b->sched);
+    rpd_emu_switch(&f.sp, b->sched_sp);
 #if EMU_ASAN
     __sanitizer_finish_switch_fiber(fake, &b->sched_stack_bottom, &b->sched_stack_size);
 #endif
@@ -104,7 +138,27 @@ void fiber_entry() {
 #if EMU_ASAN
     __sanitizer_start_switch_fiber(nullptr, b->sched_stack_bottom, b->sched_stack_size);
 #endif
-    swapcontext(&me().ctx, &b->sched);
+    void* dead = nullptr;
+    rpd_emu_switch(&dead, b->sched_sp);  // never resumed
+    __builtin_unreachable();
+}
+
+// A fresh stack laid out like one switched out by rpd_emu_switch: control
+// words, six zeroed callee-saved registers, then fiber_entry as the return
+// address, placed so that fiber_entry starts with the stack alignment of a
+// called function.
+void* make_fiber_stack(char* base, size_t size) {
+    uintptr_t top = (reinterpret_cast<uintptr_t>(base) + size) & ~uintptr_t(15);
+    uint64_t* sp = reinterpret_cast<uint64_t*>(top);
+    *--sp = 0;                                              // padding: entry sees rsp = 8 (mod 16)
+    *--sp = reinterpret_cast<uint64_t>(&fiber_entry);       // return address
+    for (int r = 0; r < 6; ++r) *--sp = 0;                  // rbp rbx r12 r13 r14 r15
+    uint32_t ctl[2];
+    asm volatile("stmxcsr %0" : "=m"(ctl[0]));
+    asm volatile("fnstcw %0" : "=m"(ctl[1]));
+    --sp;
+    std::memcpy(sp, ctl, 8);
+    return sp;
 }
 
 // scheduler -> fiber k
@@ -115,7 +169,7 @@ void resume(Block& b, int k) {
     void* fake = nullptr;
     __sanitizer_start_switch_fiber(&fake, f.stack, kStack);
 #endif
-    swapcontext(&b.sched, &f.ctx);
+    rpd_emu_switch(&b.sched_sp, f.sp);
 #if EMU_ASAN
     __sanitizer_finish_switch_fiber(fake, nullptr, nullptr);
 #endif
@@ -175,11 +229,7 @@ void run_block(Block& b) {
         f.state = RUNNABLE;
         f.tid = unsigned(t);
         b.count[size_t(t)] = 0;
-        getcontext(&f.ctx);
-        f.ctx.uc_stack.ss_sp = f.stack;
-        f.ctx.uc_stack.ss_size = kStack;
-        f.ctx.uc_link = nullptr;
-        makecontext(&f.ctx, fiber_entry, 0);
+        f.sp = make_fiber_stack(f.stack, kStack);
     }
     while (true) {
         for (int t = 0; t < n; ++t)

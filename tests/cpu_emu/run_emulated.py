@@ -19,7 +19,7 @@ def main(argv):
         build.build()
     argv = [a for a in argv if a != "--rebuild"]
     asan = subprocess.run(["g++", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
-    env = dict(os.environ, RPD_CPU_EMU="1", LD_PRELOAD=asan,
+    env = dict(os.environ, RPD_CPU_EMU="1", RPD_CPU_EMU_LIB=build.LIB, LD_PRELOAD=asan,
                ASAN_OPTIONS="detect_leaks=0:abort_on_error=0:halt_on_error=1:allocator_may_return_null=1")
     cmd = [sys.executable, "-m", "pytest", "-m", "gpu", "-p", "no:cacheprovider", *argv]
     return subprocess.call(cmd, cwd=REPO, env=env)

@@ -12,7 +12,6 @@ import numpy as np
 import pytest
 
 import oracle_bind as ob
-from fault_record import UNEXPLAINED_FAULT
 
 
 def _inputs(h, w, seed):
@@ -125,7 +124,6 @@ def test_atrous_keeps_a_linear_ramp():
 
 
 @pytest.mark.gpu
-@UNEXPLAINED_FAULT
 @pytest.mark.parametrize("h,w,sp,seed", [(37, 29, 2.0, 1), (64, 80, 0.3, 2), (20, 300, 4.5, 3)])
 def test_device_joint_bilateral_matches_oracle(engine_ctx, h, w, sp, seed):
     import rpd_engine as eng
@@ -138,7 +136,6 @@ def test_device_joint_bilateral_matches_oracle(engine_ctx, h, w, sp, seed):
 
 
 @pytest.mark.gpu
-@UNEXPLAINED_FAULT
 @pytest.mark.parametrize("h,w,iters,seed", [(37, 29, 5, 1), (130, 70, 3, 2), (16, 16, 1, 3)])
 def test_device_atrous_matches_oracle(engine_ctx, h, w, iters, seed):
     import rpd_engine as eng

@@ -12,7 +12,6 @@ import numpy as np
 import pytest
 
 import oracle_bind as ob
-from fault_record import UNEXPLAINED_FAULT
 from parity_util import rel_err
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -64,7 +63,6 @@ def test_oracle_reproduces_full_job():
 
 
 @pytest.mark.gpu
-@UNEXPLAINED_FAULT  # launches the same trace and StatMC kernels as test_gpu_parity.py
 def test_engine_against_golden_fixtures(engine_ctx):
     import rpd_engine as eng
     from parity_util import engine_camera, engine_config

@@ -7,10 +7,9 @@ import pytest
 
 import oracle_bind as ob
 import rpd_engine as eng
-from fault_record import UNEXPLAINED_FAULT
 from parity_util import engine_camera, engine_config, rel_err
 
-pytestmark = [pytest.mark.gpu, UNEXPLAINED_FAULT]
+pytestmark = pytest.mark.gpu
 
 
 def procedural_render_config(w, h, spp):

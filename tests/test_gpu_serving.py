@@ -9,10 +9,9 @@ import pytest
 
 import oracle_bind as ob
 import rpd_engine as eng
-from fault_record import UNEXPLAINED_FAULT
 from parity_util import engine_camera, engine_config
 
-pytestmark = [pytest.mark.gpu, UNEXPLAINED_FAULT]
+pytestmark = pytest.mark.gpu
 
 OUTPUTS = ("radiance", "raw_radiance", "denoised", "variance_denoised", "uncertainty", "sample_count")
 
