@@ -391,6 +391,9 @@ int reset_state(rpd_job* j) {
     HIP_TRY(hipMemsetAsync(j->pb.hit_count, 0, sizeof(int) * 2 * N, j->stream));
     HIP_TRY(hipMemsetAsync(j->pb.lum_mean, 0, sizeof(float) * 12 * N, j->stream));
     HIP_TRY(hipMemsetAsync(j->den, 0, sizeof(float) * 14 * N, j->stream));
+    // the finalized outputs too: a run that does not write one (raw radiance
+    // without StatMC) leaves it zero, as buffers.init does (core/buffers.h:121-140)
+    HIP_TRY(hipMemsetAsync(j->out_rad, 0, sizeof(float) * 16 * N, j->stream));
     HIP_TRY(hipMemsetAsync(j->grid, 0, sizeof(double) * GRID_NODE_DOUBLES * j->geom.nx * j->geom.ny, j->stream));
     HIP_TRY(hipMemsetAsync(j->tiles, 0, sizeof(float) * SF_COUNT * j->geom.nx * j->geom.ny, j->stream));
     HIP_TRY(hipMemsetAsync(j->counts, 0, sizeof(int) * (3 * N + 8), j->stream));  // plan, totals, work counter

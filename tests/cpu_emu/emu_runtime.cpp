@@ -7,6 +7,7 @@
 
 #include <dlfcn.h>
 
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -365,7 +366,12 @@ hipError_t hipMemGetInfo(size_t* f, size_t* t) { *f = size_t(12) << 30; *t = siz
 hipError_t hipMalloc(void** p, size_t bytes) {
     *p = std::malloc(bytes ? bytes : 1);
     if (!*p) return hipErrorOutOfMemory;
-    std::memset(*p, 0xA5, bytes ? bytes : 1);  // device memory is not cleared by the allocator
+    // Device memory is not cleared by the allocator, and what it holds depends
+    // on earlier allocations: each allocation gets a different fill byte, so
+    // results that read memory nobody wrote differ between otherwise equal runs.
+    static std::atomic<unsigned> n_alloc{0};
+    const unsigned char fill = (unsigned char)(0x81 + 2 * (n_alloc.fetch_add(1) % 61));
+    std::memset(*p, fill, bytes ? bytes : 1);
     return hipSuccess;
 }
 hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
