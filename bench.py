@@ -153,7 +153,7 @@ def run_c2(args):
     ctx.close()
 
 
-C3 = dict(images=32, side=(256, 2048), spp=(4, 64), seed=7, budget_gb=240.0)
+C3 = dict(images=32, side=(256, 2048), spp=(4, 64), seed=7, budget_gb=265.0)
 
 
 def synth_truth(albedo, sc):
@@ -192,6 +192,8 @@ def run_c3(args):
         jobs[-1].synthesize_records(sc)
     n_px = int((sides.astype(np.int64) ** 2).sum())
     n_samples = int((sides.astype(np.int64) ** 2 * spps).sum())
+    print(f"c3: {len(jobs)} jobs created, {n_px / 1e6:.1f} Mpx, {n_samples / 1e9:.2f} G samples", file=sys.stderr,
+          flush=True)
     # plain per-image path, each image alone on the GPU
     plain_ms = 0.0
     for j in jobs:
@@ -199,6 +201,7 @@ def run_c3(args):
         j.denoise_records()
         plain_ms += (time.perf_counter() - t) * 1e3
     plain = [j.get("denoised") for j in jobs]
+    print(f"c3: plain per-image denoise {plain_ms:.0f} ms", file=sys.stderr, flush=True)
     # batched: fresh jobs with the same records, every stage one launch over all images
     for j in jobs:
         j.release()
@@ -239,6 +242,105 @@ def run_c3(args):
     ctx.close()
 
 
+C4 = dict(jobs=64, rate=20.0, sides=(512, 1024, 1536, 2048), spp=(4, 8, 16, 32), passes=(0, 4),
+          procedural_fraction=1.0 / 3.0, budget_gb=200.0, max_in_flight=8, seed=11)
+
+
+def c4_requests():
+    """The fixed request stream: Poisson arrival times and per-job shapes drawn
+    from one seed; a third of the jobs render the procedural scene (C2 camera,
+    max depth 8), the others cornell_box (C0/C1 camera, aperture 0.14)."""
+    rng = np.random.default_rng(C4["seed"])
+    t = np.cumsum(rng.exponential(1.0 / C4["rate"], C4["jobs"]))
+    reqs = []
+    for k in range(C4["jobs"]):
+        side = int(rng.choice(C4["sides"]))
+        spp = int(rng.choice(C4["spp"]))
+        passes = int(rng.integers(C4["passes"][0], C4["passes"][1] + 1))
+        procedural = bool(rng.random() < C4["procedural_fraction"])
+        kw = dict(image_width=side, image_height=side, samples_per_pixel=spp, use_statmc=1, adaptive_spp=4,
+                  adaptive_passes=passes)
+        if procedural:
+            cfg = eng.default_config(max_depth=C2["max_depth"], **kw)
+            cam = eng.make_camera(C2["cam_pos"], C2["cam_dir"], (0.0, 1.0, 0.0), C2["fov"], C2["focus_distance"],
+                                  C2["aperture"])
+        else:
+            cfg = eng.default_config(color_window_radius=10, var_iterations=3, **kw)
+            cam = job_camera()
+        reqs.append((float(t[k]), "c2" if procedural else "cornell_box", cfg, cam, 5000 + k))
+    return reqs
+
+
+def run_c4(args):
+    """C4: the request stream through the serving loop (rpd_server_*): requests
+    arrive at their Poisson times, are admitted by slots and HBM footprint, and
+    run concurrently on their own streams. Reports stream throughput and the
+    loop's latency percentiles (submit to done)."""
+    import threading
+    ctx = eng.Context(0)
+    ctx.procedural_scene("c2", eng.procedural_config())
+    reqs = c4_requests()
+    # warm-up: one small job per scene through a separate loop (code objects, graphs, allocator)
+    warm = eng.Server(ctx, max_in_flight=2, memory_budget=int(C4["budget_gb"] * 1e9), graph_passes=True)
+    for scene in ("cornell_box", "c2"):
+        cfg = eng.default_config(image_width=128, image_height=128, samples_per_pixel=4, use_statmc=1,
+                                 adaptive_spp=4, adaptive_passes=1, max_depth=8 if scene == "c2" else 0)
+        j, _ = warm.wait(warm.submit(cfg, job_camera(), 1, scene=scene))
+        j.release()
+    warm.close()
+    srv = eng.Server(ctx, max_in_flight=C4["max_in_flight"], memory_budget=int(C4["budget_gb"] * 1e9),
+                     graph_passes=True)
+    ids = [None] * len(reqs)
+    samples = [0] * len(reqs)
+    submitted = threading.Event()
+
+    def submitter(t0):
+        for k, (at, scene, cfg, cam, seed) in enumerate(reqs):
+            delay = t0 + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids[k] = srv.submit(cfg, cam, seed, scene=scene)
+        submitted.set()
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=submitter, args=(t0,))
+    th.start()
+    done = 0
+    for k in range(len(reqs)):
+        while ids[k] is None:
+            time.sleep(0.001)
+        j, _ = srv.wait(ids[k])
+        samples[k] = int(j.get("sample_count").sum())
+        j.release()
+        done += 1
+        if done % 8 == 0:
+            print(f"c4: {done}/{len(reqs)} jobs done at {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    th.join()
+    st = srv.stats()
+    srv.close()
+    n_proc = sum(1 for r in reqs if r[1] == "c2")
+    result = {
+        "metric": "Msamples/s & denoised MP/s; p50/p99 job latency vs plain at equal relMSE",
+        "value": sum(samples) / wall / 1e6, "unit": "Msamples/s", "n_gpus": 1, "steps": len(reqs), "warmup": 2,
+        "ms_per_step": wall * 1e3 / len(reqs), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic: built-in cornell_box and procedural scenes, rendered on device",
+        "config": {"workload": f"{len(reqs)} jobs, Poisson arrivals at {C4['rate']:g}/s (seed {C4['seed']}), side "
+                               f"U{set(C4['sides'])}, base spp U{set(C4['spp'])}, adaptive passes "
+                               f"U{{{C4['passes'][0]}..{C4['passes'][1]}}} x 4 spp, {n_proc} procedural-scene jobs, "
+                               f"HBM budget {C4['budget_gb']:g} GB, {C4['max_in_flight']} in flight",
+                   "parallelism": "1 GPU, concurrent jobs on separate streams"},
+        "jobs_per_s": len(reqs) / wall,
+        "latency_ms": {"p50": st["latency_p50_ms"], "p99": st["latency_p99_ms"], "mean": st["latency_mean_ms"],
+                       "max": st["latency_max_ms"]},
+        "queue_ms": {"p50": st["queue_p50_ms"], "p99": st["queue_p99_ms"]},
+        "max_in_flight_seen": st["max_in_flight_seen"], "peak_gb": st["peak_bytes"] / 1e9,
+        "failed": st["failed"], "arrival_span_s": reqs[-1][0],
+    }
+    print(json.dumps(result))
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,9 +350,10 @@ def main():
     ap.add_argument("--size", type=int, default=WORKLOAD["width"])
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy (counter calibration) and exit")
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3"],
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4"],
                     help="c1: full-size cornell_box render+denoise job (default); c2: the 1.3M-triangle "
-                         "procedural scene at 1920x1080; c3: batched denoise of 32 synthetic per-sample buffers")
+                         "procedural scene at 1920x1080; c3: batched denoise of 32 synthetic per-sample buffers; "
+                         "c4: a Poisson stream of mixed render+denoise jobs through the serving loop")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -261,10 +364,12 @@ def main():
             sys.exit(2)
         run_c2(args)
         return
-    if args.workload == "c3":
-        print("the full-size batched-denoise workload is switched off: its first run left the GPU faulted and "
-              "the cause is not identified", file=sys.stderr)
-        sys.exit(2)
+    if args.workload in ("c3", "c4"):
+        if args.gpus != 1:
+            print("bench.py measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        (run_c3 if args.workload == "c3" else run_c4)(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): this workload runs on a single "
               "MI355X; sharding it over several GPUs is not implemented", file=sys.stderr)
