@@ -28,6 +28,44 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+// Stream capture (the hipGraph of a job's adaptive passes) must not overlap
+// with synchronous copies, allocator calls or stream/event creation in other
+// threads: the runtime rejects those while any stream is capturing (a legacy
+// stream operation would depend on the capturing stream). Captures only record
+// a few launches, so they hold this lock for their duration, and every such
+// call of the engine takes it too.
+std::mutex g_capture_mu;
+
+hipError_t sync_memcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    return hipMemcpy(dst, src, bytes, kind);
+}
+template <class T>
+hipError_t dev_malloc(T** p, size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    return hipMalloc(reinterpret_cast<void**>(p), bytes);
+}
+hipError_t dev_free(void* p) {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    return hipFree(p);
+}
+hipError_t stream_create(hipStream_t* s) {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+hipError_t stream_destroy(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    return hipStreamDestroy(s);
+}
+hipError_t event_create(hipEvent_t* e) {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    return hipEventCreate(e);
+}
+hipError_t event_destroy(hipEvent_t e) {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    return hipEventDestroy(e);
+}
+
 #define HIP_TRY(expr)                                                                          \
     do {                                                                                       \
         hipError_t e_ = (expr);                                                                \
@@ -41,7 +79,7 @@ int dalloc(T** p, size_t count, std::vector<void*>& owned) {
     *p = nullptr;
     if (count == 0) count = 1;
     void* q = nullptr;
-    const hipError_t e = hipMalloc(&q, count * sizeof(T));
+    const hipError_t e = dev_malloc(&q, count * sizeof(T));
     if (e != hipSuccess) return fail(RPD_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
     owned.push_back(q);
     *p = static_cast<T*>(q);
@@ -149,11 +187,11 @@ struct StageTimer {
     int stage;
     hipEvent_t a = nullptr;
     StageTimer(rpd_job* job, int s) : j(job), stage(s) {
-        if (!j->capturing && hipEventCreate(&a) == hipSuccess) hipEventRecord(a, j->stream);
+        if (!j->capturing && event_create(&a) == hipSuccess) hipEventRecord(a, j->stream);
     }
     ~StageTimer() {
         hipEvent_t b = nullptr;
-        if (a && !j->capturing && hipEventCreate(&b) == hipSuccess) {
+        if (a && !j->capturing && event_create(&b) == hipSuccess) {
             hipEventRecord(b, j->stream);
             j->spans.push_back({stage, a, b});
         }
@@ -166,8 +204,8 @@ void collect_spans(rpd_job* j) {
             j->stage_ms[s.stage] += ms;
             if (s.stage == 7) j->stage_ms[8] += 1.0;
         }
-        hipEventDestroy(s.a);
-        hipEventDestroy(s.b);
+        event_destroy(s.a);
+        event_destroy(s.b);
     }
     j->spans.clear();
 }
@@ -414,16 +452,23 @@ int reset_state(rpd_job* j) {
 int run_adaptive_passes_graph(rpd_job* j, int passes) {
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
-    j->capturing = true;
-    if (hipStreamBeginCapture(j->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    int st = RPD_OK;
+    hipError_t e = hipSuccess;
+    {
+        // captured work only launches kernels on the job's stream: nothing in
+        // it takes the capture lock
+        std::lock_guard<std::mutex> lk(g_capture_mu);
+        j->capturing = true;
+        if (hipStreamBeginCapture(j->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            j->capturing = false;
+            return fail(RPD_ERR_DEVICE, "hipStreamBeginCapture failed");
+        }
+        st = do_variance(j);
+        if (!st) st = do_allocation(j);
+        if (!st) st = pass_after_plan(j, 1, 1);
+        e = hipStreamEndCapture(j->stream, &g);
         j->capturing = false;
-        return fail(RPD_ERR_DEVICE, "hipStreamBeginCapture failed");
     }
-    int st = do_variance(j);
-    if (!st) st = do_allocation(j);
-    if (!st) st = pass_after_plan(j, 1, 1);
-    const hipError_t e = hipStreamEndCapture(j->stream, &g);
-    j->capturing = false;
     if (st || e != hipSuccess || !g) {
         if (g) hipGraphDestroy(g);
         return st ? st : fail(RPD_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(e));
@@ -509,8 +554,8 @@ int build_batch1(rpd_job* j, ImageDesc* dd, int64_t* dp, unsigned int* dq) {
         pf[2 * k + 1] = image_items(k, j->W, j->H, j->geom);
         j->batch1.total[k] = pf[2 * k + 1];
     }
-    HIP_TRY(hipMemcpy(dd, &j->desc, sizeof(ImageDesc), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(dp, pf, sizeof(pf), hipMemcpyHostToDevice));
+    HIP_TRY(sync_memcpy(dd, &j->desc, sizeof(ImageDesc), hipMemcpyHostToDevice));
+    HIP_TRY(sync_memcpy(dp, pf, sizeof(pf), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(dq, 0, sizeof(unsigned int) * BK_KINDS * BATCH_QUEUES));
     j->batch1.img = dd;
     j->batch1.prefix = dp;
@@ -587,11 +632,11 @@ size_t job_bytes(const rpd_render_config& c, int max_spp = 0) {
 void job_free(rpd_job* j) {
     if (j->worker.joinable()) j->worker.join();
     if (j->stream) hipStreamSynchronize(j->stream);
-    for (auto& s : j->spans) { hipEventDestroy(s.a); hipEventDestroy(s.b); }
+    for (auto& s : j->spans) { event_destroy(s.a); event_destroy(s.b); }
     j->spans.clear();
-    for (void* p : j->owned) hipFree(p);
+    for (void* p : j->owned) dev_free(p);
     j->owned.clear();
-    if (j->stream) hipStreamDestroy(j->stream);
+    if (j->stream) stream_destroy(j->stream);
     j->stream = nullptr;
     if (j->ctx) j->ctx->in_use -= j->bytes;
 }
@@ -616,7 +661,7 @@ int job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg,
     if (ctx->budget && ctx->in_use.load() + j->bytes > ctx->budget)
         return fail(RPD_ERR_OUT_OF_MEMORY, "job does not fit the context's memory budget");
     ctx->in_use += j->bytes;
-    HIP_TRY(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+    HIP_TRY(stream_create(&j->stream));
     if ((st = job_alloc(j.get()))) { job_free(j.get()); return st; }
     if ((st = reset_state(j.get()))) { job_free(j.get()); return st; }
     *out = j.release();
@@ -642,7 +687,7 @@ int job_create_denoiser(rpd_context* ctx, const rpd_render_config* cfg, int max_
     if (ctx->budget && ctx->in_use.load() + j->bytes > ctx->budget)
         return fail(RPD_ERR_OUT_OF_MEMORY, "job does not fit the context's memory budget");
     ctx->in_use += j->bytes;
-    HIP_TRY(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+    HIP_TRY(stream_create(&j->stream));
     if ((st = job_alloc(j.get()))) { job_free(j.get()); return st; }
     if ((st = reset_state(j.get()))) { job_free(j.get()); return st; }
     *out = j.release();
@@ -747,19 +792,19 @@ int rpd_ctx_create(int device, rpd_context** out) {
 int rpd_ctx_destroy(rpd_context* ctx) {
     if (!ctx) return RPD_OK;
     if (ctx->b_stream) hipStreamSynchronize(ctx->b_stream);
-    if (ctx->b_img) hipFree(ctx->b_img);
-    if (ctx->b_prefix) hipFree(ctx->b_prefix);
-    if (ctx->b_queues) hipFree(ctx->b_queues);
-    for (hipEvent_t e : ctx->b_ev) if (e) hipEventDestroy(e);
-    if (ctx->b_stream) hipStreamDestroy(ctx->b_stream);
+    if (ctx->b_img) dev_free(ctx->b_img);
+    if (ctx->b_prefix) dev_free(ctx->b_prefix);
+    if (ctx->b_queues) dev_free(ctx->b_queues);
+    for (hipEvent_t e : ctx->b_ev) if (e) event_destroy(e);
+    if (ctx->b_stream) stream_destroy(ctx->b_stream);
     if (ctx->m_stream) hipStreamSynchronize(ctx->m_stream);
-    if (ctx->m_part) hipFree(ctx->m_part);
-    if (ctx->m_out) hipFree(ctx->m_out);
-    if (ctx->m_img) hipFree(ctx->m_img);
-    if (ctx->m_stream) hipStreamDestroy(ctx->m_stream);
+    if (ctx->m_part) dev_free(ctx->m_part);
+    if (ctx->m_out) dev_free(ctx->m_out);
+    if (ctx->m_img) dev_free(ctx->m_img);
+    if (ctx->m_stream) stream_destroy(ctx->m_stream);
     if (ctx->f_stream) hipStreamSynchronize(ctx->f_stream);
-    if (ctx->f_buf) hipFree(ctx->f_buf);
-    if (ctx->f_stream) hipStreamDestroy(ctx->f_stream);
+    if (ctx->f_buf) dev_free(ctx->f_buf);
+    if (ctx->f_stream) stream_destroy(ctx->f_stream);
     delete ctx;
     return RPD_OK;
 }
@@ -872,28 +917,28 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
             (st = dalloc(&ps, ds.size(), sc->owned)) ||
             (st = dalloc(&pm, dm.size(), sc->owned)) || (st = dalloc(&pl, dl.size(), sc->owned)) ||
             (st = dalloc(&pc, cdf.size(), sc->owned))) {
-            for (void* p : sc->owned) hipFree(p);
+            for (void* p : sc->owned) dev_free(p);
             return st;
         }
         if (n_tris) {
-            HIP_TRY(hipMemcpy(pt, dt.data(), dt.size() * sizeof(DevTriangle), hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(pp, perm.data(), perm.size() * sizeof(float4), hipMemcpyHostToDevice));
+            HIP_TRY(sync_memcpy(pt, dt.data(), dt.size() * sizeof(DevTriangle), hipMemcpyHostToDevice));
+            HIP_TRY(sync_memcpy(pp, perm.data(), perm.size() * sizeof(float4), hipMemcpyHostToDevice));
         }
         if (use_bvh) {
             if ((st = dalloc(&pn, bvh.nodes.size(), sc->owned)) || (st = dalloc(&pslot, bvh.slots.size(), sc->owned)) ||
                 (st = dalloc(&pprim, bvh.slot_prim.size(), sc->owned))) {
-                for (void* p : sc->owned) hipFree(p);
+                for (void* p : sc->owned) dev_free(p);
                 return st;
             }
-            HIP_TRY(hipMemcpy(pn, bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node), hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(pslot, bvh.slots.data(), bvh.slots.size() * sizeof(float4), hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(pprim, bvh.slot_prim.data(), bvh.slot_prim.size() * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRY(sync_memcpy(pn, bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node), hipMemcpyHostToDevice));
+            HIP_TRY(sync_memcpy(pslot, bvh.slots.data(), bvh.slots.size() * sizeof(float4), hipMemcpyHostToDevice));
+            HIP_TRY(sync_memcpy(pprim, bvh.slot_prim.data(), bvh.slot_prim.size() * sizeof(int), hipMemcpyHostToDevice));
         }
-        if (n_spheres) HIP_TRY(hipMemcpy(ps, ds.data(), ds.size() * sizeof(DevSphere), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(pm, dm.data(), dm.size() * sizeof(DevMaterial), hipMemcpyHostToDevice));
+        if (n_spheres) HIP_TRY(sync_memcpy(ps, ds.data(), ds.size() * sizeof(DevSphere), hipMemcpyHostToDevice));
+        HIP_TRY(sync_memcpy(pm, dm.data(), dm.size() * sizeof(DevMaterial), hipMemcpyHostToDevice));
         if (n_lights) {
-            HIP_TRY(hipMemcpy(pl, dl.data(), dl.size() * sizeof(DevLight), hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(pc, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice));
+            HIP_TRY(sync_memcpy(pl, dl.data(), dl.size() * sizeof(DevLight), hipMemcpyHostToDevice));
+            HIP_TRY(sync_memcpy(pc, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice));
         }
     }
     sc->dev.tris = pt; sc->dev.n_tris = n_tris; sc->dev.tri_perm = pp;
@@ -993,7 +1038,7 @@ int rpd_scene_counts(const rpd_scene* s, int* nt, int* ns, int* nm, int* nl) {
 
 int rpd_scene_destroy(rpd_scene* s) {
     if (!s) return RPD_OK;
-    for (void* p : s->owned) hipFree(p);
+    for (void* p : s->owned) dev_free(p);
     delete s;
     return RPD_OK;
 }
@@ -1047,7 +1092,7 @@ int rpd_job_sync(rpd_job* j) {
     HIP_TRY(hipStreamSynchronize(j->stream));
     collect_spans(j);
     int over = 0;
-    HIP_TRY(hipMemcpy(&over, j->totals + 2, sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(sync_memcpy(&over, j->totals + 2, sizeof(int), hipMemcpyDeviceToHost));
     if (over) return fail(RPD_ERR_STATE, "an adaptive pass asked for more samples than the record capacity");
     return RPD_OK;
 }
@@ -1097,7 +1142,7 @@ int rpd_job_adaptive_counts(rpd_job* j, int* out) {
     int st = do_allocation(j);
     if (st) return st;
     if ((st = rpd_job_sync(j))) return st;
-    if (out) HIP_TRY(hipMemcpy(out, j->counts, sizeof(int) * j->N, hipMemcpyDeviceToHost));
+    if (out) HIP_TRY(sync_memcpy(out, j->counts, sizeof(int) * j->N, hipMemcpyDeviceToHost));
     return RPD_OK;
 }
 int rpd_job_adaptive_counts_from_importance(rpd_job* j, const float* importance, int* out) {
@@ -1110,7 +1155,7 @@ int rpd_job_adaptive_counts_from_importance(rpd_job* j, const float* importance,
     HIP_TRY(hipGetLastError());
     const int st = rpd_job_sync(j);
     if (st) return st;
-    if (out) HIP_TRY(hipMemcpy(out, j->counts, sizeof(int) * j->N, hipMemcpyDeviceToHost));
+    if (out) HIP_TRY(sync_memcpy(out, j->counts, sizeof(int) * j->N, hipMemcpyDeviceToHost));
     return RPD_OK;
 }
 
@@ -1223,18 +1268,18 @@ int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu
         (capture_records(jobs[k]) ? batched : alone).push_back(jobs[k]);
     }
     if (!ctx->b_stream) {
-        HIP_TRY(hipStreamCreateWithFlags(&ctx->b_stream, hipStreamNonBlocking));
-        HIP_TRY(hipEventCreate(&ctx->b_ev[0]));
-        HIP_TRY(hipEventCreate(&ctx->b_ev[1]));
-        HIP_TRY(hipMalloc(&ctx->b_queues, sizeof(unsigned int) * BK_KINDS * BATCH_QUEUES));
+        HIP_TRY(stream_create(&ctx->b_stream));
+        HIP_TRY(event_create(&ctx->b_ev[0]));
+        HIP_TRY(event_create(&ctx->b_ev[1]));
+        HIP_TRY(dev_malloc(&ctx->b_queues, sizeof(unsigned int) * BK_KINDS * BATCH_QUEUES));
     }
     const int nb = int(batched.size());
     if (nb > ctx->b_cap) {
-        if (ctx->b_img) HIP_TRY(hipFree(ctx->b_img));
-        if (ctx->b_prefix) HIP_TRY(hipFree(ctx->b_prefix));
+        if (ctx->b_img) HIP_TRY(dev_free(ctx->b_img));
+        if (ctx->b_prefix) HIP_TRY(dev_free(ctx->b_prefix));
         ctx->b_img = nullptr; ctx->b_prefix = nullptr; ctx->b_cap = 0;
-        HIP_TRY(hipMalloc(&ctx->b_img, sizeof(ImageDesc) * nb));
-        HIP_TRY(hipMalloc(&ctx->b_prefix, sizeof(int64_t) * BK_KINDS * (nb + 1)));
+        HIP_TRY(dev_malloc(&ctx->b_img, sizeof(ImageDesc) * nb));
+        HIP_TRY(dev_malloc(&ctx->b_prefix, sizeof(int64_t) * BK_KINDS * (nb + 1)));
         ctx->b_cap = nb;
     }
     hipStream_t s = ctx->b_stream;
@@ -1350,13 +1395,13 @@ static int metrics_run(rpd_context* ctx, const MetricImage& ref, const MetricIma
                        hipStream_t st, rpd_image_metrics* out) {
     const size_t need = size_t(std::max(metrics_a_blocks(W, H) * MA_FIELDS, metrics_b_blocks(W, H) * MB_FIELDS));
     if (need > ctx->m_part_cap) {
-        if (ctx->m_part) HIP_TRY(hipFree(ctx->m_part));
+        if (ctx->m_part) HIP_TRY(dev_free(ctx->m_part));
         ctx->m_part = nullptr;
         ctx->m_part_cap = 0;
-        HIP_TRY(hipMalloc(&ctx->m_part, need * sizeof(double)));
+        HIP_TRY(dev_malloc(&ctx->m_part, need * sizeof(double)));
         ctx->m_part_cap = need;
     }
-    if (!ctx->m_out) HIP_TRY(hipMalloc(&ctx->m_out, 32 * sizeof(double)));
+    if (!ctx->m_out) HIP_TRY(dev_malloc(&ctx->m_out, 32 * sizeof(double)));
     double a[MA_FIELDS], b[MB_FIELDS];
     launch_metrics_a(ref, tst, W, H, ctx->m_part, ctx->m_out, st);
     HIP_TRY(hipGetLastError());
@@ -1404,10 +1449,10 @@ static int metrics_check_dims(int width, int height) {
 // Grows the context's image staging area to `floats` and returns it.
 static int metrics_stage(rpd_context* ctx, size_t floats, float** p) {
     if (floats > ctx->m_img_cap) {
-        if (ctx->m_img) HIP_TRY(hipFree(ctx->m_img));
+        if (ctx->m_img) HIP_TRY(dev_free(ctx->m_img));
         ctx->m_img = nullptr;
         ctx->m_img_cap = 0;
-        HIP_TRY(hipMalloc(&ctx->m_img, floats * sizeof(float)));
+        HIP_TRY(dev_malloc(&ctx->m_img, floats * sizeof(float)));
         ctx->m_img_cap = floats;
     }
     *p = ctx->m_img;
@@ -1421,7 +1466,7 @@ int rpd_image_metrics_compute(rpd_context* ctx, const float* ref, const float* t
     if (st) return st;
     std::lock_guard<std::mutex> lk(ctx->metrics_mu);
     HIP_TRY(hipSetDevice(ctx->device));
-    if (!ctx->m_stream) HIP_TRY(hipStreamCreateWithFlags(&ctx->m_stream, hipStreamNonBlocking));
+    if (!ctx->m_stream) HIP_TRY(stream_create(&ctx->m_stream));
     const size_t n3 = size_t(width) * size_t(height) * 3;
     const float* dr = ref;
     const float* dt = test;
@@ -1479,12 +1524,12 @@ void rpd_atrous_config_defaults(rpd_atrous_config* c) {
 
 // The context's filter scratch of at least `floats` floats, on its stream.
 static int filters_scratch(rpd_context* ctx, size_t floats, float** p) {
-    if (!ctx->f_stream) HIP_TRY(hipStreamCreateWithFlags(&ctx->f_stream, hipStreamNonBlocking));
+    if (!ctx->f_stream) HIP_TRY(stream_create(&ctx->f_stream));
     if (floats > ctx->f_cap) {
-        if (ctx->f_buf) HIP_TRY(hipFree(ctx->f_buf));
+        if (ctx->f_buf) HIP_TRY(dev_free(ctx->f_buf));
         ctx->f_buf = nullptr;
         ctx->f_cap = 0;
-        HIP_TRY(hipMalloc(&ctx->f_buf, floats * sizeof(float)));
+        HIP_TRY(dev_malloc(&ctx->f_buf, floats * sizeof(float)));
         ctx->f_cap = floats;
     }
     *p = ctx->f_buf;
@@ -1597,7 +1642,7 @@ int rpd_job_counters(rpd_job* j, uint64_t* out, int n) {
     if (j->worker.joinable()) j->worker.join();
     HIP_TRY(hipStreamSynchronize(j->stream));
     unsigned long long q[2] = {0, 0};
-    HIP_TRY(hipMemcpy(q, j->query_count, sizeof(q), hipMemcpyDeviceToHost));
+    HIP_TRY(sync_memcpy(q, j->query_count, sizeof(q), hipMemcpyDeviceToHost));
     const uint64_t v[4] = {q[0], q[1], j->scene ? uint64_t(j->scene->n_tris) : 0,
                            j->scene ? uint64_t(j->scene->n_spheres) : 0};
     for (int k = 0; k < n && k < 4; ++k) out[k] = v[k];
@@ -1618,7 +1663,7 @@ int rpd_job_buffer_size(rpd_job* j, int id, size_t* bytes) {
         case RPD_BUF_RECORDS: {
             int tot[2] = {0, 0};
             HIP_TRY(hipStreamSynchronize(j->stream));
-            HIP_TRY(hipMemcpy(tot, j->totals, sizeof(tot), hipMemcpyDeviceToHost));
+            HIP_TRY(sync_memcpy(tot, j->totals, sizeof(tot), hipMemcpyDeviceToHost));
             *bytes = size_t(tot[1] > 0 ? tot[0] : 0) * 32 * sizeof(float);
             return RPD_OK;
         }
@@ -1643,7 +1688,7 @@ int rpd_job_read(rpd_job* j, int id, void* dst, size_t bytes) {
     const size_t N = j->N;
     float* f = static_cast<float*>(dst);
     auto scalar = [&](const float* d) -> int {
-        HIP_TRY(hipMemcpy(dst, d, 4 * N, hipMemcpyDeviceToHost));
+        HIP_TRY(sync_memcpy(dst, d, 4 * N, hipMemcpyDeviceToHost));
         return RPD_OK;
     };
     switch (id) {
@@ -1664,12 +1709,12 @@ int rpd_job_read(rpd_job* j, int id, void* dst, size_t bytes) {
         case RPD_BUF_SENSITIVITY_CONFIDENCE: return scalar(j->sconf);
         case RPD_BUF_SENSITIVITY_GRADIENT: return scalar(j->sgrad);
         case RPD_BUF_ADAPTIVE_IMPORTANCE: return scalar(j->imp);
-        case RPD_BUF_SAMPLE_COUNT: HIP_TRY(hipMemcpy(dst, j->pb.sample_count, 4 * N, hipMemcpyDeviceToHost)); return RPD_OK;
-        case RPD_BUF_HIT_COUNT: HIP_TRY(hipMemcpy(dst, j->pb.hit_count, 4 * N, hipMemcpyDeviceToHost)); return RPD_OK;
-        case RPD_BUF_PASS_COUNTS: HIP_TRY(hipMemcpy(dst, j->counts, 4 * N, hipMemcpyDeviceToHost)); return RPD_OK;
+        case RPD_BUF_SAMPLE_COUNT: HIP_TRY(sync_memcpy(dst, j->pb.sample_count, 4 * N, hipMemcpyDeviceToHost)); return RPD_OK;
+        case RPD_BUF_HIT_COUNT: HIP_TRY(sync_memcpy(dst, j->pb.hit_count, 4 * N, hipMemcpyDeviceToHost)); return RPD_OK;
+        case RPD_BUF_PASS_COUNTS: HIP_TRY(sync_memcpy(dst, j->counts, 4 * N, hipMemcpyDeviceToHost)); return RPD_OK;
         case RPD_BUF_SAMPLE_FRACTION: {  // core/buffers.h:182-195
             std::vector<int> c(N);
-            HIP_TRY(hipMemcpy(c.data(), j->pb.sample_count, 4 * N, hipMemcpyDeviceToHost));
+            HIP_TRY(sync_memcpy(c.data(), j->pb.sample_count, 4 * N, hipMemcpyDeviceToHost));
             const int mx = std::max(1, N ? *std::max_element(c.begin(), c.end()) : 1);
             const float inv = 1.0f / float(mx);
             for (size_t i = 0; i < N; ++i) f[i] = float(std::max(0, c[i])) * inv;
@@ -1677,7 +1722,7 @@ int rpd_job_read(rpd_job* j, int id, void* dst, size_t bytes) {
         }
         case RPD_BUF_STATS: {
             std::vector<float> s(12 * N);
-            HIP_TRY(hipMemcpy(s.data(), j->pb.lum_mean, 48 * N, hipMemcpyDeviceToHost));
+            HIP_TRY(sync_memcpy(s.data(), j->pb.lum_mean, 48 * N, hipMemcpyDeviceToHost));
             const int plane_of[12] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
             for (size_t i = 0; i < N; ++i)
                 for (int k = 0; k < 12; ++k) f[i * 12 + k] = s[size_t(plane_of[k]) * N + i];
@@ -1686,7 +1731,7 @@ int rpd_job_read(rpd_job* j, int id, void* dst, size_t bytes) {
         case RPD_BUF_SUMS: {
             std::vector<float> s(13 * N);
             std::vector<int> ic(2 * N);
-            HIP_TRY(hipMemcpy(s.data(), j->pb.radiance, 52 * N, hipMemcpyDeviceToHost));
+            HIP_TRY(sync_memcpy(s.data(), j->pb.radiance, 52 * N, hipMemcpyDeviceToHost));
             for (size_t i = 0; i < N; ++i)
                 for (int k = 0; k < 13; ++k) f[i * 13 + k] = s[size_t(k) * N + i];
             return RPD_OK;
@@ -1702,10 +1747,10 @@ int rpd_job_read(rpd_job* j, int id, void* dst, size_t bytes) {
             return rc;
         }
         case RPD_BUF_GRID:
-            HIP_TRY(hipMemcpy(dst, j->grid, need, hipMemcpyDeviceToHost));
+            HIP_TRY(sync_memcpy(dst, j->grid, need, hipMemcpyDeviceToHost));
             return RPD_OK;
         case RPD_BUF_TILES:
-            HIP_TRY(hipMemcpy(dst, j->tiles, need, hipMemcpyDeviceToHost));
+            HIP_TRY(sync_memcpy(dst, j->tiles, need, hipMemcpyDeviceToHost));
             return RPD_OK;
         case RPD_BUF_RECORDS: {
             const size_t ns = need / (32 * sizeof(float));
@@ -1715,12 +1760,12 @@ int rpd_job_read(rpd_job* j, int id, void* dst, size_t bytes) {
             float4* srcs[6] = {R.r0, R.r1, R.r2, R.r3, R.r4, R.r5};
             for (int k = 0; k < 6; ++k) {
                 r[k].resize(ns);
-                if (ns) HIP_TRY(hipMemcpy(r[k].data(), srcs[k], ns * sizeof(float4), hipMemcpyDeviceToHost));
+                if (ns) HIP_TRY(sync_memcpy(r[k].data(), srcs[k], ns * sizeof(float4), hipMemcpyDeviceToHost));
             }
-            if (ns) HIP_TRY(hipMemcpy(r6.data(), R.r6, ns * sizeof(float2), hipMemcpyDeviceToHost));
+            if (ns) HIP_TRY(sync_memcpy(r6.data(), R.r6, ns * sizeof(float2), hipMemcpyDeviceToHost));
             std::vector<int> offs(N), cnts(N);
-            HIP_TRY(hipMemcpy(offs.data(), j->offsets, 4 * N, hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(cnts.data(), j->counts, 4 * N, hipMemcpyDeviceToHost));
+            HIP_TRY(sync_memcpy(offs.data(), j->offsets, 4 * N, hipMemcpyDeviceToHost));
+            HIP_TRY(sync_memcpy(cnts.data(), j->counts, 4 * N, hipMemcpyDeviceToHost));
             // 32 floats per sample in pixel-major sample order (same layout as the oracle's records)
             for (size_t p = 0; p < N; ++p) {
                 for (int s = 0; s < cnts[p]; ++s) {
@@ -1760,7 +1805,7 @@ int rpd_job_read(rpd_job* j, int id, void* dst, size_t bytes) {
         const int field[9] = {SF_ALL, SF_PIXEL, SF_BRDF, SF_LENS, SF_LIGHT, SF_LIGHT_UV, SF_LIGHT_SELECT, SF_ENV, SF_RR};
         const size_t nn = size_t(j->geom.nx) * j->geom.ny;
         std::vector<float> t(nn);
-        HIP_TRY(hipMemcpy(t.data(), j->tiles + size_t(field[id - RPD_BUF_SENS_ALL]) * nn, 4 * nn, hipMemcpyDeviceToHost));
+        HIP_TRY(sync_memcpy(t.data(), j->tiles + size_t(field[id - RPD_BUF_SENS_ALL]) * nn, 4 * nn, hipMemcpyDeviceToHost));
         expand_tiles(j, t.data(), f);
         return RPD_OK;
     }
@@ -1782,22 +1827,22 @@ int rpd_job_write(rpd_job* j, int id, const void* src, size_t bytes) {
             std::vector<float> s(12 * N);
             for (size_t i = 0; i < N; ++i)
                 for (int k = 0; k < 12; ++k) s[size_t(k) * N + i] = f[i * 12 + k];
-            HIP_TRY(hipMemcpy(j->pb.lum_mean, s.data(), 48 * N, hipMemcpyHostToDevice));
+            HIP_TRY(sync_memcpy(j->pb.lum_mean, s.data(), 48 * N, hipMemcpyHostToDevice));
             return RPD_OK;
         }
         case RPD_BUF_SUMS: {
             std::vector<float> s(13 * N);
             for (size_t i = 0; i < N; ++i)
                 for (int k = 0; k < 13; ++k) s[size_t(k) * N + i] = f[i * 13 + k];
-            HIP_TRY(hipMemcpy(j->pb.radiance, s.data(), 52 * N, hipMemcpyHostToDevice));
+            HIP_TRY(sync_memcpy(j->pb.radiance, s.data(), 52 * N, hipMemcpyHostToDevice));
             return RPD_OK;
         }
-        case RPD_BUF_GRID: HIP_TRY(hipMemcpy(j->grid, src, need, hipMemcpyHostToDevice)); return RPD_OK;
-        case RPD_BUF_TILES: HIP_TRY(hipMemcpy(j->tiles, src, need, hipMemcpyHostToDevice)); return RPD_OK;
-        case RPD_BUF_HIT_COUNT: HIP_TRY(hipMemcpy(j->pb.hit_count, src, 4 * N, hipMemcpyHostToDevice)); return RPD_OK;
-        case RPD_BUF_SAMPLE_COUNT: HIP_TRY(hipMemcpy(j->pb.sample_count, src, 4 * N, hipMemcpyHostToDevice)); return RPD_OK;
-        case RPD_BUF_SENSITIVITY_GRADIENT: HIP_TRY(hipMemcpy(j->sgrad, src, 4 * N, hipMemcpyHostToDevice)); return RPD_OK;
-        case RPD_BUF_VARIANCE_DENOISED: HIP_TRY(hipMemcpy(j->vden, src, 4 * N, hipMemcpyHostToDevice)); return RPD_OK;
+        case RPD_BUF_GRID: HIP_TRY(sync_memcpy(j->grid, src, need, hipMemcpyHostToDevice)); return RPD_OK;
+        case RPD_BUF_TILES: HIP_TRY(sync_memcpy(j->tiles, src, need, hipMemcpyHostToDevice)); return RPD_OK;
+        case RPD_BUF_HIT_COUNT: HIP_TRY(sync_memcpy(j->pb.hit_count, src, 4 * N, hipMemcpyHostToDevice)); return RPD_OK;
+        case RPD_BUF_SAMPLE_COUNT: HIP_TRY(sync_memcpy(j->pb.sample_count, src, 4 * N, hipMemcpyHostToDevice)); return RPD_OK;
+        case RPD_BUF_SENSITIVITY_GRADIENT: HIP_TRY(sync_memcpy(j->sgrad, src, 4 * N, hipMemcpyHostToDevice)); return RPD_OK;
+        case RPD_BUF_VARIANCE_DENOISED: HIP_TRY(sync_memcpy(j->vden, src, 4 * N, hipMemcpyHostToDevice)); return RPD_OK;
         default: return fail(RPD_ERR_INVALID_ARG, "buffer is not writable");
     }
 }

@@ -344,6 +344,15 @@ uint64_t wave_exchange_u64(uint64_t v, int mode, int arg) {
 // ---- runtime API --------------------------------------------------------------------------
 using rpd_emu::g_last;
 
+// While a stream captures, the device runtime rejects synchronous copies,
+// allocator calls and stream/event creation from other threads; so does the
+// emulation (the capturing thread itself only enqueues work).
+static std::atomic<int> g_capturing{0};
+static thread_local int t_capturing = 0;
+static bool capture_conflict() { return g_capturing.load() > t_capturing; }
+#define EMU_NO_CAPTURE() \
+    do { if (capture_conflict()) return hipErrorStreamCaptureImplicit; } while (0)
+
 hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
 hipError_t hipSetDevice(int d) { return d == 0 ? hipSuccess : hipErrorInvalidValue; }
 hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
@@ -364,6 +373,7 @@ hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
 }
 hipError_t hipMemGetInfo(size_t* f, size_t* t) { *f = size_t(12) << 30; *t = size_t(16) << 30; return hipSuccess; }
 hipError_t hipMalloc(void** p, size_t bytes) {
+    EMU_NO_CAPTURE();
     *p = std::malloc(bytes ? bytes : 1);
     if (!*p) return hipErrorOutOfMemory;
     // Device memory is not cleared by the allocator, and what it holds depends
@@ -374,14 +384,20 @@ hipError_t hipMalloc(void** p, size_t bytes) {
     std::memset(*p, fill, bytes ? bytes : 1);
     return hipSuccess;
 }
-hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
-hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { if (n) std::memmove(d, s, n); return hipSuccess; }
+hipError_t hipFree(void* p) { EMU_NO_CAPTURE(); std::free(p); return hipSuccess; }
+hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
+    EMU_NO_CAPTURE();
+    if (n) std::memmove(d, s, n);
+    return hipSuccess;
+}
 hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t st) {
     if (st && st->capturing) {
         st->graph->nodes.push_back([d, s, n]() { if (n) std::memmove(d, s, n); });
         return hipSuccess;
     }
-    return hipMemcpy(d, s, n, k);
+    if (n) std::memmove(d, s, n);   // asynchronous on a stream: no capture conflict
+    (void)k;
+    return hipSuccess;
 }
 hipError_t hipMemset(void* d, int v, size_t n) { if (n) std::memset(d, v, n); return hipSuccess; }
 hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t st) {
@@ -391,15 +407,15 @@ hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t st) {
     }
     return hipMemset(d, v, n);
 }
-hipError_t hipStreamCreate(hipStream_t* s) { *s = new ihipStream_t(); return hipSuccess; }
+hipError_t hipStreamCreate(hipStream_t* s) { EMU_NO_CAPTURE(); *s = new ihipStream_t(); return hipSuccess; }
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { return hipStreamCreate(s); }
-hipError_t hipStreamDestroy(hipStream_t s) { delete s; return hipSuccess; }
+hipError_t hipStreamDestroy(hipStream_t s) { EMU_NO_CAPTURE(); delete s; return hipSuccess; }
 hipError_t hipStreamSynchronize(hipStream_t s) {
     return (s && s->capturing) ? hipErrorStreamCaptureUnsupported : hipSuccess;
 }
 hipError_t hipDeviceSynchronize() { return hipSuccess; }
-hipError_t hipEventCreate(hipEvent_t* e) { *e = new ihipEvent_t(); (*e)->t = std::chrono::steady_clock::now(); return hipSuccess; }
-hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
+hipError_t hipEventCreate(hipEvent_t* e) { EMU_NO_CAPTURE(); *e = new ihipEvent_t(); (*e)->t = std::chrono::steady_clock::now(); return hipSuccess; }
+hipError_t hipEventDestroy(hipEvent_t e) { EMU_NO_CAPTURE(); delete e; return hipSuccess; }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) {
     if (s && s->capturing) return hipErrorStreamCaptureUnsupported;
     e->t = std::chrono::steady_clock::now();
@@ -414,11 +430,15 @@ hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
     if (!s || s->capturing) return hipErrorInvalidValue;
     s->capturing = true;
     s->graph = new ihipGraph();
+    ++g_capturing;
+    ++t_capturing;
     return hipSuccess;
 }
 hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g) {
     if (!s || !s->capturing) return hipErrorInvalidValue;
     s->capturing = false;
+    --g_capturing;
+    --t_capturing;
     *g = s->graph;
     s->graph = nullptr;
     return hipSuccess;
@@ -443,6 +463,7 @@ const char* hipGetErrorString(hipError_t e) {
         case hipErrorOutOfMemory: return "hipErrorOutOfMemory";
         case hipErrorInvalidConfiguration: return "hipErrorInvalidConfiguration";
         case hipErrorStreamCaptureUnsupported: return "hipErrorStreamCaptureUnsupported";
+        case hipErrorStreamCaptureImplicit: return "hipErrorStreamCaptureImplicit";
         default: return "hipErrorUnknown";
     }
 }

@@ -56,7 +56,7 @@ inline double2 make_double2(double a, double b) { return {a, b}; }
 // ---- runtime types ------------------------------------------------------------------
 enum hipError_t {
     hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2, hipErrorInvalidConfiguration = 9,
-    hipErrorStreamCaptureUnsupported = 900, hipErrorUnknown = 999
+    hipErrorStreamCaptureUnsupported = 900, hipErrorStreamCaptureImplicit = 906, hipErrorUnknown = 999
 };
 enum hipMemcpyKind { hipMemcpyHostToHost = 0, hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2,
                      hipMemcpyDeviceToDevice = 3, hipMemcpyDefault = 4 };
