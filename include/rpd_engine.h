@@ -333,7 +333,8 @@ int rpd_ctx_memory(rpd_context* ctx, uint64_t* in_use, uint64_t* budget, uint64_
 typedef struct rpd_server rpd_server;
 typedef struct rpd_server_config {
     int max_in_flight;       /* jobs admitted at once (>= 1) */
-    uint64_t memory_budget;  /* bytes of job state admitted at once (0 = device memory) */
+    uint64_t memory_budget;  /* bytes of job state held at once, counting finished jobs until they are
+                                released (0 = free device memory at creation) */
     int graph_passes;        /* replay the adaptive passes from a captured hipGraph */
 } rpd_server_config;
 typedef struct rpd_server_stats {
@@ -348,7 +349,11 @@ int rpd_server_create(rpd_context* ctx, const rpd_server_config* cfg, rpd_server
 int rpd_server_submit(rpd_server* s, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam,
                       uint32_t seed, uint64_t* request_id);
 /* Blocks until the request is done; hands over its job (release it with
-   rpd_job_release) and its latency from submit to completion. */
+   rpd_job_release: its device memory counts against the budget until then)
+   and its latency from submit to completion. When nothing runs and the memory
+   is held by finished jobs, a request that a caller waits for is admitted
+   over the budget, so a caller that keeps the jobs it claimed cannot stall.
+   A failure on the scheduler thread is reported here, with its message. */
 int rpd_server_wait(rpd_server* s, uint64_t request_id, rpd_job** job, double* latency_ms);
 int rpd_server_stats_get(rpd_server* s, rpd_server_stats* out);
 /* Finishes queued and running requests, releases jobs nobody claimed. */

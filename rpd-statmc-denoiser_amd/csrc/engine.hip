@@ -752,6 +752,12 @@ void expand_tiles(const rpd_job* j, const float* t, float* out) {
 
 }  // namespace
 
+namespace rpd_internal {
+// Sets the calling thread's rpd_last_error: the serving loop hands a failure
+// caught on its scheduler thread to the thread that waits for the request.
+int set_error(int code, const char* msg) { return fail(code, msg ? msg : ""); }
+}  // namespace rpd_internal
+
 extern "C" {
 
 void rpd_render_config_defaults(rpd_render_config* c) {

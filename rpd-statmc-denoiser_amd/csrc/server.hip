@@ -2,7 +2,9 @@
 // (FIFO with a short look-ahead, so a small job can pass a large one that
 // does not fit yet) whenever a slot is free and the job's device footprint
 // fits the memory budget, starts each admitted job on its own stream, and
-// records completion. Built on the public C ABI only.
+// records completion. Built on the public C ABI, plus the engine's
+// thread-local error setter so a waiter sees the message of a failure that
+// happened on the scheduler thread.
 #include "../../include/rpd_engine.h"
 
 #include <hip/hip_runtime.h>
@@ -17,6 +19,10 @@
 #include <string>
 #include <thread>
 #include <vector>
+
+namespace rpd_internal {
+int set_error(int code, const char* msg);  // engine.hip: the calling thread's rpd_last_error
+}
 
 namespace {
 
@@ -36,6 +42,8 @@ struct Request {
     Clock::time_point t_submit, t_admit, t_done;
     rpd_job* job = nullptr;
     int status = 0;
+    std::string error;  // rpd_last_error of the failure, taken on the scheduler thread
+    bool waited = false;  // a caller is blocked in rpd_server_wait on it
     enum State { QUEUED, RUNNING, DONE, CLAIMED } state = QUEUED;
 };
 
@@ -60,12 +68,43 @@ struct rpd_server {
     std::map<uint64_t, std::unique_ptr<Request>> all;
     std::deque<Request*> queue;
     std::vector<Request*> running;
-    uint64_t next_id = 1, in_use = 0, peak = 0;
+    uint64_t next_id = 1, peak = 0;
     uint64_t submitted = 0, completed = 0, failed = 0;
     int max_seen = 0;
     std::vector<double> lat, qwait;
     bool stop = false;
     std::thread sched;
+
+    uint64_t held() const {
+        uint64_t u = 0;
+        rpd_ctx_memory(ctx, &u, nullptr, nullptr, nullptr);  // an atomic load; no device call
+        return u;
+    }
+
+    // Creates and launches the request's job; called with the lock held,
+    // which it drops around the engine call.
+    void start(Request* r, std::unique_lock<std::mutex>& lk) {
+        r->t_admit = Clock::now();
+        qwait.push_back(ms_between(r->t_submit, r->t_admit));
+        lk.unlock();
+        const int st = rpd_job_submit(ctx, r->scene, &r->cfg, &r->cam, r->seed, &r->job);
+        const std::string err = st ? rpd_last_error() : "";
+        lk.lock();
+        if (st) {
+            r->error = err;
+            r->status = st;
+            r->job = nullptr;
+            r->t_done = Clock::now();
+            r->state = Request::DONE;
+            ++failed;
+            cv_done.notify_all();
+        } else {
+            r->state = Request::RUNNING;
+            running.push_back(r);
+            max_seen = std::max(max_seen, int(running.size()));
+            peak = std::max(peak, held());
+        }
+    }
 
     void loop() {
         std::unique_lock<std::mutex> lk(mu);
@@ -77,44 +116,41 @@ struct rpd_server {
                 rpd_job_poll(r->job, &done);
                 if (!done) { ++k; continue; }
                 r->status = rpd_job_wait(r->job);  // joins the job's worker; already finished
+                if (r->status) r->error = rpd_last_error();
                 r->t_done = Clock::now();
                 r->state = Request::DONE;
-                in_use -= r->bytes;
                 lat.push_back(ms_between(r->t_submit, r->t_done));
                 if (r->status) ++failed; else ++completed;
                 running.erase(running.begin() + long(k));
                 cv_done.notify_all();
             }
-            // admission
+            // admission, against the bytes the context's jobs hold now: a finished
+            // job keeps its device memory until whoever claimed it releases it,
+            // so finished but unreleased jobs count as well as running ones.
             int looked = 0;
             for (auto it = queue.begin(); it != queue.end() && looked < kLookAhead &&
                                           int(running.size()) < cfg.max_in_flight;) {
-                Request* r = *it;
                 ++looked;
-                if (in_use + r->bytes > budget && !running.empty()) { ++it; continue; }
-                it = queue.erase(it);
-                in_use += r->bytes;
-                r->t_admit = Clock::now();
-                qwait.push_back(ms_between(r->t_submit, r->t_admit));
-                lk.unlock();
-                const int st = rpd_job_submit(ctx, r->scene, &r->cfg, &r->cam, r->seed, &r->job);
-                lk.lock();
-                if (st) {
-                    in_use -= r->bytes;
-                    r->status = st;
-                    r->job = nullptr;
-                    r->t_done = Clock::now();
-                    r->state = Request::DONE;
-                    ++failed;
-                    cv_done.notify_all();
-                } else {
-                    r->state = Request::RUNNING;
-                    running.push_back(r);
-                    max_seen = std::max(max_seen, int(running.size()));
-                    peak = std::max(peak, in_use);
-                }
+                if (held() + (*it)->bytes > budget) { ++it; continue; }
+                Request* r = *it;
+                queue.erase(it);
+                start(r, lk);
                 it = queue.begin();  // the queue changed while unlocked
                 looked = 0;
+            }
+            // Nothing runs and nothing queued fits: the memory is held by finished
+            // jobs not yet released. A caller blocked on a queued request (say,
+            // one that claims in submission order while later requests finished
+            // first) would wait for ever, so that request is admitted over the
+            // budget; so is the oldest one once the server is stopping.
+            if (running.empty() && !queue.empty()) {
+                auto it = std::find_if(queue.begin(), queue.end(), [](const Request* r) { return r->waited; });
+                if (it == queue.end() && stop) it = queue.begin();
+                if (it != queue.end()) {
+                    Request* r = *it;
+                    queue.erase(it);
+                    start(r, lk);
+                }
             }
             if (stop && queue.empty() && running.empty()) break;
             cv_sched.wait_for(lk, std::chrono::microseconds(running.empty() ? 2000 : 100));
@@ -176,6 +212,10 @@ int rpd_server_wait(rpd_server* s, uint64_t request_id, rpd_job** job, double* l
     if (it == s->all.end()) return RPD_ERR_INVALID_ARG;
     Request* r = it->second.get();
     if (r->state == Request::CLAIMED) return RPD_ERR_STATE;
+    if (r->state == Request::QUEUED) {
+        r->waited = true;
+        s->cv_sched.notify_one();
+    }
     s->cv_done.wait(lk, [r]() { return r->state == Request::DONE; });
     if (latency_ms) *latency_ms = ms_between(r->t_submit, r->t_done);
     const int status = r->status;
@@ -187,6 +227,7 @@ int rpd_server_wait(rpd_server* s, uint64_t request_id, rpd_job** job, double* l
         r->job = nullptr;
     }
     r->state = Request::CLAIMED;
+    if (status) return rpd_internal::set_error(status, r->error.c_str());
     return status;
 }
 

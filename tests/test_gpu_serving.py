@@ -82,7 +82,10 @@ def test_server_results_same_alone_and_in_the_mix():
 
 
 def test_server_admission_by_memory_budget():
-    """With room for one job at a time the loop admits one at a time and still finishes all."""
+    """With room for about one job at a time the loop still finishes all. Held
+    bytes count finished jobs until released, so a request the caller waits for
+    while a later, finished one is unclaimed goes over the budget by at most
+    that one job."""
     ctx = eng.Context(0)
     reqs = _requests()[:3]
     biggest = max(eng.job_bytes(cfg) for cfg, _, _ in reqs)
@@ -94,10 +97,42 @@ def test_server_admission_by_memory_budget():
             j.release()
         st = srv.stats()
         assert st["completed"] == 3
-        assert st["peak_bytes"] <= int(biggest * 1.2)
+        assert st["max_in_flight_seen"] <= 2
+        assert st["peak_bytes"] <= int(biggest * 1.2) + biggest
         with pytest.raises(eng.EngineError):   # larger than the whole budget: refused at submit
             big = eng.default_config(image_width=1024, image_height=1024, samples_per_pixel=64, use_statmc=1)
             srv.submit(big, eng.make_camera(), 1)
     finally:
+        srv.close()
+        ctx.close()
+
+
+def test_server_admits_a_waited_request_while_claimed_jobs_are_held():
+    """Claimed jobs hold their device memory until released; with room for one job,
+    a caller that keeps every job it claimed must still get the next request
+    (admitted over the budget once nothing runs) and the same results."""
+    ctx = eng.Context(0)
+    reqs = _requests()[:3]
+    biggest = max(eng.job_bytes(cfg) for cfg, _, _ in reqs)
+    srv = eng.Server(ctx, max_in_flight=4, memory_budget=int(biggest * 1.2), graph_passes=False)
+    kept = []
+    try:
+        alone = []
+        for cfg, cam, seed in reqs:
+            j = eng.Job(ctx, cfg, cam, seed, submit=True)
+            j.wait()
+            alone.append(_outputs(j))
+            j.release()
+        ids = [srv.submit(cfg, cam, seed) for cfg, cam, seed in reqs]
+        for k, rid in enumerate(ids):
+            j, _ = srv.wait(rid)
+            kept.append(j)
+            _same(alone[k], _outputs(j), f"request {k} with earlier jobs held")
+        st = srv.stats()
+        assert st["completed"] == 3 and st["failed"] == 0
+        assert st["peak_bytes"] >= sum(eng.job_bytes(cfg) for cfg, _, _ in reqs[:2]) > int(biggest * 1.2)
+    finally:
+        for j in kept:
+            j.release()
         srv.close()
         ctx.close()
