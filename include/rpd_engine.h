@@ -112,6 +112,12 @@ int rpd_ctx_destroy(rpd_context* ctx);
 const char* rpd_last_error(void);
 /* Device memory the context may hand to jobs (admission budget), bytes. */
 int rpd_ctx_set_memory_budget(rpd_context* ctx, uint64_t bytes);
+/* Extension (the reference allocates its buffers once per process): device
+   buffers of released jobs are kept, up to `bytes` (default 32 GiB), and
+   reused by later jobs asking for the same sizes; they do not count as held by
+   jobs. A device allocation that fails empties the cache and is tried again.
+   0 frees and disables it. `cached` (may be NULL) receives the bytes kept. */
+int rpd_ctx_set_buffer_cache(rpd_context* ctx, uint64_t bytes, uint64_t* cached);
 
 /* ---- scenes: scenes::load_scene (core/scene_loader.h:11-13) + BVH build (main.cpp:1855) ---- */
 int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, const rpd_sphere* spheres,

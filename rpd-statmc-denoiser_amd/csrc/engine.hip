@@ -116,6 +116,13 @@ struct rpd_context {
     float* f_buf = nullptr;
     size_t f_cap = 0;
     hipStream_t f_stream = nullptr;
+    // device buffers of released jobs, kept by exact byte size for the next
+    // jobs' allocations (hipMalloc and hipFree of a job's few GB cost
+    // milliseconds); at most cache_cap bytes are kept
+    std::mutex cache_mu;
+    std::multimap<size_t, void*> cache;
+    uint64_t cache_bytes = 0;
+    uint64_t cache_cap = uint64_t(32) << 30;
 };
 
 struct rpd_scene {
@@ -140,6 +147,7 @@ struct rpd_job {
     uint32_t seed = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> owned;
+    std::vector<size_t> owned_bytes;  // size of each owned buffer, for the context's cache
     uint64_t bytes = 0;
     int W = 0, H = 0, N = 0;
     GridGeom geom{};
@@ -564,14 +572,58 @@ int build_batch1(rpd_job* j, ImageDesc* dd, int64_t* dp, unsigned int* dq) {
     return RPD_OK;
 }
 
+// The context's cache of released job buffers: a buffer of exactly the asked
+// size, or null.
+void* cache_take(rpd_context* c, size_t bytes) {
+    std::lock_guard<std::mutex> lk(c->cache_mu);
+    auto it = c->cache.find(bytes);
+    if (it == c->cache.end()) return nullptr;
+    void* p = it->second;
+    c->cache.erase(it);
+    c->cache_bytes -= bytes;
+    return p;
+}
+
+// Frees cached buffers until at most `keep` bytes are cached.
+void cache_trim(rpd_context* c, uint64_t keep) {
+    std::lock_guard<std::mutex> lk(c->cache_mu);
+    for (auto it = c->cache.begin(); it != c->cache.end() && c->cache_bytes > keep;) {
+        dev_free(it->second);
+        c->cache_bytes -= it->first;
+        it = c->cache.erase(it);
+    }
+}
+
+// A job buffer: from the cache when one of the exact size is there, else from
+// hipMalloc; when hipMalloc fails the cache is emptied and it is tried once more.
+// Contents are undefined either way (reset_state clears what must start at zero).
+template <class T>
+int jalloc(rpd_job* j, T** p, size_t count) {
+    *p = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    void* q = cache_take(j->ctx, bytes);
+    if (!q) {
+        hipError_t e = dev_malloc(&q, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            cache_trim(j->ctx, 0);
+            e = dev_malloc(&q, bytes);
+        }
+        if (e != hipSuccess) return fail(RPD_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    j->owned.push_back(q);
+    j->owned_bytes.push_back(bytes);
+    *p = static_cast<T*>(q);
+    return RPD_OK;
+}
+
 int job_alloc(rpd_job* j) {
     const size_t N = j->N;
     const size_t nn = size_t(j->geom.nx) * j->geom.ny;
-    auto& o = j->owned;
     int st;
 #define A(ptr, n)                                        \
     do {                                                 \
-        if ((st = dalloc(&(ptr), (n), o))) return st;    \
+        if ((st = jalloc(j, &(ptr), (n)))) return st;    \
     } while (0)
     float* sums; A(sums, 13 * N);
     j->pb.radiance = sums; j->pb.albedo = sums + 3 * N; j->pb.normal = sums + 6 * N; j->pb.world_pos = sums + 9 * N;
@@ -634,8 +686,23 @@ void job_free(rpd_job* j) {
     if (j->stream) hipStreamSynchronize(j->stream);
     for (auto& s : j->spans) { event_destroy(s.a); event_destroy(s.b); }
     j->spans.clear();
-    for (void* p : j->owned) dev_free(p);
+    // the stream is idle: the buffers go to the context's cache, up to its cap
+    if (j->ctx) {
+        std::lock_guard<std::mutex> lk(j->ctx->cache_mu);
+        for (size_t k = 0; k < j->owned.size(); ++k) {
+            const size_t b = j->owned_bytes[k];
+            if (j->ctx->cache_bytes + b <= j->ctx->cache_cap) {
+                j->ctx->cache.emplace(b, j->owned[k]);
+                j->ctx->cache_bytes += b;
+            } else {
+                dev_free(j->owned[k]);
+            }
+        }
+    } else {
+        for (void* p : j->owned) dev_free(p);
+    }
     j->owned.clear();
+    j->owned_bytes.clear();
     if (j->stream) stream_destroy(j->stream);
     j->stream = nullptr;
     if (j->ctx) j->ctx->in_use -= j->bytes;
@@ -811,12 +878,26 @@ int rpd_ctx_destroy(rpd_context* ctx) {
     if (ctx->f_stream) hipStreamSynchronize(ctx->f_stream);
     if (ctx->f_buf) dev_free(ctx->f_buf);
     if (ctx->f_stream) stream_destroy(ctx->f_stream);
+    cache_trim(ctx, 0);
     delete ctx;
     return RPD_OK;
 }
 int rpd_ctx_set_memory_budget(rpd_context* ctx, uint64_t bytes) {
     if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
     ctx->budget = bytes;
+    return RPD_OK;
+}
+int rpd_ctx_set_buffer_cache(rpd_context* ctx, uint64_t bytes, uint64_t* cached) {
+    if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
+    {
+        std::lock_guard<std::mutex> lk(ctx->cache_mu);
+        ctx->cache_cap = bytes;
+    }
+    cache_trim(ctx, bytes);
+    if (cached) {
+        std::lock_guard<std::mutex> lk(ctx->cache_mu);
+        *cached = ctx->cache_bytes;
+    }
     return RPD_OK;
 }
 

@@ -188,6 +188,7 @@ def lib():
         for name, args in [
             ("rpd_ctx_create", [C.c_int, C.POINTER(vp)]), ("rpd_ctx_destroy", [vp]),
             ("rpd_ctx_set_memory_budget", [vp, C.c_uint64]),
+            ("rpd_ctx_set_buffer_cache", [vp, C.c_uint64, C.POINTER(C.c_uint64)]),
             ("rpd_scene_create", [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.POINTER(vp)]),
             ("rpd_scene_builtin", [vp, C.c_char_p, C.POINTER(vp)]),
             ("rpd_scene_counts", [vp, vp, vp, vp, vp]),
@@ -300,6 +301,12 @@ class Context:
 
     def set_memory_budget(self, nbytes):
         check(self.L.rpd_ctx_set_memory_budget(self.h, int(nbytes)))
+
+    def set_buffer_cache(self, nbytes):
+        """Bytes of released jobs' device buffers kept for reuse (0 disables); returns the bytes now kept."""
+        kept = C.c_uint64()
+        check(self.L.rpd_ctx_set_buffer_cache(self.h, int(nbytes), C.byref(kept)))
+        return kept.value
 
     def close(self):
         for s in self._scenes.values():

@@ -136,3 +136,33 @@ def test_server_admits_a_waited_request_while_claimed_jobs_are_held():
             j.release()
         srv.close()
         ctx.close()
+
+
+def test_reused_job_buffers_give_the_same_results():
+    """Released jobs' device buffers are kept by the context and handed to later
+    jobs of the same sizes. Each request runs after a job of the same sizes but
+    another seed, so it reuses buffers holding other data, and must match the
+    run on fresh buffers bit for bit."""
+    ctx = eng.Context(0)
+    try:
+        assert ctx.set_buffer_cache(0) == 0
+        fresh = []
+        for cfg, cam, seed in _requests():
+            j = eng.Job(ctx, cfg, cam, seed, submit=True)
+            j.wait()
+            fresh.append(_outputs(j))
+            j.release()
+        assert ctx.set_buffer_cache(1 << 34) == 0
+        for k, (cfg, cam, seed) in enumerate(_requests()):
+            other = eng.Job(ctx, cfg, cam, seed + 777, submit=True)
+            other.wait()
+            other.release()
+            kept = ctx.set_buffer_cache(1 << 34)
+            assert kept >= eng.job_bytes(cfg) // 2
+            j = eng.Job(ctx, cfg, cam, seed, submit=True)
+            j.wait()
+            _same(fresh[k], _outputs(j), f"request {k} on reused buffers")
+            j.release()
+        assert ctx.set_buffer_cache(0) == 0
+    finally:
+        ctx.close()
