@@ -1037,6 +1037,9 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
     sc->dev.light_cdf = pc;
     for (int k = 0; k < 3; ++k) sc->dev.env[k] = env ? env[k] : 0.0f;
     sc->dev.env_on = 0;
+    sc->dev.lambert_only = n_mats > 0 && std::all_of(dm.begin(), dm.end(), [](const DevMaterial& m) {
+        return m.kind == MAT_LAMBERT;
+    }) ? 1 : 0;
     sc->n_tris = n_tris; sc->n_spheres = n_spheres; sc->n_materials = n_mats; sc->n_lights = n_lights;
     *out = sc.release();
     return RPD_OK;

@@ -93,7 +93,9 @@ constexpr int TRACE_FLAT_SPHERES = 32;
 // kBvh: the scene's BVH with a per-lane traversal stack in LDS; otherwise the
 // triangle table staged in LDS and looped over. kDepth: paths stop after
 // p.max_depth shading points; without it the loop carries no depth count.
-template <bool kBvh, bool kDepth>
+// kLambert: every material of the scene is Lambertian, so the microfacet and
+// mirror code is left out (fewer registers, fewer divergent branches).
+template <bool kBvh, bool kDepth, bool kLambert>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
     const DevScene& S = p.scene;
     extern __shared__ float4 s_dyn[];
@@ -173,7 +175,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
             } else {
                 // main.cpp:480-529
                 const DevMaterial& m = S.materials[smat];
-                const bool delta_brdf = m.kind == MAT_MIRROR;
+                const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
                 if (h.hit && emitter) {
                     const DevLight& E = S.lights[li];
                     float wb = 1.0f;
@@ -283,7 +285,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
         // -- light sample of a bounce: main.cpp:405-448 ------------------------
         if (st == ST_LIGHT) {
             const DevMaterial& m = S.materials[smat];
-            const bool delta_brdf = m.kind == MAT_MIRROR;
+            const int mkind = kLambert ? int(MAT_LAMBERT) : m.kind;
+            const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
             Ld = mk3(0, 0, 0);
             ro = offset_origin(sp, sn);
             st = ST_ENV;
@@ -304,11 +307,11 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
                 if (dist > RPD_EPS_SMALL || is_delta) {
                     const float pl = light_pdf(S, li, sel_pdf, wi, dist);
                     if (pl > RPD_EPS_SMALL) {
-                        const float pw = fmaxf(RPD_EPS_SMALL, brdf_pdf(m, swo, wi, sn));
+                        const float pw = fmaxf(RPD_EPS_SMALL, brdf_pdf(mkind, m, swo, wi, sn));
                         const float wl = is_delta ? 1.0f : pl / (pl + pw);
                         const v3 em = eval_light(lt, -wi, dist);
                         if (maxc3(em) > 0.0f) {
-                            const v3 f = brdf_eval(m, swo, wi, sn);
+                            const v3 f = brdf_eval(mkind, m, swo, wi, sn);
                             const float g = delta_brdf ? 1.0f : fmaxf(0.0f, dot3(sn, wi));
                             const v3 contrib = (vmul(em, f) * g) / pl;
                             cw = wl * contrib;
@@ -326,7 +329,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
         // -- environment sample: main.cpp:450-462 -------------------------------
         if (st == ST_ENV && q == Q_NONE) {
             const DevMaterial& m = S.materials[smat];
-            const bool delta_brdf = m.kind == MAT_MIRROR;
+            const int mkind = kLambert ? int(MAT_LAMBERT) : m.kind;
+            const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
             st = ST_BRDF;
             const v2 eu = dr.next2();
             if (env_on && !delta_brdf) {
@@ -335,9 +339,9 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
                     flags |= REC_ENV;
                     p.rec.r6[slot] = make_float2(eu.x, eu.y);
                 }
-                const float bp = brdf_pdf(m, swo, ed, sn);
+                const float bp = brdf_pdf(mkind, m, swo, ed, sn);
                 const float we = pdf_env / (pdf_env + bp);
-                const v3 f = brdf_eval(m, swo, ed, sn);
+                const v3 f = brdf_eval(mkind, m, swo, ed, sn);
                 const float c = fmaxf(0.0f, dot3(sn, ed));
                 cw = (we * vmul(env, f)) * (c / pdf_env);
                 if (!is_zero3(cw)) { rd = ed; qt = inf; q = Q_SHADOW_E; st = ST_ENV; }
@@ -346,17 +350,18 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
         // -- BRDF sample: main.cpp:464-479 --------------------------------------
         if (st == ST_BRDF && q == Q_NONE) {
             const DevMaterial& m = S.materials[smat];
-            const bool delta_brdf = m.kind == MAT_MIRROR;
+            const int mkind = kLambert ? int(MAT_LAMBERT) : m.kind;
+            const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
             const v2 bu = dr.next2();
             if (p.capture && !delta_brdf && !(flags & REC_BRDF)) {
                 flags |= REC_BRDF;
                 reinterpret_cast<float2*>(&p.rec.r5[slot])[0] = make_float2(bu.x, bu.y);
             }
             v3 dir; float pdf;
-            brdf_sample(m, swo, sn, bu, dir, pdf);
+            brdf_sample(mkind, m, swo, sn, bu, dir, pdf);
             if (pdf > RPD_EPS_SMALL && sqnorm3(dir) > 0.0f && (!kDepth || p.max_depth <= 0 || depth < p.max_depth)) {
                 bpdf = pdf;
-                cw = brdf_sample_weight(m, swo, dir, sn, pdf);
+                cw = brdf_sample_weight(mkind, m, swo, dir, sn, pdf);
                 rd = dir; qt = inf; q = Q_EXT;
             } else {
                 L = L + vmul(T, Ld);
@@ -383,28 +388,38 @@ static size_t trace_lds_bytes(const DevScene& s, const DevBvh& b) {
 
 // BVH scenes always take the depth-limited variant (max_depth 0 still means
 // unbounded there); flat scenes take it only when a limit is set.
-static const void* trace_kernel(const DevBvh& b, int max_depth) {
-    if (b.nodes) return reinterpret_cast<const void*>(&k_trace<true, true>);
-    return max_depth > 0 ? reinterpret_cast<const void*>(&k_trace<false, true>)
-                         : reinterpret_cast<const void*>(&k_trace<false, false>);
+template <bool kLambert>
+static const void* trace_kernel_m(const DevBvh& b, int max_depth) {
+    if (b.nodes) return reinterpret_cast<const void*>(&k_trace<true, true, kLambert>);
+    return max_depth > 0 ? reinterpret_cast<const void*>(&k_trace<false, true, kLambert>)
+                         : reinterpret_cast<const void*>(&k_trace<false, false, kLambert>);
+}
+static const void* trace_kernel(const DevScene& s, const DevBvh& b, int max_depth) {
+    return s.lambert_only ? trace_kernel_m<true>(b, max_depth) : trace_kernel_m<false>(b, max_depth);
 }
 
 int trace_blocks_per_cu(const DevScene& s, const DevBvh& b, int max_depth) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, trace_kernel(b, max_depth), TRACE_BLOCK,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, trace_kernel(s, b, max_depth), TRACE_BLOCK,
                                                      trace_lds_bytes(s, b)) != hipSuccess || nb <= 0)
         nb = 1;
     return nb;
 }
 
+template <bool kLambert>
+static void launch_trace_m(const TraceParams& p, int n_blocks, size_t lds, hipStream_t st) {
+    if (p.bvh.nodes)
+        hipLaunchKernelGGL((k_trace<true, true, kLambert>), dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
+    else if (p.max_depth > 0)
+        hipLaunchKernelGGL((k_trace<false, true, kLambert>), dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
+    else
+        hipLaunchKernelGGL((k_trace<false, false, kLambert>), dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
+}
+
 void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st) {
     const size_t lds = trace_lds_bytes(p.scene, p.bvh);
-    if (p.bvh.nodes)
-        hipLaunchKernelGGL((k_trace<true, true>), dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
-    else if (p.max_depth > 0)
-        hipLaunchKernelGGL((k_trace<false, true>), dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
-    else
-        hipLaunchKernelGGL((k_trace<false, false>), dim3(n_blocks), dim3(TRACE_BLOCK), lds, st, p);
+    if (p.scene.lambert_only) launch_trace_m<true>(p, n_blocks, lds, st);
+    else launch_trace_m<false>(p, n_blocks, lds, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -229,6 +229,7 @@ struct DevScene {
     const float* light_cdf;  // inclusive cumulative power
     float env[3];
     int env_on;
+    int lambert_only;        // every material is Lambertian: the trace kernel's specialised variant applies
 };
 
 // BVH over a scene's triangles and spheres, for scenes too large for the flat
@@ -486,9 +487,12 @@ __device__ __forceinline__ float mf_pdf(const DevMaterial& m, v3 wo_w, v3 wi_w, 
 }
 __device__ __forceinline__ v3 reflect3(v3 d, v3 n) { return d - (2.0f * dot3(d, n)) * n; }
 
-__device__ inline v3 brdf_eval(const DevMaterial& m, v3 wo_w, v3 wi_w, v3 n) {
-    if (m.kind == MAT_LAMBERT) return ld3(m.albedo) / float(RPD_PI);
-    if (m.kind == MAT_MICROFACET) {
+// The BRDF functions take the material kind as an argument: a kernel that
+// knows every material of its scene is Lambertian passes MAT_LAMBERT, and the
+// other kinds' code drops out of it.
+__device__ inline v3 brdf_eval(int kind, const DevMaterial& m, v3 wo_w, v3 wi_w, v3 n) {
+    if (kind == MAT_LAMBERT) return ld3(m.albedo) / float(RPD_PI);
+    if (kind == MAT_MICROFACET) {
         const v3 wo = normalized3(world_to_local(wo_w, n)), wi = normalized3(world_to_local(wi_w, n));
         if (wo.z <= RPD_EPS_SMALL || wi.z <= RPD_EPS_SMALL) return mk3(0, 0, 0);
         const v3 h = normalized3(wo + wi);
@@ -504,18 +508,18 @@ __device__ inline v3 brdf_eval(const DevMaterial& m, v3 wo_w, v3 wi_w, v3 n) {
     const v3 in = normalized3(wi_w);
     return (sqnorm3(e - in) <= RPD_EPS_SMALL) ? ld3(m.albedo) : mk3(0, 0, 0);
 }
-__device__ inline float brdf_pdf(const DevMaterial& m, v3 wo_w, v3 wi_w, v3 n) {
-    if (m.kind == MAT_LAMBERT) return float(double(fmaxf(0.0f, dot3(wi_w, n))) / RPD_PI);
-    if (m.kind == MAT_MICROFACET) return mf_pdf(m, wo_w, wi_w, n);
+__device__ inline float brdf_pdf(int kind, const DevMaterial& m, v3 wo_w, v3 wi_w, v3 n) {
+    if (kind == MAT_LAMBERT) return float(double(fmaxf(0.0f, dot3(wi_w, n))) / RPD_PI);
+    if (kind == MAT_MICROFACET) return mf_pdf(m, wo_w, wi_w, n);
     const v3 nn = normalized3(n);
     if (dot3(wo_w, nn) <= RPD_EPS_SMALL || dot3(wi_w, nn) <= RPD_EPS_SMALL) return 0.0f;
     const v3 e = normalized3(reflect3(-normalized3(wo_w), nn));
     const v3 in = normalized3(wi_w);
     return (sqnorm3(e - in) <= RPD_EPS_SMALL) ? 1.0f : 0.0f;
 }
-__device__ inline void brdf_sample(const DevMaterial& m, v3 wo_w, v3 n, v2 u, v3& wi, float& pdf) {
+__device__ inline void brdf_sample(int kind, const DevMaterial& m, v3 wo_w, v3 n, v2 u, v3& wi, float& pdf) {
     wi = mk3(0, 0, 0); pdf = 0.0f;
-    if (m.kind == MAT_LAMBERT) {
+    if (kind == MAT_LAMBERT) {
         const float r = sqrtf(u.x);
         const float theta = float(2.0 * RPD_PI * double(u.y));
         const v3 l = mk3(r * cosf(theta), r * sinf(theta), sqrtf(fmaxf(0.0f, 1.0f - u.x)));
@@ -523,7 +527,7 @@ __device__ inline void brdf_sample(const DevMaterial& m, v3 wo_w, v3 n, v2 u, v3
         pdf = float(double(l.z) / RPD_PI);
         return;
     }
-    if (m.kind == MAT_MICROFACET) {
+    if (kind == MAT_MICROFACET) {
         const v3 wo = normalized3(world_to_local(wo_w, n));
         if (wo.z <= RPD_EPS_SMALL) return;
         const float u1 = clampf(u.x, RPD_EPS_SMALL, 1.0f - RPD_EPS_SMALL);
@@ -553,10 +557,10 @@ __device__ inline void brdf_sample(const DevMaterial& m, v3 wo_w, v3 n, v2 u, v3
     if (!finite3(r)) return;
     wi = r; pdf = 1.0f;
 }
-__device__ inline v3 brdf_sample_weight(const DevMaterial& m, v3 wo, v3 wi, v3 n, float pdf) {
+__device__ inline v3 brdf_sample_weight(int kind, const DevMaterial& m, v3 wo, v3 wi, v3 n, float pdf) {
     if (pdf <= RPD_EPS_SMALL) return mk3(0, 0, 0);
-    if (m.kind == MAT_MIRROR) return ld3(m.albedo);
-    return brdf_eval(m, wo, wi, n) * (fmaxf(0.0f, dot3(n, wi)) / pdf);
+    if (kind == MAT_MIRROR) return ld3(m.albedo);
+    return brdf_eval(kind, m, wo, wi, n) * (fmaxf(0.0f, dot3(n, wi)) / pdf);
 }
 
 // ---- lights: path_tracer/core/light.h, core/scene.h --------------------------
