@@ -51,13 +51,18 @@ void launch_batch_queues_reset(const Batch& b, hipStream_t st) {
 // Grid gather. One workgroup owns one node and walks the pixels of the node's
 // support in row-major order and each pixel's samples in order, which is the
 // order a single sampling thread of the reference feeds that node. Phase 1
-// turns up to 256 (pixel, sample) contributions into (weight, luminance,
-// 13 bin indices) in LDS; phase 2 gives each (accumulator, bin) pair one thread
-// that adds the chunk sequentially in FP64, so every sum is formed in sample
-// order without atomics. The pass totals are then added to the node.
+// turns up to 256 (pixel, sample) contributions into their FP64 terms (w,
+// w*y, w*w, w*y*y) and 13 bin indices in LDS; phase 2 gives each (accumulator,
+// bin) pair and each accumulator's two total pairs one thread that adds the
+// chunk sequentially in FP64, so every sum is formed in sample order without
+// atomics. The pass totals are then added to the node.
 //
-// The support list is built by all threads at once (weights, counts, then an
-// ordered block scan), and phase 2 reads four contributions per LDS access.
+// Threads 0-207 are the bin sums (accumulator tid / 16, bin tid % 16) and
+// 208-233 the totals, so three of the four waves run only the bin loop. A
+// contribution that does not belong to a thread's sum adds -0.0, which leaves
+// every FP64 value unchanged (signed zeros and NaN included), so the loops
+// need no branch. The support list is built by all threads at once (weights,
+// counts, then an ordered block scan).
 constexpr int GATHER_BLOCK = 256;
 constexpr int GATHER_MAX_PIX = 1024;  // support window of up to (2*step)^2 pixels, step <= 16
 
@@ -88,11 +93,13 @@ struct GatherLds {
     float w[GATHER_MAX_PIX];
     int pre[GATHER_MAX_PIX + 1];
     int scan[GATHER_BLOCK];
-    float4 w4[GATHER_BLOCK / 4];
-    float4 y4[GATHER_BLOCK / 4];
+    double2 wy[GATHER_BLOCK];    // (w, w*y) of each contribution of the chunk
+    double2 sq[GATHER_BLOCK];    // (w*w, w*y*y)
     uint32_t bin4[GRID_ACC][GATHER_BLOCK / 4];
     int64_t item;
 };
+constexpr int GATHER_BIN_THREADS = GRID_ACC * 16;                      // 208
+constexpr int GATHER_SUM_THREADS = GATHER_BIN_THREADS + 2 * GRID_ACC;  // 234
 
 // One node of one image; called by all threads of the block.
 __device__ void gather_node(const ImageDesc& d, const int node, GatherLds& L) {
@@ -105,10 +112,6 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds& L) {
     float* s_w = L.w;
     int* s_pre = L.pre;
     int* s_scan = L.scan;
-    float4* c_w4 = L.w4;
-    float4* c_y4 = L.y4;
-    float* c_w = reinterpret_cast<float*>(L.w4);
-    float* c_y = reinterpret_cast<float*>(L.y4);
     signed char* c_bin = reinterpret_cast<signed char*>(L.bin4);
 
     const int gx = node % g.nx, gy = node / g.nx;
@@ -152,9 +155,12 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds& L) {
     if (tid == 0) s_pre[npix] = total;
     __syncthreads();
 
-    // this thread's accumulator slot: k = tid / 18, b = tid % 18
-    const int acc = tid / 18, bsel = tid % 18;
-    const bool active_t = acc < GRID_ACC && (bsel >= 16 || bsel < grid_acc_bins(acc));
+    // this thread's sum: bin (acc, bsel) for tid < 208; totals of accumulator
+    // acc for 208 <= tid < 234 (bsel 16: w and w*w, bsel 17: w*y and w*y*y)
+    const bool bin_t = tid < GATHER_BIN_THREADS;
+    const int acc = bin_t ? tid / 16 : (tid - GATHER_BIN_THREADS) >> 1;
+    const int bsel = bin_t ? tid % 16 : 16 + ((tid - GATHER_BIN_THREADS) & 1);
+    const bool active_t = tid < GATHER_SUM_THREADS && (bsel >= 16 || bsel < grid_acc_bins(acc));
     double a0 = 0.0, a1 = 0.0;
 
     const float half = 0.5f * float(g.support);
@@ -200,29 +206,38 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds& L) {
                 if (fl & REC_RR) bins[11] = (signed char)((fl & REC_RR_SURVIVED) ? 1 : 0);
             }
         }
-        // entries past the end of the chunk carry bin -1 and add nothing
-        c_w[tid] = wc;
-        c_y[tid] = y;
+        // entries past the end of the chunk carry bin -1 and add nothing; the
+        // FP64 terms are the products the sums add, each rounded once
+        {
+            const double wd = double(wc), yd = double(y);
+            const double wyd = wd * yd;
+            L.wy[tid] = make_double2(wd, wyd);
+            L.sq[tid] = make_double2(wd * wd, wyd * yd);
+        }
         for (int k = 0; k < GRID_ACC; ++k) c_bin[k * GATHER_BLOCK + tid] = bins[k];
         __syncthreads();
         const int nquad = (min(GATHER_BLOCK, total - base) + 3) >> 2;
-        if (active_t) {
+        if (bin_t) {
             for (int i4 = 0; i4 < nquad; ++i4) {
                 const uint32_t b4 = L.bin4[acc][i4];
-                const float4 w4 = c_w4[i4], y4 = c_y4[i4];
-                const float wv[4] = {w4.x, w4.y, w4.z, w4.w}, yv[4] = {y4.x, y4.y, y4.z, y4.w};
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int bi = int(int8_t(b4 >> (8 * e)));
-                    if (bi < 0) continue;
-                    const double wd = double(wv[e]), yd = double(yv[e]);
-                    if (bsel < 16) {
-                        if (bi == bsel) { a0 += wd; a1 += wd * yd; }
-                    } else if (bsel == 16) {
-                        a0 += wd; a1 += wd * wd;
-                    } else {
-                        a0 += wd * yd; a1 += wd * yd * yd;
-                    }
+                    const bool mine = int(int8_t(b4 >> (8 * e))) == bsel;
+                    const double2 v = L.wy[4 * i4 + e];
+                    a0 += mine ? v.x : -0.0;
+                    a1 += mine ? v.y : -0.0;
+                }
+            }
+        } else if (tid < GATHER_SUM_THREADS) {
+            const bool second = bsel == 17;
+            for (int i4 = 0; i4 < nquad; ++i4) {
+                const uint32_t b4 = L.bin4[acc][i4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const bool in = int(int8_t(b4 >> (8 * e))) >= 0;
+                    const double2 v = L.wy[4 * i4 + e], q = L.sq[4 * i4 + e];
+                    a0 += in ? (second ? v.y : v.x) : -0.0;
+                    a1 += in ? (second ? q.y : q.x) : -0.0;
                 }
             }
         }
