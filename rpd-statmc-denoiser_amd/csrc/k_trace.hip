@@ -15,10 +15,17 @@ namespace rpd {
 // pixel_sample_seed(idx, n) and performs the first twist. The seeding words
 // x[k] come from the recurrence x[k] = 1812433253 * (x[k-1] ^ x[k-1] >> 30) + k,
 // walked by two cursors (at k+1 and at k+397) so the seeded state is never
-// stored; words k >= 227 of the twist read back the tape words it wrote 227
-// steps earlier.
+// stored; word k >= 227 of the twist reads back the tape word k - 227 it wrote
+// earlier. Words are written four at a time (one uint4 per lane, so a wave
+// writes one contiguous KiB) and read back the same way: the chunk holding
+// words 4c-227 .. 4c-224 is read one chunk ahead of its use, so the read-back
+// latency overlaps the next four words' work.
 __device__ __forceinline__ uint32_t mt_seed_step(uint32_t prev, uint32_t k) {
     return 1812433253u * (prev ^ (prev >> 30)) + k;
+}
+
+__device__ __forceinline__ uint32_t u4_at(const uint4& v, int i) {
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
 }
 
 __global__ void __launch_bounds__(256) k_tape(TraceParams p) {
@@ -31,26 +38,37 @@ __global__ void __launch_bounds__(256) k_tape(TraceParams p) {
     uint32_t lo = s;               // x[k]
     uint32_t far = s;              // x[k + 397] while k + 397 < 624
     for (uint32_t k = 1; k <= (uint32_t)MT_M; ++k) far = mt_seed_step(far, k);
-    // four words per store: the 64 lanes of a wave write one contiguous KiB
+    constexpr int kBack = MT_N - MT_M;  // 227: word k >= kBack mixes in tape word k - kBack
+    constexpr int kLastBackChunk = (MT_N - 2 - kBack) / 4;  // 98: chunk of the last such word (622 - 227)
+    // Window of read-back chunks: at the start of iteration c, m57 = chunk c-57
+    // and m56 = chunk c-56 (word k = 4c+i reads word 4(c-57)+i+1); chunk c-54 is
+    // requested at the end of iteration c, after the iteration that wrote it.
+    uint4 m57 = make_uint4(0, 0, 0, 0), m56 = make_uint4(0, 0, 0, 0), m55 = make_uint4(0, 0, 0, 0);
     for (int c = 0; c < MT_N / 4; ++c) {
         uint32_t w4[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int k = 4 * c + i;
-            if (k < MT_N - MT_M) {                       // far word from the seeded state
-                const uint32_t nxt = mt_seed_step(lo, (uint32_t)(k + 1));
-                w4[i] = mt_mix(lo, nxt, far);
-                lo = nxt;
-                if (k + 1 < MT_N - MT_M) far = mt_seed_step(far, (uint32_t)(k + 1 + MT_M));
-            } else if (k < MT_N - 1) {                   // far word is an already twisted word
-                const uint32_t nxt = mt_seed_step(lo, (uint32_t)(k + 1));
-                w4[i] = mt_mix(lo, nxt, t[tape_word(k + MT_M - MT_N)]);
-                lo = nxt;
+            if (k < kBack) {                             // far word from the seeded state
+                const uint32_t x1 = mt_seed_step(lo, (uint32_t)(k + 1));
+                w4[i] = mt_mix(lo, x1, far);
+                lo = x1;
+                if (k + 1 < kBack) far = mt_seed_step(far, (uint32_t)(k + 1 + MT_M));
+            } else if (k < MT_N - 1) {                   // far word: tape word k - 227
+                const uint32_t back = i < 3 ? u4_at(m57, i + 1) : m56.x;
+                const uint32_t x1 = mt_seed_step(lo, (uint32_t)(k + 1));
+                w4[i] = mt_mix(lo, x1, back);
+                lo = x1;
             } else {                                     // last word wraps to the new word 0
                 w4[i] = mt_mix(lo, t[tape_word(0)], t[tape_word(MT_M - 1)]);
             }
         }
         *reinterpret_cast<uint4*>(t + tape_word(4 * c)) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+        m57 = m56;
+        m56 = m55;
+        const int ahead = c - 54;
+        if (ahead >= 0 && ahead <= kLastBackChunk)
+            m55 = *reinterpret_cast<const uint4*>(t + tape_word(4 * ahead));
     }
 }
 

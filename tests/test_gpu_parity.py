@@ -59,6 +59,24 @@ def test_trace_records_match_oracle(engine_ctx, aperture):
     assert rel_err(orec[~bad, 1:4], grec[~bad, 1:4]).max() <= 1e-4
 
 
+def test_trace_records_past_the_first_twist_match_oracle(engine_ctx):
+    """32 samples per pixel draw about a thousand numbers per pixel: tape words
+    227-623 (whose twist reads back words the tape kernel wrote earlier) and the
+    in-kernel re-twist after word 623 are all consumed."""
+    rc = ob.config_c0(16, 12)
+    rc.spp = 32
+    oj = _pair(rc)
+    oj.run_pass(np.full(16 * 12, rc.spp, np.int32), statmc=True)
+    gj = _gpu_job(engine_ctx, rc)
+    gj.run_pass(None, statmc=True)
+    orec, grec = oj.get("records"), gj.get("records")
+    bad = _compare_records(orec, grec)[0]
+    print(f"samples {len(orec)} outliers {bad.sum()}")
+    assert len(orec) == 16 * 12 * 32
+    assert bad.mean() <= 2e-3
+    assert rel_err(orec[~bad, 1:4], grec[~bad, 1:4]).max() <= 1e-4
+
+
 def test_trace_uniform_render_matches_oracle(engine_ctx):
     rc = ob.config_c0(24, 20)
     rc.use_statmc = 0
