@@ -295,7 +295,14 @@ struct AllocBufs {
     unsigned char* clipped;
     AllocState* st;
     SeqSum seq;
+    // min and max importance over the unclipped pixels of each chunk of 64
+    // pixels and of each super-chunk of 64 chunks (NaN: none left)
+    float* cmin;
+    float* cmax;
+    float* smin;
+    float* smax;
 };
+constexpr int ACH = 64;  // pixels per chunk, chunks per super-chunk
 
 __global__ void k_alloc_begin(int n, AllocConfig ac, int* counts, AllocBufs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -323,58 +330,129 @@ __global__ void k_alloc_first_clip(int n, AllocConfig ac, const float* imp, Allo
     if (want > double(max_pp)) atomicMin(&a.st->first_clip, i);
 }
 
-// The clip-and-redistribute sweeps (main.cpp:1387-1412), one workgroup.
-// The first sweep starts at the first clip found by k_alloc_first_clip.
-__global__ void __launch_bounds__(SB) k_alloc_sweeps(int n, AllocConfig ac, const float* imp, AllocBufs a) {
-    __shared__ int wave_first[SB / 64];
-    __shared__ int sh_found;
-    const int t = threadIdx.x;
+__device__ __forceinline__ float wave_fmin(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_fmax(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Chunk and super-chunk min/max of the importance before any clip: one wave
+// per chunk (or super-chunk).
+__global__ void __launch_bounds__(256) k_alloc_chunk_minmax(int n, const float* imp, AllocBufs a) {
+    const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    const int nch = (n + ACH - 1) / ACH;
+    if (c >= nch || !a.st->active) return;
+    const int i = c * ACH + lane;
+    const float v = i < n ? imp[i] : __builtin_nanf("");
+    const float mn = wave_fmin(v), mx = wave_fmax(v);
+    if (lane == (c & 63)) { a.cmin[c] = mn; a.cmax[c] = mx; }
+}
+__global__ void __launch_bounds__(256) k_alloc_super_minmax(int n, AllocBufs a) {
+    const int s = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    const int nch = (n + ACH - 1) / ACH, nsc = (nch + ACH - 1) / ACH;
+    if (s >= nsc || !a.st->active) return;
+    const int c = s * ACH + lane;
+    const float mn = wave_fmin(c < nch ? a.cmin[c] : __builtin_nanf(""));
+    const float mx = wave_fmax(c < nch ? a.cmax[c] : __builtin_nanf(""));
+    if (lane == (s & 63)) { a.smin[s] = mn; a.smax[s] = mx; }
+}
+
+// The clip-and-redistribute sweeps (main.cpp:1387-1412), one wave.
+//
+// Each sweep visits the unclipped pixels in index order, and clips a pixel
+// when double(imp) / rem_imp * remaining exceeds the cap, which changes
+// rem_imp and remaining for the pixels after it. For fixed (rem_imp,
+// remaining) that expression is monotone in imp (every step is a correctly
+// rounded division or multiplication by a fixed operand), so a set of pixels
+// holds a clip exactly when its smallest or its largest importance clips. The
+// sweep therefore tests whole chunks by their min and max, and whole
+// super-chunks likewise, and only walks the pixels of a chunk that holds a
+// clip; within such a chunk the 64 pixels stay in registers while their clips
+// are taken one after another, in order, each under the state left by the one
+// before. Chunk min/max are recomputed when a chunk loses pixels; super-chunk
+// min/max are refreshed whenever the sweep descends into one (between
+// refreshes they can only bound a superset of its pixels, so a stale value
+// costs a look, never a missed clip). Every array element is read and written
+// by lane (index & 63) only, so the wave needs no fences between lanes.
+__global__ void __launch_bounds__(64) k_alloc_sweeps(int n, AllocConfig ac, const float* imp, AllocBufs a) {
+    const int lane = threadIdx.x;
     if (!a.st->active) return;
     const int max_pp = max(4, 8 * max(1, ac.adaptive_spp));
+    const float qnan = __builtin_nanf("");
     double rem_imp = a.st->rem_imp;
     int remaining = a.st->remaining;
     int active_count = n;
     const int first = a.st->first_clip;
+    const int nch = (n + ACH - 1) / ACH, nsc = (nch + ACH - 1) / ACH;
     bool final_assign = false;
     bool first_sweep = true;
+    auto clips = [&](float v) { return double(v) / rem_imp * double(remaining) > double(max_pp); };
     while (active_count > 0 && remaining > 0 && rem_imp > double(RPD_EPS_SMALL)) {
         bool clipped_any = false;
-        int pos = 0;
-        while (pos < n) {
-            int found = 0x7fffffff;
-            if (first_sweep && pos == 0) {
-                found = first;                 // no earlier index clips under the initial state
-            } else {
-                for (int base = pos; base < n && found == 0x7fffffff; base += SB) {
-                    const int i = base + t;
-                    bool hitc = false;
-                    if (i < n && !a.clipped[i]) hitc = double(imp[i]) / rem_imp * double(remaining) > double(max_pp);
-                    const uint64_t b = __ballot(hitc);
-                    if ((t & 63) == 0) wave_first[t >> 6] = b ? (base + (t & ~63) + __ffsll((long long)b) - 1) : 0x7fffffff;
-                    __syncthreads();
-                    if (t == 0) {
-                        int f = 0x7fffffff;
-                        for (int w = 0; w < SB / 64; ++w) f = min(f, wave_first[w]);
-                        sh_found = f;
-                    }
-                    __syncthreads();
-                    found = sh_found;
-                    __syncthreads();
-                }
-            }
-            if (found == 0x7fffffff) break;
-            if (t == 0) { a.desired[found] = float(max_pp); a.clipped[found] = 1; }
-            remaining -= max_pp;
-            rem_imp -= double(imp[found]);
-            --active_count;
-            clipped_any = true;
-            pos = found + 1;
-            __syncthreads();
-        }
+        // under the initial state no pixel before `first` clips
+        int pos = first_sweep ? min(first, n) : 0;
         first_sweep = false;
+        while (pos < n) {
+            // the rest of pos's chunk, pixel by pixel
+            const int c = pos / ACH;
+            const int i = c * ACH + lane;
+            const bool in_range = i < n;
+            const float v = in_range ? imp[i] : 0.0f;
+            bool cl = in_range ? a.clipped[i] != 0 : true;
+            bool changed = false;
+            while (true) {
+                const uint64_t b = __ballot(!cl && i >= pos && clips(v));
+                if (!b) break;
+                const int fl = __ffsll((long long)b) - 1;
+                const float vf = __shfl(v, fl);
+                if (lane == fl) {
+                    cl = true;
+                    a.desired[i] = float(max_pp);
+                    a.clipped[i] = 1;
+                }
+                remaining -= max_pp;
+                rem_imp -= double(vf);
+                --active_count;
+                clipped_any = true;
+                changed = true;
+                pos = c * ACH + fl + 1;
+            }
+            if (changed) {
+                const float mn = wave_fmin(cl ? qnan : v), mx = wave_fmax(cl ? qnan : v);
+                if (lane == (c & 63)) { a.cmin[c] = mn; a.cmax[c] = mx; }
+            }
+            // the next chunk that holds a clip under the current state
+            int next = nch;
+            int c1 = c + 1;
+            while (c1 < nch) {
+                const int sc = c1 / ACH;
+                const int j = sc * ACH + lane;
+                const float mn = j < nch ? a.cmin[j] : qnan, mx = j < nch ? a.cmax[j] : qnan;
+                const float smn = wave_fmin(mn), smx = wave_fmax(mx);
+                if (lane == (sc & 63)) { a.smin[sc] = smn; a.smax[sc] = smx; }
+                const uint64_t b = __ballot(j < nch && j >= c1 && (clips(mn) || clips(mx)));
+                if (b) { next = sc * ACH + __ffsll((long long)b) - 1; break; }
+                // later super-chunks, 64 per look, aligned so lane k & 63 owns super-chunk k
+                int s1 = sc + 1, found = -1;
+                for (int s0 = s1 & ~63; s0 < nsc && found < 0; s0 += 64) {
+                    const int k = s0 + lane;
+                    const bool ok = k >= s1 && k < nsc;
+                    const float kmn = ok ? a.smin[k] : qnan, kmx = ok ? a.smax[k] : qnan;
+                    const uint64_t bs = __ballot(ok && (clips(kmn) || clips(kmx)));
+                    if (bs) found = s0 + __ffsll((long long)bs) - 1;
+                }
+                if (found < 0) break;
+                c1 = found * ACH;
+            }
+            if (next >= nch) break;
+            pos = next * ACH;
+        }
         if (!clipped_any) { final_assign = true; break; }
     }
-    if (t == 0) {
+    if (lane == 0) {
         a.st->rem_imp = rem_imp;
         a.st->remaining = remaining;
         a.st->final_assign = final_assign ? 1 : 0;
@@ -429,9 +507,10 @@ static size_t al(size_t b) { return (b + 255) & ~size_t(255); }
 size_t alloc_scratch_bytes(int n) {
     const size_t N = size_t(n) + 1;
     const size_t nb = N / SB + 2;
+    const size_t nch = N / ACH + 2, nsc = nch / ACH + 2;
     return al(N * 4) + al(N) + al(sizeof(AllocState)) + al(N * 4) + al(N * 4) + al(N * 8) + al(N * 8) +
            al(N * 4) + al(N * 4) + al(N * 4) + al(N * 8) + al(N * 8) + al(8) + al(4) + al(nb * 8) + al(nb * 8) +
-           al(nb * 4);
+           al(nb * 4) + 2 * al(nch * 4) + 2 * al(nsc * 4);
 }
 
 struct Carved {
@@ -465,6 +544,11 @@ static Carved carve(void* scratch, int n) {
     s.bsum_d = reinterpret_cast<double*>(take(nb * 8));
     s.bsum_u = reinterpret_cast<uint64_t*>(take(nb * 8));
     s.bsum_i = reinterpret_cast<int*>(take(nb * 4));
+    const size_t nch = N / ACH + 2, nsc = nch / ACH + 2;
+    c.a.cmin = reinterpret_cast<float*>(take(nch * 4));
+    c.a.cmax = reinterpret_cast<float*>(take(nch * 4));
+    c.a.smin = reinterpret_cast<float*>(take(nsc * 4));
+    c.a.smax = reinterpret_cast<float*>(take(nsc * 4));
     return c;
 }
 
@@ -489,7 +573,10 @@ void launch_alloc_counts(int n, const AllocConfig& ac, const float* imp, int* co
     seq_sum_launch(c.a.seq, false, st);
     hipLaunchKernelGGL(k_alloc_begin, g, b, 0, st, n, ac, counts, c.a);
     hipLaunchKernelGGL(k_alloc_first_clip, g, b, 0, st, n, ac, imp, c.a);
-    hipLaunchKernelGGL(k_alloc_sweeps, dim3(1), dim3(SB), 0, st, n, ac, imp, c.a);
+    const int nch = (n + ACH - 1) / ACH, nsc = (nch + ACH - 1) / ACH;
+    hipLaunchKernelGGL(k_alloc_chunk_minmax, dim3((nch + 3) / 4), dim3(256), 0, st, n, imp, c.a);
+    hipLaunchKernelGGL(k_alloc_super_minmax, dim3((nsc + 3) / 4), dim3(256), 0, st, n, c.a);
+    hipLaunchKernelGGL(k_alloc_sweeps, dim3(1), dim3(64), 0, st, n, ac, imp, c.a);
     hipLaunchKernelGGL(k_alloc_floor, g, b, 0, st, n, imp, counts, c.a, c.resid);
     // residual prefix sums as the reference's running `cumulative` forms them
     c.a.seq.x = c.resid;

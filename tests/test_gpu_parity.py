@@ -214,6 +214,8 @@ def _importance_fields(n, rng):
     yield "constant", np.full(n, 0.37, np.float32)
     ties = (np.arange(n) % 7 == 0).astype(np.float32) * 2.5 + np.float32(2.0 ** -20)
     yield "ties", ties
+    # heavy tail: each sweep's clips raise the share of the rest, so clips cascade over several sweeps
+    yield "pareto", (rng.pareto(0.6, n) + 1e-3).astype(np.float32)
 
 
 @pytest.mark.parametrize("base_samples,spp,w,h", [(0, 4, 64, 48), (1, 4, 64, 48), (0, 1, 64, 48), (3, 8, 64, 48),
