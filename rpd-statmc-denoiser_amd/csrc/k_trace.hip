@@ -108,6 +108,250 @@ __device__ __forceinline__ bool is_zero3(v3 a) { return a.x == 0.0f && a.y == 0.
 constexpr int TRACE_LDS_TRIS = 256;
 constexpr int TRACE_FLAT_SPHERES = 32;
 
+// One step of a path lane, shared by the one-kernel trace and the split
+// shade/intersect pair: resolves the lane's last query (its result in h),
+// writes a finished sample, hands out pixels (a wave-wide operation), then
+// advances the path through its stages until it issues its next query (q) or
+// has none this step. Returns false once every lane of the wave is out of work.
+template <bool kDepth, bool kLambert>
+__device__ __forceinline__ bool path_step(const TraceParams& p, const int lane, const v3 env, const bool env_on,
+                                          int& c_next, int& c_end, const int n_active, const HitRec& h, int& st,
+                                          int& q, int& pix, int& left, int64_t& slot, Draws& dr, v3& ro, v3& rd,
+                                          float& qt, v3& T, v3& L, v3& Ld, v3& sp, v3& sn, v3& swo, v3& cw,
+                                          float& bpdf, int& smat, int& depth, uint32_t& flags, uint32_t& light_bits) {
+    const DevScene& S = p.scene;
+    const float pdf_env = float(1.0 / (4.0 * RPD_PI));
+    const float inf = __builtin_inff();
+    // -- resolve the last query -------------------------------------------
+    if (q == Q_SHADOW_L || q == Q_SHADOW_E) {
+        if (!h.hit) Ld = Ld + cw;
+        st = q == Q_SHADOW_L ? ST_ENV : ST_BRDF;
+    } else if (q == Q_PRIMARY || q == Q_EXT) {
+        bool emitter = false;
+        int li = -1, hm = 0;
+        v3 hn = mk3(0, 0, 0);
+        if (h.hit) {
+            if (h.sphere) {
+                const DevSphere& s = S.spheres[h.index];
+                emitter = s.emitter != 0; li = s.light; hn = h.sn; hm = s.material;
+            } else {
+                const DevTriangle& t = S.tris[h.index];
+                emitter = t.emitter != 0; li = t.light; hn = ld3(t.n); hm = t.material;
+            }
+        }
+        if (q == Q_PRIMARY) {
+            // main.cpp:540-585
+            if (!h.hit) {
+                L = env;
+                st = ST_DONE;
+            } else {
+                const v3 hp = ro + h.t * rd;
+                flags |= REC_HIT;
+                p.rec.r1[slot] = make_float4(hn.x, hn.y, hn.z, h.t);
+                const DevMaterial& hmat = S.materials[hm];
+                p.rec.r2[slot] = make_float4(hmat.albedo[0], hmat.albedo[1], hmat.albedo[2], 0.0f);
+                p.rec.r3[slot] = make_float4(hp.x, hp.y, hp.z, 0.0f);
+                if (emitter) {
+                    L = eval_light(S.lights[li], -rd, h.t);
+                    st = ST_DONE;
+                } else {
+                    sp = hp; sn = hn; swo = -normalized3(rd); smat = hm;
+                    depth = 1;
+                    st = ST_LIGHT;
+                }
+            }
+        } else {
+            // main.cpp:480-529
+            const DevMaterial& m = S.materials[smat];
+            const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
+            if (h.hit && emitter) {
+                const DevLight& E = S.lights[li];
+                float wb = 1.0f;
+                if (!delta_brdf && !light_is_delta(E)) {
+                    const float plb = light_pdf(S, li, light_select_pdf(S, li), rd, h.t);
+                    if (plb > RPD_EPS_SMALL) wb = bpdf / (bpdf + plb);
+                }
+                Ld = Ld + wb * vmul(eval_light(E, -rd, h.t), cw);
+            } else if (!h.hit) {
+                const float wb = delta_brdf ? 1.0f : bpdf / (bpdf + pdf_env);
+                Ld = Ld + wb * vmul(env, cw);
+            }
+            L = L + vmul(T, Ld);
+            st = ST_DONE;
+            if (h.hit && !emitter) {
+                const float ksi = dr.next();
+                if (p.capture && !(flags & REC_RR)) { flags |= REC_RR; if (ksi < 0.8f) flags |= REC_RR_SURVIVED; }
+                if (ksi < 0.8f) {
+                    T = vmul(T, cw) / 0.8f;
+                    sp = ro + h.t * rd;
+                    sn = hn;
+                    swo = -rd;
+                    smat = hm;
+                    ++depth;
+                    st = ST_LIGHT;
+                }
+            }
+        }
+    }
+    q = Q_NONE;
+
+    // -- finish samples ------------------------------------------------
+    if (st == ST_DONE) {
+        if (finite3(L)) flags |= REC_FINITE;
+        flags |= (light_bits & 255u) << REC_LIGHT_SHIFT;
+        p.rec.r0[slot] = make_float4(L.x, L.y, L.z, __uint_as_float(flags));
+        ++slot;
+        --left;
+        st = left > 0 ? ST_CAMERA : ST_NEED;
+    }
+    // -- hand out pixels ------------------------------------------------
+    uint64_t need = ballot64(st == ST_NEED);
+    while (need) {
+        if (c_next >= c_end) {
+            int base = 0;
+            if (lane == 0) base = atomicAdd(p.work_counter, TRACE_CHUNK);
+            base = __shfl(base, 0);
+            c_next = base;
+            c_end = min(base + TRACE_CHUNK, n_active);
+            if (c_next >= c_end) break;
+        }
+        const int avail = c_end - c_next;
+        const int rank = __popcll(need & ((1ull << lane) - 1ull));
+        const bool mine = (st == ST_NEED) && rank < avail;
+        if (mine) {
+            const int a = c_next + rank;
+            pix = p.plan.active[a];
+            left = p.plan.counts[pix];
+            slot = p.plan.offsets[pix];
+            dr.reset(p.tape + tape_base(a));
+            st = ST_CAMERA;
+        }
+        const int taken = min(__popcll(need), avail);
+        c_next += taken;
+        need = ballot64(st == ST_NEED);
+    }
+    if (st == ST_NEED) st = ST_DEAD;
+    if (ballot64(st != ST_DEAD) == 0) return false;
+
+    // -- camera rays: main.cpp:691-705 ----------------------------------
+    if (st == ST_CAMERA) {
+        const int x = pix % p.width, y = pix / p.width;
+        const v2 c_pix = dr.next2();
+        const float u = (float(x) + c_pix.x) / float(p.width);
+        const float v = (float(y) + c_pix.y) / float(p.height);
+        const v2 c_lens = dr.next2();
+        v2 lens; lens.x = 0.0f; lens.y = 0.0f;
+        if (p.cam.lens_radius > 0.0f) lens = sample_disk(c_lens);
+        flags = 0;
+        light_bits = 255u;
+        if (p.capture) {
+            flags |= REC_PIXEL;
+            if (p.cam.lens_radius > 0.0f) flags |= REC_LENS;
+            p.rec.r4[slot] = make_float4(c_pix.x, c_pix.y, c_lens.x, c_lens.y);
+        }
+        v3 o = ld3(p.cam.pos);
+        if (p.cam.lens_radius > 0.0f) {
+            const float lx = p.cam.lens_radius * lens.x, ly = p.cam.lens_radius * lens.y;
+            o = o + (lx * ld3(p.cam.right) + ly * ld3(p.cam.up));
+        }
+        const v3 px = ld3(p.cam.llc) + u * ld3(p.cam.horiz) + (1.0f - v) * ld3(p.cam.vert);
+        ro = o;
+        rd = normalized3(px - o);
+        qt = inf;
+        T = mk3(1, 1, 1);
+        L = mk3(0, 0, 0);
+        q = Q_PRIMARY;
+    }
+
+    // -- light sample of a bounce: main.cpp:405-448 ------------------------
+    if (st == ST_LIGHT) {
+        const DevMaterial& m = S.materials[smat];
+        const int mkind = kLambert ? int(MAT_LAMBERT) : m.kind;
+        const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
+        Ld = mk3(0, 0, 0);
+        ro = offset_origin(sp, sn);
+        st = ST_ENV;
+        const float sel_u = dr.next();
+        int li; float sel_pdf;
+        sample_light(S, sel_u, li, sel_pdf);
+        if (p.capture && !(flags & REC_LSEL) && li >= 0 && li < 16) { flags |= REC_LSEL; light_bits = (uint32_t)li; }
+        if (li >= 0 && (!delta_brdf || light_is_delta(S.lights[li]))) {
+            const DevLight& lt = S.lights[li];
+            const bool is_delta = light_is_delta(lt);
+            const v2 lu = dr.next2();
+            if (p.capture && !(flags & REC_LIGHT) && !is_delta) {
+                flags |= REC_LIGHT;
+                reinterpret_cast<float2*>(&p.rec.r5[slot])[1] = make_float2(lu.x, lu.y);
+            }
+            v3 wi; float dist;
+            light_sample(lt, sp, lu, wi, dist);
+            if (dist > RPD_EPS_SMALL || is_delta) {
+                const float pl = light_pdf(S, li, sel_pdf, wi, dist);
+                if (pl > RPD_EPS_SMALL) {
+                    const float pw = fmaxf(RPD_EPS_SMALL, brdf_pdf(mkind, m, swo, wi, sn));
+                    const float wl = is_delta ? 1.0f : pl / (pl + pw);
+                    const v3 em = eval_light(lt, -wi, dist);
+                    if (maxc3(em) > 0.0f) {
+                        const v3 f = brdf_eval(mkind, m, swo, wi, sn);
+                        const float g = delta_brdf ? 1.0f : fmaxf(0.0f, dot3(sn, wi));
+                        const v3 contrib = (vmul(em, f) * g) / pl;
+                        cw = wl * contrib;
+                        const float eff = dist - RPD_EPS_ANYHIT;  // ray_tracer.h:65-75
+                        if (eff <= RPD_EPS_SMALL) {
+                            Ld = Ld + cw;
+                        } else if (!is_zero3(cw)) {
+                            rd = wi; qt = eff; q = Q_SHADOW_L; st = ST_LIGHT;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // -- environment sample: main.cpp:450-462 -------------------------------
+    if (st == ST_ENV && q == Q_NONE) {
+        const DevMaterial& m = S.materials[smat];
+        const int mkind = kLambert ? int(MAT_LAMBERT) : m.kind;
+        const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
+        st = ST_BRDF;
+        const v2 eu = dr.next2();
+        if (env_on && !delta_brdf) {
+            const v3 ed = sample_sphere(eu);
+            if (p.capture && !(flags & REC_ENV)) {
+                flags |= REC_ENV;
+                p.rec.r6[slot] = make_float2(eu.x, eu.y);
+            }
+            const float bp = brdf_pdf(mkind, m, swo, ed, sn);
+            const float we = pdf_env / (pdf_env + bp);
+            const v3 f = brdf_eval(mkind, m, swo, ed, sn);
+            const float c = fmaxf(0.0f, dot3(sn, ed));
+            cw = (we * vmul(env, f)) * (c / pdf_env);
+            if (!is_zero3(cw)) { rd = ed; qt = inf; q = Q_SHADOW_E; st = ST_ENV; }
+        }
+    }
+    // -- BRDF sample: main.cpp:464-479 --------------------------------------
+    if (st == ST_BRDF && q == Q_NONE) {
+        const DevMaterial& m = S.materials[smat];
+        const int mkind = kLambert ? int(MAT_LAMBERT) : m.kind;
+        const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
+        const v2 bu = dr.next2();
+        if (p.capture && !delta_brdf && !(flags & REC_BRDF)) {
+            flags |= REC_BRDF;
+            reinterpret_cast<float2*>(&p.rec.r5[slot])[0] = make_float2(bu.x, bu.y);
+        }
+        v3 dir; float pdf;
+        brdf_sample(mkind, m, swo, sn, bu, dir, pdf);
+        if (pdf > RPD_EPS_SMALL && sqnorm3(dir) > 0.0f && (!kDepth || p.max_depth <= 0 || depth < p.max_depth)) {
+            bpdf = pdf;
+            cw = brdf_sample_weight(mkind, m, swo, dir, sn, pdf);
+            rd = dir; qt = inf; q = Q_EXT;
+        } else {
+            L = L + vmul(T, Ld);
+            st = ST_DONE;   // no query: the sample is written next iteration
+        }
+    }
+    return true;
+}
+
 // kBvh: the scene's BVH with a per-lane traversal stack in LDS; otherwise the
 // triangle table staged in LDS and looped over. kDepth: paths stop after
 // p.max_depth shading points; without it the loop carries no depth count.
@@ -128,7 +372,6 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
     const int lane = threadIdx.x & 63;
     const v3 env = mk3(S.env[0], S.env[1], S.env[2]);
     const bool env_on = maxc3(env) > 0.0f;
-    const float pdf_env = float(1.0 / (4.0 * RPD_PI));
     const float inf = __builtin_inff();
 
     // wave-uniform chunk cursor over the active list
@@ -152,116 +395,9 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
     uint32_t n_queries = 0;
 
     while (true) {
-        // -- resolve the last query -------------------------------------------
-        if (q == Q_SHADOW_L || q == Q_SHADOW_E) {
-            if (!h.hit) Ld = Ld + cw;
-            st = q == Q_SHADOW_L ? ST_ENV : ST_BRDF;
-        } else if (q == Q_PRIMARY || q == Q_EXT) {
-            bool emitter = false;
-            int li = -1, hm = 0;
-            v3 hn = mk3(0, 0, 0);
-            if (h.hit) {
-                if (h.sphere) {
-                    const DevSphere& s = S.spheres[h.index];
-                    emitter = s.emitter != 0; li = s.light; hn = h.sn; hm = s.material;
-                } else {
-                    const DevTriangle& t = S.tris[h.index];
-                    emitter = t.emitter != 0; li = t.light; hn = ld3(t.n); hm = t.material;
-                }
-            }
-            if (q == Q_PRIMARY) {
-                // main.cpp:540-585
-                if (!h.hit) {
-                    L = env;
-                    st = ST_DONE;
-                } else {
-                    const v3 hp = ro + h.t * rd;
-                    flags |= REC_HIT;
-                    p.rec.r1[slot] = make_float4(hn.x, hn.y, hn.z, h.t);
-                    const DevMaterial& hmat = S.materials[hm];
-                    p.rec.r2[slot] = make_float4(hmat.albedo[0], hmat.albedo[1], hmat.albedo[2], 0.0f);
-                    p.rec.r3[slot] = make_float4(hp.x, hp.y, hp.z, 0.0f);
-                    if (emitter) {
-                        L = eval_light(S.lights[li], -rd, h.t);
-                        st = ST_DONE;
-                    } else {
-                        sp = hp; sn = hn; swo = -normalized3(rd); smat = hm;
-                        depth = 1;
-                        st = ST_LIGHT;
-                    }
-                }
-            } else {
-                // main.cpp:480-529
-                const DevMaterial& m = S.materials[smat];
-                const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
-                if (h.hit && emitter) {
-                    const DevLight& E = S.lights[li];
-                    float wb = 1.0f;
-                    if (!delta_brdf && !light_is_delta(E)) {
-                        const float plb = light_pdf(S, li, light_select_pdf(S, li), rd, h.t);
-                        if (plb > RPD_EPS_SMALL) wb = bpdf / (bpdf + plb);
-                    }
-                    Ld = Ld + wb * vmul(eval_light(E, -rd, h.t), cw);
-                } else if (!h.hit) {
-                    const float wb = delta_brdf ? 1.0f : bpdf / (bpdf + pdf_env);
-                    Ld = Ld + wb * vmul(env, cw);
-                }
-                L = L + vmul(T, Ld);
-                st = ST_DONE;
-                if (h.hit && !emitter) {
-                    const float ksi = dr.next();
-                    if (p.capture && !(flags & REC_RR)) { flags |= REC_RR; if (ksi < 0.8f) flags |= REC_RR_SURVIVED; }
-                    if (ksi < 0.8f) {
-                        T = vmul(T, cw) / 0.8f;
-                        sp = ro + h.t * rd;
-                        sn = hn;
-                        swo = -rd;
-                        smat = hm;
-                        ++depth;
-                        st = ST_LIGHT;
-                    }
-                }
-            }
-        }
-        q = Q_NONE;
-
-        // -- finish samples ------------------------------------------------
-        if (st == ST_DONE) {
-            if (finite3(L)) flags |= REC_FINITE;
-            flags |= (light_bits & 255u) << REC_LIGHT_SHIFT;
-            p.rec.r0[slot] = make_float4(L.x, L.y, L.z, __uint_as_float(flags));
-            ++slot;
-            --left;
-            st = left > 0 ? ST_CAMERA : ST_NEED;
-        }
-        // -- hand out pixels ------------------------------------------------
-        uint64_t need = ballot64(st == ST_NEED);
-        while (need) {
-            if (c_next >= c_end) {
-                int base = 0;
-                if (lane == 0) base = atomicAdd(p.work_counter, TRACE_CHUNK);
-                base = __shfl(base, 0);
-                c_next = base;
-                c_end = min(base + TRACE_CHUNK, n_active);
-                if (c_next >= c_end) break;
-            }
-            const int avail = c_end - c_next;
-            const int rank = __popcll(need & ((1ull << lane) - 1ull));
-            const bool mine = (st == ST_NEED) && rank < avail;
-            if (mine) {
-                const int a = c_next + rank;
-                pix = p.plan.active[a];
-                left = p.plan.counts[pix];
-                slot = p.plan.offsets[pix];
-                dr.reset(p.tape + tape_base(a));
-                st = ST_CAMERA;
-            }
-            const int taken = min(__popcll(need), avail);
-            c_next += taken;
-            need = ballot64(st == ST_NEED);
-        }
-        if (st == ST_NEED) st = ST_DEAD;
-        if (ballot64(st != ST_DEAD) == 0) {
+        if (!path_step<kDepth, kLambert>(p, lane, env, env_on, c_next, c_end, n_active, h, st, q, pix, left, slot,
+                                         dr, ro, rd, qt, T, L, Ld, sp, sn, swo, cw, bpdf, smat, depth, flags,
+                                         light_bits)) {
             if (p.query_count) {
                 unsigned long long tot = n_queries;
                 for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
@@ -269,124 +405,6 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
             }
             break;
         }
-
-        // -- camera rays: main.cpp:691-705 ----------------------------------
-        if (st == ST_CAMERA) {
-            const int x = pix % p.width, y = pix / p.width;
-            const v2 c_pix = dr.next2();
-            const float u = (float(x) + c_pix.x) / float(p.width);
-            const float v = (float(y) + c_pix.y) / float(p.height);
-            const v2 c_lens = dr.next2();
-            v2 lens; lens.x = 0.0f; lens.y = 0.0f;
-            if (p.cam.lens_radius > 0.0f) lens = sample_disk(c_lens);
-            flags = 0;
-            light_bits = 255u;
-            if (p.capture) {
-                flags |= REC_PIXEL;
-                if (p.cam.lens_radius > 0.0f) flags |= REC_LENS;
-                p.rec.r4[slot] = make_float4(c_pix.x, c_pix.y, c_lens.x, c_lens.y);
-            }
-            v3 o = ld3(p.cam.pos);
-            if (p.cam.lens_radius > 0.0f) {
-                const float lx = p.cam.lens_radius * lens.x, ly = p.cam.lens_radius * lens.y;
-                o = o + (lx * ld3(p.cam.right) + ly * ld3(p.cam.up));
-            }
-            const v3 px = ld3(p.cam.llc) + u * ld3(p.cam.horiz) + (1.0f - v) * ld3(p.cam.vert);
-            ro = o;
-            rd = normalized3(px - o);
-            qt = inf;
-            T = mk3(1, 1, 1);
-            L = mk3(0, 0, 0);
-            q = Q_PRIMARY;
-        }
-
-        // -- light sample of a bounce: main.cpp:405-448 ------------------------
-        if (st == ST_LIGHT) {
-            const DevMaterial& m = S.materials[smat];
-            const int mkind = kLambert ? int(MAT_LAMBERT) : m.kind;
-            const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
-            Ld = mk3(0, 0, 0);
-            ro = offset_origin(sp, sn);
-            st = ST_ENV;
-            const float sel_u = dr.next();
-            int li; float sel_pdf;
-            sample_light(S, sel_u, li, sel_pdf);
-            if (p.capture && !(flags & REC_LSEL) && li >= 0 && li < 16) { flags |= REC_LSEL; light_bits = (uint32_t)li; }
-            if (li >= 0 && (!delta_brdf || light_is_delta(S.lights[li]))) {
-                const DevLight& lt = S.lights[li];
-                const bool is_delta = light_is_delta(lt);
-                const v2 lu = dr.next2();
-                if (p.capture && !(flags & REC_LIGHT) && !is_delta) {
-                    flags |= REC_LIGHT;
-                    reinterpret_cast<float2*>(&p.rec.r5[slot])[1] = make_float2(lu.x, lu.y);
-                }
-                v3 wi; float dist;
-                light_sample(lt, sp, lu, wi, dist);
-                if (dist > RPD_EPS_SMALL || is_delta) {
-                    const float pl = light_pdf(S, li, sel_pdf, wi, dist);
-                    if (pl > RPD_EPS_SMALL) {
-                        const float pw = fmaxf(RPD_EPS_SMALL, brdf_pdf(mkind, m, swo, wi, sn));
-                        const float wl = is_delta ? 1.0f : pl / (pl + pw);
-                        const v3 em = eval_light(lt, -wi, dist);
-                        if (maxc3(em) > 0.0f) {
-                            const v3 f = brdf_eval(mkind, m, swo, wi, sn);
-                            const float g = delta_brdf ? 1.0f : fmaxf(0.0f, dot3(sn, wi));
-                            const v3 contrib = (vmul(em, f) * g) / pl;
-                            cw = wl * contrib;
-                            const float eff = dist - RPD_EPS_ANYHIT;  // ray_tracer.h:65-75
-                            if (eff <= RPD_EPS_SMALL) {
-                                Ld = Ld + cw;
-                            } else if (!is_zero3(cw)) {
-                                rd = wi; qt = eff; q = Q_SHADOW_L; st = ST_LIGHT;
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        // -- environment sample: main.cpp:450-462 -------------------------------
-        if (st == ST_ENV && q == Q_NONE) {
-            const DevMaterial& m = S.materials[smat];
-            const int mkind = kLambert ? int(MAT_LAMBERT) : m.kind;
-            const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
-            st = ST_BRDF;
-            const v2 eu = dr.next2();
-            if (env_on && !delta_brdf) {
-                const v3 ed = sample_sphere(eu);
-                if (p.capture && !(flags & REC_ENV)) {
-                    flags |= REC_ENV;
-                    p.rec.r6[slot] = make_float2(eu.x, eu.y);
-                }
-                const float bp = brdf_pdf(mkind, m, swo, ed, sn);
-                const float we = pdf_env / (pdf_env + bp);
-                const v3 f = brdf_eval(mkind, m, swo, ed, sn);
-                const float c = fmaxf(0.0f, dot3(sn, ed));
-                cw = (we * vmul(env, f)) * (c / pdf_env);
-                if (!is_zero3(cw)) { rd = ed; qt = inf; q = Q_SHADOW_E; st = ST_ENV; }
-            }
-        }
-        // -- BRDF sample: main.cpp:464-479 --------------------------------------
-        if (st == ST_BRDF && q == Q_NONE) {
-            const DevMaterial& m = S.materials[smat];
-            const int mkind = kLambert ? int(MAT_LAMBERT) : m.kind;
-            const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
-            const v2 bu = dr.next2();
-            if (p.capture && !delta_brdf && !(flags & REC_BRDF)) {
-                flags |= REC_BRDF;
-                reinterpret_cast<float2*>(&p.rec.r5[slot])[0] = make_float2(bu.x, bu.y);
-            }
-            v3 dir; float pdf;
-            brdf_sample(mkind, m, swo, sn, bu, dir, pdf);
-            if (pdf > RPD_EPS_SMALL && sqnorm3(dir) > 0.0f && (!kDepth || p.max_depth <= 0 || depth < p.max_depth)) {
-                bpdf = pdf;
-                cw = brdf_sample_weight(mkind, m, swo, dir, sn, pdf);
-                rd = dir; qt = inf; q = Q_EXT;
-            } else {
-                L = L + vmul(T, Ld);
-                st = ST_DONE;   // no query: the sample is written next iteration
-            }
-        }
-
         // -- one intersection query per lane ----------------------------------
         if (q != Q_NONE) {
             h = kBvh ? bvh_query(p.bvh, ro, rd, qt, q >= Q_SHADOW_L, stack, TRACE_BLOCK)
