@@ -367,12 +367,25 @@ RPD_HD HitRec trace_query(const DevScene& S, const float4* tp, v3 o, v3 d, float
     return h;
 }
 
-// The same query through the BVH. Nodes are visited near child first; the
-// stack holds node indices at stack[k * stride]. Triangle ties go to the lower
-// index and sphere ties to the lower index, as in the index-ordered loop, and
+// Traversal stack of node indices, entry k at base[k * stride] (LDS or a
+// host array); pushes past BVH_STACK entries are dropped, which the builder's
+// depth limit rules out.
+struct StridedStack {
+    int* base;
+    int stride;
+    int sp;
+    RPD_HD bool empty() const { return sp == 0; }
+    RPD_HD void push(int v) { if (sp < BVH_STACK) base[(sp++) * stride] = v; }
+    RPD_HD int pop() { return base[(--sp) * stride]; }
+};
+
+// The same query through the BVH. Nodes are visited near child first. Triangle
+// ties go to the lower index and sphere ties to the lower index, as in the
+// index-ordered loop, so the result does not depend on the visiting order; and
 // every culling test is widened by 2^-20 so that rounding in the slab test
 // never drops a primitive the exact tests accept.
-RPD_HD HitRec bvh_query(const DevBvh& B, v3 o, v3 d, float t_max, bool occlusion, int* stack, int stride) {
+template <class Stack>
+RPD_HD HitRec bvh_query_t(const DevBvh& B, v3 o, v3 d, float t_max, bool occlusion, Stack& stk) {
     HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
     const RayPre rp = ray_pre(o, d);
     const float ix = 1.0f / (d.x != 0.0f ? d.x : copysignf(1e-30f, d.x));
@@ -382,7 +395,7 @@ RPD_HD HitRec bvh_query(const DevBvh& B, v3 o, v3 d, float t_max, bool occlusion
     float best = t_max, sbest = t_max;
     int bi = -1, si = -1;
     v3 sn = mk3(0, 0, 0);
-    int sp = 0, node = 0;
+    int node = 0;
     while (true) {
         const Bvh4Node& N = B.nodes[node];
         const float cull = fminf(best, sbest) * widen;
@@ -428,15 +441,18 @@ RPD_HD HitRec bvh_query(const DevBvh& B, v3 o, v3 d, float t_max, bool occlusion
                 const float tt = ct[b]; ct[b] = ct[b - 1]; ct[b - 1] = tt;
                 const int tc = cn[b]; cn[b] = cn[b - 1]; cn[b - 1] = tc;
             }
-        for (int a = nc - 1; a >= 1; --a)
-            if (sp < BVH_STACK) stack[(sp++) * stride] = cn[a];
+        for (int a = nc - 1; a >= 1; --a) stk.push(cn[a]);
         if (nc > 0) { node = cn[0]; continue; }
-        if (sp == 0) break;
-        node = stack[(--sp) * stride];
+        if (stk.empty()) break;
+        node = stk.pop();
     }
     if (bi >= 0 && (si < 0 || best < sbest)) { h.hit = 1; h.index = bi; h.t = best; }
     else if (si >= 0) { h.hit = 1; h.sphere = 1; h.index = si; h.t = sbest; h.sn = sn; }
     return h;
+}
+RPD_HD HitRec bvh_query(const DevBvh& B, v3 o, v3 d, float t_max, bool occlusion, int* stack, int stride) {
+    StridedStack stk{stack, stride, 0};
+    return bvh_query_t(B, o, d, t_max, occlusion, stk);
 }
 
 // ---- materials: path_tracer/core/material.h --------------------------------
