@@ -384,19 +384,36 @@ struct StridedStack {
 // index-ordered loop, so the result does not depend on the visiting order; and
 // every culling test is widened by 2^-20 so that rounding in the slab test
 // never drops a primitive the exact tests accept.
-template <class Stack>
-RPD_HD HitRec bvh_query_t(const DevBvh& B, v3 o, v3 d, float t_max, bool occlusion, Stack& stk) {
-    HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
-    const RayPre rp = ray_pre(o, d);
-    const float ix = 1.0f / (d.x != 0.0f ? d.x : copysignf(1e-30f, d.x));
-    const float iy = 1.0f / (d.y != 0.0f ? d.y : copysignf(1e-30f, d.y));
-    const float iz = 1.0f / (d.z != 0.0f ? d.z : copysignf(1e-30f, d.z));
-    const float widen = 1.0f + 0x1p-20f;
-    float best = t_max, sbest = t_max;
-    int bi = -1, si = -1;
-    v3 sn = mk3(0, 0, 0);
-    int node = 0;
-    while (true) {
+// The query in resumable form: init, then step (one node visit with the
+// primitives of its leaf children) until step reports the end, then result.
+// A kernel can interleave other work between the steps of a lane's walk.
+struct BvhWalk {
+    RayPre rp;
+    float ix, iy, iz;
+    float best, sbest;
+    int bi, si;
+    v3 sn;
+    int node;
+    bool occlusion;
+    bool any;  // occlusion: a hit ended the walk
+
+    RPD_HD void init(v3 o, v3 d, float t_max, bool occl) {
+        rp = ray_pre(o, d);
+        ix = 1.0f / (d.x != 0.0f ? d.x : copysignf(1e-30f, d.x));
+        iy = 1.0f / (d.y != 0.0f ? d.y : copysignf(1e-30f, d.y));
+        iz = 1.0f / (d.z != 0.0f ? d.z : copysignf(1e-30f, d.z));
+        best = t_max; sbest = t_max;
+        bi = -1; si = -1;
+        sn = mk3(0, 0, 0);
+        node = 0;
+        occlusion = occl;
+        any = false;
+    }
+
+    // One node visit for the ray (o, d) given to init; true once the walk has ended.
+    template <class Stack>
+    RPD_HD bool step(const DevBvh& B, v3 o, v3 d, Stack& stk) {
+        const float widen = 1.0f + 0x1p-20f;
         const Bvh4Node& N = B.nodes[node];
         const float cull = fminf(best, sbest) * widen;
         int cn[4];
@@ -432,7 +449,7 @@ RPD_HD HitRec bvh_query_t(const DevBvh& B, v3 o, v3 d, float t_max, bool occlusi
                         sbest = t; si = sidx; sn = n;
                     }
                 }
-                if (occlusion && (bi >= 0 || si >= 0)) { h.hit = 1; return h; }
+                if (occlusion && (bi >= 0 || si >= 0)) { any = true; return true; }
             }
         }
         // nearest inner child next, the others on the stack farthest first
@@ -442,13 +459,27 @@ RPD_HD HitRec bvh_query_t(const DevBvh& B, v3 o, v3 d, float t_max, bool occlusi
                 const int tc = cn[b]; cn[b] = cn[b - 1]; cn[b - 1] = tc;
             }
         for (int a = nc - 1; a >= 1; --a) stk.push(cn[a]);
-        if (nc > 0) { node = cn[0]; continue; }
-        if (stk.empty()) break;
+        if (nc > 0) { node = cn[0]; return false; }
+        if (stk.empty()) return true;
         node = stk.pop();
+        return false;
     }
-    if (bi >= 0 && (si < 0 || best < sbest)) { h.hit = 1; h.index = bi; h.t = best; }
-    else if (si >= 0) { h.hit = 1; h.sphere = 1; h.index = si; h.t = sbest; h.sn = sn; }
-    return h;
+
+    RPD_HD HitRec result() const {
+        HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
+        if (any) { h.hit = 1; return h; }
+        if (bi >= 0 && (si < 0 || best < sbest)) { h.hit = 1; h.index = bi; h.t = best; }
+        else if (si >= 0) { h.hit = 1; h.sphere = 1; h.index = si; h.t = sbest; h.sn = sn; }
+        return h;
+    }
+};
+
+template <class Stack>
+RPD_HD HitRec bvh_query_t(const DevBvh& B, v3 o, v3 d, float t_max, bool occlusion, Stack& stk) {
+    BvhWalk w;
+    w.init(o, d, t_max, occlusion);
+    while (!w.step(B, o, d, stk)) {}
+    return w.result();
 }
 RPD_HD HitRec bvh_query(const DevBvh& B, v3 o, v3 d, float t_max, bool occlusion, int* stack, int stride) {
     StridedStack stk{stack, stride, 0};
