@@ -164,7 +164,11 @@ __global__ void k_seq_classify(SeqSum s) {
     if (!serial) {
         const double lo_prev = s.approx[k - 1] * (1.0 - SEQ_MARGIN);
         const double hi_cur = s.approx[k] * (1.0 + SEQ_MARGIN);
-        if (!(lo_prev > 0.0)) {
+        if (s.approx[k - 1] == 0.0) {
+            // the terms are non-negative, so a zero prefix means every term so far is zero and
+            // the sum is exactly +0; adding another zero keeps it (a simple step of no units)
+            serial = xf != 0.0f;
+        } else if (!(lo_prev > 0.0)) {
             serial = true;                   // running sum may still be zero
         } else {
             E = binade_of(lo_prev);
@@ -403,22 +407,55 @@ __global__ void __launch_bounds__(64) k_alloc_sweeps(int n, AllocConfig ac, cons
             const float v = in_range ? imp[i] : 0.0f;
             bool cl = in_range ? a.clipped[i] != 0 : true;
             bool changed = false;
+            // Clips come in runs (about 27 per chunk at 1920x1080), and each changes the
+            // state for the next, so they are resolved a round at a time: every pixel that
+            // clips under the round's starting state is taken as clipped, one lane-ordered
+            // FP64 chain gives each lane the exact state it would see, and every lane
+            // re-tests itself under that state. Up to the first lane whose outcome differs
+            // from the guess, every outcome is exact (by induction, each lane saw the true
+            // state); that lane's own outcome is exact too, and the next round starts after it.
             while (true) {
-                const uint64_t b = __ballot(!cl && i >= pos && clips(v));
-                if (!b) break;
-                const int fl = __ffsll((long long)b) - 1;
-                const float vf = __shfl(v, fl);
-                if (lane == fl) {
+                const uint64_t guess = __ballot(!cl && i >= pos && clips(v));
+                if (!guess) break;
+                double I = rem_imp, Ib = rem_imp;   // running state; state before this lane
+                int R = remaining, Rb = remaining;
+                for (uint64_t m = guess; m; m &= m - 1) {
+                    const int k = __ffsll((long long)m) - 1;
+                    I -= double(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k)));
+                    R -= max_pp;
+                    if (lane > k) { Ib = I; Rb = R; }
+                }
+                const bool mine = (guess >> lane) & 1ull;
+                const bool clip = !cl && i >= pos && double(v) / Ib * double(Rb) > double(max_pp);
+                const uint64_t miss = __ballot(clip != mine);
+                const int fb = miss ? __ffsll((long long)miss) - 1 : 64;
+                // taken: the guessed clips before lane fb, and fb itself if it clips
+                const bool take = clip && lane <= fb;
+                const uint64_t taken = __ballot(take);
+                if (take) {
                     cl = true;
                     a.desired[i] = float(max_pp);
                     a.clipped[i] = 1;
                 }
-                remaining -= max_pp;
-                rem_imp -= double(vf);
-                --active_count;
-                clipped_any = true;
-                changed = true;
-                pos = c * ACH + fl + 1;
+                if (taken) {
+                    active_count -= __popcll(taken);
+                    clipped_any = true;
+                    changed = true;
+                }
+                if (fb == 64) {          // every guess held: the chain's end is the state
+                    rem_imp = I;
+                    remaining = R;
+                    break;
+                }
+                const int hi = __builtin_amdgcn_readlane(int(uint64_t(__double_as_longlong(Ib)) >> 32), fb);
+                const int lo = __builtin_amdgcn_readlane(int(uint32_t(uint64_t(__double_as_longlong(Ib)))), fb);
+                rem_imp = __longlong_as_double((long long)((uint64_t(uint32_t(hi)) << 32) | uint64_t(uint32_t(lo))));
+                remaining = __builtin_amdgcn_readlane(Rb, fb);
+                if ((taken >> fb) & 1ull) {
+                    rem_imp -= double(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), fb)));
+                    remaining -= max_pp;
+                }
+                pos = c * ACH + fb + 1;
             }
             if (changed) {
                 const float mn = wave_fmin(cl ? qnan : v), mx = wave_fmax(cl ? qnan : v);

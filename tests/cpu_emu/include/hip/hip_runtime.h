@@ -167,6 +167,8 @@ template <class T> inline T __shfl_xor(T v, int mask, int width = 64) {
     std::memcpy(&r, &bits, sizeof(T));
     return r;
 }
+// v of lane `lane` (a wave-uniform index, as the device instruction requires)
+inline int __builtin_amdgcn_readlane(int v, int lane) { return __shfl(v, lane); }
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 inline int __popc(unsigned v) { return __builtin_popcount(v); }
 inline int __ffsll(unsigned long long v) { return __builtin_ffsll(static_cast<long long>(v)); }
@@ -175,6 +177,8 @@ inline float __uint_as_float(unsigned v) { float f; std::memcpy(&f, &v, 4); retu
 inline unsigned __float_as_uint(float f) { unsigned v; std::memcpy(&v, &f, 4); return v; }
 inline int __float_as_int(float f) { int v; std::memcpy(&v, &f, 4); return v; }
 inline float __int_as_float(int v) { float f; std::memcpy(&f, &v, 4); return f; }
+inline long long __double_as_longlong(double d) { long long v; std::memcpy(&v, &d, 8); return v; }
+inline double __longlong_as_double(long long v) { double d; std::memcpy(&d, &v, 8); return d; }
 
 template <class T, class U> inline T atomicAdd(T* p, U v) { return __atomic_fetch_add(p, T(v), __ATOMIC_SEQ_CST); }
 template <class F> inline F atomic_add_fp(F* p, F v) {

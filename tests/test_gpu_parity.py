@@ -216,6 +216,11 @@ def _importance_fields(n, rng):
     yield "ties", ties
     # heavy tail: each sweep's clips raise the share of the rest, so clips cascade over several sweeps
     yield "pareto", (rng.pareto(0.6, n) + 1e-3).astype(np.float32)
+    # a long run of zeros first (pixels without importance), then zeros among the rest
+    lead = np.exp(rng.normal(-1.0, 1.0, n)).astype(np.float32)
+    lead[: n // 3] = 0.0
+    lead[rng.random(n) < 0.2] = 0.0
+    yield "leading_zeros", lead
 
 
 @pytest.mark.parametrize("base_samples,spp,w,h", [(0, 4, 64, 48), (1, 4, 64, 48), (0, 1, 64, 48), (3, 8, 64, 48),
