@@ -118,6 +118,15 @@ int rpd_ctx_set_memory_budget(rpd_context* ctx, uint64_t bytes);
    jobs. A device allocation that fails empties the cache and is tried again.
    0 frees and disables it. `cached` (may be NULL) receives the bytes kept. */
 int rpd_ctx_set_buffer_cache(rpd_context* ctx, uint64_t bytes, uint64_t* cached);
+/* How scenes created next build their BVH (only scenes too large for the flat
+   loop get one; the reference builds a binary median-split BVH on the host,
+   core/ray_tracer.h:49-226): RPD_BVH_HOST_SAH (default) bins a surface-area
+   heuristic on the host; RPD_BVH_DEVICE_LBVH builds a linear BVH from Morton
+   codes on the device in milliseconds, trees slower to traverse, and falls back to
+   the host build when its tree is too deep for the traversal stack. Hits do
+   not depend on the tree. */
+enum { RPD_BVH_HOST_SAH = 0, RPD_BVH_DEVICE_LBVH = 1 };
+int rpd_ctx_set_bvh_builder(rpd_context* ctx, int builder);
 
 /* ---- scenes: scenes::load_scene (core/scene_loader.h:11-13) + BVH build (main.cpp:1855) ---- */
 int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, const rpd_sphere* spheres,
@@ -317,6 +326,8 @@ int rpd_scene_query(const rpd_scene* s, int n, const float* origins, const float
                     const int* occlusion, int use_bvh, int* hit, float* t, int* index, int* is_sphere);
 /* Nodes and 4-wide depth of the scene's BVH (0, 0 for scenes looped over). */
 int rpd_scene_bvh_info(const rpd_scene* s, int* nodes, int* depth);
+/* Which builder made the scene's BVH: RPD_BVH_HOST_SAH or RPD_BVH_DEVICE_LBVH. */
+int rpd_scene_bvh_builder(const rpd_scene* scene, int* builder);
 
 /* Capture each job's adaptive-pass sequence (variance smoothing, allocation,
    sampling pass) as a hipGraph on its first pass and replay it for every

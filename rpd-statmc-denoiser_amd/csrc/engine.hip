@@ -123,6 +123,7 @@ struct rpd_context {
     std::multimap<size_t, void*> cache;
     uint64_t cache_bytes = 0;
     uint64_t cache_cap = uint64_t(32) << 30;
+    int bvh_builder = RPD_BVH_HOST_SAH;  // for the scenes created next
 };
 
 struct rpd_scene {
@@ -132,6 +133,7 @@ struct rpd_scene {
     DevBvh bvh{};          // device BVH, or null nodes for the flat loop
     int n_tris = 0, n_spheres = 0, n_materials = 0, n_lights = 0;
     int trace_per_cu[2] = {0, 0};  // resident trace blocks per CU without / with a depth limit, set on first use
+    int bvh_builder = RPD_BVH_HOST_SAH;  // which builder made the BVH (if the scene has one)
     // host copies for host-side queries (rpd_scene_query)
     std::vector<DevTriangle> h_tris;
     std::vector<DevSphere> h_spheres;
@@ -819,6 +821,56 @@ void expand_tiles(const rpd_job* j, const float* t, float* out) {
 
 }  // namespace
 
+namespace {
+
+hipError_t read_back_sync(void* dst, const void* src, size_t bytes) {
+    return sync_memcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+}
+
+// The scene's BVH built on the device from its uploaded primitives, with a
+// host copy for host-side queries. A tree too deep for the traversal stack
+// leaves pn null (its buffers freed) so that the caller can use the host build.
+int device_bvh_build(rpd_scene* sc, const DevTriangle* pt, int n_tris, const DevSphere* ps, int n_spheres,
+                     BvhHost& bvh, Bvh4Node*& pn, float4*& pslot, int*& pprim) {
+    const int n = n_tris + n_spheres;
+    int st;
+    if ((st = dalloc(&pn, size_t(n), sc->owned)) || (st = dalloc(&pslot, size_t(n) * BVH_SLOT_FLOAT4, sc->owned)) ||
+        (st = dalloc(&pprim, size_t(n), sc->owned)))
+        return st;
+    void* scratch = nullptr;
+    hipStream_t s = nullptr;
+    HIP_TRY(dev_malloc(&scratch, bvh4_device_scratch_bytes(n)));
+    if (stream_create(&s) != hipSuccess) {
+        dev_free(scratch);
+        return fail(RPD_ERR_DEVICE, "stream for the BVH build");
+    }
+    int count = 0, depth = 0;
+    const int r = build_bvh4_device(pt, n_tris, ps, n_spheres, pn, pslot, pprim, scratch, &count, &depth,
+                                    &read_back_sync, s);
+    stream_destroy(s);
+    dev_free(scratch);
+    if (r == -2) {
+        for (void* p : {static_cast<void*>(pn), static_cast<void*>(pslot), static_cast<void*>(pprim)}) {
+            sc->owned.erase(std::find(sc->owned.begin(), sc->owned.end(), p));
+            dev_free(p);
+        }
+        pn = nullptr; pslot = nullptr; pprim = nullptr;
+        return RPD_OK;
+    }
+    if (r == -1) return fail(RPD_ERR_INVALID_ARG, "scene has no primitives");
+    if (r) return fail(RPD_ERR_DEVICE, std::string("device BVH build: ") + hipGetErrorString(hipError_t(r)));
+    bvh.nodes.resize(size_t(count));
+    bvh.slots.resize(size_t(n) * BVH_SLOT_FLOAT4);
+    bvh.slot_prim.resize(size_t(n));
+    bvh.depth = depth;
+    HIP_TRY(read_back_sync(bvh.nodes.data(), pn, bvh.nodes.size() * sizeof(Bvh4Node)));
+    HIP_TRY(read_back_sync(bvh.slots.data(), pslot, bvh.slots.size() * sizeof(float4)));
+    HIP_TRY(read_back_sync(bvh.slot_prim.data(), pprim, bvh.slot_prim.size() * sizeof(int)));
+    return RPD_OK;
+}
+
+}  // namespace
+
 namespace rpd_internal {
 // Sets the calling thread's rpd_last_error: the serving loop hands a failure
 // caught on its scheduler thread to the thread that waits for the request.
@@ -887,6 +939,13 @@ int rpd_ctx_set_memory_budget(rpd_context* ctx, uint64_t bytes) {
     ctx->budget = bytes;
     return RPD_OK;
 }
+int rpd_ctx_set_bvh_builder(rpd_context* ctx, int builder) {
+    if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
+    if (builder != RPD_BVH_HOST_SAH && builder != RPD_BVH_DEVICE_LBVH) return fail(RPD_ERR_INVALID_ARG, "unknown BVH builder");
+    ctx->bvh_builder = builder;
+    return RPD_OK;
+}
+
 int rpd_ctx_set_buffer_cache(rpd_context* ctx, uint64_t bytes, uint64_t* cached) {
     if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
     {
@@ -990,7 +1049,8 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
     // scenes too large for the flat LDS loop get a BVH over triangles and spheres
     BvhHost bvh;
     const bool use_bvh = trace_needs_bvh(n_tris, n_spheres);
-    if (use_bvh) {
+    bool device_bvh = use_bvh && ctx && ctx->bvh_builder == RPD_BVH_DEVICE_LBVH;
+    if (use_bvh && !device_bvh) {
         const int bst = build_bvh4(dt.data(), n_tris, ds.data(), n_spheres, bvh);
         if (bst == -2) return fail(RPD_ERR_UNSUPPORTED, "BVH deeper than the traversal stack allows");
         if (bst) return fail(RPD_ERR_INVALID_ARG, "scene has no primitives");
@@ -1011,7 +1071,17 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
             HIP_TRY(sync_memcpy(pt, dt.data(), dt.size() * sizeof(DevTriangle), hipMemcpyHostToDevice));
             HIP_TRY(sync_memcpy(pp, perm.data(), perm.size() * sizeof(float4), hipMemcpyHostToDevice));
         }
-        if (use_bvh) {
+        if (n_spheres) HIP_TRY(sync_memcpy(ps, ds.data(), ds.size() * sizeof(DevSphere), hipMemcpyHostToDevice));
+        if (device_bvh) {
+            if ((st = device_bvh_build(sc.get(), pt, n_tris, ps, n_spheres, bvh, pn, pslot, pprim))) return st;
+            if (!pn) {  // deeper than the traversal stack allows: the host's SAH tree instead
+                device_bvh = false;
+                const int bst = build_bvh4(dt.data(), n_tris, ds.data(), n_spheres, bvh);
+                if (bst == -2) return fail(RPD_ERR_UNSUPPORTED, "BVH deeper than the traversal stack allows");
+                if (bst) return fail(RPD_ERR_INVALID_ARG, "scene has no primitives");
+            }
+        }
+        if (use_bvh && !device_bvh) {
             if ((st = dalloc(&pn, bvh.nodes.size(), sc->owned)) || (st = dalloc(&pslot, bvh.slots.size(), sc->owned)) ||
                 (st = dalloc(&pprim, bvh.slot_prim.size(), sc->owned))) {
                 for (void* p : sc->owned) dev_free(p);
@@ -1021,7 +1091,6 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
             HIP_TRY(sync_memcpy(pslot, bvh.slots.data(), bvh.slots.size() * sizeof(float4), hipMemcpyHostToDevice));
             HIP_TRY(sync_memcpy(pprim, bvh.slot_prim.data(), bvh.slot_prim.size() * sizeof(int), hipMemcpyHostToDevice));
         }
-        if (n_spheres) HIP_TRY(sync_memcpy(ps, ds.data(), ds.size() * sizeof(DevSphere), hipMemcpyHostToDevice));
         HIP_TRY(sync_memcpy(pm, dm.data(), dm.size() * sizeof(DevMaterial), hipMemcpyHostToDevice));
         if (n_lights) {
             HIP_TRY(sync_memcpy(pl, dl.data(), dl.size() * sizeof(DevLight), hipMemcpyHostToDevice));
@@ -1030,6 +1099,7 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
     }
     sc->dev.tris = pt; sc->dev.n_tris = n_tris; sc->dev.tri_perm = pp;
     sc->bvh.nodes = pn; sc->bvh.slots = pslot; sc->bvh.slot_prim = pprim;
+    sc->bvh_builder = device_bvh ? RPD_BVH_DEVICE_LBVH : RPD_BVH_HOST_SAH;
     sc->h_tris = dt; sc->h_spheres = ds; sc->h_perm = std::move(perm); sc->h_bvh = std::move(bvh);
     sc->dev.spheres = ps; sc->dev.n_spheres = n_spheres;
     sc->dev.materials = pm;
@@ -1107,6 +1177,12 @@ int rpd_scene_query(const rpd_scene* s, int n, const float* origins, const float
                                  : trace_query(hs, hs.tri_perm, o, d, t_max[i], occlusion[i] != 0);
         hit[i] = h.hit; t[i] = h.t; index[i] = h.index; is_sphere[i] = h.sphere;
     }
+    return RPD_OK;
+}
+
+int rpd_scene_bvh_builder(const rpd_scene* s, int* builder) {
+    if (!s || !builder) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    *builder = s->bvh_builder;
     return RPD_OK;
 }
 

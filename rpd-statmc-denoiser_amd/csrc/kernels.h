@@ -183,6 +183,11 @@ __device__ inline int64_t batch_take(const Batch& b, int kind, int& cursor) {
     return -1;
 }
 
+// stable radix sort of (key, value) pairs on the low key_bits bits (sort_keys.hip)
+size_t sort_pairs_u32_temp_bytes(int n);
+hipError_t sort_pairs_u32(const uint32_t* keys_in, uint32_t* keys_out, const int* vals_in, int* vals_out, int n,
+                          int key_bits, void* temp, size_t temp_bytes, hipStream_t st);
+
 // ---- launch wrappers (each enqueues on `st`) ----
 void launch_tape(const TraceParams& p, hipStream_t st);
 void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st);

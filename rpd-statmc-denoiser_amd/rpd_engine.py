@@ -188,6 +188,7 @@ def lib():
         for name, args in [
             ("rpd_ctx_create", [C.c_int, C.POINTER(vp)]), ("rpd_ctx_destroy", [vp]),
             ("rpd_ctx_set_memory_budget", [vp, C.c_uint64]),
+            ("rpd_ctx_set_bvh_builder", [vp, C.c_int]),
             ("rpd_ctx_set_buffer_cache", [vp, C.c_uint64, C.POINTER(C.c_uint64)]),
             ("rpd_scene_create", [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.POINTER(vp)]),
             ("rpd_scene_builtin", [vp, C.c_char_p, C.POINTER(vp)]),
@@ -222,6 +223,7 @@ def lib():
             ("rpd_scene_procedural", [vp, C.POINTER(ProceduralConfig), C.POINTER(vp)]),
             ("rpd_scene_query", [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]),
             ("rpd_scene_bvh_info", [vp, vp, vp]),
+            ("rpd_scene_bvh_builder", [vp, vp]),
             ("rpd_job_bytes", [C.POINTER(RenderConfig), C.POINTER(C.c_uint64)]),
             ("rpd_ctx_memory", [vp, vp, vp, vp, vp]),
             ("rpd_server_create", [vp, C.POINTER(ServerConfig), C.POINTER(vp)]),
@@ -301,6 +303,22 @@ class Context:
 
     def set_memory_budget(self, nbytes):
         check(self.L.rpd_ctx_set_memory_budget(self.h, int(nbytes)))
+
+    def set_bvh_builder(self, builder):
+        """RPD_BVH_HOST_SAH or RPD_BVH_DEVICE_LBVH for the scenes created next."""
+        check(self.L.rpd_ctx_set_bvh_builder(self.h, int(builder)))
+
+    def scene_query(self, name, origins, dirs, t_max, occlusion, use_bvh):
+        """Host-side queries of scene `name` (its BVH as built, or the exhaustive loop)."""
+        return _scene_query(self.L, self.scene(name), origins, dirs, t_max, occlusion, use_bvh)
+
+    def bvh_info(self, name):
+        return _bvh_info(self.L, self.scene(name))
+
+    def bvh_builder(self, name):
+        b = C.c_int()
+        check(self.L.rpd_scene_bvh_builder(self.scene(name), C.byref(b)))
+        return b.value
 
     def set_buffer_cache(self, nbytes):
         """Bytes of released jobs' device buffers kept for reuse (0 disables); returns the bytes now kept."""
@@ -415,6 +433,28 @@ def procedural_arrays(pc):
     return tris, sph, mats, lights, env, [x.value for x in n]
 
 
+RPD_BVH_HOST_SAH, RPD_BVH_DEVICE_LBVH = 0, 1
+
+
+def _scene_query(L, h, origins, dirs, t_max, occlusion, use_bvh):
+    o = np.ascontiguousarray(origins, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32)
+    tm = np.ascontiguousarray(t_max, np.float32)
+    oc = np.ascontiguousarray(occlusion, np.int32)
+    n = tm.size
+    hit, idx, sph = (np.zeros(n, np.int32) for _ in range(3))
+    t = np.zeros(n, np.float32)
+    check(L.rpd_scene_query(h, n, _ptr(o), _ptr(d), _ptr(tm), _ptr(oc), int(use_bvh), _ptr(hit), _ptr(t), _ptr(idx),
+                            _ptr(sph)))
+    return hit, t, idx, sph
+
+
+def _bvh_info(L, h):
+    a, b = C.c_int(), C.c_int()
+    check(L.rpd_scene_bvh_info(h, C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
 class HostScene:
     """A scene kept on the host (no device) for rpd_scene_query checks."""
 
@@ -425,21 +465,10 @@ class HostScene:
         self.h = h
 
     def query(self, origins, dirs, t_max, occlusion, use_bvh):
-        o = np.ascontiguousarray(origins, np.float32)
-        d = np.ascontiguousarray(dirs, np.float32)
-        tm = np.ascontiguousarray(t_max, np.float32)
-        oc = np.ascontiguousarray(occlusion, np.int32)
-        n = tm.size
-        hit, idx, sph = (np.zeros(n, np.int32) for _ in range(3))
-        t = np.zeros(n, np.float32)
-        check(self.L.rpd_scene_query(self.h, n, _ptr(o), _ptr(d), _ptr(tm), _ptr(oc), int(use_bvh), _ptr(hit),
-                                     _ptr(t), _ptr(idx), _ptr(sph)))
-        return hit, t, idx, sph
+        return _scene_query(self.L, self.h, origins, dirs, t_max, occlusion, use_bvh)
 
     def bvh_info(self):
-        a, b = C.c_int(), C.c_int()
-        check(self.L.rpd_scene_bvh_info(self.h, C.byref(a), C.byref(b)))
-        return a.value, b.value
+        return _bvh_info(self.L, self.h)
 
     def close(self):
         if self.h:

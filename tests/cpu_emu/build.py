@@ -35,6 +35,8 @@ def prepare():
     shutil.copy(os.path.join(REPO, "include", "rpd_engine.h"), os.path.join(OUT, "include", "rpd_engine.h"))
     units = []
     for name in sorted(os.listdir(SRC)):
+        if name in REPLACED:
+            continue
         text = open(os.path.join(SRC, name)).read()
         text = DYN_LDS.sub(r"\1* \2 = static_cast<\1*>(::rpd_emu::dynamic_lds());", text)
         if name.endswith(".hip"):
@@ -43,6 +45,11 @@ def prepare():
         with open(os.path.join(src_dir, name), "w") as f:
             f.write(text)
     return units
+
+
+# engine sources that use device libraries the emulation has no g++ build of,
+# each replaced by a CPU file of the same interface
+REPLACED = {"sort_keys.hip": "sort_keys_emu.cpp"}
 
 
 def compile_unit(cpp):
@@ -58,6 +65,7 @@ def compile_unit(cpp):
 def build():
     units = prepare()
     units.append(os.path.join(HERE, "emu_runtime.cpp"))
+    units += [os.path.join(HERE, f) for f in REPLACED.values()]
     with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(compile_unit, units))
     subprocess.run(["g++", "-shared", "-fsanitize=address", "-fsanitize=bounds", "-pthread", "-o", LIB, *objs, "-ldl"], check=True)

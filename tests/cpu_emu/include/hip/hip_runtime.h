@@ -50,6 +50,7 @@ inline float2 make_float2(float a, float b) { return {a, b}; }
 inline float3 make_float3(float a, float b, float c) { return {a, b, c}; }
 inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
 inline int4 make_int4(int a, int b, int c, int d) { return {a, b, c, d}; }
+inline int2 make_int2(int a, int b) { return {a, b}; }
 inline uint4 make_uint4(unsigned a, unsigned b, unsigned c, unsigned d) { return {a, b, c, d}; }
 inline double2 make_double2(double a, double b) { return {a, b}; }
 
@@ -170,6 +171,10 @@ template <class T> inline T __shfl_xor(T v, int mask, int width = 64) {
 // v of lane `lane` (a wave-uniform index, as the device instruction requires)
 inline int __builtin_amdgcn_readlane(int v, int lane) { return __shfl(v, lane); }
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
+inline int __clz(unsigned v) { return v ? __builtin_clz(v) : 32; }
+inline int __clz(int v) { return __clz(unsigned(v)); }
+// fibers of one process share memory; atomics are sequentially consistent
+inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 inline int __popc(unsigned v) { return __builtin_popcount(v); }
 inline int __ffsll(unsigned long long v) { return __builtin_ffsll(static_cast<long long>(v)); }
 inline int __ffsll(long long v) { return __builtin_ffsll(v); }
