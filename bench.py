@@ -65,6 +65,16 @@ def trace_traffic_bytes():
     return None
 
 
+def check_sample_counter(job):
+    """Timed steps count samples with the job's device counter (every traced path sample), which
+    equals the per-pixel counts of finite samples unless some samples were non-finite. Checked
+    once on a warm-up job, outside the timing."""
+    finite = int(job.get("sample_count").sum())
+    traced = job.counters()["samples"]
+    if not finite <= traced <= finite * 1.001:
+        print(f"note: {traced} traced samples against {finite} finite ones", file=sys.stderr)
+
+
 def calibrate_copy(nbytes=1 << 30):
     """A plain device copy of known size, for scaling the FETCH_SIZE and
     WRITE_SIZE counters to bytes (run under rocprofv3 --pmc)."""
@@ -119,6 +129,7 @@ def run_c2(args):
     for _ in range(args.warmup):
         j = eng.Job(ctx, cfg, cam, C2["seed"], scene="c2", submit=True)
         j.wait()
+        check_sample_counter(j)
         j.release()
     samples = queries = 0
     stage = np.zeros(11)
@@ -126,9 +137,10 @@ def run_c2(args):
     for _ in range(args.steps):
         j = eng.Job(ctx, cfg, cam, C2["seed"], scene="c2", submit=True)
         j.wait()
-        samples += int(j.get("sample_count").sum())
         stage += j.stage_ms()
-        queries += j.counters()["queries"]
+        cnt = j.counters()
+        samples += cnt["samples"]
+        queries += cnt["queries"]
         j.release()
     wall = time.perf_counter() - t0
     n_px = C2["width"] * C2["height"]
@@ -380,6 +392,7 @@ def main():
     for _ in range(args.warmup):
         j = eng.Job(ctx, cfg, cam, WORKLOAD["seed"], submit=True)
         j.wait()
+        check_sample_counter(j)
         j.release()
     samples = 0
     queries = 0
@@ -388,9 +401,9 @@ def main():
     for k in range(args.steps):
         j = eng.Job(ctx, cfg, cam, WORKLOAD["seed"], submit=True)
         j.wait()
-        samples += int(j.get("sample_count").sum())
         stage += j.stage_ms()
         cnt = j.counters()
+        samples += cnt["samples"]
         queries += cnt["queries"]
         j.release()
     wall = time.perf_counter() - t0

@@ -126,7 +126,11 @@ __global__ void k_filter_pack(Batch b) {
 // neighbour exactly as the reference: Welch t^2 per channel against the
 // t critical value at Welch-Satterthwaite df, the maximum over channels,
 // RPD relaxation, hard rejection above 25, weight exp(-q/2)/(1+d^2).
-__device__ void statmc_pixel(const ImageDesc& d, const int x, const int y) {
+// 1 / (1 + d^2) for integer squared offsets d^2 <= STATMC_SP_MAX (window radius up to 24),
+// formed once per block; larger windows compute it in the loop
+constexpr int STATMC_SP_MAX = 2 * 24 * 24;
+
+__device__ void statmc_pixel(const ImageDesc& d, const int x, const int y, const double* sp_tab) {
     const int W = d.W, H = d.H, n = d.N;
     const FilterConfig fc = d.fc;
     const float4* __restrict__ P = reinterpret_cast<const float4*>(d.filter_scratch + 4 * size_t(n));
@@ -168,6 +172,15 @@ __device__ void statmc_pixel(const ImageDesc& d, const int x, const int y) {
     vtot[i] = var_i;
     veff[i] = vm_i;
     vrat[i] = (var_i > RPD_EPS_SMALL) ? (vm_i * float(n_i)) / var_i : 0.0f;
+    // this pixel's term of each channel's Welch-Satterthwaite denominator, formed once
+    double df_i[3];
+    {
+        const float vsi0[3] = {p3.x, p3.y, p3.z};
+        for (int ch = 0; ch < 3; ++ch) {
+            const double vi = double(vsi0[ch]);
+            df_i[ch] = (vi * vi) / fmax(double(n_i - 1), 1.0);
+        }
+    }
 
     double ws = 0, wss = 0, wvs = 0, c0 = 0, c1 = 0, c2 = 0, accepted = 0;
     const int R = fc.radius;
@@ -195,7 +208,7 @@ __device__ void statmc_pixel(const ImageDesc& d, const int x, const int y) {
                 const double vs = vi + vj;
                 const double md = double(smi[ch] - smj[ch]);
                 const double t2 = md * md / fmax(double(RPD_EPS_SMALL), vs);
-                const double den_df = (vi * vi) / fmax(double(n_i - 1), 1.0) + (vj * vj) / fmax(double(n_j - 1), 1.0);
+                const double den_df = df_i[ch] + (vj * vj) / fmax(double(n_j - 1), 1.0);
                 const double df = (den_df > double(RPD_EPS_SMALL)) ? (vs * vs) / den_df : double(n_i + n_j - 2);
                 const double tc = double(t_critical(float(df), table, z));
                 mx = fmax(mx, t2 / (tc * tc));
@@ -209,7 +222,8 @@ __device__ void statmc_pixel(const ImageDesc& d, const int x, const int y) {
             const float adj = clampf(fc.rpd_scale * shared, 0.0f, 1.0f);
             const double qv = mx / (1.0 + double(adj));
             if (qv > 25.0) continue;
-            const double sp = 1.0 / (1.0 + double(dx * dx + dy * dy));
+            const int d2 = dx * dx + dy * dy;
+            const double sp = d2 <= STATMC_SP_MAX ? sp_tab[d2] : 1.0 / (1.0 + double(d2));
             const double wj = sp * exp(-0.5 * qv);
             const float4 q4 = P[4 * n + q];
             c0 += double(q4.x) * wj; c1 += double(q4.y) * wj; c2 += double(q4.z) * wj;
@@ -241,6 +255,8 @@ __device__ void statmc_pixel(const ImageDesc& d, const int x, const int y) {
 // the XCD-partitioned work ranges; one thread per pixel of the tile.
 __global__ void __launch_bounds__(256) k_statmc(Batch b) {
     __shared__ int64_t s_item;
+    __shared__ double s_sp[STATMC_SP_MAX + 1];
+    for (int k = threadIdx.x; k <= STATMC_SP_MAX; k += blockDim.x) s_sp[k] = 1.0 / (1.0 + double(k));
     int cursor = 0;
     const int tx = threadIdx.x & (STATMC_TILE - 1), ty = threadIdx.x / STATMC_TILE;
     while (true) {
@@ -253,7 +269,7 @@ __global__ void __launch_bounds__(256) k_statmc(Batch b) {
         const ImageDesc& d = b.img[batch_find(b, BK_TILE, item, lt)];
         const int tiles_x = (d.W + STATMC_TILE - 1) / STATMC_TILE;
         const int x = int(lt % tiles_x) * STATMC_TILE + tx, y = int(lt / tiles_x) * STATMC_TILE + ty;
-        if (x < d.W && y < d.H) statmc_pixel(d, x, y);
+        if (x < d.W && y < d.H) statmc_pixel(d, x, y, s_sp);
     }
 }
 
