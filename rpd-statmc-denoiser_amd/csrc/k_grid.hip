@@ -58,11 +58,17 @@ void launch_batch_queues_reset(const Batch& b, hipStream_t st) {
 // atomics. The pass totals are then added to the node.
 //
 // Threads 0-207 are the bin sums (accumulator tid / 16, bin tid % 16) and
-// 208-233 the totals, so three of the four waves run only the bin loop. A
-// contribution that does not belong to a thread's sum adds -0.0, which leaves
-// every FP64 value unchanged (signed zeros and NaN included), so the loops
-// need no branch. The support list is built by all threads at once (weights,
-// counts, then an ordered block scan).
+// 208-233 the totals. Every thread runs the same loop: it reads its own pair
+// of terms ((w, w*y) for a bin, (w, w*w) or (w*y, w*y*y) for the totals),
+// tests the contribution's bin against its range, and adds with
+// fma(m, term, sum), m = 1 for a member (fma(1, x, s) rounds x + s once,
+// exactly the addition) and m = -0 otherwise (a sum that starts at +0 never
+// becomes -0 under round-to-nearest, so adding the zero -0 * term leaves it
+// unchanged). That needs finite terms, and they are: weights lie in [0, 1],
+// and the luminance of finite channels is finite, because its float weights
+// sum to exactly 1 and its unfused evaluation is monotone in each channel, so
+// |y| <= luminance3(FLT_MAX, FLT_MAX, FLT_MAX) = FLT_MAX. The support list is
+// built by all threads at once (weights, counts, then an ordered block scan).
 constexpr int GATHER_BLOCK = 256;
 constexpr int GATHER_MAX_PIX = 1024;  // support window of up to (2*step)^2 pixels, step <= 16
 
@@ -93,8 +99,7 @@ struct GatherLds {
     float w[GATHER_MAX_PIX];
     int pre[GATHER_MAX_PIX + 1];
     int scan[GATHER_BLOCK];
-    double2 wy[GATHER_BLOCK];    // (w, w*y) of each contribution of the chunk
-    double2 sq[GATHER_BLOCK];    // (w*w, w*y*y)
+    double2 pr[3][GATHER_BLOCK];  // per contribution: (w, w*y), (w, w*w), (w*y, w*y*y)
     uint32_t bin4[GRID_ACC][GATHER_BLOCK / 4];
     int64_t item;
 };
@@ -211,33 +216,25 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds& L) {
         {
             const double wd = double(wc), yd = double(y);
             const double wyd = wd * yd;
-            L.wy[tid] = make_double2(wd, wyd);
-            L.sq[tid] = make_double2(wd * wd, wyd * yd);
+            L.pr[0][tid] = make_double2(wd, wyd);
+            L.pr[1][tid] = make_double2(wd, wd * wd);
+            L.pr[2][tid] = make_double2(wyd, wyd * yd);
         }
         for (int k = 0; k < GRID_ACC; ++k) c_bin[k * GATHER_BLOCK + tid] = bins[k];
         __syncthreads();
         const int nquad = (min(GATHER_BLOCK, total - base) + 3) >> 2;
-        if (bin_t) {
+        if (tid < GATHER_SUM_THREADS) {
+            const double2* src = L.pr[bin_t ? 0 : bsel - 15];
+            const unsigned lo = bin_t ? unsigned(bsel) : 0u, span = bin_t ? 0u : 15u;
             for (int i4 = 0; i4 < nquad; ++i4) {
                 const uint32_t b4 = L.bin4[acc][i4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const bool mine = int(int8_t(b4 >> (8 * e))) == bsel;
-                    const double2 v = L.wy[4 * i4 + e];
-                    a0 += mine ? v.x : -0.0;
-                    a1 += mine ? v.y : -0.0;
-                }
-            }
-        } else if (tid < GATHER_SUM_THREADS) {
-            const bool second = bsel == 17;
-            for (int i4 = 0; i4 < nquad; ++i4) {
-                const uint32_t b4 = L.bin4[acc][i4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const bool in = int(int8_t(b4 >> (8 * e))) >= 0;
-                    const double2 v = L.wy[4 * i4 + e], q = L.sq[4 * i4 + e];
-                    a0 += in ? (second ? v.y : v.x) : -0.0;
-                    a1 += in ? (second ? q.y : q.x) : -0.0;
+                    const unsigned bin = unsigned(int(int8_t(b4 >> (8 * e))));  // -1: none
+                    const double m = bin - lo <= span ? 1.0 : -0.0;
+                    const double2 v = src[4 * i4 + e];
+                    a0 = fma(m, v.x, a0);
+                    a1 = fma(m, v.y, a1);
                 }
             }
         }

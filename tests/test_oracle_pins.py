@@ -363,3 +363,27 @@ def test_record_fold_equals_traced_fold():
     a.recompute_sensitivity(); a.statmc()
     b.recompute_sensitivity(); b.statmc()
     assert np.array_equal(a.get("denoised"), b.get("denoised"))
+
+
+def test_luminance_of_finite_channels_is_finite():
+    """The grid gather adds its FP64 terms with fma(m, term, sum), m in {1, -0},
+    which needs every term finite. Terms come from finite records, weights in
+    [0, 1] and the float luminance 0.2126f r + 0.7152f g + 0.0722f b evaluated
+    unfused left to right. Its float weights sum to exactly 1 and each rounded
+    step is monotone in each channel, so the extremes are the all-FLT_MAX
+    channels; those evaluate to FLT_MAX, not infinity."""
+    from fractions import Fraction
+    c = np.array([0.2126, 0.7152, 0.0722], np.float32)
+    assert sum(Fraction(float(x)) for x in c) == 1
+    m = np.finfo(np.float32).max
+    for s in (1.0, -1.0):
+        v = np.full(3, s * m, np.float32)
+        with np.errstate(over="raise"):
+            lum = (c[0] * v[0] + c[1] * v[1]) + c[2] * v[2]
+        assert lum == np.float32(s) * m
+    rng = np.random.default_rng(2)
+    bits = (m.view(np.uint32) - rng.integers(0, 1 << 23, (200000, 3))).astype(np.uint32)
+    v = bits.view(np.float32) * rng.choice(np.array([-1.0, 1.0], np.float32), (200000, 3))
+    with np.errstate(over="raise"):
+        lum = (c[0] * v[:, 0] + c[1] * v[:, 1]) + c[2] * v[:, 2]
+    assert np.isfinite(lum).all()
