@@ -60,7 +60,9 @@ void launch_batch_queues_reset(const Batch& b, hipStream_t st) {
 // Threads 0-207 are the bin sums (accumulator tid / 16, bin tid % 16) and
 // 208-233 the totals. Every thread runs the same loop: it reads its own pair
 // of terms ((w, w*y) for a bin, (w, w*w) or (w*y, w*y*y) for the totals),
-// tests the contribution's bin against its range, and adds with
+// compares the contribution's bin with its key (its own bin, or -1, "no
+// bin", for the totals) and picks m from two per-thread values by the result
+// (bins: equal gives 1; totals: equal gives -0), and adds with
 // fma(m, term, sum), m = 1 for a member (fma(1, x, s) rounds x + s once,
 // exactly the addition) and m = -0 otherwise (a sum that starts at +0 never
 // becomes -0 under round-to-nearest, so adding the zero -0 * term leaves it
@@ -94,13 +96,18 @@ __device__ __forceinline__ int gather_scan(int v, int* sh) {
     return r;
 }
 
+constexpr int GATHER_BIN_ROW = GATHER_BLOCK / 4 + 1;  // dwords per accumulator row of bin bytes
+
 struct GatherLds {
     int pix[GATHER_MAX_PIX];
     float w[GATHER_MAX_PIX];
     int pre[GATHER_MAX_PIX + 1];
     int scan[GATHER_BLOCK];
-    double2 pr[3][GATHER_BLOCK];  // per contribution: (w, w*y), (w, w*w), (w*y, w*y*y)
-    uint32_t bin4[GRID_ACC][GATHER_BLOCK / 4];
+    // Rows are padded by one element so that the rows one wave reads at the
+    // same index (several accumulators' bins, the three pair arrays) start in
+    // different LDS banks.
+    double2 pr[3][GATHER_BLOCK + 1];  // per contribution: (w, w*y), (w, w*w), (w*y, w*y*y)
+    uint32_t bin4[GRID_ACC][GATHER_BIN_ROW];
     int64_t item;
 };
 constexpr int GATHER_BIN_THREADS = GRID_ACC * 16;                      // 208
@@ -220,18 +227,19 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds& L) {
             L.pr[1][tid] = make_double2(wd, wd * wd);
             L.pr[2][tid] = make_double2(wyd, wyd * yd);
         }
-        for (int k = 0; k < GRID_ACC; ++k) c_bin[k * GATHER_BLOCK + tid] = bins[k];
+        for (int k = 0; k < GRID_ACC; ++k) c_bin[k * 4 * GATHER_BIN_ROW + tid] = bins[k];
         __syncthreads();
         const int nquad = (min(GATHER_BLOCK, total - base) + 3) >> 2;
         if (tid < GATHER_SUM_THREADS) {
             const double2* src = L.pr[bin_t ? 0 : bsel - 15];
-            const unsigned lo = bin_t ? unsigned(bsel) : 0u, span = bin_t ? 0u : 15u;
+            const int key = bin_t ? bsel : -1;
+            const double m_eq = bin_t ? 1.0 : -0.0, m_ne = bin_t ? -0.0 : 1.0;
             for (int i4 = 0; i4 < nquad; ++i4) {
                 const uint32_t b4 = L.bin4[acc][i4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const unsigned bin = unsigned(int(int8_t(b4 >> (8 * e))));  // -1: none
-                    const double m = bin - lo <= span ? 1.0 : -0.0;
+                    const int bin = int(int8_t(b4 >> (8 * e)));  // -1: none
+                    const double m = bin == key ? m_eq : m_ne;
                     const double2 v = src[4 * i4 + e];
                     a0 = fma(m, v.x, a0);
                     a1 = fma(m, v.y, a1);
