@@ -660,7 +660,7 @@ int job_alloc(rpd_job* j) {
     A(j->grid, nn * GRID_NODE_DOUBLES);
     A(j->tiles, nn * SF_COUNT);
     A(j->node_scratch, nn * 33);
-    A(j->filter_scratch, 32 * N);
+    A(j->filter_scratch, FILTER_SCRATCH_FLOATS * N);
     char* as; A(as, alloc_scratch_bytes(j->N));
     j->alloc_scratch = as;
     ImageDesc* dd; A(dd, 1);
@@ -679,7 +679,7 @@ size_t job_bytes(const rpd_render_config& c, int max_spp = 0) {
                       size_t(std::max(2, (c.image_height + step - 1) / step + 1));
     const size_t per_px = max_spp > 0 ? size_t(max_spp) : size_t(std::max(c.samples_per_pixel, c.adaptive_spp + 1));
     const size_t tape = max_spp > 0 ? 0 : ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_N * 4;
-    return N * 4 * (13 + 2 + 12 + 7 + 14 + 13 + 1 + 3 + 3 + 32) + tape + per_px * N * (6 * 16 + 8) +
+    return N * 4 * (13 + 2 + 12 + 7 + 14 + 13 + 1 + 3 + 3 + FILTER_SCRATCH_FLOATS) + tape + per_px * N * (6 * 16 + 8) +
            nn * (GRID_NODE_DOUBLES * 8 + SF_COUNT * 4 + 33 * 4) + alloc_scratch_bytes(int(N)) + sizeof(ImageDesc) + 512;
 }
 

@@ -49,7 +49,14 @@ __device__ __forceinline__ float t_critical(float df, int table, float z) {
 //   P3 = (stabilised sqrt variance of the mean rgb, variance of the mean)
 //   P4 = (colour mean rgb, 0)
 //   P5 = (reliable components 0..3), P6 = (reliable components 4..5, 0, 0)
+// followed by 2 double2 planes D0 = (s0, s1), D1 = (s2, 0): the pixel's term
+// s_ch = v_ch^2 / max(n - 1, 1) of each channel's Welch-Satterthwaite
+// denominator (v = P3 channel, n = P1.w), formed once per pixel instead of once
+// per (pixel, neighbour) pair; the same FP64 operations give the same value.
 constexpr int FILTER_PLANES = 7;
+__device__ __forceinline__ double2* filter_ws_planes(const ImageDesc& d) {
+    return reinterpret_cast<double2*>(d.filter_scratch + (4 + 4 * FILTER_PLANES) * size_t(d.N));
+}
 
 // sqrt-domain variance of the mean, floored (main.cpp:977-980)
 __global__ void k_sqrt_var_mean(Batch b) {
@@ -108,6 +115,16 @@ __global__ void k_filter_pack(Batch b) {
     P[1 * n + i] = make_float4(f.albedo[i], f.albedo[n + i], f.albedo[2 * n + i], float(ni));
     P[2 * n + i] = make_float4(pb.sqrt_mean[i], pb.sqrt_mean[n + i], pb.sqrt_mean[2 * n + i], var);
     P[3 * n + i] = make_float4(svs.x, svs.y, svs.z, vmean);
+    {
+        double ws3[3];
+        for (int ch = 0; ch < 3; ++ch) {
+            const double v = double(ch == 0 ? svs.x : ch == 1 ? svs.y : svs.z);
+            ws3[ch] = (v * v) / fmax(double(ni - 1), 1.0);
+        }
+        double2* D = filter_ws_planes(d);
+        D[i] = make_double2(ws3[0], ws3[1]);
+        D[n + i] = make_double2(ws3[2], 0.0);
+    }
     P[4 * n + i] = make_float4(pb.color_mean[i], pb.color_mean[n + i], pb.color_mean[2 * n + i], 0.0f);
     // main.cpp:313-327, 305-310: six reliable components attenuated by the gradient
     const int nn = g.nx * g.ny;
@@ -172,14 +189,12 @@ __device__ void statmc_pixel(const ImageDesc& d, const int x, const int y, const
     vtot[i] = var_i;
     veff[i] = vm_i;
     vrat[i] = (var_i > RPD_EPS_SMALL) ? (vm_i * float(n_i)) / var_i : 0.0f;
-    // this pixel's term of each channel's Welch-Satterthwaite denominator, formed once
+    // each pixel's term of each channel's Welch-Satterthwaite denominator
+    const double2* __restrict__ D = filter_ws_planes(d);
     double df_i[3];
     {
-        const float vsi0[3] = {p3.x, p3.y, p3.z};
-        for (int ch = 0; ch < 3; ++ch) {
-            const double vi = double(vsi0[ch]);
-            df_i[ch] = (vi * vi) / fmax(double(n_i - 1), 1.0);
-        }
+        const double2 s01 = D[i], s2 = D[n + i];
+        df_i[0] = s01.x; df_i[1] = s01.y; df_i[2] = s2.x;
     }
 
     double ws = 0, wss = 0, wvs = 0, c0 = 0, c1 = 0, c2 = 0, accepted = 0;
@@ -199,6 +214,8 @@ __device__ void statmc_pixel(const ImageDesc& d, const int x, const int y, const
             if (norm3(ai - mk3(q1.x, q1.y, q1.z)) > 0.12f) continue;
             const float4 q2 = P[2 * n + q], q3 = P[3 * n + q];
             const int n_j = int(q1.w);
+            const double2 sj01 = D[q], sj2 = D[n + q];
+            const double df_j[3] = {sj01.x, sj01.y, sj2.x};
             double mx = 0.0;
             const float smi[3] = {p2.x, p2.y, p2.z}, smj[3] = {q2.x, q2.y, q2.z};
             const float vsi[3] = {p3.x, p3.y, p3.z}, vsj[3] = {q3.x, q3.y, q3.z};
@@ -208,7 +225,7 @@ __device__ void statmc_pixel(const ImageDesc& d, const int x, const int y, const
                 const double vs = vi + vj;
                 const double md = double(smi[ch] - smj[ch]);
                 const double t2 = md * md / fmax(double(RPD_EPS_SMALL), vs);
-                const double den_df = df_i[ch] + (vj * vj) / fmax(double(n_j - 1), 1.0);
+                const double den_df = df_i[ch] + df_j[ch];
                 const double df = (den_df > double(RPD_EPS_SMALL)) ? (vs * vs) / den_df : double(n_i + n_j - 2);
                 const double tc = double(t_critical(float(df), table, z));
                 mx = fmax(mx, t2 / (tc * tc));

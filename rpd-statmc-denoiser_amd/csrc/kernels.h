@@ -93,6 +93,10 @@ struct Features {            // path_tracer/main.cpp:94-122
 //   pixel, brdf, screen_brdf, lens, screen_lens, light_uv, screen_light_uv,
 //   environment, screen_environment, light_select(16 bins),
 //   screen_light_select, rr(2 bins), screen_rr
+// Per-pixel filter scratch: 4 floats of sqrt-domain variance of the mean, 7
+// float4 planes of packed filter inputs, 2 double2 planes of Welch-Satterthwaite
+// terms.
+constexpr int FILTER_SCRATCH_FLOATS = 4 + 7 * 4 + 2 * 4;
 constexpr int GRID_ACC = 13;
 constexpr int GRID_NODE_DOUBLES = 440;
 __host__ __device__ constexpr int grid_acc_bins(int k) { return k == 11 ? 2 : 16; }
@@ -134,7 +138,8 @@ struct ImageDesc {
     float* tiles;            // SF_COUNT planes over nodes
     float* conf;             // per-pixel sensitivity confidence
     float* grad;             // per-pixel sensitivity gradient
-    float* filter_scratch;   // 32 floats per pixel: 3 planes of variance of the mean + 7 float4 planes
+    float* filter_scratch;   // FILTER_SCRATCH_FLOATS per pixel: variance of the mean (3 planes + 1
+                             // spare), 7 float4 planes, 2 double2 planes
     float *den, *unc, *alp, *vtot, *veff, *vrat;
     FilterConfig fc;
     float conf_n0;
