@@ -554,6 +554,10 @@ void launch_plan_scan(int n, const int* counts, int* offsets, int* active, int* 
 // Fold a pass's samples into the per-pixel accumulators in sample order
 // (main.cpp:572-576, 707-717; core/stats.h:25-41). Features are summed for every
 // sample with a primary hit, statistics and radiance only for finite samples.
+// The records of FOLD_AHEAD samples are loaded before the first of them is
+// folded (all four planes, hit or not), so a lane waits on memory once per
+// group rather than twice per sample; the folding order is unchanged.
+constexpr int FOLD_AHEAD = 2;
 __global__ void __launch_bounds__(256) k_fold(Batch b, int statmc) {
     const int64_t item = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (item >= b.total[BK_PIXEL]) return;
@@ -583,34 +587,46 @@ __global__ void __launch_bounds__(256) k_fold(Batch b, int statmc) {
         sm = mk3(pb.sqrt_mean[i], pb.sqrt_mean[N + i], pb.sqrt_mean[2 * N + i]);
         s2 = mk3(pb.sqrt_m2[i], pb.sqrt_m2[N + i], pb.sqrt_m2[2 * N + i]);
     }
-    for (int s = 0; s < cnt; ++s) {
-        const float4 r0 = rec.r0[off + s];
-        const uint32_t fl = __float_as_uint(r0.w);
-        if (fl & REC_HIT) {
-            const float4 r1 = rec.r1[off + s], r2 = rec.r2[off + s], r3 = rec.r3[off + s];
-            ++hits;
-            nx += r1.x; ny += r1.y; nz += r1.z;
-            ar += r2.x; ag += r2.y; ab += r2.z;
-            px += r3.x; py += r3.y; pz += r3.z;
-            dep += r1.w;
+    for (int s0 = 0; s0 < cnt; s0 += FOLD_AHEAD) {
+        float4 q0[FOLD_AHEAD], q1[FOLD_AHEAD], q2[FOLD_AHEAD], q3[FOLD_AHEAD];
+#pragma unroll
+        for (int k = 0; k < FOLD_AHEAD; ++k) {
+            if (s0 + k < cnt) {
+                const int64_t o = off + s0 + k;
+                q0[k] = rec.r0[o]; q1[k] = rec.r1[o]; q2[k] = rec.r2[o]; q3[k] = rec.r3[o];
+            }
         }
-        if (!(fl & REC_FINITE)) continue;
-        const v3 c = mk3(r0.x, r0.y, r0.z);
-        if (statmc) {
-            const float lum = luminance3(c);
-            const float d = lum - mean;
-            ++n;
-            mean += d / float(n);
-            const float d2 = lum - mean;
-            m2 += d * d2;
-            cm = cm + (c - cm) / float(n);
-            const v3 sc = mk3(sqrtf(fmaxf(0.0f, c.x)), sqrtf(fmaxf(0.0f, c.y)), sqrtf(fmaxf(0.0f, c.z)));
-            const v3 sd = sc - sm;
-            sm = sm + sd / float(n);
-            s2 = s2 + vmul(sd, sc - sm);
+#pragma unroll
+        for (int k = 0; k < FOLD_AHEAD; ++k) {
+            if (s0 + k >= cnt) break;
+            const float4 r0 = q0[k];
+            const uint32_t fl = __float_as_uint(r0.w);
+            if (fl & REC_HIT) {
+                const float4 r1 = q1[k], r2 = q2[k], r3 = q3[k];
+                ++hits;
+                nx += r1.x; ny += r1.y; nz += r1.z;
+                ar += r2.x; ag += r2.y; ab += r2.z;
+                px += r3.x; py += r3.y; pz += r3.z;
+                dep += r1.w;
+            }
+            if (!(fl & REC_FINITE)) continue;
+            const v3 c = mk3(r0.x, r0.y, r0.z);
+            if (statmc) {
+                const float lum = luminance3(c);
+                const float d = lum - mean;
+                ++n;
+                mean += d / float(n);
+                const float d2 = lum - mean;
+                m2 += d * d2;
+                cm = cm + (c - cm) / float(n);
+                const v3 sc = mk3(sqrtf(fmaxf(0.0f, c.x)), sqrtf(fmaxf(0.0f, c.y)), sqrtf(fmaxf(0.0f, c.z)));
+                const v3 sd = sc - sm;
+                sm = sm + sd / float(n);
+                s2 = s2 + vmul(sd, sc - sm);
+            }
+            rr += c.x; rg += c.y; rb += c.z;
+            ++scount;
         }
-        rr += c.x; rg += c.y; rb += c.z;
-        ++scount;
     }
     pb.radiance[i] = rr; pb.radiance[N + i] = rg; pb.radiance[2 * N + i] = rb;
     pb.albedo[i] = ar; pb.albedo[N + i] = ag; pb.albedo[2 * N + i] = ab;
