@@ -190,13 +190,15 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds& L) {
             const int p = s_pix[j];
             wc = s_w[j];
             const int64_t slot = int64_t(plan.offsets[p]) + (c - s_pre[j]);
+            // all four planes at once: one memory round trip (the last three
+            // are unused for a non-finite sample)
             const float4 r0 = rec.r0[slot];
+            const float4 r4 = rec.r4[slot];
+            const float4 r5 = rec.r5[slot];
+            const float2 r6 = rec.r6[slot];
             const uint32_t fl = __float_as_uint(r0.w);
             if (fl & REC_FINITE) {
                 y = luminance3(mk3(r0.x, r0.y, r0.z));
-                const float4 r4 = rec.r4[slot];
-                const float4 r5 = rec.r5[slot];
-                const float2 r6 = rec.r6[slot];
                 const int x = p % W, yy = p / W;
                 if (fl & REC_PIXEL) {
                     const float su = clampf((float(x) + r4.x - left) / float(g.support), 0.0f, 1.0f);
