@@ -496,17 +496,26 @@ __global__ void __launch_bounds__(64) k_alloc_sweeps(int n, AllocConfig ac, cons
     }
 }
 
-// shares, floors and residuals (main.cpp:1403-1408, 1414-1423)
+// shares, floors and residuals (main.cpp:1403-1408, 1414-1423). The floors are
+// summed per wave and added to the assigned count with one atomic per wave
+// (an integer sum, the same in any order). Every lane of the grid reaches the
+// shuffles: lanes past n add 0, and the inactive case returns for all lanes.
 __global__ void k_alloc_floor(int n, const float* imp, int* counts, AllocBufs a, float* resid) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (!a.st->active) { resid[i] = 0.0f; return; }
-    float d = a.desired[i];
-    if (a.st->final_assign && !a.clipped[i]) d = float(double(imp[i]) / a.st->rem_imp * double(a.st->remaining));
-    const int add = int(floorf(d));
-    counts[i] += add;
-    resid[i] = fmaxf(0.0f, d - float(add));
-    if (add) atomicAdd(&a.st->assigned, add);
+    if (!a.st->active) {
+        if (i < n) resid[i] = 0.0f;
+        return;
+    }
+    int add = 0;
+    if (i < n) {
+        float d = a.desired[i];
+        if (a.st->final_assign && !a.clipped[i]) d = float(double(imp[i]) / a.st->rem_imp * double(a.st->remaining));
+        add = int(floorf(d));
+        counts[i] += add;
+        resid[i] = fmaxf(0.0f, d - float(add));
+    }
+    for (int o = 32; o > 0; o >>= 1) add += __shfl_xor(add, o);
+    if ((threadIdx.x & 63) == 0 && add) atomicAdd(&a.st->assigned, add);
 }
 
 // systematic residual sampling (main.cpp:1425-1438), counted per pixel
