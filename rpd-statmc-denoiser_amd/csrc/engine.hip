@@ -564,6 +564,7 @@ int build_batch1(rpd_job* j, ImageDesc* dd, int64_t* dp, unsigned int* dq) {
         pf[2 * k + 1] = image_items(k, j->W, j->H, j->geom);
         j->batch1.total[k] = pf[2 * k + 1];
     }
+    j->batch1.max_step = j->geom.step;
     HIP_TRY(sync_memcpy(dd, &j->desc, sizeof(ImageDesc), hipMemcpyHostToDevice));
     HIP_TRY(sync_memcpy(dp, pf, sizeof(pf), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(dq, 0, sizeof(unsigned int) * BK_KINDS * BATCH_QUEUES));
@@ -1454,7 +1455,10 @@ int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu
         std::vector<ImageDesc> descs(nb);
         std::vector<int64_t> pf(size_t(BK_KINDS) * (nb + 1), 0);
         Batch b{};
-        for (int k = 0; k < nb; ++k) descs[k] = batched[k]->desc;
+        for (int k = 0; k < nb; ++k) {
+            descs[k] = batched[k]->desc;
+            b.max_step = std::max(b.max_step, descs[k].g.step);
+        }
         for (int kind = 0; kind < BK_KINDS; ++kind) {
             for (int k = 0; k < nb; ++k)
                 pf[size_t(kind) * (nb + 1) + k + 1] =

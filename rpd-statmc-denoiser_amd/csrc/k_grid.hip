@@ -72,7 +72,12 @@ void launch_batch_queues_reset(const Batch& b, hipStream_t st) {
 // |y| <= luminance3(FLT_MAX, FLT_MAX, FLT_MAX) = FLT_MAX. The support list is
 // built by all threads at once (weights, counts, then an ordered block scan).
 constexpr int GATHER_BLOCK = 256;
-constexpr int GATHER_MAX_PIX = 1024;  // support window of up to (2*step)^2 pixels, step <= 16
+// The support window of a node holds up to (2*step)^2 pixels. The kernel is
+// built for windows of 64, 256 and 1024 pixels (steps up to 4, 8 and 16) and
+// launched with the smallest that holds every image of the batch, so the
+// default step of 4 leaves LDS room for more resident blocks.
+constexpr int GATHER_MAX_STEP = 16;
+__host__ __device__ constexpr int gather_max_pix(int max_step) { return 4 * max_step * max_step; }
 
 __device__ __forceinline__ int dep_bin(float u, float v) {  // core/stats.h:165-169
     const int x = min(max(int(u * 4), 0), 3);
@@ -98,10 +103,11 @@ __device__ __forceinline__ int gather_scan(int v, int* sh) {
 
 constexpr int GATHER_BIN_ROW = GATHER_BLOCK / 4 + 1;  // dwords per accumulator row of bin bytes
 
+template <int kMaxPix>
 struct GatherLds {
-    int pix[GATHER_MAX_PIX];
-    float w[GATHER_MAX_PIX];
-    int pre[GATHER_MAX_PIX + 1];
+    int pix[kMaxPix];
+    float w[kMaxPix];
+    int pre[kMaxPix + 1];
     int scan[GATHER_BLOCK];
     // Rows are padded by one element so that the rows one wave reads at the
     // same index (several accumulators' bins, the three pair arrays) start in
@@ -114,7 +120,8 @@ constexpr int GATHER_BIN_THREADS = GRID_ACC * 16;                      // 208
 constexpr int GATHER_SUM_THREADS = GATHER_BIN_THREADS + 2 * GRID_ACC;  // 234
 
 // One node of one image; called by all threads of the block.
-__device__ void gather_node(const ImageDesc& d, const int node, GatherLds& L) {
+template <int kMaxPix>
+__device__ void gather_node(const ImageDesc& d, const int node, GatherLds<kMaxPix>& L) {
     const GridGeom g = d.g;
     const int W = d.W, H = d.H;
     const PassPlan& plan = d.plan;
@@ -259,8 +266,9 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds& L) {
     }
 }
 
+template <int kMaxStep>
 __global__ void __launch_bounds__(GATHER_BLOCK) k_grid_gather(Batch b) {
-    __shared__ GatherLds L;
+    __shared__ GatherLds<gather_max_pix(kMaxStep)> L;
     int cursor = 0;
     while (true) {
         if (threadIdx.x == 0) L.item = batch_take(b, BK_NODE, cursor);
@@ -274,10 +282,11 @@ __global__ void __launch_bounds__(GATHER_BLOCK) k_grid_gather(Batch b) {
     }
 }
 
-void launch_grid_gather(const Batch& b, hipStream_t st) {
+template <int kMaxStep>
+static void launch_grid_gather_t(const Batch& b, hipStream_t st) {
     static int per_cu = 0, cus = 0;
     if (per_cu <= 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_grid_gather),
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_grid_gather<kMaxStep>),
                                                          GATHER_BLOCK, 0) != hipSuccess || per_cu <= 0) per_cu = 1;
         int dev = 0;
         hipGetDevice(&dev);
@@ -285,7 +294,14 @@ void launch_grid_gather(const Batch& b, hipStream_t st) {
     }
     const int64_t blocks = std::min<int64_t>(b.total[BK_NODE], int64_t(per_cu) * cus);
     if (blocks <= 0) return;
-    hipLaunchKernelGGL(k_grid_gather, dim3(unsigned(blocks)), dim3(GATHER_BLOCK), 0, st, b);
+    hipLaunchKernelGGL(k_grid_gather<kMaxStep>, dim3(unsigned(blocks)), dim3(GATHER_BLOCK), 0, st, b);
+}
+
+// b.max_step <= 0 (not known) takes the largest window
+void launch_grid_gather(const Batch& b, hipStream_t st) {
+    if (b.max_step > 0 && b.max_step <= 4) launch_grid_gather_t<4>(b, st);
+    else if (b.max_step > 0 && b.max_step <= 8) launch_grid_gather_t<8>(b, st);
+    else launch_grid_gather_t<GATHER_MAX_STEP>(b, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -153,6 +153,7 @@ struct Batch {
     unsigned int* queues;            // device: BK_KINDS * BATCH_QUEUES work counters
     int n;
     int64_t total[BK_KINDS];         // host copies of the last prefix entries (launch sizing)
+    int max_step;                    // host: largest grid step of the images (0: not known)
 };
 __host__ __device__ inline int64_t image_items(int kind, int W, int H, GridGeom g) {
     return kind == BK_PIXEL ? int64_t(W) * H

@@ -55,8 +55,10 @@ def test_traced_records_denoise_like_oracle(engine_ctx, aperture):
     _compare_denoised(oj, gj, "traced records")
 
 
-def test_uneven_counts_and_nonfinite_records(engine_ctx):
+@pytest.mark.parametrize("tile", [8, 16, 32])   # grid steps 4, 8, 16: each gather support-window size
+def test_uneven_counts_and_nonfinite_records(engine_ctx, tile):
     rc = ob.config_c0(37, 29)
+    rc.rpf_tile_size = tile
     rng = np.random.default_rng(5)
     counts = rng.integers(0, 9, rc.width * rc.height).astype(np.int32)
     n = int(counts.sum())
@@ -137,14 +139,18 @@ def test_synthetic_record_statistics(engine_ctx):
 
 
 def test_batch_identical_to_alone(engine_ctx):
-    """Uneven images and sample counts in one batch give the same bits as each image alone."""
+    """Uneven images, sample counts and grid steps in one batch give the same bits as each
+    image alone (alone, each runs the gather built for its own step; in the batch, all run the
+    one built for the largest)."""
     shapes = [(64, 48, 4), (100, 37, 7), (33, 70, 16), (128, 128, 2), (17, 9, 5)]
+    tiles = [8, 16, 6, 32, 8]
     outs = []
     for batched in (False, True):
         jobs = []
         for k, (w, h, spp) in enumerate(shapes):
             rc = ob.config_c0(w, h)
             rc.color_window_radius = 3 + k
+            rc.rpf_tile_size = tiles[k]
             j = _denoiser(engine_ctx, rc, spp)
             j.synthesize_records(eng.synth_config(seed=100 + k, samples_per_pixel=spp, region_size=4 + 3 * k))
             jobs.append(j)
