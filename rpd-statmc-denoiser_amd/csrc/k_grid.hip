@@ -267,7 +267,7 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds<kMaxPi
 }
 
 template <int kMaxStep>
-__global__ void __launch_bounds__(GATHER_BLOCK) k_grid_gather(Batch b) {
+__global__ void __launch_bounds__(GATHER_BLOCK, 8) k_grid_gather(Batch b) {
     __shared__ GatherLds<gather_max_pix(kMaxStep)> L;
     int cursor = 0;
     while (true) {
