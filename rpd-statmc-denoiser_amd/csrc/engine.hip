@@ -132,7 +132,9 @@ struct rpd_scene {
     DevScene dev{};
     DevBvh bvh{};          // device BVH, or null nodes for the flat loop
     int n_tris = 0, n_spheres = 0, n_materials = 0, n_lights = 0;
-    int trace_per_cu[2] = {0, 0};  // resident trace blocks per CU without / with a depth limit, set on first use
+    // resident trace blocks per CU without / with a depth limit, set on first
+    // use; jobs of several threads may share the scene
+    std::atomic<int> trace_per_cu[2] = {{0}, {0}};
     int bvh_builder = RPD_BVH_HOST_SAH;  // which builder made the BVH (if the scene has one)
     // host copies for host-side queries (rpd_scene_query)
     std::vector<DevTriangle> h_tris;
@@ -321,8 +323,11 @@ TraceParams trace_params(rpd_job* j, int statmc) {
 
 int trace_blocks(rpd_job* j) {
     const int limited = j->cfg.max_depth > 0 ? 1 : 0;
-    int& per_cu = j->scene->trace_per_cu[limited];
-    if (per_cu <= 0) per_cu = trace_blocks_per_cu(j->scene->dev, j->scene->bvh, j->cfg.max_depth);
+    int per_cu = j->scene->trace_per_cu[limited].load();
+    if (per_cu <= 0) {
+        per_cu = trace_blocks_per_cu(j->scene->dev, j->scene->bvh, j->cfg.max_depth);
+        j->scene->trace_per_cu[limited].store(per_cu);
+    }
     const int by_work = (j->N + trace_block_size() - 1) / trace_block_size();
     return std::max(1, std::min(by_work, per_cu * std::max(1, j->ctx->num_cus)));
 }

@@ -296,15 +296,8 @@ void launch_statmc_prepare(const Batch& b, hipStream_t st) {
     hipLaunchKernelGGL(k_filter_pack, dim3((np + 255) / 256), dim3(256), 0, st, b);
 }
 void launch_statmc_window(const Batch& b, hipStream_t st) {
-    static int per_cu = 0, cus = 0;
-    if (per_cu <= 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_statmc), 256, 0) !=
-                hipSuccess || per_cu <= 0) per_cu = 1;
-        int dev = 0;
-        hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
-    }
-    const int64_t blocks = std::min<int64_t>(b.total[BK_TILE], int64_t(per_cu) * cus);
+    static const int cap = device_resident_blocks(reinterpret_cast<const void*>(&k_statmc), 256);
+    const int64_t blocks = std::min<int64_t>(b.total[BK_TILE], cap);
     if (blocks <= 0) return;
     hipLaunchKernelGGL(k_statmc, dim3(unsigned(blocks)), dim3(256), 0, st, b);
 }

@@ -282,17 +282,21 @@ __global__ void __launch_bounds__(GATHER_BLOCK, 8) k_grid_gather(Batch b) {
     }
 }
 
+int device_resident_blocks(const void* kernel, int block_threads) {
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block_threads, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
+    return per_cu * cus;
+}
+
 template <int kMaxStep>
 static void launch_grid_gather_t(const Batch& b, hipStream_t st) {
-    static int per_cu = 0, cus = 0;
-    if (per_cu <= 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_grid_gather<kMaxStep>),
-                                                         GATHER_BLOCK, 0) != hipSuccess || per_cu <= 0) per_cu = 1;
-        int dev = 0;
-        hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
-    }
-    const int64_t blocks = std::min<int64_t>(b.total[BK_NODE], int64_t(per_cu) * cus);
+    // resident blocks over the device; a function-local static is initialised
+    // once even when several job threads launch at the same time
+    static const int cap = device_resident_blocks(reinterpret_cast<const void*>(&k_grid_gather<kMaxStep>), GATHER_BLOCK);
+    const int64_t blocks = std::min<int64_t>(b.total[BK_NODE], cap);
     if (blocks <= 0) return;
     hipLaunchKernelGGL(k_grid_gather<kMaxStep>, dim3(unsigned(blocks)), dim3(GATHER_BLOCK), 0, st, b);
 }

@@ -189,6 +189,10 @@ __device__ inline int64_t batch_take(const Batch& b, int kind, int& cursor) {
     return -1;
 }
 
+// Blocks of `kernel` (block_threads threads, no dynamic LDS) that fit on the
+// current device at once: occupancy per CU times the CU count, at least 1.
+int device_resident_blocks(const void* kernel, int block_threads);
+
 // stable radix sort of (key, value) pairs on the low key_bits bits (sort_keys.hip)
 size_t sort_pairs_u32_temp_bytes(int n);
 hipError_t sort_pairs_u32(const uint32_t* keys_in, uint32_t* keys_out, const int* vals_in, int* vals_out, int n,

@@ -30,3 +30,85 @@ def rel_err(a, b, floor=1e-6):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), floor)
+
+
+# Fraction of traced samples allowed to take another branch at a rounding
+# boundary (an intersection within an ulp of an edge or of t_max, a threshold
+# compare on a transcendental result): cornell_box (32 triangles, Lambertian)
+# and the procedural scene (BVH, GGX microfacets, spheres). Measured on the
+# MI355X: at most 5 of 262,144 cornell samples and 7 of 49,152 procedural
+# samples per pass.
+RECORD_OUTLIER_BUDGET = {"cornell_box": 1e-4, "procedural": 5e-4}
+
+
+def records_outliers(orec, grec):
+    """Per-sample records (32-float rows) of identical RNG streams: the pixel of
+    every slot and the first four draws must be equal; returns the mask of
+    samples that differ (radiance beyond 1e-4 relative, any captured dimension,
+    flag or light index, or primary features beyond 1e-5)."""
+    assert orec.shape == grec.shape, (orec.shape, grec.shape)
+    np.testing.assert_array_equal(orec[:, 0], grec[:, 0])              # pixel of every slot
+    np.testing.assert_array_equal(orec[:, 16:20], grec[:, 16:20])      # pixel_u, lens_u: first four draws
+    rad_bad = (rel_err(orec[:, 1:4], grec[:, 1:4]) > 1e-4).any(axis=1)
+    dims_bad = (orec[:, 20:29] != grec[:, 20:29]).any(axis=1)
+    flag_bad = (orec[:, 4:6] != grec[:, 4:6]).any(axis=1)
+    feat_bad = (rel_err(orec[:, 6:16], grec[:, 6:16], floor=1.0) > 1e-5).any(axis=1)
+    return rad_bad | dims_bad | flag_bad | feat_bad
+
+
+def check_records(orec, grec, budget, what=""):
+    """Outliers within `budget` (a fraction of the samples) and every other
+    sample's radiance within 1e-4 relative; returns the outlier mask."""
+    bad = records_outliers(orec, grec)
+    n_bad = int(bad.sum())
+    print(f"{what} samples {len(orec)} outliers {n_bad} (budget {budget * len(orec):.1f})")
+    assert n_bad <= budget * len(orec), (what, n_bad, len(orec))
+    if (~bad).any():
+        assert rel_err(orec[~bad, 1:4], grec[~bad, 1:4]).max() <= 1e-4
+    return bad
+
+
+def relmse(x, r):
+    """mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2)."""
+    x, r = np.asarray(x, np.float64), np.asarray(r, np.float64)
+    return float(np.mean((x - r) ** 2 / (r ** 2 + 1e-2)))
+
+
+# A free-running job that diverged from the oracle after an outlier sample:
+# largest relMSE(engine, oracle) as a fraction of relMSE(oracle, oracle run
+# with another seed). Measured on the MI355X: 0.002 to 0.092.
+DIVERGED_RELMSE_RATIO = 0.25
+
+
+def compare_whole_job(oj, gj, what, oracle_other_seed):
+    """A finished job against the oracle's render() of the same request (oj).
+
+    Equal within 1e-4 everywhere when no pass diverged. The allocation is one
+    sequential scan over the whole image (main.cpp:1373-1438), so one outlier
+    sample changes the running sums and the systematic residual draw of every
+    later pixel: from that pass on the job samples many pixels differently, as
+    two runs of the reference would after any perturbation. Such a job must
+    keep the total sample count and differ from the oracle by much less than
+    two oracle runs with different seeds (oracle_other_seed: a callable that
+    renders one, run only then), with the image mean within 1e-3."""
+    n = oj.w * oj.ht
+    sc_diff = oj.get("sample_count") != gj.get("sample_count")
+    raw = rel_err(oj.get("raw_radiance"), gj.get("raw_radiance")).max(axis=1) > 1e-4
+    den = rel_err(oj.get("radiance"), gj.get("radiance")).max(axis=1) > 1e-4
+    var = rel_err(oj.get("variance_denoised"), gj.get("variance_denoised"), floor=1e-8) > 1e-4
+    line = (f"{what}: {n} px; sample counts differ {int(sc_diff.sum())}, raw radiance beyond 1e-4 "
+            f"{int(raw.sum())}, denoised beyond 1e-4 {int(den.sum())}, variance beyond 1e-4 {int(var.sum())}")
+    assert oj.get("sample_count").sum() == gj.get("sample_count").sum(), line
+    if not (sc_diff.any() or raw.any() or den.any() or var.any()):
+        print("\n  " + line + "; identical within 1e-4")
+        return line
+    other = oracle_other_seed()
+    for name in ("raw_radiance", "radiance"):
+        o, g, o2 = oj.get(name), gj.get(name), other.get(name)
+        r_gpu, r_seed = relmse(g, o), relmse(o2, o)
+        bias = abs(float(g.mean()) - float(o.mean())) / float(o.mean())
+        line += f"; {name}: relMSE vs oracle {r_gpu:.2e}, oracle seed-to-seed {r_seed:.2e}, mean {bias:.1e}"
+        assert r_gpu <= DIVERGED_RELMSE_RATIO * r_seed, line
+        assert bias <= 1e-3, line
+    print("\n  " + line)
+    return line

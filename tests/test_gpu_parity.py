@@ -5,14 +5,18 @@ from identical bits) and compares the stage output; the trace tests compare
 per-sample records of identical RNG streams. Tolerances: random draws, bin
 inputs, flags and sample counts are compared exactly; radiance within 1e-4
 relative per sample, with samples whose path took a different branch at an
-intersection rounding boundary counted as outliers.
+intersection rounding boundary counted as outliers and held to
+RECORD_OUTLIER_BUDGET; StatMC and variance smoothing from identical inputs have
+no pixel beyond 1e-4; allocation counts are exact.
 """
 import numpy as np
 import pytest
 
 import oracle_bind as ob
 import rpd_engine as eng
-from parity_util import engine_camera, engine_config, rel_err
+from parity_util import RECORD_OUTLIER_BUDGET, check_records, compare_whole_job, engine_camera, engine_config, rel_err
+
+BUDGET = RECORD_OUTLIER_BUDGET["cornell_box"]
 
 pytestmark = pytest.mark.gpu
 
@@ -28,18 +32,6 @@ def _gpu_job(ctx, rc, seed=SEED):
     return eng.Job(ctx, engine_config(rc), engine_camera(rc), seed)
 
 
-def _compare_records(orec, grec):
-    assert orec.shape == grec.shape, (orec.shape, grec.shape)
-    np.testing.assert_array_equal(orec[:, 0], grec[:, 0])              # pixel of every slot
-    np.testing.assert_array_equal(orec[:, 16:20], grec[:, 16:20])      # pixel_u, lens_u: first four draws
-    rad_bad = (rel_err(orec[:, 1:4], grec[:, 1:4]) > 1e-4).any(axis=1)
-    dims_bad = (orec[:, 20:29] != grec[:, 20:29]).any(axis=1)
-    flag_bad = (orec[:, 4:6] != grec[:, 4:6]).any(axis=1)
-    feat_bad = (rel_err(orec[:, 6:16], grec[:, 6:16], floor=1.0) > 1e-5).any(axis=1)
-    bad = rad_bad | dims_bad | flag_bad | feat_bad
-    return bad, rad_bad, dims_bad, flag_bad, feat_bad
-
-
 @pytest.mark.parametrize("aperture", [0.0, 0.14])
 def test_trace_records_match_oracle(engine_ctx, aperture):
     rc = ob.config_c0(40, 32)
@@ -49,14 +41,7 @@ def test_trace_records_match_oracle(engine_ctx, aperture):
     oj.run_pass(np.full(40 * 32, rc.spp, np.int32), statmc=True)
     gj = _gpu_job(engine_ctx, rc)
     gj.run_pass(None, statmc=True)
-    orec, grec = oj.get("records"), gj.get("records")
-    bad, rad_bad, dims_bad, flag_bad, feat_bad = _compare_records(orec, grec)
-    print(f"samples {len(orec)} outliers {bad.sum()} (radiance {rad_bad.sum()}, dims {dims_bad.sum()}, "
-          f"flags {flag_bad.sum()}, features {feat_bad.sum()})")
-    # samples whose path split at an intersection rounding boundary
-    assert bad.mean() <= 2e-3
-    # the per-sample radiance of agreeing samples is within 1e-4 relative
-    assert rel_err(orec[~bad, 1:4], grec[~bad, 1:4]).max() <= 1e-4
+    check_records(oj.get("records"), gj.get("records"), BUDGET, f"aperture {aperture}:")
 
 
 def test_trace_records_past_the_first_twist_match_oracle(engine_ctx):
@@ -70,11 +55,8 @@ def test_trace_records_past_the_first_twist_match_oracle(engine_ctx):
     gj = _gpu_job(engine_ctx, rc)
     gj.run_pass(None, statmc=True)
     orec, grec = oj.get("records"), gj.get("records")
-    bad = _compare_records(orec, grec)[0]
-    print(f"samples {len(orec)} outliers {bad.sum()}")
     assert len(orec) == 16 * 12 * 32
-    assert bad.mean() <= 2e-3
-    assert rel_err(orec[~bad, 1:4], grec[~bad, 1:4]).max() <= 1e-4
+    check_records(orec, grec, BUDGET, "32 spp:")
 
 
 def test_trace_uniform_render_matches_oracle(engine_ctx):
@@ -84,9 +66,7 @@ def test_trace_uniform_render_matches_oracle(engine_ctx):
     oj.run_pass(np.full(24 * 20, rc.spp, np.int32), statmc=False)
     gj = _gpu_job(engine_ctx, rc)
     gj.run_pass(None, statmc=False)
-    orec, grec = oj.get("records"), gj.get("records")
-    bad = _compare_records(orec, grec)[0]
-    assert bad.mean() <= 2e-3
+    check_records(oj.get("records"), gj.get("records"), BUDGET, "uniform render:")
 
 
 def test_pass_state_matches_oracle(engine_ctx):
@@ -95,19 +75,25 @@ def test_pass_state_matches_oracle(engine_ctx):
     oj.run_pass(np.full(40 * 32, rc.spp, np.int32), statmc=True)
     gj = _gpu_job(engine_ctx, rc)
     gj.run_pass(None, statmc=True)
+    bad = check_records(oj.get("records"), gj.get("records"), BUDGET, "pass state:")
+    out_px = np.zeros(rc.width * rc.height, bool)
+    out_px[oj.get("records")[bad, 0].astype(np.int64)] = True
     np.testing.assert_array_equal(oj.get("hit_count"), gj.get("hit_count"))
     np.testing.assert_array_equal(oj.get("sample_count"), gj.get("sample_count"))
     os_, gs = oj.get("stats"), gj.get("stats")
     np.testing.assert_array_equal(os_[:, 2].view(np.int32), gs[:, 2].view(np.int32))
     cols = [0, 3, 4, 5, 6, 7, 8]
     px_bad = (rel_err(os_[:, cols], gs[:, cols], floor=1e-3) > 1e-4).any(axis=1)
-    print("pixels with stat differences:", px_bad.sum())
-    assert px_bad.mean() <= 0.02
+    print("pixels with stat differences:", px_bad.sum(), "outlier pixels:", out_px.sum())
+    assert not (px_bad & ~out_px).any()
+    # grid accumulators are sums of non-negative per-sample terms: within the
+    # per-sample tolerance, except nodes an outlier splatted into (at most 4 each)
     og, gg = oj.get("grid"), gj.get("grid")
     assert og.shape == gg.shape
-    node_bad = (np.abs(og - gg) > 1e-6 * np.maximum(np.abs(og), 1.0)).any(axis=1)
-    print("grid nodes with differences:", node_bad.sum(), "of", len(og))
-    assert node_bad.mean() <= 0.1
+    node_err = (np.abs(og - gg) / np.maximum(np.abs(og), 1e-300)).max(axis=1)
+    node_err[np.abs(og - gg).max(axis=1) == 0.0] = 0.0
+    print("grid max rel err", node_err.max(), "nodes beyond 1e-4:", (node_err > 1e-4).sum(), "of", len(og))
+    assert (node_err > 1e-4).sum() <= 4 * bad.sum()
 
 
 def _oracle_after_base(rc, seed=SEED):
@@ -156,10 +142,11 @@ def test_statmc_stage_identical_inputs(engine_ctx, radius, alpha):
     gj.statmc()
     od, gd = oj.get("denoised"), gj.get("denoised")
     e = rel_err(od, gd)
-    print("denoised max rel err", e.max(), "pixels > 1e-4:", (e > 1e-4).any(axis=1).sum())
-    assert (e > 1e-4).any(axis=1).mean() <= 1e-3
-    np.testing.assert_allclose(gj.get("alpha"), oj.get("alpha"), atol=1e-5)
-    assert rel_err(oj.get("uncertainty"), gj.get("uncertainty"), floor=1e-9).max() <= 1e-3
+    print("denoised max rel err", e.max(), "pixels > 1e-4:", (e > 1e-4).any(axis=1).sum(),
+          "bit-identical pixels:", (od == gd).all(axis=1).mean())
+    assert not (e > 1e-4).any()
+    np.testing.assert_allclose(gj.get("alpha"), oj.get("alpha"), rtol=0, atol=1e-6)
+    assert rel_err(oj.get("uncertainty"), gj.get("uncertainty"), floor=1e-12).max() <= 1e-5
     for name in ("var_total", "var_eff", "var_ratio"):
         np.testing.assert_array_equal(oj.get(name), gj.get(name))
 
@@ -246,13 +233,10 @@ def test_full_pipeline_c0_small(engine_ctx):
     oj.render()
     gj = eng.Job(engine_ctx, engine_config(rc), engine_camera(rc), SEED, submit=True)
     gj.wait()
-    og, gg = oj.get("radiance"), gj.get("radiance")
-    oraw, graw = oj.get("raw_radiance"), gj.get("raw_radiance")
-    e = rel_err(og, gg).max(axis=1)
-    er = rel_err(oraw, graw).max(axis=1)
-    sc_same = (oj.get("sample_count") == gj.get("sample_count")).mean()
-    print(f"denoised px >1e-4: {(e > 1e-4).mean():.4f}, raw px >1e-4: {(er > 1e-4).mean():.4f}, "
-          f"sample counts equal: {sc_same:.4f}")
-    assert sc_same >= 0.98
-    assert (e > 1e-4).mean() <= 0.05
-    assert np.median(e) <= 1e-5
+
+    def other():
+        o = ob.OracleJob(rc, SEED + 1, nthreads=4, skip_dead=True)
+        o.render()
+        return o
+
+    compare_whole_job(oj, gj, "c0 48x40", other)
