@@ -394,6 +394,17 @@ typedef struct rpd_image_metrics {
    device memory of the context's device (else host memory, copied in). */
 int rpd_image_metrics_compute(rpd_context* ctx, const float* ref, const float* test, int width, int height,
                               int on_device, rpd_image_metrics* out);
+/* No counterpart in the reference. rpd_image_metrics_compute for images in
+   device memory, for callers that cannot allocate device memory (code under
+   stream capture, tight serving loops, harnesses that forbid allocation): it
+   allocates nothing. Its partial sums live in a fixed device array of the
+   library (5.2 MB, held by every device the library runs on until the process
+   ends), so images are limited to 65536 16x16 tiles (4096 x 4096 pixels);
+   larger ones are refused with RPD_ERR_INVALID_ARG. Calls on one context are
+   serialised; calls through different contexts on the same device share the
+   array and must not overlap. Results equal rpd_image_metrics_compute's. */
+int rpd_image_metrics_compute_noalloc(rpd_context* ctx, const float* ref, const float* test, int width, int height,
+                                      rpd_image_metrics* out);
 /* A job's RPD_BUF_RADIANCE / RPD_BUF_RAW_RADIANCE (finalized jobs) or
    RPD_BUF_DENOISED buffer, read in place on the device, against `ref`. */
 int rpd_job_image_metrics(rpd_job* job, int buffer_id, const float* ref, int ref_on_device, rpd_image_metrics* out);

@@ -86,6 +86,18 @@ int dalloc(T** p, size_t count, std::vector<void*>& owned) {
     return RPD_OK;
 }
 
+// Device scratch of rpd_image_metrics_compute_noalloc: the metric kernels'
+// per-block partial sums followed by the reduced fields. Every device the
+// library runs on holds its own copy until the process ends. Host code never
+// uses `scratch` itself, only the address hipGetSymbolAddress returns for it
+// (rpd_context::scratch), taken once per context with its device current.
+constexpr int NOALLOC_METRICS_MAX_TILES = 65536;      // 16x16 tiles: up to 4096 x 4096 pixels
+constexpr size_t NOALLOC_METRICS_OUT = 32;           // reduced fields (MA_FIELDS, MB_FIELDS <= 32)
+constexpr size_t NOALLOC_METRICS_PARTS =             // partials of the larger pass
+    size_t(NOALLOC_METRICS_MAX_TILES) * MB_FIELDS > size_t(METRICS_A_MAX_BLOCKS) * MA_FIELDS
+        ? size_t(NOALLOC_METRICS_MAX_TILES) * MB_FIELDS : size_t(METRICS_A_MAX_BLOCKS) * MA_FIELDS;
+static __device__ __align__(256) double scratch[NOALLOC_METRICS_PARTS + NOALLOC_METRICS_OUT];
+
 }  // namespace
 
 struct rpd_context {
@@ -111,6 +123,8 @@ struct rpd_context {
     float* m_img = nullptr;
     size_t m_img_cap = 0;
     hipStream_t m_stream = nullptr;
+    // device address of this device's `scratch` array (set by rpd_ctx_create)
+    double* scratch = nullptr;
     // comparison denoisers: staging and scratch floats, grown on demand
     std::mutex filters_mu;
     float* f_buf = nullptr;
@@ -917,6 +931,10 @@ int rpd_ctx_create(int device, rpd_context** out) {
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     c->num_cus = prop.multiProcessorCount;
+    // set-up of the allocation-free metrics call: this device's scratch array
+    // and the stream it runs on, so that the call itself creates nothing
+    HIP_TRY(hipGetSymbolAddress(reinterpret_cast<void**>(&c->scratch), HIP_SYMBOL(scratch)));
+    HIP_TRY(stream_create(&c->m_stream));
     *out = c.release();
     return RPD_OK;
 }
@@ -1566,9 +1584,14 @@ int rpd_job_stage_ms(rpd_job* j, double* ms, int n) {
 // hdr_metrics (shared/tools/eval_tools.cpp:236-279) and
 // compute_extended_metrics (eval_tools.cpp:145, 172-174, 191, 212).
 // The caller holds ctx->metrics_mu with the context's device current.
-static int metrics_run(rpd_context* ctx, const MetricImage& ref, const MetricImage& tst, int W, int H,
-                       hipStream_t st, rpd_image_metrics* out) {
-    const size_t need = size_t(std::max(metrics_a_blocks(W, H) * MA_FIELDS, metrics_b_blocks(W, H) * MB_FIELDS));
+// Partial sums of both passes for a W x H image, in doubles.
+static size_t metrics_part_doubles(int W, int H) {
+    return size_t(std::max(metrics_a_blocks(W, H) * MA_FIELDS, metrics_b_blocks(W, H) * MB_FIELDS));
+}
+
+// Grows the context's partial and output buffers for a W x H image.
+static int metrics_buffers(rpd_context* ctx, int W, int H) {
+    const size_t need = metrics_part_doubles(W, H);
     if (need > ctx->m_part_cap) {
         if (ctx->m_part) HIP_TRY(dev_free(ctx->m_part));
         ctx->m_part = nullptr;
@@ -1576,11 +1599,19 @@ static int metrics_run(rpd_context* ctx, const MetricImage& ref, const MetricIma
         HIP_TRY(dev_malloc(&ctx->m_part, need * sizeof(double)));
         ctx->m_part_cap = need;
     }
-    if (!ctx->m_out) HIP_TRY(dev_malloc(&ctx->m_out, 32 * sizeof(double)));
+    if (!ctx->m_out) HIP_TRY(dev_malloc(&ctx->m_out, NOALLOC_METRICS_OUT * sizeof(double)));
+    return RPD_OK;
+}
+
+// `part` holds metrics_part_doubles(W, H) doubles and `dout` 32; every
+// partial the reductions read is written by the pass before, so neither needs
+// clearing between calls.
+static int metrics_run(const MetricImage& ref, const MetricImage& tst, int W, int H, double* part, double* dout,
+                       hipStream_t st, rpd_image_metrics* out) {
     double a[MA_FIELDS], b[MB_FIELDS];
-    launch_metrics_a(ref, tst, W, H, ctx->m_part, ctx->m_out, st);
+    launch_metrics_a(ref, tst, W, H, part, dout, st);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(a, ctx->m_out, sizeof(a), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(a, dout, sizeof(a), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     const double pixels = double(W) * double(H);
     MetricsPassB pb;
@@ -1588,9 +1619,9 @@ static int metrics_run(rpd_context* ctx, const MetricImage& ref, const MetricIma
     const double log_peak = std::max(1e-6, a[MA_LOG_PEAK]);
     pb.c1 = std::pow(0.01 * log_peak, 2.0);
     pb.c2 = std::pow(0.03 * log_peak, 2.0);
-    launch_metrics_b(ref, tst, W, H, pb, ctx->m_part, ctx->m_out, st);
+    launch_metrics_b(ref, tst, W, H, pb, part, dout, st);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(b, ctx->m_out, sizeof(b), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(b, dout, sizeof(b), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
 
     const double peak = a[MA_PEAK] > 0.0 ? a[MA_PEAK] : 1.0;
@@ -1641,7 +1672,6 @@ int rpd_image_metrics_compute(rpd_context* ctx, const float* ref, const float* t
     if (st) return st;
     std::lock_guard<std::mutex> lk(ctx->metrics_mu);
     HIP_TRY(hipSetDevice(ctx->device));
-    if (!ctx->m_stream) HIP_TRY(stream_create(&ctx->m_stream));
     const size_t n3 = size_t(width) * size_t(height) * 3;
     const float* dr = ref;
     const float* dt = test;
@@ -1653,7 +1683,25 @@ int rpd_image_metrics_compute(rpd_context* ctx, const float* ref, const float* t
         dr = stage;
         dt = stage + n3;
     }
-    return metrics_run(ctx, MetricImage{dr, 1, 3}, MetricImage{dt, 1, 3}, width, height, ctx->m_stream, out);
+    if ((st = metrics_buffers(ctx, width, height))) return st;
+    return metrics_run(MetricImage{dr, 1, 3}, MetricImage{dt, 1, 3}, width, height, ctx->m_part, ctx->m_out,
+                       ctx->m_stream, out);
+}
+
+int rpd_image_metrics_compute_noalloc(rpd_context* ctx, const float* ref, const float* test, int width, int height,
+                                      rpd_image_metrics* out) {
+    if (!ctx || !ref || !test || !out) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    int st = metrics_check_dims(width, height);
+    if (st) return st;
+    if (metrics_b_blocks(width, height) > NOALLOC_METRICS_MAX_TILES ||
+        metrics_part_doubles(width, height) > NOALLOC_METRICS_PARTS)
+        return fail(RPD_ERR_INVALID_ARG, "image too large for rpd_image_metrics_compute_noalloc: at most 65536 "
+                                         "16x16 tiles (4096 x 4096 pixels)");
+    if (!ctx->scratch || !ctx->m_stream) return fail(RPD_ERR_STATE, "context has no metrics scratch");
+    std::lock_guard<std::mutex> lk(ctx->metrics_mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    return metrics_run(MetricImage{ref, 1, 3}, MetricImage{test, 1, 3}, width, height, ctx->scratch,
+                       ctx->scratch + NOALLOC_METRICS_PARTS, ctx->m_stream, out);
 }
 
 int rpd_job_image_metrics(rpd_job* j, int buffer_id, const float* ref, int ref_on_device, rpd_image_metrics* out) {
@@ -1684,7 +1732,10 @@ int rpd_job_image_metrics(rpd_job* j, int buffer_id, const float* ref, int ref_o
         HIP_TRY(hipMemcpyAsync(stage, ref, n3 * sizeof(float), hipMemcpyHostToDevice, j->stream));
         dr = stage;
     }
-    return metrics_run(ctx, MetricImage{dr, 1, 3}, MetricImage{planes, int64_t(j->N), 1}, j->W, j->H, j->stream, out);
+    const int st = metrics_buffers(ctx, j->W, j->H);
+    if (st) return st;
+    return metrics_run(MetricImage{dr, 1, 3}, MetricImage{planes, int64_t(j->N), 1}, j->W, j->H, ctx->m_part,
+                       ctx->m_out, j->stream, out);
 }
 
 // ---- comparison denoisers ---------------------------------------------------

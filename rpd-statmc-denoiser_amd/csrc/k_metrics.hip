@@ -19,7 +19,7 @@ namespace rpd {
 namespace {
 
 constexpr int MA_BLOCK = 256;
-constexpr int MA_MAX_BLOCKS = 1024;
+constexpr int MA_MAX_BLOCKS = METRICS_A_MAX_BLOCKS;
 constexpr int MB_TILE = 16;
 constexpr int MB_HALO = 5;
 constexpr int MB_SPAN = MB_TILE + 2 * MB_HALO;   // 26

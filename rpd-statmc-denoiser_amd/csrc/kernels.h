@@ -262,6 +262,7 @@ enum : int {
 // pass B partial fields (sums)
 enum : int { MB_VAR_REF = 0, MB_VAR_TST = 3, MB_COV = 6, MB_SSIM = 9, MB_FIELDS = 10 };
 struct MetricsPassB { double mean_ref[3], mean_tst[3], c1, c2; };
+constexpr int METRICS_A_MAX_BLOCKS = 1024;   // pass A grid-strides over at most this many blocks
 int64_t metrics_a_blocks(int W, int H);
 int64_t metrics_b_blocks(int W, int H);
 // each writes its reduced fields (MA_FIELDS or MB_FIELDS doubles) to `out`;

@@ -214,6 +214,7 @@ def lib():
             ("rpd_ctx_set_graph_passes", [vp, C.c_int]),
             ("rpd_abi_struct_sizes", [vp, C.c_int]),
             ("rpd_image_metrics_compute", [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.POINTER(ImageMetrics)]),
+            ("rpd_image_metrics_compute_noalloc", [vp, vp, vp, C.c_int, C.c_int, C.POINTER(ImageMetrics)]),
             ("rpd_job_image_metrics", [vp, C.c_int, vp, C.c_int, C.POINTER(ImageMetrics)]),
             ("rpd_builtin_scene_arrays", [C.c_char_p, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
             ("rpd_obj_load", [C.c_char_p, C.c_int, vp, vp]),
