@@ -236,7 +236,7 @@ void collect_spans(rpd_job* j) {
     j->spans.clear();
 }
 
-// ---- config validation: core/config.h:460-539 -------------------------------
+// ---- config validation: core/config.h:116-195 -------------------------------
 int validate(const rpd_render_config& c) {
     auto bad = [](const char* m) { return fail(RPD_ERR_INVALID_ARG, m); };
     if (c.image_width <= 0 || c.image_height <= 0) return bad("image_width and image_height must be positive");

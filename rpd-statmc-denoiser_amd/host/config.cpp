@@ -22,7 +22,7 @@ bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1])
 float sq3(const float* v) { return v[0] * v[0] + (v[1] * v[1] + v[2] * v[2]); }
 }  // namespace
 
-// core/config.h:51-545
+// core/config.h:51-201
 Settings parse_settings(const FlatYaml& y) {
     for (const auto& key : y.keys_in_order()) {
         if (std::find_if(kKnownKeys.begin(), kKnownKeys.end(), [&](const char* k) { return key == k; }) == kKnownKeys.end())
@@ -41,7 +41,7 @@ Settings parse_settings(const FlatYaml& y) {
     if (pos.size() != 3 || dir.size() != 3) throw std::runtime_error("Camera vectors must have three components");
     if (up.size() != 3) throw std::runtime_error("world_up must have three components");
     for (int k = 0; k < 3; ++k) { s.camera_position[k] = pos[k]; s.camera_direction[k] = dir[k]; s.camera_up[k] = up[k]; }
-    // focus_distance defaults to the length of camera_direction (config.h:434)
+    // focus_distance defaults to the length of camera_direction (config.h:90)
     s.focus_distance = y.has("focus_distance") ? y.as_float("focus_distance") : std::sqrt(sq3(s.camera_direction));
     s.aperture = y.has("aperture") ? y.as_float("aperture") : 0.0f;
     if (y.has("scene")) s.scene_name = y.as_string("scene");

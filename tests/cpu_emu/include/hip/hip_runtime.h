@@ -31,6 +31,7 @@
 #define __shared__ static thread_local
 #define __constant__
 #define HIP_SYMBOL(x) (&(x))
+#define __align__(n) alignas(n)
 
 // ---- vector types -----------------------------------------------------------------
 struct dim3 {
@@ -94,6 +95,8 @@ hipError_t hipMemGetInfo(size_t* free_b, size_t* total_b);
 hipError_t hipMalloc(void** p, size_t bytes);
 template <class T> inline hipError_t hipMalloc(T** p, size_t bytes) { return hipMalloc(reinterpret_cast<void**>(p), bytes); }
 hipError_t hipFree(void* p);
+// a file-scope device array is ordinary host memory here: its address is itself
+inline hipError_t hipGetSymbolAddress(void** p, const void* symbol) { *p = const_cast<void*>(symbol); return hipSuccess; }
 hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind k);
 hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind k, hipStream_t s = nullptr);
 hipError_t hipMemset(void* dst, int v, size_t bytes);
