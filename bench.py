@@ -93,7 +93,7 @@ def cpu_baseline(seconds_hint=20.0):
     its overwritten in-loop StatMC calls) on one whole job of the same workload."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_bind as ob
-    threads = min(16, os.cpu_count() or 1)
+    threads = len(os.sched_getaffinity(0))      # every host core this process may run on (nproc)
     rc = ob.RenderConfig(width=WORKLOAD["width"], height=WORKLOAD["height"], spp=WORKLOAD["spp"], adaptive_spp=WORKLOAD["adaptive_spp"],
                          adaptive_passes=WORKLOAD["adaptive_passes"], color_window_radius=WORKLOAD["color_window_radius"],
                          var_iterations=WORKLOAD["var_iterations"], aperture=WORKLOAD["aperture"],
@@ -105,7 +105,32 @@ def cpu_baseline(seconds_hint=20.0):
     samples = int(j.get("sample_count").sum())
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
             "sample": f"oracle, one whole job of the same workload ({samples} samples, {dt:.1f} s, "
-                      f"{threads} OpenMP threads)"}
+                      f"{threads} OpenMP threads = all {threads} host cores from nproc)"}
+
+
+REFERENCE_SPP = (8, 128)   # jobs x spp of the high-spp reference image (1024 spp)
+
+
+def c1_quality(ctx, cfg, cam, w, h):
+    """relMSE and NRMSE (shared/tools/eval_tools.cpp:146) of one job's final
+    image and of its raw mean, against a 1024-spp uniform render of the same
+    scene with other seeds (outside every timed region)."""
+    ref = np.zeros((w * h, 3), np.float64)
+    jobs, spp = REFERENCE_SPP
+    rcfg = eng.default_config(image_width=w, image_height=h, samples_per_pixel=spp, use_statmc=0)
+    for k in range(jobs):
+        r = eng.Job(ctx, rcfg, cam, 7000 + k, submit=True)
+        r.wait()
+        ref += r.get("radiance")
+        r.release()
+    ref = (ref / jobs).astype(np.float32)
+    j = eng.Job(ctx, cfg, cam, WORKLOAD["seed"], submit=True)
+    j.wait()
+    den, raw = j.metrics("radiance", ref), j.metrics("raw_radiance", ref)
+    j.release()
+    return {"reference": f"uniform render, {jobs * spp} spp, seeds 7000..{7000 + jobs - 1}",
+            "relmse_denoised": den["relmse"], "relmse_raw": raw["relmse"],
+            "nrmse_denoised": den["nrmse"], "nrmse_raw": raw["nrmse"]}
 
 
 C2 = dict(width=1920, height=1080, spp=8, adaptive_spp=8, adaptive_passes=4, max_depth=8, aperture=0.06,
@@ -184,73 +209,91 @@ def relmse(x, ref):
 
 def run_c3(args):
     """Batched denoise of 32 synthetic per-sample buffers (side U{256..2048},
-    spp U{4..64}): the plain per-image path (one image after another) against
-    rpd_denoise_batch (every stage one launch over all images)."""
+    spp U{4..64}): the plain per-image path (one rpd_job_denoise_records call
+    per image, one image after another) against rpd_denoise_batch (every stage
+    one launch over all images). Both are timed the same way, host wall clock
+    around the calls (each returns with its work done), after a warm-up of
+    each, over args.steps repetitions on freshly synthesised jobs; medians."""
     rng = np.random.default_rng(C3["seed"])
     sides = rng.integers(C3["side"][0], C3["side"][1] + 1, C3["images"])
     spps = rng.integers(C3["spp"][0], C3["spp"][1] + 1, C3["images"])
     ctx = eng.Context(0)
     cfg0 = eng.default_config(image_width=8, image_height=8, use_statmc=1)
     radius = cfg0.color_window_radius
-    budget = int(C3["budget_gb"] * 1e9)
-    ctx.set_memory_budget(budget)
-    jobs, scs = [], []
-    for k in range(C3["images"]):
-        side, spp = int(sides[k]), int(spps[k])
-        cfg = eng.default_config(image_width=side, image_height=side, samples_per_pixel=spp, use_statmc=1)
-        jobs.append(eng.Job(ctx, cfg, None, 0, denoiser_spp=spp))
-        sc = eng.synth_config(seed=1000 + k, samples_per_pixel=spp)
-        scs.append(sc)
-        jobs[-1].synthesize_records(sc)
+    ctx.set_memory_budget(int(C3["budget_gb"] * 1e9))
+    scs = [eng.synth_config(seed=1000 + k, samples_per_pixel=int(spps[k])) for k in range(C3["images"])]
     n_px = int((sides.astype(np.int64) ** 2).sum())
     n_samples = int((sides.astype(np.int64) ** 2 * spps).sum())
-    print(f"c3: {len(jobs)} jobs created, {n_px / 1e6:.1f} Mpx, {n_samples / 1e9:.2f} G samples", file=sys.stderr,
-          flush=True)
-    # plain per-image path, each image alone on the GPU
-    plain_ms = 0.0
-    for j in jobs:
+
+    def fresh_jobs():
+        jobs = []
+        for k in range(C3["images"]):
+            side, spp = int(sides[k]), int(spps[k])
+            cfg = eng.default_config(image_width=side, image_height=side, samples_per_pixel=spp, use_statmc=1)
+            jobs.append(eng.Job(ctx, cfg, None, 0, denoiser_spp=spp))
+            jobs[-1].synthesize_records(scs[k])
+        return jobs
+
+    def plain_run():
+        jobs = fresh_jobs()
         t = time.perf_counter()
-        j.denoise_records()
-        plain_ms += (time.perf_counter() - t) * 1e3
-    plain = [j.get("denoised") for j in jobs]
-    print(f"c3: plain per-image denoise {plain_ms:.0f} ms", file=sys.stderr, flush=True)
-    # batched: fresh jobs with the same records, every stage one launch over all images
-    for j in jobs:
-        j.release()
-    jobs = []
-    for k in range(C3["images"]):
-        side, spp = int(sides[k]), int(spps[k])
-        cfg = eng.default_config(image_width=side, image_height=side, samples_per_pixel=spp, use_statmc=1)
-        jobs.append(eng.Job(ctx, cfg, None, 0, denoiser_spp=spp))
-        jobs[-1].synthesize_records(scs[k])
-    t = time.perf_counter()
-    batch_ms = [eng.denoise_batch(ctx, jobs)]
-    batch_wall = [(time.perf_counter() - t) * 1e3]
+        for j in jobs:
+            j.denoise_records()
+        return (time.perf_counter() - t) * 1e3, jobs
+
+    def batch_run():
+        jobs = fresh_jobs()
+        t = time.perf_counter()
+        dev_ms = eng.denoise_batch(ctx, jobs)
+        return (time.perf_counter() - t) * 1e3, dev_ms, jobs
+
+    print(f"c3: {C3['images']} images, {n_px / 1e6:.1f} Mpx, {n_samples / 1e9:.2f} G samples", file=sys.stderr,
+          flush=True)
+    plain_ms, batch_ms, batch_dev_ms = [], [], []
+    plain = None
+    for rep in range(1 + args.steps):                 # repetition 0 is the warm-up of both paths
+        p_ms, jobs = plain_run()
+        if rep == 0:
+            plain = [j.get("denoised") for j in jobs]
+        else:
+            plain_ms.append(p_ms)
+        for j in jobs:
+            j.release()
+        b_ms, dev, jobs = batch_run()
+        if rep > 0:
+            batch_ms.append(b_ms)
+            batch_dev_ms.append(dev)
+        if rep < args.steps:
+            for j in jobs:
+                j.release()
+        print(f"c3: repetition {rep}: plain {p_ms:.0f} ms, batched {b_ms:.0f} ms", file=sys.stderr, flush=True)
     identical = all(np.array_equal(a, j.get("denoised")) for a, j in zip(plain, jobs))
     err_den = err_in = 0.0
     for j, sc in zip(jobs, scs):
         truth = synth_truth(j.get("albedo") / np.maximum(j.get("hit_count"), 1)[:, None], sc)
-        w = j.w * j.h_px / n_px
-        err_den += w * relmse(j.get("denoised"), truth)
-        err_in += w * relmse(j.get("stats")[:, 3:6], truth)
-    gpu_ms = float(np.median(batch_ms))
+        wgt = j.w * j.h_px / n_px
+        err_den += wgt * relmse(j.get("denoised"), truth)
+        err_in += wgt * relmse(j.get("stats")[:, 3:6], truth)
+    for j in jobs:
+        j.release()
+    batch = float(np.median(batch_ms))
+    plain_med = float(np.median(plain_ms))
     result = {
         "metric": "denoised MP/s, batched denoise of synthetic per-sample buffers",
-        "value": n_px / (gpu_ms / 1e3) / 1e6, "unit": "MP/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
-        "ms_per_step": gpu_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
+        "value": n_px / (batch / 1e3) / 1e6, "unit": "MP/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": batch, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
         "data": "synthetic: counter-based per-sample records generated on device",
         "config": {"workload": f"{C3['images']} images, side U{{{C3['side'][0]}..{C3['side'][1]}}}, "
                                f"spp U{{{C3['spp'][0]}..{C3['spp'][1]}}}, seed {C3['seed']}, StatMC r{radius}",
                    "megapixels": n_px / 1e6, "gsamples": n_samples / 1e9},
-        "plain_ms": plain_ms, "plain_mp_per_s": n_px / (plain_ms / 1e3) / 1e6,
-        "batched_wall_ms": float(np.median(batch_wall)), "speedup_vs_plain": plain_ms / gpu_ms,
+        "timing": "host wall clock around the calls for both paths, median of steps after one warm-up",
+        "plain_ms": plain_med, "plain_mp_per_s": n_px / (plain_med / 1e3) / 1e6,
+        "batched_device_ms": float(np.median(batch_dev_ms)), "speedup_vs_plain": plain_med / batch,
         "identical_to_plain": identical,
         "relmse": {"denoised": err_den, "input_mean": err_in},
         "filter_tile": {"radius": radius, "tile": [16, 16]},
     }
     print(json.dumps(result))
-    for j in jobs:
-        j.release()
     ctx.close()
 
 
@@ -359,6 +402,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-quality", action="store_true", help="skip the relMSE against a high-spp reference")
     ap.add_argument("--size", type=int, default=WORKLOAD["width"])
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy (counter calibration) and exit")
@@ -437,12 +481,20 @@ def main():
     if launches > 0 and stage[7] > 0:
         flops = queries * (cnt["triangles"] * TRI_TEST_FLOPS + cnt["spheres"] * SPHERE_TEST_FLOPS)
         achieved = flops / (stage[7] / 1e3) / 1e12
-        result["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                              "frac": achieved / FP32_PEAK_TFLOPS, "traffic": trace_traffic_bytes(),
-                              "kernel": "k_trace (FP32 VALU, no MFMA; fp32 peak)",
+        avg_launch_s = stage[7] / launches / 1e3
+        traffic = trace_traffic_bytes()
+        result["roofline"] = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+                              "kernel": "k_trace (FP32 vector ALU, no matrix-core work; FP32 vector peak)",
                               "avg_launch_ms": stage[7] / launches,
                               "ray_queries_per_job": queries // args.steps,
                               "ray_queries_per_s": queries / (stage[7] / 1e3)}
+        if traffic:
+            # the same kernel against the HBM roof: counter bytes per launch over its launch time
+            result["roofline"]["hbm_gbs"] = traffic / avg_launch_s / 1e9
+            result["roofline"]["hbm_frac"] = traffic / avg_launch_s / 1e9 / HBM_PEAK_GBS
+    if not args.no_quality:
+        result["quality"] = c1_quality(ctx, cfg, cam, w, h)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result))
