@@ -291,7 +291,8 @@ def run_c3(args):
         "batched_device_ms": float(np.median(batch_dev_ms)), "speedup_vs_plain": plain_med / batch,
         "identical_to_plain": identical,
         "relmse": {"denoised": err_den, "input_mean": err_in},
-        "filter_tile": {"radius": radius, "tile": [16, 16]},
+        "filter_tile": {"radius": radius, "tile": list(ctx.statmc_tile(radius, n_px)),
+                        "chosen_by": "autotuner (every tile shape timed on the first batch)"},
     }
     print(json.dumps(result))
     ctx.close()
@@ -473,6 +474,7 @@ def main():
                      "variance": stage[4] / args.steps, "allocation": stage[5] / args.steps,
                      "trace_kernel": trace_ms_total, "grid_gather": stage[10] / args.steps},
         "samples_per_job": samples // args.steps,
+        "statmc_tile": list(ctx.statmc_tile(WORKLOAD["color_window_radius"], n_px)),
     }
     # roofline of the dominant kernel, the trace kernel: FP32 VALU-bound.
     # Algorithmic work = ray queries x (triangles x TRI_TEST_FLOPS + spheres x

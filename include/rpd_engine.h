@@ -329,6 +329,19 @@ int rpd_scene_bvh_info(const rpd_scene* s, int* nodes, int* depth);
 /* Which builder made the scene's BVH: RPD_BVH_HOST_SAH or RPD_BVH_DEVICE_LBVH. */
 int rpd_scene_bvh_builder(const rpd_scene* scene, int* builder);
 
+/* StatMC window tile shapes (no counterpart in the reference, which filters
+   pixel by pixel on the CPU, main.cpp:998-1127). The window kernel runs one
+   thread per pixel over tiles of one of the shapes rpd_statmc_tile_shapes
+   lists ((w, h) pairs; *count receives how many there are). By default a
+   context tunes: the first StatMC launch of each (window radius, image-size
+   class) times every shape on its own images and keeps the fastest.
+   rpd_ctx_set_statmc_tile fixes one shape instead; 0, 0 returns to tuning.
+   Every shape gives the same bits. rpd_ctx_statmc_tile reports the shape used
+   for a radius and pixel count (0, 0: not tuned yet). */
+int rpd_statmc_tile_shapes(int* wh, int n, int* count);
+int rpd_ctx_set_statmc_tile(rpd_context* ctx, int tile_w, int tile_h);
+int rpd_ctx_statmc_tile(rpd_context* ctx, int radius, int64_t pixels, int* tile_w, int* tile_h);
+
 /* Capture each job's adaptive-pass sequence (variance smoothing, allocation,
    sampling pass) as a hipGraph on its first pass and replay it for every
    pass; results are identical to launching the kernels one by one. */

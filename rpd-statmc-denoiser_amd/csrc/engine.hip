@@ -4,6 +4,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <map>
 #include <mutex>
@@ -138,6 +139,10 @@ struct rpd_context {
     uint64_t cache_bytes = 0;
     uint64_t cache_cap = uint64_t(32) << 30;
     int bvh_builder = RPD_BVH_HOST_SAH;  // for the scenes created next
+    // StatMC window tile shapes: tuned per (radius, image-size class), or forced
+    std::mutex tune_mu;
+    std::map<uint64_t, int> tuned;
+    int forced_shape = -1;
 };
 
 struct rpd_scene {
@@ -387,15 +392,79 @@ int do_recompute_sensitivity(rpd_job* j) {
     return RPD_OK;
 }
 
+// The StatMC window kernel's tile shape for a batch: a shape forced on the
+// context, one tuned earlier for this (largest radius, size class), or one
+// tuned now. Tuning launches the window kernel once per shape on this batch,
+// timed with events, and keeps the fastest; every run computes the whole
+// output with the same bits, so the batch's result is complete whichever run
+// came last. `set_shape` points the batch's BK_TILE row at a shape's tiles.
+// Under stream capture (no events, no waiting) an untuned key takes shape 0.
+uint64_t tune_key(const Batch& b) {
+    int size_class = 0;
+    for (int64_t px = b.total[BK_PIXEL]; px > 1; px >>= 1) ++size_class;
+    return (uint64_t(b.max_radius) << 8) | uint64_t(size_class);
+}
+
+int run_statmc_window(rpd_context* ctx, Batch& b, hipStream_t st, bool can_time,
+                      const std::function<int(Batch&, int)>& set_shape) {
+    int shape = ctx->forced_shape;
+    const uint64_t key = tune_key(b);
+    if (shape < 0) {
+        std::lock_guard<std::mutex> lk(ctx->tune_mu);
+        auto it = ctx->tuned.find(key);
+        if (it != ctx->tuned.end()) shape = it->second;
+    }
+    if (shape < 0 && can_time) {
+        hipEvent_t ev[STATMC_SHAPE_COUNT + 1];
+        for (auto& e : ev) HIP_TRY(event_create(&e));
+        int st2 = RPD_OK;
+        HIP_TRY(hipEventRecord(ev[0], st));
+        for (int k = 0; k < STATMC_SHAPE_COUNT && !st2; ++k) {
+            st2 = set_shape(b, k);
+            launch_batch_queues_reset(b, st);
+            launch_statmc_window(b, st);
+            if (!st2 && hipEventRecord(ev[k + 1], st) != hipSuccess) st2 = fail(RPD_ERR_DEVICE, "hipEventRecord failed");
+        }
+        int best = 0;
+        float best_ms = 0.0f;
+        if (!st2 && hipEventSynchronize(ev[STATMC_SHAPE_COUNT]) == hipSuccess) {
+            for (int k = 0; k < STATMC_SHAPE_COUNT; ++k) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) continue;
+                if (k == 0 || ms < best_ms) { best = k; best_ms = ms; }
+            }
+            std::lock_guard<std::mutex> lk(ctx->tune_mu);
+            ctx->tuned.emplace(key, best);
+        }
+        for (auto& e : ev) event_destroy(e);
+        HIP_TRY(hipGetLastError());
+        return st2;
+    }
+    if (shape < 0) shape = 0;
+    const int st2 = set_shape(b, shape);
+    if (st2) return st2;
+    launch_batch_queues_reset(b, st);
+    launch_statmc_window(b, st);
+    return RPD_OK;
+}
+
 int do_statmc(rpd_job* j) {
     StageTimer tm(j, 3);
     ensure_features(j);
     launch_statmc_prepare(j->batch1, j->stream);
+    int st = RPD_OK;
     {
         StageTimer tw(j, 9);
-        launch_batch_queues_reset(j->batch1, j->stream);
-        launch_statmc_window(j->batch1, j->stream);
+        auto set_shape = [j](Batch& b, int k) -> int {
+            b.tile_w = STATMC_SHAPES[k][0];
+            b.tile_h = STATMC_SHAPES[k][1];
+            b.total[BK_TILE] = image_items(BK_TILE, j->W, j->H, j->geom, b.tile_w, b.tile_h);
+            launch_set_tile_prefix1(b, b.total[BK_TILE], j->stream);
+            return RPD_OK;
+        };
+        st = run_statmc_window(j->ctx, j->batch1, j->stream, !j->capturing, set_shape);
     }
+    if (st) return st;
     HIP_TRY(hipGetLastError());
     return RPD_OK;
 }
@@ -584,6 +653,9 @@ int build_batch1(rpd_job* j, ImageDesc* dd, int64_t* dp, unsigned int* dq) {
         j->batch1.total[k] = pf[2 * k + 1];
     }
     j->batch1.max_step = j->geom.step;
+    j->batch1.tile_w = STATMC_SHAPES[0][0];
+    j->batch1.tile_h = STATMC_SHAPES[0][1];
+    j->batch1.max_radius = j->cfg.color_window_radius;
     HIP_TRY(sync_memcpy(dd, &j->desc, sizeof(ImageDesc), hipMemcpyHostToDevice));
     HIP_TRY(sync_memcpy(dp, pf, sizeof(pf), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(dq, 0, sizeof(unsigned int) * BK_KINDS * BATCH_QUEUES));
@@ -1478,9 +1550,12 @@ int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu
         std::vector<ImageDesc> descs(nb);
         std::vector<int64_t> pf(size_t(BK_KINDS) * (nb + 1), 0);
         Batch b{};
+        b.tile_w = STATMC_SHAPES[0][0];
+        b.tile_h = STATMC_SHAPES[0][1];
         for (int k = 0; k < nb; ++k) {
             descs[k] = batched[k]->desc;
             b.max_step = std::max(b.max_step, descs[k].g.step);
+            b.max_radius = std::max(b.max_radius, descs[k].fc.radius);
         }
         for (int kind = 0; kind < BK_KINDS; ++kind) {
             for (int k = 0; k < nb; ++k)
@@ -1497,8 +1572,25 @@ int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu
         launch_features(b, s);
         launch_sensitivity(b, s);
         launch_statmc_prepare(b, s);
-        launch_batch_queues_reset(b, s);
-        launch_statmc_window(b, s);
+        // the BK_TILE prefix row for a tile shape, copied before the launch that uses it
+        // one host row per launch: each stays alive until the batch has finished
+        std::vector<std::vector<int64_t>> tile_rows;
+        auto set_shape = [&](Batch& bb, int k) -> int {
+            tile_rows.emplace_back(size_t(nb) + 1);
+            std::vector<int64_t>& tile_row = tile_rows.back();
+            bb.tile_w = STATMC_SHAPES[k][0];
+            bb.tile_h = STATMC_SHAPES[k][1];
+            tile_row[0] = 0;
+            for (int m = 0; m < nb; ++m)
+                tile_row[m + 1] = tile_row[m] + image_items(BK_TILE, descs[m].W, descs[m].H, descs[m].g, bb.tile_w,
+                                                            bb.tile_h);
+            bb.total[BK_TILE] = tile_row[nb];
+            HIP_TRY(hipMemcpyAsync(ctx->b_prefix + size_t(BK_TILE) * (nb + 1), tile_row.data(),
+                                   sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice, s));
+            return RPD_OK;
+        };
+        const int st = run_statmc_window(ctx, b, s, true, set_shape);
+        if (st) return st;
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(ctx->b_ev[1], s));
@@ -1516,6 +1608,40 @@ int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu
         if (st) return st;
     }
     if (gpu_ms) *gpu_ms = double(ms);
+    return RPD_OK;
+}
+
+int rpd_statmc_tile_shapes(int* wh, int n, int* count) {
+    if (count) *count = STATMC_SHAPE_COUNT;
+    for (int k = 0; wh && k < n && k < STATMC_SHAPE_COUNT; ++k) {
+        wh[2 * k] = STATMC_SHAPES[k][0];
+        wh[2 * k + 1] = STATMC_SHAPES[k][1];
+    }
+    return RPD_OK;
+}
+
+int rpd_ctx_set_statmc_tile(rpd_context* ctx, int tile_w, int tile_h) {
+    if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
+    if (tile_w == 0 && tile_h == 0) { ctx->forced_shape = -1; return RPD_OK; }
+    const int k = statmc_shape_of(tile_w, tile_h);
+    if (k < 0) return fail(RPD_ERR_INVALID_ARG, "not a StatMC tile shape (see rpd_statmc_tile_shapes)");
+    ctx->forced_shape = k;
+    return RPD_OK;
+}
+
+int rpd_ctx_statmc_tile(rpd_context* ctx, int radius, int64_t pixels, int* tile_w, int* tile_h) {
+    if (!ctx || !tile_w || !tile_h) return fail(RPD_ERR_INVALID_ARG, "null argument");
+    Batch b{};
+    b.max_radius = radius;
+    b.total[BK_PIXEL] = pixels;
+    int k = ctx->forced_shape;
+    if (k < 0) {
+        std::lock_guard<std::mutex> lk(ctx->tune_mu);
+        auto it = ctx->tuned.find(tune_key(b));
+        if (it != ctx->tuned.end()) k = it->second;
+    }
+    *tile_w = k < 0 ? 0 : STATMC_SHAPES[k][0];
+    *tile_h = k < 0 ? 0 : STATMC_SHAPES[k][1];
     return RPD_OK;
 }
 

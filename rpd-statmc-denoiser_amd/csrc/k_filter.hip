@@ -147,7 +147,27 @@ __global__ void k_filter_pack(Batch b) {
 // formed once per block; larger windows compute it in the loop
 constexpr int STATMC_SP_MAX = 2 * 24 * 24;
 
-__device__ void statmc_pixel(const ImageDesc& d, const int x, const int y, const double* sp_tab) {
+// The gate inputs of the window (P0: unit normal and depth, P1: albedo and n)
+// of a tile and its halo, staged in LDS when they fit; g0 null reads them from
+// the packed planes in global memory.
+struct GateTile {
+    const float4* g0;
+    const float4* g1;
+    int span_w;      // row pitch of the staged rectangle
+    int ox, oy;      // image coordinates of its first element
+};
+
+// Two operations of the reference's per-channel loop are replaced by ones that
+// give the same bits:
+//  * the t critical value depends on df only through float(df), and is the
+//    constant z once float(df) >= 30.5 (main.cpp:216-230). df = vs^2 / den is
+//    divided in FP64 only when an FP32 comparison vs^2 >= 31 den (a few float
+//    roundings, < 1e-6 relative, against a 1.6% margin) cannot show df >= 31;
+//  * when every channel's critical value is the same tc, the channel maximum of
+//    RN(t2_c / RN(tc tc)) is RN(max_c t2_c / RN(tc tc)): correctly rounded
+//    division by one divisor is monotone. One FP64 division instead of three.
+__device__ __forceinline__ void statmc_pixel(const ImageDesc& d, const int x, const int y, const double* sp_tab,
+                                             const GateTile& gt) {
     const int W = d.W, H = d.H, n = d.N;
     const FilterConfig fc = d.fc;
     const float4* __restrict__ P = reinterpret_cast<const float4*>(d.filter_scratch + 4 * size_t(n));
@@ -177,6 +197,7 @@ __device__ void statmc_pixel(const ImageDesc& d, const int x, const int y, const
         else if (a <= 0.10f) z = 1.64f;
         else z = 1.28f;
     }
+    const double zz = double(z) * double(z);
     const v3 ni = mk3(p0.x, p0.y, p0.z);
     const float di = p0.w;
     const float4 p1 = P[n + i], p2 = P[2 * n + i], p3 = P[3 * n + i], p4 = P[4 * n + i], p5 = P[5 * n + i],
@@ -206,29 +227,56 @@ __device__ void statmc_pixel(const ImageDesc& d, const int x, const int y, const
             const int xx = x + dx;
             if (xx < 0 || xx >= W) continue;
             const int q = yy * W + xx;
-            const float4 q0 = P[q];
-            if (q0.w < 0.0f) continue;
-            if (dot3(ni, mk3(q0.x, q0.y, q0.z)) < fc.normal_threshold) continue;
-            if (relative_depth_delta(di, q0.w) > fc.depth_threshold) continue;
-            const float4 q1 = P[n + q];
+            float4 q0, q1;
+            if (gt.g0) {
+                const int k = (yy - gt.oy) * gt.span_w + (xx - gt.ox);
+                q0 = gt.g0[k];
+                if (q0.w < 0.0f) continue;
+                if (dot3(ni, mk3(q0.x, q0.y, q0.z)) < fc.normal_threshold) continue;
+                if (relative_depth_delta(di, q0.w) > fc.depth_threshold) continue;
+                q1 = gt.g1[k];
+            } else {
+                q0 = P[q];
+                if (q0.w < 0.0f) continue;
+                if (dot3(ni, mk3(q0.x, q0.y, q0.z)) < fc.normal_threshold) continue;
+                if (relative_depth_delta(di, q0.w) > fc.depth_threshold) continue;
+                q1 = P[n + q];
+            }
             if (norm3(ai - mk3(q1.x, q1.y, q1.z)) > 0.12f) continue;
             const float4 q2 = P[2 * n + q], q3 = P[3 * n + q];
             const int n_j = int(q1.w);
             const double2 sj01 = D[q], sj2 = D[n + q];
             const double df_j[3] = {sj01.x, sj01.y, sj2.x};
-            double mx = 0.0;
             const float smi[3] = {p2.x, p2.y, p2.z}, smj[3] = {q2.x, q2.y, q2.z};
             const float vsi[3] = {p3.x, p3.y, p3.z}, vsj[3] = {q3.x, q3.y, q3.z};
+            double t2[3], tcs[3];
+            bool all_z = true;
 #pragma unroll
             for (int ch = 0; ch < 3; ++ch) {
                 const double vi = double(vsi[ch]), vj = double(vsj[ch]);
                 const double vs = vi + vj;
                 const double md = double(smi[ch] - smj[ch]);
-                const double t2 = md * md / fmax(double(RPD_EPS_SMALL), vs);
+                t2[ch] = md * md / fmax(double(RPD_EPS_SMALL), vs);
                 const double den_df = df_i[ch] + df_j[ch];
-                const double df = (den_df > double(RPD_EPS_SMALL)) ? (vs * vs) / den_df : double(n_i + n_j - 2);
-                const double tc = double(t_critical(float(df), table, z));
-                mx = fmax(mx, t2 / (tc * tc));
+                tcs[ch] = double(z);
+                if (den_df > double(RPD_EPS_SMALL)) {
+                    const float vsf = float(vs);
+                    if (!(vsf * vsf >= 31.0f * float(den_df))) {
+                        tcs[ch] = double(t_critical(float((vs * vs) / den_df), table, z));
+                        all_z = false;
+                    }
+                } else {
+                    tcs[ch] = double(t_critical(float(double(n_i + n_j - 2)), table, z));
+                    all_z = false;
+                }
+            }
+            double mx;
+            if (all_z) {
+                mx = fmax(fmax(t2[0], t2[1]), t2[2]) / zz;
+            } else {
+                mx = 0.0;
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) mx = fmax(mx, t2[ch] / (tcs[ch] * tcs[ch]));
             }
             const float4 q5 = P[5 * n + q], q6 = P[6 * n + q];
             const float cj[6] = {q5.x, q5.y, q5.z, q5.w, q6.x, q6.y};
@@ -268,14 +316,30 @@ __device__ void statmc_pixel(const ImageDesc& d, const int x, const int y, const
     alp[i] = a;
 }
 
-// Persistent StatMC: blocks take 16x16 tiles of any image of the batch from
-// the XCD-partitioned work ranges; one thread per pixel of the tile.
-__global__ void __launch_bounds__(256) k_statmc(Batch b) {
+// Tile shapes of the window kernel: TW x TH pixels, one thread each. Every
+// shape computes every pixel with the same operations, so all give the same
+// bits; which runs fastest depends on the window radius and the image sizes,
+// and the engine's tuner picks one by timing them.
+const int STATMC_SHAPES[STATMC_SHAPE_COUNT][2] = {{16, 16}, {32, 8}, {8, 32}, {64, 4}, {16, 8}, {8, 16}, {8, 8}};
+
+// LDS for the staged gate inputs of one tile and halo: two float4 per pixel.
+__host__ __device__ inline size_t statmc_gate_bytes(int tw, int th, int radius) {
+    return size_t(tw + 2 * radius) * size_t(th + 2 * radius) * 2 * sizeof(float4);
+}
+constexpr size_t STATMC_GATE_LDS_MAX = 48 * 1024;   // larger halos read the gate inputs from global memory
+
+// Persistent StatMC: blocks take TW x TH tiles of any image of the batch from
+// the XCD-partitioned work ranges; one thread per pixel of the tile. With
+// `stage`, the block first copies the gate inputs of the tile and its halo
+// (every radius of the batch fits in the dynamic LDS it was given) to LDS.
+template <int TW, int TH>
+__global__ void __launch_bounds__(TW * TH) k_statmc(Batch b, int stage) {
     __shared__ int64_t s_item;
     __shared__ double s_sp[STATMC_SP_MAX + 1];
+    extern __shared__ float4 s_gate[];
     for (int k = threadIdx.x; k <= STATMC_SP_MAX; k += blockDim.x) s_sp[k] = 1.0 / (1.0 + double(k));
     int cursor = 0;
-    const int tx = threadIdx.x & (STATMC_TILE - 1), ty = threadIdx.x / STATMC_TILE;
+    const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
     while (true) {
         if (threadIdx.x == 0) s_item = batch_take(b, BK_TILE, cursor);
         __syncthreads();
@@ -284,9 +348,28 @@ __global__ void __launch_bounds__(256) k_statmc(Batch b) {
         if (item < 0) break;
         int64_t lt;
         const ImageDesc& d = b.img[batch_find(b, BK_TILE, item, lt)];
-        const int tiles_x = (d.W + STATMC_TILE - 1) / STATMC_TILE;
-        const int x = int(lt % tiles_x) * STATMC_TILE + tx, y = int(lt / tiles_x) * STATMC_TILE + ty;
-        if (x < d.W && y < d.H) statmc_pixel(d, x, y, s_sp);
+        const int tiles_x = (d.W + TW - 1) / TW;
+        const int x0 = int(lt % tiles_x) * TW, y0 = int(lt / tiles_x) * TH;
+        const int x = x0 + tx, y = y0 + ty;
+        GateTile gt;
+        gt.g0 = nullptr; gt.g1 = nullptr; gt.span_w = 0; gt.ox = 0; gt.oy = 0;
+        if (stage) {
+            const int r = d.fc.radius;
+            const int sw = TW + 2 * r, sh = TH + 2 * r, cnt = sw * sh;
+            const float4* P = reinterpret_cast<const float4*>(d.filter_scratch + 4 * size_t(d.N));
+            for (int k = threadIdx.x; k < cnt; k += TW * TH) {
+                const int gx = x0 - r + k % sw, gy = y0 - r + k / sw;
+                if (gx >= 0 && gx < d.W && gy >= 0 && gy < d.H) {
+                    const int q = gy * d.W + gx;
+                    s_gate[k] = P[q];
+                    s_gate[cnt + k] = P[d.N + q];
+                }
+            }
+            __syncthreads();
+            gt.g0 = s_gate; gt.g1 = s_gate + cnt; gt.span_w = sw; gt.ox = x0 - r; gt.oy = y0 - r;
+        }
+        if (x < d.W && y < d.H) statmc_pixel(d, x, y, s_sp, gt);
+        if (stage) __syncthreads();   // the next tile's copy overwrites the staged inputs
     }
 }
 
@@ -295,11 +378,45 @@ void launch_statmc_prepare(const Batch& b, hipStream_t st) {
     hipLaunchKernelGGL(k_sqrt_var_mean, dim3((np + 255) / 256), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_filter_pack, dim3((np + 255) / 256), dim3(256), 0, st, b);
 }
-void launch_statmc_window(const Batch& b, hipStream_t st) {
-    static const int cap = device_resident_blocks(reinterpret_cast<const void*>(&k_statmc), 256);
-    const int64_t blocks = std::min<int64_t>(b.total[BK_TILE], cap);
+
+template <int TW, int TH>
+static void launch_statmc_t(const Batch& b, hipStream_t st) {
+    // resident blocks over the device (a function-local static is initialised once
+    // even when several job threads launch at the same time), counted without the
+    // staged inputs' LDS; a launch that stages them keeps to what fits beside it
+    static const int cap = device_resident_blocks(reinterpret_cast<const void*>(&k_statmc<TW, TH>), TW * TH);
+    const size_t lds = statmc_gate_bytes(TW, TH, b.max_radius);
+    const int stage = b.max_radius > 0 && lds <= STATMC_GATE_LDS_MAX ? 1 : 0;
+    int per_cu = 0, dev = 0, cus = 1;
+    if (stage && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_statmc<TW, TH>),
+                                                              TW * TH, lds) == hipSuccess && per_cu > 0) {
+        hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
+    } else {
+        per_cu = 0;
+    }
+    const int64_t limit = stage && per_cu > 0 ? int64_t(per_cu) * cus : cap;
+    const int64_t blocks = std::min<int64_t>(b.total[BK_TILE], limit);
     if (blocks <= 0) return;
-    hipLaunchKernelGGL(k_statmc, dim3(unsigned(blocks)), dim3(256), 0, st, b);
+    hipLaunchKernelGGL((k_statmc<TW, TH>), dim3(unsigned(blocks)), dim3(TW * TH), stage ? lds : 0, st, b, stage);
+}
+
+int statmc_shape_of(int tile_w, int tile_h) {
+    for (int k = 0; k < STATMC_SHAPE_COUNT; ++k)
+        if (STATMC_SHAPES[k][0] == tile_w && STATMC_SHAPES[k][1] == tile_h) return k;
+    return -1;
+}
+
+void launch_statmc_window(const Batch& b, hipStream_t st) {
+    switch (statmc_shape_of(b.tile_w, b.tile_h)) {
+        case 1: launch_statmc_t<32, 8>(b, st); break;
+        case 2: launch_statmc_t<8, 32>(b, st); break;
+        case 3: launch_statmc_t<64, 4>(b, st); break;
+        case 4: launch_statmc_t<16, 8>(b, st); break;
+        case 5: launch_statmc_t<8, 16>(b, st); break;
+        case 6: launch_statmc_t<8, 8>(b, st); break;
+        default: launch_statmc_t<16, 16>(b, st); break;
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -43,6 +43,14 @@ void launch_features(const Batch& b, hipStream_t st) {
 __global__ void k_queues_zero(unsigned int* q) {
     if (threadIdx.x < BK_KINDS * BATCH_QUEUES) q[threadIdx.x] = 0u;
 }
+__global__ void k_set_prefix_row(int64_t* row, int64_t total) {
+    if (threadIdx.x == 0) { row[0] = 0; row[1] = total; }
+}
+void launch_set_tile_prefix1(const Batch& b, int64_t tiles, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_prefix_row, dim3(1), dim3(64), 0, st, const_cast<int64_t*>(b.prefix) + BK_TILE * 2,
+                       tiles);
+}
+
 void launch_batch_queues_reset(const Batch& b, hipStream_t st) {
     hipLaunchKernelGGL(k_queues_zero, dim3(1), dim3(64), 0, st, b.queues);
 }

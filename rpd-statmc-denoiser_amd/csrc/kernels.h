@@ -145,8 +145,12 @@ struct ImageDesc {
     float conf_n0;
 };
 enum : int { BK_PIXEL = 0, BK_NODE = 1, BK_TILE = 2, BK_KINDS = 3 };
-constexpr int STATMC_TILE = 16;     // StatMC tiles are 16x16 pixels
 constexpr int BATCH_QUEUES = 8;     // one work range per XCD for persistent kernels
+// StatMC window tile shapes (width, height): STATMC_SHAPES[k], k < STATMC_SHAPE_COUNT;
+// shape 0 (16 x 16) is the default
+constexpr int STATMC_SHAPE_COUNT = 7;
+extern const int STATMC_SHAPES[STATMC_SHAPE_COUNT][2];
+int statmc_shape_of(int tile_w, int tile_h);   // index of a shape, -1 if it is not one
 struct Batch {
     const ImageDesc* img;            // device array, n entries
     const int64_t* prefix;           // device: BK_KINDS rows of n + 1 exclusive prefix sums
@@ -154,12 +158,16 @@ struct Batch {
     int n;
     int64_t total[BK_KINDS];         // host copies of the last prefix entries (launch sizing)
     int max_step;                    // host: largest grid step of the images (0: not known)
+    int tile_w, tile_h;              // StatMC tile shape the BK_TILE row counts
+    int max_radius;                  // host: largest StatMC window radius of the images
 };
-__host__ __device__ inline int64_t image_items(int kind, int W, int H, GridGeom g) {
+__host__ __device__ inline int64_t image_items(int kind, int W, int H, GridGeom g, int tile_w = 16, int tile_h = 16) {
     return kind == BK_PIXEL ? int64_t(W) * H
          : kind == BK_NODE  ? int64_t(g.nx) * g.ny
-                            : int64_t((W + STATMC_TILE - 1) / STATMC_TILE) * ((H + STATMC_TILE - 1) / STATMC_TILE);
+                            : int64_t((W + tile_w - 1) / tile_w) * ((H + tile_h - 1) / tile_h);
 }
+// batch of one: sets its BK_TILE prefix row to {0, tiles} on the stream
+void launch_set_tile_prefix1(const Batch& b, int64_t tiles, hipStream_t st);
 // image of global item `item` of `kind` and its local index
 __device__ __forceinline__ int batch_find(const Batch& b, int kind, int64_t item, int64_t& local) {
     const int64_t* pf = b.prefix + kind * (b.n + 1);

@@ -212,6 +212,9 @@ def lib():
             ("rpd_job_denoise_records", [vp]),
             ("rpd_denoise_batch", [vp, vp, C.c_int, C.POINTER(C.c_double)]),
             ("rpd_ctx_set_graph_passes", [vp, C.c_int]),
+            ("rpd_statmc_tile_shapes", [vp, C.c_int, C.POINTER(C.c_int)]),
+            ("rpd_ctx_set_statmc_tile", [vp, C.c_int, C.c_int]),
+            ("rpd_ctx_statmc_tile", [vp, C.c_int, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             ("rpd_abi_struct_sizes", [vp, C.c_int]),
             ("rpd_image_metrics_compute", [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.POINTER(ImageMetrics)]),
             ("rpd_image_metrics_compute_noalloc", [vp, vp, vp, C.c_int, C.c_int, C.POINTER(ImageMetrics)]),
@@ -293,6 +296,16 @@ class Context:
             check(self.L.rpd_scene_procedural(self.h, C.byref(pc), C.byref(s)))
             self._scenes[name] = s
         return self._scenes[name]
+
+    def set_statmc_tile(self, tile_w, tile_h):
+        """Fix the StatMC window tile shape; (0, 0) returns to tuning."""
+        check(self.L.rpd_ctx_set_statmc_tile(self.h, int(tile_w), int(tile_h)))
+
+    def statmc_tile(self, radius, pixels):
+        """The tile shape used for this radius and pixel count, (0, 0) if not tuned yet."""
+        w, h = C.c_int(), C.c_int()
+        check(self.L.rpd_ctx_statmc_tile(self.h, int(radius), int(pixels), C.byref(w), C.byref(h)))
+        return w.value, h.value
 
     def set_graph_passes(self, on):
         check(self.L.rpd_ctx_set_graph_passes(self.h, int(on)))
@@ -380,6 +393,15 @@ def atrous(ctx, raw, normal, albedo, **kw):
     h, w = raw.shape[:2]
     check(lib().rpd_atrous(ctx.h, _ptr(raw), _ptr(normal), _ptr(albedo), w, h, C.byref(cfg), _ptr(out), 0))
     return out
+
+
+def statmc_tile_shapes():
+    """The StatMC window kernel's tile shapes, [(w, h), ...]."""
+    n = C.c_int()
+    check(lib().rpd_statmc_tile_shapes(None, 0, C.byref(n)))
+    wh = np.zeros(2 * n.value, np.int32)
+    check(lib().rpd_statmc_tile_shapes(_ptr(wh), n.value, C.byref(n)))
+    return [(int(wh[2 * k]), int(wh[2 * k + 1])) for k in range(n.value)]
 
 
 def procedural_config(**kw):
