@@ -204,7 +204,14 @@ enum rpd_buffer_id {
     RPD_BUF_GRID = 62,             /* 440 doubles/node (RPF grid accumulators) */
     RPD_BUF_TILES = 63,            /* 18 planes of node floats (SensitivityTiles fields) */
     RPD_BUF_FEATURES = 64,         /* 7 floats/pixel: unit normal[3], depth, albedo[3] */
-    RPD_BUF_RECORDS = 65           /* last pass: 32 floats per sample (read only) */
+    RPD_BUF_RECORDS = 65,          /* last pass: 32 floats per sample (read only) */
+    /* debug_statmc_outputs (main.cpp:991-1157, 1283-1321), kept only for jobs
+       created after rpd_ctx_set_debug_outputs(ctx, 1): per pixel of the last
+       StatMC filter the accepted neighbours, weight sum and effective
+       neighbours (0 without a primary hit), and the clamped variance source of
+       the last adaptive allocation. Floats per pixel. */
+    RPD_BUF_DEBUG_NEIGHBORS = 70, RPD_BUF_DEBUG_WEIGHT_SUM = 71, RPD_BUF_DEBUG_NEFF = 72,
+    RPD_BUF_DEBUG_VARIANCE_SOURCE = 73
 };
 /* Size in bytes of a buffer for this job (RECORDS: of the last pass). */
 int rpd_job_buffer_size(rpd_job* job, int buffer_id, size_t* bytes);
@@ -341,6 +348,10 @@ int rpd_scene_bvh_builder(const rpd_scene* scene, int* builder);
 int rpd_statmc_tile_shapes(int* wh, int n, int* count);
 int rpd_ctx_set_statmc_tile(rpd_context* ctx, int tile_w, int tile_h);
 int rpd_ctx_statmc_tile(rpd_context* ctx, int radius, int64_t pixels, int* tile_w, int* tile_h);
+
+/* Jobs created after this call with on != 0 keep the debug buffers of the
+   reference's debug_statmc_outputs (RPD_BUF_DEBUG_*); 0 stops it. */
+int rpd_ctx_set_debug_outputs(rpd_context* ctx, int on);
 
 /* Capture each job's adaptive-pass sequence (variance smoothing, allocation,
    sampling pass) as a hipGraph on its first pass and replay it for every

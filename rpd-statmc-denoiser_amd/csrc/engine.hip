@@ -143,6 +143,7 @@ struct rpd_context {
     std::mutex tune_mu;
     std::map<uint64_t, int> tuned;
     int forced_shape = -1;
+    bool debug_outputs = false;  // jobs created next keep the StatMC and allocation debug planes
 };
 
 struct rpd_scene {
@@ -189,6 +190,7 @@ struct rpd_job {
     double* grid = nullptr;
     float *tiles = nullptr, *node_scratch = nullptr, *filter_scratch = nullptr;
     void* alloc_scratch = nullptr;
+    float* dbg = nullptr;    // debug planes (neighbours, weight sum, n_eff, variance source), or null
     bool denoiser = false;   // created for denoising records: no scene, camera or tape
     bool capturing = false;  // the stream is being captured into a graph: no per-stage timing events
     int max_spp = 0;         // record capacity per pixel of a denoiser job
@@ -494,6 +496,7 @@ int do_allocation(rpd_job* j) {
     ac.radius = j->cfg.adaptive_importance_smoothing_radius;
     ac.sigma_max = j->cfg.adaptive_sigma_max;
     ac.seed = j->seed;
+    ac.dbg_variance_source = j->dbg ? j->dbg + 3 * size_t(j->N) : nullptr;
     launch_importance(j->W, j->H, ac, j->pb, j->feat, j->vden, j->imp, j->alloc_scratch, j->stream);
     launch_alloc_counts(j->N, ac, j->imp, j->counts, j->alloc_scratch, j->stream);
     launch_plan_scan(j->N, j->counts, j->offsets, j->active, j->totals, j->scan_scratch, j->rec.capacity, j->stream);
@@ -508,6 +511,7 @@ AllocConfig alloc_config(const rpd_job* j) {
     ac.radius = j->cfg.adaptive_importance_smoothing_radius;
     ac.sigma_max = j->cfg.adaptive_sigma_max;
     ac.seed = j->seed;
+    ac.dbg_variance_source = nullptr;
     return ac;
 }
 
@@ -534,6 +538,7 @@ int reset_state(rpd_job* j) {
     HIP_TRY(hipMemsetAsync(j->tiles, 0, sizeof(float) * SF_COUNT * j->geom.nx * j->geom.ny, j->stream));
     HIP_TRY(hipMemsetAsync(j->counts, 0, sizeof(int) * (3 * N + 8), j->stream));  // plan, totals, work counter
     HIP_TRY(hipMemsetAsync(j->query_count, 0, sizeof(unsigned long long) * 2, j->stream));
+    if (j->dbg) HIP_TRY(hipMemsetAsync(j->dbg, 0, sizeof(float) * 4 * N, j->stream));
     j->features_valid = false;
     j->finalized = false;
     return RPD_OK;
@@ -641,6 +646,7 @@ ImageDesc make_desc(const rpd_job* j) {
     d.fc.sigma_max = j->cfg.color_sigma_max;
     d.fc.rpd_scale = j->cfg.rpf_shrinkage_scale;
     d.conf_n0 = j->cfg.rpf_confidence_samples;
+    d.dbg = j->dbg;
     return d;
 }
 
@@ -755,6 +761,7 @@ int job_alloc(rpd_job* j) {
     A(j->filter_scratch, FILTER_SCRATCH_FLOATS * N);
     char* as; A(as, alloc_scratch_bytes(j->N));
     j->alloc_scratch = as;
+    if (j->ctx->debug_outputs) A(j->dbg, 4 * N);
     ImageDesc* dd; A(dd, 1);
     int64_t* dp; A(dp, BK_KINDS * 2);
     unsigned int* dq; A(dq, BK_KINDS * BATCH_QUEUES);
@@ -1645,6 +1652,12 @@ int rpd_ctx_statmc_tile(rpd_context* ctx, int radius, int64_t pixels, int* tile_
     return RPD_OK;
 }
 
+int rpd_ctx_set_debug_outputs(rpd_context* ctx, int on) {
+    if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
+    ctx->debug_outputs = on != 0;
+    return RPD_OK;
+}
+
 int rpd_ctx_set_graph_passes(rpd_context* ctx, int on) {
     if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
     ctx->graph_passes = on != 0;
@@ -2020,6 +2033,12 @@ int rpd_job_buffer_size(rpd_job* j, int id, size_t* bytes) {
             return RPD_OK;
         }
         default:
+            if (id >= RPD_BUF_DEBUG_NEIGHBORS && id <= RPD_BUF_DEBUG_VARIANCE_SOURCE) {
+                if (!j->dbg) return fail(RPD_ERR_STATE, "debug buffers are kept only for jobs created after "
+                                                        "rpd_ctx_set_debug_outputs(ctx, 1)");
+                *bytes = 4 * N;
+                return RPD_OK;
+            }
             if ((id >= RPD_BUF_DEPTH && id <= RPD_BUF_SAMPLE_FRACTION) || (id >= RPD_BUF_SENS_ALL && id <= RPD_BUF_SENS_RR) ||
                 (id >= RPD_BUF_SAMPLE_COUNT && id <= RPD_BUF_PASS_COUNTS)) {
                 *bytes = 4 * N;
@@ -2061,6 +2080,10 @@ int rpd_job_read(rpd_job* j, int id, void* dst, size_t bytes) {
         case RPD_BUF_SENSITIVITY_CONFIDENCE: return scalar(j->sconf);
         case RPD_BUF_SENSITIVITY_GRADIENT: return scalar(j->sgrad);
         case RPD_BUF_ADAPTIVE_IMPORTANCE: return scalar(j->imp);
+        case RPD_BUF_DEBUG_NEIGHBORS: return scalar(j->dbg);
+        case RPD_BUF_DEBUG_WEIGHT_SUM: return scalar(j->dbg + N);
+        case RPD_BUF_DEBUG_NEFF: return scalar(j->dbg + 2 * N);
+        case RPD_BUF_DEBUG_VARIANCE_SOURCE: return scalar(j->dbg + 3 * N);
         case RPD_BUF_SAMPLE_COUNT: HIP_TRY(sync_memcpy(dst, j->pb.sample_count, 4 * N, hipMemcpyDeviceToHost)); return RPD_OK;
         case RPD_BUF_HIT_COUNT: HIP_TRY(sync_memcpy(dst, j->pb.hit_count, 4 * N, hipMemcpyDeviceToHost)); return RPD_OK;
         case RPD_BUF_PASS_COUNTS: HIP_TRY(sync_memcpy(dst, j->counts, 4 * N, hipMemcpyDeviceToHost)); return RPD_OK;

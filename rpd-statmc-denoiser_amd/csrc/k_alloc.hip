@@ -25,7 +25,7 @@
 namespace rpd {
 
 // main.cpp:1305-1328: variance source and sqrt importance
-__global__ void k_importance(int n, PixelBuffers pb, const float* vden, float smax, float* imp) {
+__global__ void k_importance(int n, PixelBuffers pb, const float* vden, float smax, float* imp, float* dbg_src) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int cnt = pb.n[i];
@@ -34,7 +34,9 @@ __global__ void k_importance(int n, PixelBuffers pb, const float* vden, float sm
     raw = fmaxf(RPD_EPS_SMALL, raw);
     float d = vden[i];
     if (!isfinite(d) || d <= 0.0f) d = raw;
-    imp[i] = sqrtf(clampf(fmaxf(raw, d), RPD_EPS_SMALL, smax * smax));
+    const float src = clampf(fmaxf(raw, d), RPD_EPS_SMALL, smax * smax);
+    if (dbg_src) dbg_src[i] = src;
+    imp[i] = sqrtf(src);
 }
 
 // main.cpp:1331-1368: edge-aware smoothing
@@ -604,7 +606,7 @@ void launch_importance(int W, int H, const AllocConfig& ac, const PixelBuffers& 
     const Carved c = carve(scratch, n);
     const dim3 grid((n + 255) / 256);
     if (ac.adaptive_spp <= 0) return;
-    hipLaunchKernelGGL(k_importance, grid, dim3(256), 0, st, n, pb, vden, ac.sigma_max, imp);
+    hipLaunchKernelGGL(k_importance, grid, dim3(256), 0, st, n, pb, vden, ac.sigma_max, imp, ac.dbg_variance_source);
     if (ac.radius > 0) {
         hipLaunchKernelGGL(k_importance_smooth, grid, dim3(256), 0, st, W, H, ac.radius, pb, f, imp, c.tmp);
         hipMemcpyAsync(imp, c.tmp, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st);

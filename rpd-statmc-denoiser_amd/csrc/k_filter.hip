@@ -314,6 +314,11 @@ __device__ __forceinline__ void statmc_pixel(const ImageDesc& d, const int x, co
     unc[i] = float(double(a) * double(a) * double(vm_i) + blend * blend * vn +
                    2.0 * double(a) * blend * (1.0 / fmax(ws, double(RPD_EPS_SMALL))) * double(vm_i));
     alp[i] = a;
+    if (d.dbg) {
+        d.dbg[i] = float(accepted);
+        d.dbg[n + i] = float(ws);
+        d.dbg[2 * n + i] = float(neff);
+    }
 }
 
 // Tile shapes of the window kernel: TW x TH pixels, one thread each. Every

@@ -143,6 +143,8 @@ struct ImageDesc {
     float *den, *unc, *alp, *vtot, *veff, *vrat;
     FilterConfig fc;
     float conf_n0;
+    float* dbg;              // null, or 3 planes the window filter fills per pixel with a primary hit:
+                             // accepted neighbours, weight sum, effective neighbours (main.cpp:991-1108)
 };
 enum : int { BK_PIXEL = 0, BK_NODE = 1, BK_TILE = 2, BK_KINDS = 3 };
 constexpr int BATCH_QUEUES = 8;     // one work range per XCD for persistent kernels
@@ -247,6 +249,7 @@ struct AllocConfig {
     int adaptive_spp, base_samples, radius;
     float sigma_max;
     uint32_t seed;
+    float* dbg_variance_source;   // null, or a plane for the clamped variance source (main.cpp:1303-1321)
 };
 // importance field (main.cpp:1295-1371) into `importance`
 void launch_importance(int width, int height, const AllocConfig& ac, const PixelBuffers& pb, const Features& f,

@@ -281,10 +281,10 @@ int main(int argc, char** argv) {
     INFO("Color window radius: %d", r.color_window_radius);
     INFO("Sample-variance window radius: %d", r.var_window_radius);
     INFO("Tonemap: %s", tonemap_name.c_str());
-    if (st.debug_statmc_outputs) INFO("debug_statmc_outputs: per-neighbour debug images are not produced by this engine");
 
     rpd_context* ctx = nullptr;
     CHECK(rpd_ctx_create(0, &ctx));
+    if (statmc && st.debug_statmc_outputs) CHECK(rpd_ctx_set_debug_outputs(ctx, 1));
     rpd_scene* sc = nullptr;
     if (rpd_scene_builtin(ctx, scene.c_str(), &sc) != 0) {
         ERROR("Failed to load scene '%s': %s", scene.c_str(), rpd_last_error());
@@ -392,6 +392,59 @@ int main(int argc, char** argv) {
         scalar(RPD_BUF_VAR_TOTAL, "_var_total");
         scalar(RPD_BUF_VAR_EFF, "_var_eff");
         scalar(RPD_BUF_VAR_RATIO, "_var_ratio");
+    }
+    // debug_statmc_outputs: written to the working directory, as statmc_denoise
+    // (main.cpp:1129-1157) and compute_adaptive_sample_counts (1283-1289,
+    // 1319-1321) do on their last call
+    if (statmc && st.debug_statmc_outputs) {
+        auto debug_scalar = [&](int id, const char* name) {
+            if (rpd_job_read(job, id, sc1.data(), sc1.size() * 4) != 0) { ok = false; return; }
+            ok &= host::write_hdr_scalar(name, W, H, sc1.data());
+        };
+        debug_scalar(RPD_BUF_DEBUG_NEIGHBORS, "debug_neighbors.hdr");
+        debug_scalar(RPD_BUF_ALPHA, "debug_alpha.hdr");
+        debug_scalar(RPD_BUF_VAR_TOTAL, "debug_var_total.hdr");
+        debug_scalar(RPD_BUF_VAR_EFF, "debug_var_eff.hdr");
+        debug_scalar(RPD_BUF_VAR_RATIO, "debug_var_ratio.hdr");
+        debug_scalar(RPD_BUF_DEBUG_WEIGHT_SUM, "debug_weight_sum.hdr");
+        debug_scalar(RPD_BUF_DEBUG_NEFF, "debug_neff.hdr");
+        debug_scalar(RPD_BUF_SENS_ALL, "debug_sensitivity_all.hdr");
+        debug_scalar(RPD_BUF_SENS_PIXEL, "debug_sensitivity_pixel.hdr");
+        debug_scalar(RPD_BUF_SENS_BRDF, "debug_sensitivity_brdf.hdr");
+        debug_scalar(RPD_BUF_SENS_LENS, "debug_sensitivity_lens.hdr");
+        debug_scalar(RPD_BUF_SENS_LIGHT, "debug_sensitivity_light.hdr");
+        debug_scalar(RPD_BUF_SENS_LIGHT_UV, "debug_sensitivity_light_uv.hdr");
+        debug_scalar(RPD_BUF_SENS_LIGHT_SELECT, "debug_sensitivity_light_select.hdr");
+        debug_scalar(RPD_BUF_SENS_ENVIRONMENT, "debug_sensitivity_environment.hdr");
+        debug_scalar(RPD_BUF_SENS_RR, "debug_sensitivity_rr.hdr");
+        debug_scalar(RPD_BUF_SENSITIVITY_CONFIDENCE, "debug_sensitivity_confidence.hdr");
+        debug_scalar(RPD_BUF_SENSITIVITY_GRADIENT, "debug_sensitivity_gradient.hdr");
+        // SensitivityState::reliable / reliable_denoise (main.cpp:297-304) of every pixel
+        std::vector<float> all(N), grad(N), rel(N), rel_den(N);
+        if (rpd_job_read(job, RPD_BUF_SENS_ALL, all.data(), N * 4) != 0 ||
+            rpd_job_read(job, RPD_BUF_SENSITIVITY_GRADIENT, grad.data(), N * 4) != 0) {
+            ok = false;
+        } else {
+            for (size_t i = 0; i < N; ++i) {
+                const float a = std::min(1.0f, std::max(0.0f, all[i]));
+                const float g = std::min(1.0f, std::max(0.0f, grad[i]));
+                rel[i] = a;
+                rel_den[i] = std::min(1.0f, std::max(0.0f, a * (1.0f - g)));
+            }
+            ok &= host::write_hdr_scalar("debug_reliability_base.hdr", W, H, rel.data());
+            ok &= host::write_hdr_scalar("debug_reliability_denoise.hdr", W, H, rel_den.data());
+        }
+        if (r.adaptive_passes > 0) {
+            // with adaptive_spp 0 the allocation returns its floor of 0 before any pass
+            std::vector<int> counts(N, 0);
+            if (r.adaptive_spp > 0 && rpd_job_read(job, RPD_BUF_PASS_COUNTS, counts.data(), N * 4) != 0) {
+                ok = false;
+            } else {
+                for (size_t i = 0; i < N; ++i) sc1[i] = float(counts[i]);
+                ok &= host::write_hdr_scalar("debug_adaptive_extra_counts.hdr", W, H, sc1.data());
+            }
+            if (r.adaptive_spp > 0) debug_scalar(RPD_BUF_DEBUG_VARIANCE_SOURCE, "debug_variance_source_adaptive.hdr");
+        }
     }
     rpd_job_release(job);
     rpd_scene_destroy(sc);

@@ -22,6 +22,7 @@ BUF = dict(
     sens_light_select=36, sens_environment=37, sens_rr=38,
     sample_count=50, hit_count=51, pass_counts=52,
     stats=60, sums=61, grid=62, tiles=63, features=64, records=65,
+    debug_neighbors=70, debug_weight_sum=71, debug_neff=72, debug_variance_source=73,
 )
 _RGB = {"radiance", "raw_radiance", "albedo", "normal", "world_pos", "denoised"}
 _INT = {"sample_count", "hit_count", "pass_counts"}
@@ -212,6 +213,7 @@ def lib():
             ("rpd_job_denoise_records", [vp]),
             ("rpd_denoise_batch", [vp, vp, C.c_int, C.POINTER(C.c_double)]),
             ("rpd_ctx_set_graph_passes", [vp, C.c_int]),
+            ("rpd_ctx_set_debug_outputs", [vp, C.c_int]),
             ("rpd_statmc_tile_shapes", [vp, C.c_int, C.POINTER(C.c_int)]),
             ("rpd_ctx_set_statmc_tile", [vp, C.c_int, C.c_int]),
             ("rpd_ctx_statmc_tile", [vp, C.c_int, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -306,6 +308,10 @@ class Context:
         w, h = C.c_int(), C.c_int()
         check(self.L.rpd_ctx_statmc_tile(self.h, int(radius), int(pixels), C.byref(w), C.byref(h)))
         return w.value, h.value
+
+    def set_debug_outputs(self, on):
+        """Jobs created next keep the debug_statmc_outputs buffers (debug_*)."""
+        check(self.L.rpd_ctx_set_debug_outputs(self.h, int(on)))
 
     def set_graph_passes(self, on):
         check(self.L.rpd_ctx_set_graph_passes(self.h, int(on)))
