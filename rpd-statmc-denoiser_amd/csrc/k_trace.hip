@@ -84,18 +84,23 @@ void launch_tape(const TraceParams& p, hipStream_t st) {
 // pixel's next sample, and a lane whose pixel ended takes the next pixel from
 // a wave-level chunk of the active list.
 //
-// Every loop iteration resolves the lane's last ray query, advances the lane
-// through the path stages (camera, light sample, environment sample, BRDF
-// sample) until it issues its next query, and then runs one intersection loop
-// for all lanes at once: camera and extension rays as closest-hit queries,
-// both shadow rays of a bounce as occlusion queries. All random draws of a
-// bounce are independent of the shadow-ray outcomes, so splitting a bounce
-// over three queries keeps the draw order of main.cpp:394-533.
+// Every loop iteration resolves the lane's queries of the last iteration,
+// then shades one bounce in full (main.cpp:394-533): light selection and light
+// sample, environment sample, BRDF sample, in the reference's draw order, and
+// issues that bounce's rays together: the light's shadow ray, the environment
+// shadow ray and the BRDF (extension) ray. All random draws of a bounce are
+// independent of the shadow-ray outcomes, and the Russian-roulette draw
+// follows the extension ray's result, so the draw order is the reference's.
+// Because every lane shades at most one bounce per iteration, the lanes of a
+// wave run the same shading code together instead of splitting between a
+// light step and a BRDF step.
 //
 // Radiance is accumulated forward (throughput times each bounce's direct
 // term); the reference's recursion sums the same terms innermost-first.
-enum : int { ST_NEED = 0, ST_CAMERA = 1, ST_LIGHT = 2, ST_ENV = 3, ST_BRDF = 4, ST_DONE = 5, ST_DEAD = 6 };
-enum : int { Q_NONE = 0, Q_PRIMARY = 1, Q_EXT = 2, Q_SHADOW_L = 3, Q_SHADOW_E = 4 };
+enum : int { ST_NEED = 0, ST_CAMERA = 1, ST_SHADE = 2, ST_WAIT = 3, ST_DONE = 5, ST_DEAD = 6 };
+// queries of one iteration (bit mask): the closest-hit ray (camera or BRDF
+// extension), the light's shadow ray, the environment's shadow ray
+enum : int { Q_EXT = 1, Q_SHADOW_L = 2, Q_SHADOW_E = 4, Q_PRIMARY = 8 };
 
 constexpr int TRACE_BLOCK = 256;
 constexpr int TRACE_CHUNK = 64;
@@ -108,92 +113,117 @@ __device__ __forceinline__ bool is_zero3(v3 a) { return a.x == 0.0f && a.y == 0.
 constexpr int TRACE_LDS_TRIS = 256;
 constexpr int TRACE_FLAT_SPHERES = 32;
 
-// One step of a path lane, shared by the one-kernel trace and the split
-// shade/intersect pair: resolves the lane's last query (its result in h),
-// writes a finished sample, hands out pixels (a wave-wide operation), then
-// advances the path through its stages until it issues its next query (q) or
-// has none this step. Returns false once every lane of the wave is out of work.
+// The rays a lane issues in one iteration: all start at `ro` (the camera, or
+// the offset shading point), with their own directions.
+struct LaneRays {
+    v3 ro;
+    v3 rd_ext, rd_l, rd_e;   // closest-hit ray, light shadow ray, environment shadow ray
+    float t_l;               // light shadow ray range (distance - any-hit epsilon)
+};
+
+// One step of a path lane: resolves the lane's queries (h: the closest-hit
+// result, occ_l / occ_e: whether the shadow rays were blocked) once none is
+// pending, writes a finished sample, hands out pixels (a wave-wide operation),
+// then starts a camera ray or shades one bounce, setting q and pend to the
+// queries to run. Returns false once every lane of the wave is out of work.
 template <bool kDepth, bool kLambert>
 __device__ __forceinline__ bool path_step(const TraceParams& p, const int lane, const v3 env, const bool env_on,
-                                          int& c_next, int& c_end, const int n_active, const HitRec& h, int& st,
-                                          int& q, int& pix, int& left, int64_t& slot, Draws& dr, v3& ro, v3& rd,
-                                          float& qt, v3& T, v3& L, v3& Ld, v3& sp, v3& sn, v3& swo, v3& cw,
-                                          float& bpdf, int& smat, int& depth, uint32_t& flags, uint32_t& light_bits) {
+                                          int& c_next, int& c_end, const int n_active, const HitRec& h,
+                                          const bool occ_l, const bool occ_e, int& st, int& q, int& pend,
+                                          int& pix, int& left,
+                                          int64_t& slot, Draws& dr, LaneRays& R, v3& T, v3& L, v3& Ld, v3& sp,
+                                          v3& sn, v3& swo, v3& cw_l, v3& cw_e, v3& cw_x, float& bpdf, int& smat,
+                                          int& depth, uint32_t& flags, uint32_t& light_bits) {
     const DevScene& S = p.scene;
     const float pdf_env = float(1.0 / (4.0 * RPD_PI));
-    const float inf = __builtin_inff();
-    // -- resolve the last query -------------------------------------------
-    if (q == Q_SHADOW_L || q == Q_SHADOW_E) {
-        if (!h.hit) Ld = Ld + cw;
-        st = q == Q_SHADOW_L ? ST_ENV : ST_BRDF;
-    } else if (q == Q_PRIMARY || q == Q_EXT) {
-        bool emitter = false;
-        int li = -1, hm = 0;
-        v3 hn = mk3(0, 0, 0);
-        if (h.hit) {
-            if (h.sphere) {
-                const DevSphere& s = S.spheres[h.index];
-                emitter = s.emitter != 0; li = s.light; hn = h.sn; hm = s.material;
-            } else {
-                const DevTriangle& t = S.tris[h.index];
-                emitter = t.emitter != 0; li = t.light; hn = ld3(t.n); hm = t.material;
-            }
-        }
-        if (q == Q_PRIMARY) {
+    // -- resolve the last queries -------------------------------------------
+    if (st == ST_WAIT && pend == 0) {
+        if (q & Q_PRIMARY) {
             // main.cpp:540-585
+            st = ST_DONE;
             if (!h.hit) {
                 L = env;
-                st = ST_DONE;
             } else {
-                const v3 hp = ro + h.t * rd;
+                bool emitter;
+                int li, hm;
+                v3 hn;
+                if (h.sphere) {
+                    const DevSphere& sp_ = S.spheres[h.index];
+                    emitter = sp_.emitter != 0; li = sp_.light; hn = h.sn; hm = sp_.material;
+                } else {
+                    const DevTriangle& t = S.tris[h.index];
+                    emitter = t.emitter != 0; li = t.light; hn = ld3(t.n); hm = t.material;
+                }
+                const v3 hp = R.ro + h.t * R.rd_ext;
                 flags |= REC_HIT;
                 p.rec.r1[slot] = make_float4(hn.x, hn.y, hn.z, h.t);
                 const DevMaterial& hmat = S.materials[hm];
                 p.rec.r2[slot] = make_float4(hmat.albedo[0], hmat.albedo[1], hmat.albedo[2], 0.0f);
                 p.rec.r3[slot] = make_float4(hp.x, hp.y, hp.z, 0.0f);
                 if (emitter) {
-                    L = eval_light(S.lights[li], -rd, h.t);
-                    st = ST_DONE;
+                    L = eval_light(S.lights[li], -R.rd_ext, h.t);
                 } else {
-                    sp = hp; sn = hn; swo = -normalized3(rd); smat = hm;
+                    sp = hp; sn = hn; swo = -normalized3(R.rd_ext); smat = hm;
                     depth = 1;
-                    st = ST_LIGHT;
+                    st = ST_SHADE;
                 }
             }
         } else {
-            // main.cpp:480-529
-            const DevMaterial& m = S.materials[smat];
-            const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
-            if (h.hit && emitter) {
-                const DevLight& E = S.lights[li];
-                float wb = 1.0f;
-                if (!delta_brdf && !light_is_delta(E)) {
-                    const float plb = light_pdf(S, li, light_select_pdf(S, li), rd, h.t);
-                    if (plb > RPD_EPS_SMALL) wb = bpdf / (bpdf + plb);
-                }
-                Ld = Ld + wb * vmul(eval_light(E, -rd, h.t), cw);
-            } else if (!h.hit) {
-                const float wb = delta_brdf ? 1.0f : bpdf / (bpdf + pdf_env);
-                Ld = Ld + wb * vmul(env, cw);
-            }
-            L = L + vmul(T, Ld);
+            // the shadow rays of the bounce (main.cpp:436-448, 456-461)
+            if ((q & Q_SHADOW_L) && !occ_l) Ld = Ld + cw_l;
+            if ((q & Q_SHADOW_E) && !occ_e) Ld = Ld + cw_e;
             st = ST_DONE;
-            if (h.hit && !emitter) {
-                const float ksi = dr.next();
-                if (p.capture && !(flags & REC_RR)) { flags |= REC_RR; if (ksi < 0.8f) flags |= REC_RR_SURVIVED; }
-                if (ksi < 0.8f) {
-                    T = vmul(T, cw) / 0.8f;
-                    sp = ro + h.t * rd;
-                    sn = hn;
-                    swo = -rd;
-                    smat = hm;
-                    ++depth;
-                    st = ST_LIGHT;
+            if (!(q & Q_EXT)) {
+                L = L + vmul(T, Ld);      // no BRDF ray: the bounce ends the path
+            } else {
+                // the BRDF ray: main.cpp:480-529
+                const DevMaterial& m = S.materials[smat];
+                const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
+                bool emitter = false;
+                int li = -1, hm = 0;
+                v3 hn = mk3(0, 0, 0);
+                if (h.hit) {
+                    if (h.sphere) {
+                        const DevSphere& sp_ = S.spheres[h.index];
+                        emitter = sp_.emitter != 0; li = sp_.light; hn = h.sn; hm = sp_.material;
+                    } else {
+                        const DevTriangle& t = S.tris[h.index];
+                        emitter = t.emitter != 0; li = t.light; hn = ld3(t.n); hm = t.material;
+                    }
+                }
+                if (h.hit && emitter) {
+                    const DevLight& E = S.lights[li];
+                    float wb = 1.0f;
+                    if (!delta_brdf && !light_is_delta(E)) {
+                        const float plb = light_pdf(S, li, light_select_pdf(S, li), R.rd_ext, h.t);
+                        if (plb > RPD_EPS_SMALL) wb = bpdf / (bpdf + plb);
+                    }
+                    Ld = Ld + wb * vmul(eval_light(E, -R.rd_ext, h.t), cw_x);
+                } else if (!h.hit) {
+                    const float wb = delta_brdf ? 1.0f : bpdf / (bpdf + pdf_env);
+                    Ld = Ld + wb * vmul(env, cw_x);
+                }
+                L = L + vmul(T, Ld);
+                if (h.hit && !emitter) {
+                    const float ksi = dr.next();
+                    if (p.capture && !(flags & REC_RR)) {
+                        flags |= REC_RR;
+                        if (ksi < 0.8f) flags |= REC_RR_SURVIVED;
+                    }
+                    if (ksi < 0.8f) {
+                        T = vmul(T, cw_x) / 0.8f;
+                        sp = R.ro + h.t * R.rd_ext;
+                        sn = hn;
+                        swo = -R.rd_ext;
+                        smat = hm;
+                        ++depth;
+                        st = ST_SHADE;
+                    }
                 }
             }
         }
+        q = 0;
     }
-    q = Q_NONE;
 
     // -- finish samples ------------------------------------------------
     if (st == ST_DONE) {
@@ -255,22 +285,23 @@ __device__ __forceinline__ bool path_step(const TraceParams& p, const int lane, 
             o = o + (lx * ld3(p.cam.right) + ly * ld3(p.cam.up));
         }
         const v3 px = ld3(p.cam.llc) + u * ld3(p.cam.horiz) + (1.0f - v) * ld3(p.cam.vert);
-        ro = o;
-        rd = normalized3(px - o);
-        qt = inf;
+        R.ro = o;
+        R.rd_ext = normalized3(px - o);
         T = mk3(1, 1, 1);
         L = mk3(0, 0, 0);
-        q = Q_PRIMARY;
+        q = Q_EXT | Q_PRIMARY;
+        pend = Q_EXT;
+        st = ST_WAIT;
     }
 
-    // -- light sample of a bounce: main.cpp:405-448 ------------------------
-    if (st == ST_LIGHT) {
+    // -- one bounce: light sample (main.cpp:405-448), environment sample
+    //    (450-462) and BRDF sample (464-479), in the reference's draw order ----
+    if (st == ST_SHADE) {
         const DevMaterial& m = S.materials[smat];
         const int mkind = kLambert ? int(MAT_LAMBERT) : m.kind;
         const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
         Ld = mk3(0, 0, 0);
-        ro = offset_origin(sp, sn);
-        st = ST_ENV;
+        R.ro = offset_origin(sp, sn);
         const float sel_u = dr.next();
         int li; float sel_pdf;
         sample_light(S, sel_u, li, sel_pdf);
@@ -295,24 +326,17 @@ __device__ __forceinline__ bool path_step(const TraceParams& p, const int lane, 
                         const v3 f = brdf_eval(mkind, m, swo, wi, sn);
                         const float g = delta_brdf ? 1.0f : fmaxf(0.0f, dot3(sn, wi));
                         const v3 contrib = (vmul(em, f) * g) / pl;
-                        cw = wl * contrib;
+                        cw_l = wl * contrib;
                         const float eff = dist - RPD_EPS_ANYHIT;  // ray_tracer.h:65-75
                         if (eff <= RPD_EPS_SMALL) {
-                            Ld = Ld + cw;
-                        } else if (!is_zero3(cw)) {
-                            rd = wi; qt = eff; q = Q_SHADOW_L; st = ST_LIGHT;
+                            Ld = Ld + cw_l;
+                        } else if (!is_zero3(cw_l)) {
+                            R.rd_l = wi; R.t_l = eff; q |= Q_SHADOW_L;
                         }
                     }
                 }
             }
         }
-    }
-    // -- environment sample: main.cpp:450-462 -------------------------------
-    if (st == ST_ENV && q == Q_NONE) {
-        const DevMaterial& m = S.materials[smat];
-        const int mkind = kLambert ? int(MAT_LAMBERT) : m.kind;
-        const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
-        st = ST_BRDF;
         const v2 eu = dr.next2();
         if (env_on && !delta_brdf) {
             const v3 ed = sample_sphere(eu);
@@ -324,15 +348,9 @@ __device__ __forceinline__ bool path_step(const TraceParams& p, const int lane, 
             const float we = pdf_env / (pdf_env + bp);
             const v3 f = brdf_eval(mkind, m, swo, ed, sn);
             const float c = fmaxf(0.0f, dot3(sn, ed));
-            cw = (we * vmul(env, f)) * (c / pdf_env);
-            if (!is_zero3(cw)) { rd = ed; qt = inf; q = Q_SHADOW_E; st = ST_ENV; }
+            cw_e = (we * vmul(env, f)) * (c / pdf_env);
+            if (!is_zero3(cw_e)) { R.rd_e = ed; q |= Q_SHADOW_E; }
         }
-    }
-    // -- BRDF sample: main.cpp:464-479 --------------------------------------
-    if (st == ST_BRDF && q == Q_NONE) {
-        const DevMaterial& m = S.materials[smat];
-        const int mkind = kLambert ? int(MAT_LAMBERT) : m.kind;
-        const bool delta_brdf = !kLambert && m.kind == MAT_MIRROR;
         const v2 bu = dr.next2();
         if (p.capture && !delta_brdf && !(flags & REC_BRDF)) {
             flags |= REC_BRDF;
@@ -342,12 +360,13 @@ __device__ __forceinline__ bool path_step(const TraceParams& p, const int lane, 
         brdf_sample(mkind, m, swo, sn, bu, dir, pdf);
         if (pdf > RPD_EPS_SMALL && sqnorm3(dir) > 0.0f && (!kDepth || p.max_depth <= 0 || depth < p.max_depth)) {
             bpdf = pdf;
-            cw = brdf_sample_weight(mkind, m, swo, dir, sn, pdf);
-            rd = dir; qt = inf; q = Q_EXT;
-        } else {
-            L = L + vmul(T, Ld);
-            st = ST_DONE;   // no query: the sample is written next iteration
+            cw_x = brdf_sample_weight(mkind, m, swo, dir, sn, pdf);
+            R.rd_ext = dir;
+            q |= Q_EXT;
         }
+        // a bounce with no query ends here; it is written next iteration
+        pend = q & (Q_EXT | Q_SHADOW_L | Q_SHADOW_E);
+        st = ST_WAIT;
     }
     return true;
 }
@@ -358,7 +377,7 @@ __device__ __forceinline__ bool path_step(const TraceParams& p, const int lane, 
 // kLambert: every material of the scene is Lambertian, so the microfacet and
 // mirror code is left out (fewer registers, fewer divergent branches).
 template <bool kBvh, bool kDepth, bool kLambert>
-__global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
+__global__ void __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) k_trace(TraceParams p) {
     const DevScene& S = p.scene;
     extern __shared__ float4 s_dyn[];
     if (!kBvh) {
@@ -378,26 +397,27 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
     int c_next = 0, c_end = 0;
     const int n_active = p.plan.totals[1];
 
-    int st = ST_NEED, q = Q_NONE;
+    int st = ST_NEED, q = 0, pend = 0;
     int pix = -1, left = 0;
     int64_t slot = 0;
     Draws dr;
     dr.tape = nullptr; dr.pos = 0; dr.cache = make_uint4(0, 0, 0, 0);
-    v3 ro = mk3(0, 0, 0), rd = mk3(0, 0, 0);   // current query ray
-    float qt = inf;                             // query range
+    LaneRays R;
+    R.ro = mk3(0, 0, 0); R.rd_ext = R.ro; R.rd_l = R.ro; R.rd_e = R.ro; R.t_l = 0.0f;
     v3 T = mk3(1, 1, 1), L = mk3(0, 0, 0), Ld = mk3(0, 0, 0);
     v3 sp = mk3(0, 0, 0), sn = mk3(0, 0, 0), swo = mk3(0, 0, 0);  // shading point
-    v3 cw = mk3(0, 0, 0);                       // shadow-ray contribution, or BRDF weight of the extension ray
+    v3 cw_l = mk3(0, 0, 0), cw_e = mk3(0, 0, 0), cw_x = mk3(0, 0, 0);  // shadow contributions, BRDF weight
     float bpdf = 0.0f;
     int smat = 0, depth = 0;
     uint32_t flags = 0, light_bits = 255u;
     HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
+    bool occ_l = false, occ_e = false;
     uint32_t n_queries = 0;
 
     while (true) {
-        if (!path_step<kDepth, kLambert>(p, lane, env, env_on, c_next, c_end, n_active, h, st, q, pix, left, slot,
-                                         dr, ro, rd, qt, T, L, Ld, sp, sn, swo, cw, bpdf, smat, depth, flags,
-                                         light_bits)) {
+        if (!path_step<kDepth, kLambert>(p, lane, env, env_on, c_next, c_end, n_active, h, occ_l, occ_e, st, q,
+                                         pend, pix, left, slot, dr, R, T, L, Ld, sp, sn, swo, cw_l, cw_e, cw_x,
+                                         bpdf, smat, depth, flags, light_bits)) {
             if (p.query_count) {
                 unsigned long long tot = n_queries;
                 for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
@@ -405,11 +425,37 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceParams p) {
             }
             break;
         }
-        // -- one intersection query per lane ----------------------------------
-        if (q != Q_NONE) {
-            h = kBvh ? bvh_query(p.bvh, ro, rd, qt, q >= Q_SHADOW_L, stack, TRACE_BLOCK)
-                     : trace_query(S, tp, ro, rd, qt, q >= Q_SHADOW_L);
-            ++n_queries;
+        // -- the lane's queries ------------------------------------------------
+        // BVH scenes trace one pending query per lane per iteration (the
+        // closest-hit ray first), so every lane has one walk in the divergent
+        // traversal loop; the flat loop over the LDS triangle table costs the
+        // same for every ray, and runs all of a lane's queries at once.
+        if (kBvh) {
+            const int cur = pend & -pend;
+            if (cur) {
+                const v3 d = cur == Q_EXT ? R.rd_ext : (cur == Q_SHADOW_L ? R.rd_l : R.rd_e);
+                const HitRec r = bvh_query(p.bvh, R.ro, d, cur == Q_SHADOW_L ? R.t_l : inf, cur != Q_EXT, stack,
+                                           TRACE_BLOCK);
+                if (cur == Q_EXT) h = r;
+                else if (cur == Q_SHADOW_L) occ_l = r.hit != 0;
+                else occ_e = r.hit != 0;
+                pend &= ~cur;
+                ++n_queries;
+            }
+        } else {
+            if (pend & Q_EXT) {
+                h = trace_query(S, tp, R.ro, R.rd_ext, inf, false);
+                ++n_queries;
+            }
+            if (pend & Q_SHADOW_L) {
+                occ_l = trace_query(S, tp, R.ro, R.rd_l, R.t_l, true).hit != 0;
+                ++n_queries;
+            }
+            if (pend & Q_SHADOW_E) {
+                occ_e = trace_query(S, tp, R.ro, R.rd_e, inf, true).hit != 0;
+                ++n_queries;
+            }
+            pend = 0;
         }
     }
 }
