@@ -250,7 +250,6 @@ int validate(const rpd_render_config& c) {
     if ((int64_t)c.image_width * c.image_height > (1 << 26)) return bad("image too large (more than 2^26 pixels)");
     if (c.samples_per_pixel <= 0) return bad("samples_per_pixel must be positive");
     if (c.rpf_tile_size <= 0) return bad("rpf_tile_size must be positive");
-    if (c.rpf_tile_size > 33) return bad("rpf_tile_size above 33 is not supported (grid step must be <= 16)");
     if (c.color_window_radius <= 0) return bad("color_window_radius must be positive");
     if (!std::isfinite(c.color_normal_threshold) || c.color_normal_threshold <= 0.0f || c.color_normal_threshold > 1.0f)
         return bad("color_normal_threshold must be finite and in (0,1]");

@@ -83,7 +83,8 @@ constexpr int GATHER_BLOCK = 256;
 // The support window of a node holds up to (2*step)^2 pixels. The kernel is
 // built for windows of 64, 256 and 1024 pixels (steps up to 4, 8 and 16) and
 // launched with the smallest that holds every image of the batch, so the
-// default step of 4 leaves LDS room for more resident blocks.
+// default step of 4 leaves LDS room for more resident blocks; a larger step
+// walks its window in segments of 1024 positions.
 constexpr int GATHER_MAX_STEP = 16;
 __host__ __device__ constexpr int gather_max_pix(int max_step) { return 4 * max_step * max_step; }
 
@@ -149,39 +150,6 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds<kMaxPi
     const int nwin = (wx > 0 && wy > 0) ? wx * wy : 0;
     const bool any_active = plan.totals[1] > 0;
 
-    // support pixels with their node weight (main.cpp:332-349), row-major
-    int npix = 0, total = 0;
-    for (int q0 = 0; q0 < nwin; q0 += GATHER_BLOCK) {
-        const int q = q0 + tid;
-        float w = 0.0f;
-        int c = 0, p = 0;
-        if (q < nwin) {
-            const int x = x_lo + q % wx, y = y_lo + q / wx;
-            const float fx = float(x) / float(g.step), fy = float(y) / float(g.step);
-            const int x0 = min(max(int(floorf(fx)), 0), g.nx - 1), y0 = min(max(int(floorf(fy)), 0), g.ny - 1);
-            const int x1 = min(x0 + 1, g.nx - 1), y1 = min(y0 + 1, g.ny - 1);
-            const float tx = clampf(fx - float(x0), 0.0f, 1.0f), ty = clampf(fy - float(y0), 0.0f, 1.0f);
-            if (x0 == gx && y0 == gy) w = (1.0f - tx) * (1.0f - ty);
-            else if (x1 == gx && y0 == gy) w = tx * (1.0f - ty);
-            else if (x0 == gx && y1 == gy) w = (1.0f - tx) * ty;
-            else if (x1 == gx && y1 == gy) w = tx * ty;
-            p = y * W + x;
-            if (w > RPD_EPS_SMALL && any_active) c = max(plan.counts[p], 0);
-        }
-        const bool keep = c > 0;
-        const int kpos = gather_scan(keep ? 1 : 0, s_scan) - (keep ? 1 : 0);
-        const int kcnt = s_scan[GATHER_BLOCK - 1];
-        __syncthreads();
-        const int cpos = gather_scan(c, s_scan) - c;
-        const int ccnt = s_scan[GATHER_BLOCK - 1];
-        if (keep) { s_pix[npix + kpos] = p; s_w[npix + kpos] = w; s_pre[npix + kpos] = total + cpos; }
-        npix += kcnt;
-        total += ccnt;
-        __syncthreads();
-    }
-    if (tid == 0) s_pre[npix] = total;
-    __syncthreads();
-
     // this thread's sum: bin (acc, bsel) for tid < 208; totals of accumulator
     // acc for 208 <= tid < 234 (bsel 16: w and w*w, bsel 17: w*y and w*y*y)
     const bool bin_t = tid < GATHER_BIN_THREADS;
@@ -193,79 +161,122 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds<kMaxPi
     const float half = 0.5f * float(g.support);
     const float left = float(gx * g.step) - half, top = float(gy * g.step) - half;
 
-    for (int base = 0; base < total; base += GATHER_BLOCK) {
-        const int c = base + tid;
-        signed char bins[GRID_ACC];
-        for (int k = 0; k < GRID_ACC; ++k) bins[k] = -1;
-        float y = 0.0f, wc = 0.0f;
-        if (c < total) {
-            int lo = 0, hi = npix - 1;  // pixel of contribution c: last j with s_pre[j] <= c
-            while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (s_pre[mid] <= c) lo = mid; else hi = mid - 1; }
-            const int j = lo;
-            const int p = s_pix[j];
-            wc = s_w[j];
-            const int64_t slot = int64_t(plan.offsets[p]) + (c - s_pre[j]);
-            // all four planes at once: one memory round trip (the last three
-            // are unused for a non-finite sample)
-            const float4 r0 = rec.r0[slot];
-            const float4 r4 = rec.r4[slot];
-            const float4 r5 = rec.r5[slot];
-            const float2 r6 = rec.r6[slot];
-            const uint32_t fl = __float_as_uint(r0.w);
-            if (fl & REC_FINITE) {
-                y = luminance3(mk3(r0.x, r0.y, r0.z));
-                const int x = p % W, yy = p / W;
-                if (fl & REC_PIXEL) {
-                    const float su = clampf((float(x) + r4.x - left) / float(g.support), 0.0f, 1.0f);
-                    const float sv = clampf((float(yy) + r4.y - top) / float(g.support), 0.0f, 1.0f);
-                    const int sb = dep_bin(su, sv);
-                    bins[0] = (signed char)sb;
-                    if (fl & REC_BRDF) bins[2] = (signed char)sb;
-                    if (fl & REC_LENS) bins[4] = (signed char)sb;
-                    if (fl & REC_LIGHT) bins[6] = (signed char)sb;
-                    if (fl & REC_ENV) bins[8] = (signed char)sb;
-                    if (fl & REC_LSEL) bins[10] = (signed char)sb;
-                    if (fl & REC_RR) bins[12] = (signed char)sb;
-                }
-                if (fl & REC_BRDF) bins[1] = (signed char)dep_bin(r5.x, r5.y);
-                if (fl & REC_LENS) bins[3] = (signed char)dep_bin(r4.z, r4.w);
-                if (fl & REC_LIGHT) bins[5] = (signed char)dep_bin(r5.z, r5.w);
-                if (fl & REC_ENV) bins[7] = (signed char)dep_bin(r6.x, r6.y);
-                if (fl & REC_LSEL) bins[9] = (signed char)min(max(int((fl >> REC_LIGHT_SHIFT) & 255u), 0), 15);
-                if (fl & REC_RR) bins[11] = (signed char)((fl & REC_RR_SURVIVED) ? 1 : 0);
+    // The window is walked in segments of at most kMaxPix positions (one
+    // segment unless the grid step exceeds the kernel's window), each with its
+    // own support list; the sums run on across segments, so the order of the
+    // contributions is the same.
+    int grand = 0;
+    for (int seg = 0; seg < nwin; seg += kMaxPix) {
+        const int seg_end = min(nwin, seg + kMaxPix);
+        // support pixels with their node weight (main.cpp:332-349), row-major,
+        // over window positions [seg, seg_end)
+        int npix = 0, total = 0;
+        for (int q0 = seg; q0 < seg_end; q0 += GATHER_BLOCK) {
+            const int q = q0 + tid;
+            float w = 0.0f;
+            int c = 0, p = 0;
+            if (q < seg_end) {
+                const int x = x_lo + q % wx, y = y_lo + q / wx;
+                const float fx = float(x) / float(g.step), fy = float(y) / float(g.step);
+                const int x0 = min(max(int(floorf(fx)), 0), g.nx - 1), y0 = min(max(int(floorf(fy)), 0), g.ny - 1);
+                const int x1 = min(x0 + 1, g.nx - 1), y1 = min(y0 + 1, g.ny - 1);
+                const float tx = clampf(fx - float(x0), 0.0f, 1.0f), ty = clampf(fy - float(y0), 0.0f, 1.0f);
+                if (x0 == gx && y0 == gy) w = (1.0f - tx) * (1.0f - ty);
+                else if (x1 == gx && y0 == gy) w = tx * (1.0f - ty);
+                else if (x0 == gx && y1 == gy) w = (1.0f - tx) * ty;
+                else if (x1 == gx && y1 == gy) w = tx * ty;
+                p = y * W + x;
+                if (w > RPD_EPS_SMALL && any_active) c = max(plan.counts[p], 0);
             }
+            const bool keep = c > 0;
+            const int kpos = gather_scan(keep ? 1 : 0, s_scan) - (keep ? 1 : 0);
+            const int kcnt = s_scan[GATHER_BLOCK - 1];
+            __syncthreads();
+            const int cpos = gather_scan(c, s_scan) - c;
+            const int ccnt = s_scan[GATHER_BLOCK - 1];
+            if (keep) { s_pix[npix + kpos] = p; s_w[npix + kpos] = w; s_pre[npix + kpos] = total + cpos; }
+            npix += kcnt;
+            total += ccnt;
+            __syncthreads();
         }
-        // entries past the end of the chunk carry bin -1 and add nothing; the
-        // FP64 terms are the products the sums add, each rounded once
-        {
-            const double wd = double(wc), yd = double(y);
-            const double wyd = wd * yd;
-            L.pr[0][tid] = make_double2(wd, wyd);
-            L.pr[1][tid] = make_double2(wd, wd * wd);
-            L.pr[2][tid] = make_double2(wyd, wyd * yd);
-        }
-        for (int k = 0; k < GRID_ACC; ++k) c_bin[k * 4 * GATHER_BIN_ROW + tid] = bins[k];
+        if (tid == 0) s_pre[npix] = total;
         __syncthreads();
-        const int nquad = (min(GATHER_BLOCK, total - base) + 3) >> 2;
-        if (tid < GATHER_SUM_THREADS) {
-            const double2* src = L.pr[bin_t ? 0 : bsel - 15];
-            const int key = bin_t ? bsel : -1;
-            const double m_eq = bin_t ? 1.0 : -0.0, m_ne = bin_t ? -0.0 : 1.0;
-            for (int i4 = 0; i4 < nquad; ++i4) {
-                const uint32_t b4 = L.bin4[acc][i4];
+
+        for (int base = 0; base < total; base += GATHER_BLOCK) {
+            const int c = base + tid;
+            signed char bins[GRID_ACC];
+            for (int k = 0; k < GRID_ACC; ++k) bins[k] = -1;
+            float y = 0.0f, wc = 0.0f;
+            if (c < total) {
+                int lo = 0, hi = npix - 1;  // pixel of contribution c: last j with s_pre[j] <= c
+                while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (s_pre[mid] <= c) lo = mid; else hi = mid - 1; }
+                const int j = lo;
+                const int p = s_pix[j];
+                wc = s_w[j];
+                const int64_t slot = int64_t(plan.offsets[p]) + (c - s_pre[j]);
+                // all four planes at once: one memory round trip (the last three
+                // are unused for a non-finite sample)
+                const float4 r0 = rec.r0[slot];
+                const float4 r4 = rec.r4[slot];
+                const float4 r5 = rec.r5[slot];
+                const float2 r6 = rec.r6[slot];
+                const uint32_t fl = __float_as_uint(r0.w);
+                if (fl & REC_FINITE) {
+                    y = luminance3(mk3(r0.x, r0.y, r0.z));
+                    const int x = p % W, yy = p / W;
+                    if (fl & REC_PIXEL) {
+                        const float su = clampf((float(x) + r4.x - left) / float(g.support), 0.0f, 1.0f);
+                        const float sv = clampf((float(yy) + r4.y - top) / float(g.support), 0.0f, 1.0f);
+                        const int sb = dep_bin(su, sv);
+                        bins[0] = (signed char)sb;
+                        if (fl & REC_BRDF) bins[2] = (signed char)sb;
+                        if (fl & REC_LENS) bins[4] = (signed char)sb;
+                        if (fl & REC_LIGHT) bins[6] = (signed char)sb;
+                        if (fl & REC_ENV) bins[8] = (signed char)sb;
+                        if (fl & REC_LSEL) bins[10] = (signed char)sb;
+                        if (fl & REC_RR) bins[12] = (signed char)sb;
+                    }
+                    if (fl & REC_BRDF) bins[1] = (signed char)dep_bin(r5.x, r5.y);
+                    if (fl & REC_LENS) bins[3] = (signed char)dep_bin(r4.z, r4.w);
+                    if (fl & REC_LIGHT) bins[5] = (signed char)dep_bin(r5.z, r5.w);
+                    if (fl & REC_ENV) bins[7] = (signed char)dep_bin(r6.x, r6.y);
+                    if (fl & REC_LSEL) bins[9] = (signed char)min(max(int((fl >> REC_LIGHT_SHIFT) & 255u), 0), 15);
+                    if (fl & REC_RR) bins[11] = (signed char)((fl & REC_RR_SURVIVED) ? 1 : 0);
+                }
+            }
+            // entries past the end of the chunk carry bin -1 and add nothing; the
+            // FP64 terms are the products the sums add, each rounded once
+            {
+                const double wd = double(wc), yd = double(y);
+                const double wyd = wd * yd;
+                L.pr[0][tid] = make_double2(wd, wyd);
+                L.pr[1][tid] = make_double2(wd, wd * wd);
+                L.pr[2][tid] = make_double2(wyd, wyd * yd);
+            }
+            for (int k = 0; k < GRID_ACC; ++k) c_bin[k * 4 * GATHER_BIN_ROW + tid] = bins[k];
+            __syncthreads();
+            const int nquad = (min(GATHER_BLOCK, total - base) + 3) >> 2;
+            if (tid < GATHER_SUM_THREADS) {
+                const double2* src = L.pr[bin_t ? 0 : bsel - 15];
+                const int key = bin_t ? bsel : -1;
+                const double m_eq = bin_t ? 1.0 : -0.0, m_ne = bin_t ? -0.0 : 1.0;
+                for (int i4 = 0; i4 < nquad; ++i4) {
+                    const uint32_t b4 = L.bin4[acc][i4];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int bin = int(int8_t(b4 >> (8 * e)));  // -1: none
-                    const double m = bin == key ? m_eq : m_ne;
-                    const double2 v = src[4 * i4 + e];
-                    a0 = fma(m, v.x, a0);
-                    a1 = fma(m, v.y, a1);
+                    for (int e = 0; e < 4; ++e) {
+                        const int bin = int(int8_t(b4 >> (8 * e)));  // -1: none
+                        const double m = bin == key ? m_eq : m_ne;
+                        const double2 v = src[4 * i4 + e];
+                        a0 = fma(m, v.x, a0);
+                        a1 = fma(m, v.y, a1);
+                    }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
+        grand += total;
     }
-    if (active_t && total > 0) {
+    if (active_t && grand > 0) {
         double* nd = grid + (size_t)node * GRID_NODE_DOUBLES + grid_acc_offset(acc);
         const int nb = grid_acc_bins(acc);
         if (bsel < 16) { nd[bsel] += a0; nd[nb + bsel] += a1; }

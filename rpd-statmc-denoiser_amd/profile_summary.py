@@ -12,7 +12,7 @@ import os
 import sys
 from collections import defaultdict
 
-CMD = "rpd-statmc-denoiser_amd/profile_kernels.sh (rocprofv3 on: python3 bench.py --steps 1 --warmup 0 --no-cpu-baseline)"
+CMD = "rpd-statmc-denoiser_amd/profile_kernels.sh (rocprofv3 on: python3 bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-quality)"
 COPY_BYTES = 1 << 30
 
 
@@ -43,6 +43,11 @@ def main():
     with open(os.path.join(summ, "kernels_cornell1024_c1_job.md"), "w") as f:
         f.write("# Kernel time of one cornell_box 1024x1024 job (16 + 3x4 spp, StatMC r10, var 3, aperture 0.14)\n\n")
         f.write(f"Command: `{CMD}`, step 1 (`--kernel-trace --stats`).\n\n")
+        shapes = [short(r["Name"]) for r in stats if short(r["Name"]).startswith("rpd::k_statmc<")]
+        if len(shapes) > 1:
+            f.write(f"The run is the process's first job, so the StatMC window kernel runs once in each of its "
+                    f"{len(shapes)} tile shapes (the context times them and keeps the fastest for this radius and "
+                    f"image size); a later job launches only the fastest.\n\n")
         f.write("| kernel | calls | total ms | avg us | % |\n|---|---|---|---|---|\n")
         for r in stats[:20]:
             f.write(f"| {short(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "

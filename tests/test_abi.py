@@ -40,9 +40,12 @@ def test_defaults_and_validation_without_device():
     cfg = eng.default_config(image_width=64, image_height=48)
     assert cfg.samples_per_pixel >= 1 and cfg.rpf_tile_size == 8 and cfg.max_depth == 0
     assert eng.job_bytes(cfg) > 64 * 48 * 4
-    bad = eng.default_config(image_width=64, image_height=48, rpf_tile_size=40)
+    # any positive tile size is accepted, as by the reference's config check (core/config.h:136-138)
+    big = eng.default_config(image_width=64, image_height=48, rpf_tile_size=70)
+    assert eng.job_bytes(big) > 0
+    bad = eng.default_config(image_width=64, image_height=48, rpf_tile_size=0)
     try:
         eng.job_bytes(bad)
-        raise AssertionError("an unsupported tile size was accepted")
+        raise AssertionError("a tile size of 0 was accepted")
     except eng.EngineError as e:
-        assert "rpf_tile_size" in str(e)
+        assert "rpf_tile_size must be positive" in str(e)

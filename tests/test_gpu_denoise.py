@@ -56,9 +56,12 @@ def test_traced_records_denoise_like_oracle(engine_ctx, aperture):
     _compare_denoised(oj, gj, "traced records")
 
 
-@pytest.mark.parametrize("tile", [8, 16, 32])   # grid steps 4, 8, 16: each gather support-window size
-def test_uneven_counts_and_nonfinite_records(engine_ctx, tile):
-    rc = ob.config_c0(37, 29)
+# grid steps 4, 8, 16: each gather support-window size; steps 20 and 35 walk
+# their windows (up to 40x40 and 70x70 positions) in segments of 1024
+@pytest.mark.parametrize("tile,size", [(8, (37, 29)), (16, (37, 29)), (32, (37, 29)), (40, (53, 47)),
+                                       (70, (80, 75))])
+def test_uneven_counts_and_nonfinite_records(engine_ctx, tile, size):
+    rc = ob.config_c0(*size)
     rc.rpf_tile_size = tile
     rng = np.random.default_rng(5)
     counts = rng.integers(0, 9, rc.width * rc.height).astype(np.int32)
@@ -142,9 +145,9 @@ def test_synthetic_record_statistics(engine_ctx):
 def test_batch_identical_to_alone(engine_ctx):
     """Uneven images, sample counts and grid steps in one batch give the same bits as each
     image alone (alone, each runs the gather built for its own step; in the batch, all run the
-    one built for the largest)."""
-    shapes = [(64, 48, 4), (100, 37, 7), (33, 70, 16), (128, 128, 2), (17, 9, 5)]
-    tiles = [8, 16, 6, 32, 8]
+    one built for the largest; a step above 16 walks each window in segments)."""
+    shapes = [(64, 48, 4), (100, 37, 7), (33, 70, 16), (128, 128, 2), (17, 9, 5), (90, 60, 3)]
+    tiles = [8, 16, 6, 32, 8, 44]
     outs = []
     for batched in (False, True):
         jobs = []
