@@ -327,12 +327,15 @@ def c4_requests():
     return reqs
 
 
-def c4_stream(ctx, reqs, rate_scale, max_in_flight):
+def c4_stream(ctx, reqs, rate_scale, max_in_flight, cobatch_tail=False):
     """The request stream through the serving loop, arrivals at their Poisson
     times divided by rate_scale (1 = the stream's own rate); returns wall
-    seconds, samples, the loop's statistics and every job's image checksum."""
+    seconds, samples, the loop's statistics and every job's image checksum.
+    cobatch_tail: the denoise tails of jobs whose passes finish together run
+    as one batch."""
     import threading
-    srv = eng.Server(ctx, max_in_flight=max_in_flight, memory_budget=int(C4["budget_gb"] * 1e9), graph_passes=True)
+    srv = eng.Server(ctx, max_in_flight=max_in_flight, memory_budget=int(C4["budget_gb"] * 1e9), graph_passes=True,
+                     cobatch_tail=cobatch_tail)
     ids = [None] * len(reqs)
     samples = [0] * len(reqs)
     sums = [0.0] * len(reqs)
@@ -364,11 +367,13 @@ def c4_stream(ctx, reqs, rate_scale, max_in_flight):
 
 
 def run_c4(args):
-    """C4: one fixed request stream three ways, on one context:
+    """C4: one fixed request stream four ways, on one context:
       * plain: the per-image GPU path, one whole job after another
         (rpd_job_submit + wait, no overlap), arrival times ignored;
       * served at the stream's rate (20 jobs/s: more than the device serves,
         so this is the saturated throughput and its queueing latency);
+      * the same with the denoise tails of jobs whose passes finish together
+        batched into shared launches;
       * served at a sustainable rate, 0.8 of the plain path's job rate.
     Served jobs equal the plain ones bit for bit (a job's result does not
     depend on what else runs), so the images, and their relMSE, are the same;
@@ -402,10 +407,11 @@ def run_c4(args):
     plain_rate = len(reqs) / plain_wall
     # served, at the stream's rate and at a sustainable one
     sat_wall, sat_samples, sat, sat_sums = c4_stream(ctx, reqs, 1.0, C4["max_in_flight"])
+    cob_wall, cob_samples, cob, cob_sums = c4_stream(ctx, reqs, 1.0, C4["max_in_flight"], cobatch_tail=True)
     arrival_rate = len(reqs) / reqs[-1][0]
     sus_scale = 0.8 * plain_rate / arrival_rate
     sus_wall, sus_samples, sus, sus_sums = c4_stream(ctx, reqs, sus_scale, C4["max_in_flight"])
-    same = sat_sums == plain_sums and sus_sums == plain_sums
+    same = sat_sums == plain_sums and sus_sums == plain_sums and cob_sums == plain_sums
     n_proc = sum(1 for r in reqs if r[1] == "c2")
 
     def lat(st):
@@ -427,10 +433,15 @@ def run_c4(args):
                   "wall_s": plain_wall},
         "served_20_per_s": {"jobs_per_s": len(reqs) / sat_wall, "msamples_per_s": sat_samples / sat_wall / 1e6,
                             "wall_s": sat_wall, "latency_ms": lat(sat)},
+        "served_20_per_s_cobatched_tail": {
+            "jobs_per_s": len(reqs) / cob_wall, "msamples_per_s": cob_samples / cob_wall / 1e6, "wall_s": cob_wall,
+            "latency_ms": lat(cob), "tail_batches": cob["tail_batches"],
+            "jobs_per_tail_batch": cob["tail_jobs"] / max(1, cob["tail_batches"])},
         "served_sustainable": {"arrival_jobs_per_s": arrival_rate * sus_scale,
                                "jobs_per_s": len(reqs) / sus_wall, "msamples_per_s": sus_samples / sus_wall / 1e6,
                                "wall_s": sus_wall, "latency_ms": lat(sus)},
         "served_over_plain": (len(reqs) / sat_wall) / plain_rate,
+        "cobatched_over_plain": (len(reqs) / cob_wall) / plain_rate,
         "images_equal_to_plain": same,
     }
     print(json.dumps(result))

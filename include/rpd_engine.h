@@ -112,12 +112,18 @@ int rpd_ctx_destroy(rpd_context* ctx);
 const char* rpd_last_error(void);
 /* Device memory the context may hand to jobs (admission budget), bytes. */
 int rpd_ctx_set_memory_budget(rpd_context* ctx, uint64_t bytes);
-/* Extension (the reference allocates its buffers once per process): device
-   buffers of released jobs are kept, up to `bytes` (default 32 GiB), and
-   reused by later jobs asking for the same sizes; they do not count as held by
+/* Extension (the reference allocates its buffers once per process): a job's
+   device buffers live in one slab, sized in classes of eight per doubling.
+   Slabs of released jobs are kept, up to `bytes` (default 32 GiB; the oldest
+   are freed first), and a later job takes the smallest kept slab that holds
+   it and is at most half as large again; kept slabs do not count as held by
    jobs. A device allocation that fails empties the cache and is tried again.
    0 frees and disables it. `cached` (may be NULL) receives the bytes kept. */
 int rpd_ctx_set_buffer_cache(rpd_context* ctx, uint64_t bytes, uint64_t* cached);
+/* Extension: a job takes a kept slab larger than it needs only while the
+   bytes held by jobs stay within `bytes` (0, the default: no limit). The
+   serving loop sets its budget here, so that admission by footprint holds. */
+int rpd_ctx_set_held_limit(rpd_context* ctx, uint64_t bytes);
 /* How scenes created next build their BVH (only scenes too large for the flat
    loop get one; the reference builds a binary median-split BVH on the host,
    core/ray_tracer.h:49-226): RPD_BVH_HOST_SAH (default) bins a surface-area
@@ -158,6 +164,20 @@ int rpd_job_submit(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* 
 int rpd_job_poll(rpd_job* job, int* done);
 int rpd_job_wait(rpd_job* job);
 int rpd_job_release(rpd_job* job);
+/* render() in two parts, for a caller that batches the second part of several
+   jobs (the serving loop does). rpd_job_submit_passes queues the sampling part
+   only: the base pass and the adaptive passes (main.cpp:1543-1547, 1451-1538);
+   poll and wait then report that part. A job without StatMC is complete after
+   it. rpd_render_finish_batch runs the denoise tail of every listed job whose
+   passes have finished (main.cpp:1548-1564: sensitivity from the final grid,
+   StatMC, variance denoise, outputs), the sensitivity and StatMC stages as one
+   launch each over all the images; it returns with the outputs written,
+   identical to rpd_job_submit's. gpu_ms (optional) receives the batch's device
+   time. RPD_ERR_STATE for a job whose passes were not started this way or
+   whose tail already ran. */
+int rpd_job_submit_passes(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam,
+                          uint32_t seed, rpd_job** out);
+int rpd_render_finish_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu_ms);
 
 /* ---- step-wise jobs: the reference's stage functions on one job's state ---- */
 /* Allocates a job's device state (buffers.init + pixel_stats + rpf_grid.init,
@@ -357,7 +377,8 @@ int rpd_ctx_set_debug_outputs(rpd_context* ctx, int on);
    sampling pass) as a hipGraph on its first pass and replay it for every
    pass; results are identical to launching the kernels one by one. */
 int rpd_ctx_set_graph_passes(rpd_context* ctx, int on);
-/* Device bytes a job of this configuration holds (admission by HBM footprint). */
+/* Device bytes a job of this configuration needs (its slab size; admission by
+   HBM footprint). A job that takes a larger kept slab holds that slab's size. */
 int rpd_job_bytes(const rpd_render_config* cfg, uint64_t* bytes);
 /* Bytes held by the context's jobs, its budget (0 = none) and the device's free and total memory. */
 int rpd_ctx_memory(rpd_context* ctx, uint64_t* in_use, uint64_t* budget, uint64_t* device_free,
@@ -377,6 +398,8 @@ typedef struct rpd_server_config {
     uint64_t memory_budget;  /* bytes of job state held at once, counting finished jobs until they are
                                 released (0 = free device memory at creation) */
     int graph_passes;        /* replay the adaptive passes from a captured hipGraph */
+    int cobatch_tail;        /* run the denoise tails of the jobs whose passes finished together as one
+                                batch (rpd_render_finish_batch) instead of each on its own stream */
 } rpd_server_config;
 typedef struct rpd_server_stats {
     uint64_t submitted, completed, failed;
@@ -384,7 +407,11 @@ typedef struct rpd_server_stats {
     uint64_t peak_bytes;
     double latency_p50_ms, latency_p99_ms, latency_mean_ms, latency_max_ms;  /* submit -> done */
     double queue_p50_ms, queue_p99_ms;                                        /* submit -> admitted */
+    uint64_t tail_batches, tail_jobs;  /* cobatch_tail: denoise-tail batches run, and the jobs in them */
 } rpd_server_stats;
+/* Also sets the context's slab cache (rpd_ctx_set_buffer_cache) to what the
+   budget leaves of the device less 16 GiB, and at least 32 GiB, and its held
+   limit (rpd_ctx_set_held_limit) to the budget. */
 int rpd_server_create(rpd_context* ctx, const rpd_server_config* cfg, rpd_server** out);
 /* Queues one job; returns at once with the request id. */
 int rpd_server_submit(rpd_server* s, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam,

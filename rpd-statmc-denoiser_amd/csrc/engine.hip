@@ -131,13 +131,17 @@ struct rpd_context {
     float* f_buf = nullptr;
     size_t f_cap = 0;
     hipStream_t f_stream = nullptr;
-    // device buffers of released jobs, kept by exact byte size for the next
-    // jobs' allocations (hipMalloc and hipFree of a job's few GB cost
-    // milliseconds); at most cache_cap bytes are kept
+    // device slabs of released jobs, by size, for the next jobs: hipMalloc and
+    // hipFree of a job's few GB cost milliseconds, and hipFree waits for all
+    // work queued on the device, which stalls the caller while other jobs run.
+    // Each entry is (slab, insertion number); at most cache_cap bytes are kept,
+    // the oldest freed first.
     std::mutex cache_mu;
-    std::multimap<size_t, void*> cache;
+    std::multimap<size_t, std::pair<void*, uint64_t>> cache;
     uint64_t cache_bytes = 0;
+    uint64_t cache_seq = 0;
     uint64_t cache_cap = uint64_t(32) << 30;
+    uint64_t held_limit = 0;  // a larger kept slab only while jobs hold at most this (0: no limit)
     int bvh_builder = RPD_BVH_HOST_SAH;  // for the scenes created next
     // StatMC window tile shapes: tuned per (radius, image-size class), or forced
     std::mutex tune_mu;
@@ -170,9 +174,8 @@ struct rpd_job {
     rpd::Camera cam{};
     uint32_t seed = 0;
     hipStream_t stream = nullptr;
-    std::vector<void*> owned;
-    std::vector<size_t> owned_bytes;  // size of each owned buffer, for the context's cache
-    uint64_t bytes = 0;
+    void* slab = nullptr;   // every device buffer of the job, carved from one allocation
+    uint64_t bytes = 0;     // the slab's size
     int W = 0, H = 0, N = 0;
     GridGeom geom{};
     // device state
@@ -191,6 +194,9 @@ struct rpd_job {
     float *tiles = nullptr, *node_scratch = nullptr, *filter_scratch = nullptr;
     void* alloc_scratch = nullptr;
     float* dbg = nullptr;    // debug planes (neighbours, weight sum, n_eff, variance source), or null
+    ImageDesc* b1_img = nullptr;        // device copies of the batch of one
+    int64_t* b1_prefix = nullptr;
+    unsigned int* b1_queues = nullptr;
     bool denoiser = false;   // created for denoising records: no scene, camera or tape
     bool capturing = false;  // the stream is being captured into a graph: no per-stage timing events
     int max_spp = 0;         // record capacity per pixel of a denoiser job
@@ -199,6 +205,7 @@ struct rpd_job {
     Batch batch1{};
     bool features_valid = false;
     bool finalized = false;
+    bool tail_pending = false;  // the sampling passes ran; the denoise tail has not
     bool last_pass_statmc = false;
     int last_pass_capture = 0;
     // timing
@@ -470,9 +477,7 @@ int do_statmc(rpd_job* j) {
     return RPD_OK;
 }
 
-int do_variance(rpd_job* j) {
-    StageTimer tm(j, 4);
-    ensure_features(j);
+void launch_job_variance(rpd_job* j, hipStream_t st) {
     VarianceConfig vc;
     vc.radius = j->cfg.var_window_radius;
     vc.iterations = j->cfg.var_iterations;
@@ -481,7 +486,13 @@ int do_variance(rpd_job* j) {
     vc.compat_sigma = j->cfg.var_compat_sigma;
     vc.shrinkage_k = j->cfg.var_shrinkage_k;
     vc.sigma_max = j->cfg.adaptive_sigma_max;
-    launch_variance(j->W, j->H, vc, j->pb, j->feat, j->vtmp, j->vden, j->stream);
+    launch_variance(j->W, j->H, vc, j->pb, j->feat, j->vtmp, j->vden, st);
+}
+
+int do_variance(rpd_job* j) {
+    StageTimer tm(j, 4);
+    ensure_features(j);
+    launch_job_variance(j, j->stream);
     HIP_TRY(hipGetLastError());
     return RPD_OK;
 }
@@ -514,14 +525,15 @@ AllocConfig alloc_config(const rpd_job* j) {
     return ac;
 }
 
-int do_finalize(rpd_job* j, bool statmc) {
-    launch_finalize(j->N, j->pb, statmc ? j->out_raw : j->out_rad, j->out_alb, j->out_nrm, j->out_pos, j->out_dep,
-                    j->stream);
-    if (statmc) HIP_TRY(hipMemcpyAsync(j->out_rad, j->den, sizeof(float) * 3 * j->N, hipMemcpyDeviceToDevice, j->stream));
+int finalize_on(rpd_job* j, bool statmc, hipStream_t st) {
+    launch_finalize(j->N, j->pb, statmc ? j->out_raw : j->out_rad, j->out_alb, j->out_nrm, j->out_pos, j->out_dep, st);
+    if (statmc) HIP_TRY(hipMemcpyAsync(j->out_rad, j->den, sizeof(float) * 3 * j->N, hipMemcpyDeviceToDevice, st));
     j->finalized = true;
     HIP_TRY(hipGetLastError());
     return RPD_OK;
 }
+
+int do_finalize(rpd_job* j, bool statmc) { return finalize_on(j, statmc, j->stream); }
 
 int reset_state(rpd_job* j) {
     const size_t N = j->N;
@@ -592,7 +604,11 @@ int run_adaptive_passes_graph(rpd_job* j, int passes) {
     return RPD_OK;
 }
 
-int run_pipeline(rpd_job* j) {
+// The sampling part: base pass and adaptive passes (main.cpp:1543-1547). A job
+// without StatMC is finished here; otherwise tail_pending is set and
+// run_tail, or a batch of several jobs' tails, completes it.
+int run_passes(rpd_job* j) {
+    j->tail_pending = false;
     int st = reset_state(j);
     if (st) return st;
     const bool statmc = j->cfg.use_statmc != 0;
@@ -614,10 +630,25 @@ int run_pipeline(rpd_job* j) {
             if ((st = pass_after_plan(j, 1, 1))) return st;
         }
     }
-    if ((st = do_recompute_sensitivity(j))) return st;
-    if ((st = do_statmc(j))) return st;
-    if ((st = do_variance(j))) return st;
-    return do_finalize(j, true);
+    j->tail_pending = true;
+    return RPD_OK;
+}
+
+// The denoise tail (main.cpp:1548-1564): sensitivity from the final grid,
+// StatMC, variance denoise of the final statistics, outputs.
+int run_tail(rpd_job* j) {
+    int st = do_recompute_sensitivity(j);
+    if (!st) st = do_statmc(j);
+    if (!st) st = do_variance(j);
+    if (!st) st = do_finalize(j, true);
+    j->tail_pending = false;
+    return st;
+}
+
+int run_pipeline(rpd_job* j) {
+    const int st = run_passes(j);
+    if (st || !j->tail_pending) return st;
+    return run_tail(j);
 }
 
 // The job's image descriptor (everything a denoise-stage kernel reads) and
@@ -671,77 +702,89 @@ int build_batch1(rpd_job* j, ImageDesc* dd, int64_t* dp, unsigned int* dq) {
     return RPD_OK;
 }
 
-// The context's cache of released job buffers: a buffer of exactly the asked
-// size, or null.
-void* cache_take(rpd_context* c, size_t bytes) {
+// Job slabs are sized in classes of eight per doubling (at most 12.5% above
+// the job's need, 1 MiB at least), so that slabs of released jobs fit later
+// jobs of similar size.
+size_t slab_class(size_t bytes) {
+    const size_t min_slab = size_t(1) << 20;
+    if (bytes <= min_slab) return min_slab;
+    const int e = 63 - __builtin_clzll(uint64_t(bytes));
+    const size_t step = size_t(1) << (e - 3);
+    return (bytes + step - 1) / step * step;
+}
+
+// A cached slab of at least `want` bytes and at most half as much again (the
+// smallest such), or null; *got receives its size. A larger slab than `want`
+// is taken only while the bytes jobs hold stay within the context's limit.
+void* cache_take(rpd_context* c, size_t want, size_t* got) {
     std::lock_guard<std::mutex> lk(c->cache_mu);
-    auto it = c->cache.find(bytes);
-    if (it == c->cache.end()) return nullptr;
-    void* p = it->second;
+    size_t most = want + want / 2;
+    if (c->held_limit) {
+        const uint64_t held = c->in_use.load();
+        most = held + want >= c->held_limit ? want : std::min<uint64_t>(most, c->held_limit - held);
+    }
+    auto it = c->cache.lower_bound(want);
+    if (it == c->cache.end() || it->first > most) return nullptr;
+    void* p = it->second.first;
+    *got = it->first;
+    c->cache_bytes -= it->first;
     c->cache.erase(it);
-    c->cache_bytes -= bytes;
     return p;
 }
 
-// Frees cached buffers until at most `keep` bytes are cached.
+// Frees cached slabs, oldest first, until at most `keep` bytes are cached.
+void cache_trim_locked(rpd_context* c, uint64_t keep) {
+    while (c->cache_bytes > keep && !c->cache.empty()) {
+        auto old = c->cache.begin();
+        for (auto it = c->cache.begin(); it != c->cache.end(); ++it)
+            if (it->second.second < old->second.second) old = it;
+        dev_free(old->second.first);
+        c->cache_bytes -= old->first;
+        c->cache.erase(old);
+    }
+}
 void cache_trim(rpd_context* c, uint64_t keep) {
     std::lock_guard<std::mutex> lk(c->cache_mu);
-    for (auto it = c->cache.begin(); it != c->cache.end() && c->cache_bytes > keep;) {
-        dev_free(it->second);
-        c->cache_bytes -= it->first;
-        it = c->cache.erase(it);
-    }
+    cache_trim_locked(c, keep);
 }
 
-// A job buffer: from the cache when one of the exact size is there, else from
-// hipMalloc; when hipMalloc fails the cache is emptied and it is tried once more.
-// Contents are undefined either way (reset_state clears what must start at zero).
-template <class T>
-int jalloc(rpd_job* j, T** p, size_t count) {
-    *p = nullptr;
-    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    void* q = cache_take(j->ctx, bytes);
-    if (!q) {
-        hipError_t e = dev_malloc(&q, bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            cache_trim(j->ctx, 0);
-            e = dev_malloc(&q, bytes);
-        }
-        if (e != hipSuccess) return fail(RPD_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+// The layout of a job's buffers in its slab: each buffer at a 256-byte
+// aligned offset. With base null it only adds up the size.
+struct SlabLayout {
+    char* base = nullptr;
+    size_t off = 0;
+    template <class T>
+    void take(T** p, size_t count) {
+        off = (off + 255) & ~size_t(255);
+        if (base) *p = reinterpret_cast<T*>(base + off);
+        off += std::max<size_t>(count, 1) * sizeof(T);
     }
-    j->owned.push_back(q);
-    j->owned_bytes.push_back(bytes);
-    *p = static_cast<T*>(q);
-    return RPD_OK;
-}
+};
 
-int job_alloc(rpd_job* j) {
+// Places every device buffer of job j (W, H, geom, cfg, denoiser, max_spp
+// and ctx->debug_outputs set) in L; fills j's pointers when L has a base.
+void job_layout(rpd_job* j, SlabLayout& L) {
     const size_t N = j->N;
     const size_t nn = size_t(j->geom.nx) * j->geom.ny;
-    int st;
-#define A(ptr, n)                                        \
-    do {                                                 \
-        if ((st = jalloc(j, &(ptr), (n)))) return st;    \
-    } while (0)
-    float* sums; A(sums, 13 * N);
+#define A(ptr, n) L.take(&(ptr), (n))
+    float* sums = nullptr; A(sums, 13 * N);
     j->pb.radiance = sums; j->pb.albedo = sums + 3 * N; j->pb.normal = sums + 6 * N; j->pb.world_pos = sums + 9 * N;
     j->pb.depth = sums + 12 * N;
-    int* ic; A(ic, 2 * N);
+    int* ic = nullptr; A(ic, 2 * N);
     j->pb.hit_count = ic; j->pb.sample_count = ic + N;
-    float* stats; A(stats, 12 * N);
+    float* stats = nullptr; A(stats, 12 * N);
     j->pb.lum_mean = stats; j->pb.lum_m2 = stats + N; j->pb.n = reinterpret_cast<int*>(stats + 2 * N);
     j->pb.color_mean = stats + 3 * N; j->pb.sqrt_mean = stats + 6 * N; j->pb.sqrt_m2 = stats + 9 * N;
-    float* fe; A(fe, 7 * N);
+    float* fe = nullptr; A(fe, 7 * N);
     j->feat.normal = fe; j->feat.depth = fe + 3 * N; j->feat.albedo = fe + 4 * N;
-    float* misc; A(misc, 14 * N);
+    float* misc = nullptr; A(misc, 14 * N);
     j->den = misc; j->unc = misc + 3 * N; j->alp = misc + 4 * N; j->vtot = misc + 5 * N; j->veff = misc + 6 * N;
     j->vrat = misc + 7 * N; j->vden = misc + 8 * N; j->vtmp = misc + 9 * N; j->imp = misc + 10 * N;
     j->sconf = misc + 11 * N; j->sgrad = misc + 12 * N;
-    float* outs; A(outs, 16 * N);
+    float* outs = nullptr; A(outs, 16 * N);
     j->out_rad = outs; j->out_raw = outs + 3 * N; j->out_alb = outs + 6 * N; j->out_nrm = outs + 9 * N;
     j->out_pos = outs + 12 * N; j->out_dep = outs + 15 * N;
-    int* plan; A(plan, 3 * N + 8);
+    int* plan = nullptr; A(plan, 3 * N + 8);
     j->counts = plan; j->offsets = plan + N; j->active = plan + 2 * N; j->totals = plan + 3 * N;
     j->work = plan + 3 * N + 4;
     A(j->scan_scratch, (size_t)plan_scan_scratch_elems(j->N));
@@ -758,27 +801,62 @@ int job_alloc(rpd_job* j) {
     A(j->tiles, nn * SF_COUNT);
     A(j->node_scratch, nn * 33);
     A(j->filter_scratch, FILTER_SCRATCH_FLOATS * N);
-    char* as; A(as, alloc_scratch_bytes(j->N));
+    char* as = nullptr; A(as, alloc_scratch_bytes(j->N));
     j->alloc_scratch = as;
-    if (j->ctx->debug_outputs) A(j->dbg, 4 * N);
-    ImageDesc* dd; A(dd, 1);
-    int64_t* dp; A(dp, BK_KINDS * 2);
-    unsigned int* dq; A(dq, BK_KINDS * BATCH_QUEUES);
+    if (j->ctx && j->ctx->debug_outputs) A(j->dbg, 4 * N);
+    A(j->b1_img, 1);
+    A(j->b1_prefix, BK_KINDS * 2);
+    A(j->b1_queues, BK_KINDS * BATCH_QUEUES);
 #undef A
-    return build_batch1(j, dd, dp, dq);
 }
 
-// Device bytes of a job; a denoiser job (max_spp > 0) has no tape and room
-// for max_spp records per pixel.
+// One slab for all of the job's buffers: a cached one when one fits, else
+// hipMalloc (when that fails the cache is emptied and it is tried once more).
+// Contents are undefined either way (reset_state clears what must start at zero).
+int job_alloc(rpd_job* j) {
+    SlabLayout L;
+    job_layout(j, L);
+    const size_t want = slab_class(L.off);
+    size_t got = 0;
+    void* q = cache_take(j->ctx, want, &got);
+    if (!q) {
+        got = want;
+        hipError_t e = dev_malloc(&q, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            cache_trim(j->ctx, 0);
+            e = dev_malloc(&q, want);
+        }
+        if (e != hipSuccess) return fail(RPD_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    j->slab = q;
+    j->ctx->in_use += got;
+    j->bytes = got;
+    L.base = static_cast<char*>(q);
+    L.off = 0;
+    job_layout(j, L);
+    return build_batch1(j, j->b1_img, j->b1_prefix, j->b1_queues);
+}
+
+void set_geometry(rpd_job* j) {
+    j->geom.support = j->cfg.rpf_tile_size;
+    j->geom.step = std::max(1, j->cfg.rpf_tile_size / 2);
+    j->geom.nx = std::max(2, (j->W + j->geom.step - 1) / j->geom.step + 1);
+    j->geom.ny = std::max(2, (j->H + j->geom.step - 1) / j->geom.step + 1);
+}
+
+// Device bytes of a job (its slab, without debug planes); a denoiser job
+// (max_spp > 0) has no tape and room for max_spp records per pixel.
 size_t job_bytes(const rpd_render_config& c, int max_spp = 0) {
-    const size_t N = size_t(c.image_width) * c.image_height;
-    const int step = std::max(1, c.rpf_tile_size / 2);
-    const size_t nn = size_t(std::max(2, (c.image_width + step - 1) / step + 1)) *
-                      size_t(std::max(2, (c.image_height + step - 1) / step + 1));
-    const size_t per_px = max_spp > 0 ? size_t(max_spp) : size_t(std::max(c.samples_per_pixel, c.adaptive_spp + 1));
-    const size_t tape = max_spp > 0 ? 0 : ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_N * 4;
-    return N * 4 * (13 + 2 + 12 + 7 + 14 + 13 + 1 + 3 + 3 + FILTER_SCRATCH_FLOATS) + tape + per_px * N * (6 * 16 + 8) +
-           nn * (GRID_NODE_DOUBLES * 8 + SF_COUNT * 4 + 33 * 4) + alloc_scratch_bytes(int(N)) + sizeof(ImageDesc) + 512;
+    rpd_job t;
+    t.cfg = c;
+    t.W = c.image_width; t.H = c.image_height; t.N = t.W * t.H;
+    t.denoiser = max_spp > 0;
+    t.max_spp = max_spp;
+    set_geometry(&t);
+    SlabLayout L;
+    job_layout(&t, L);
+    return slab_class(L.off);
 }
 
 void job_free(rpd_job* j) {
@@ -786,26 +864,19 @@ void job_free(rpd_job* j) {
     if (j->stream) hipStreamSynchronize(j->stream);
     for (auto& s : j->spans) { event_destroy(s.a); event_destroy(s.b); }
     j->spans.clear();
-    // the stream is idle: the buffers go to the context's cache, up to its cap
-    if (j->ctx) {
+    // the stream is idle: the slab goes to the context's cache, which frees its
+    // oldest slabs beyond the cap
+    if (j->slab) {
         std::lock_guard<std::mutex> lk(j->ctx->cache_mu);
-        for (size_t k = 0; k < j->owned.size(); ++k) {
-            const size_t b = j->owned_bytes[k];
-            if (j->ctx->cache_bytes + b <= j->ctx->cache_cap) {
-                j->ctx->cache.emplace(b, j->owned[k]);
-                j->ctx->cache_bytes += b;
-            } else {
-                dev_free(j->owned[k]);
-            }
-        }
-    } else {
-        for (void* p : j->owned) dev_free(p);
+        j->ctx->cache.emplace(size_t(j->bytes), std::make_pair(j->slab, j->ctx->cache_seq++));
+        j->ctx->cache_bytes += j->bytes;
+        cache_trim_locked(j->ctx, j->ctx->cache_cap);
+        j->ctx->in_use -= j->bytes;
     }
-    j->owned.clear();
-    j->owned_bytes.clear();
+    j->slab = nullptr;
+    j->bytes = 0;
     if (j->stream) stream_destroy(j->stream);
     j->stream = nullptr;
-    if (j->ctx) j->ctx->in_use -= j->bytes;
 }
 
 int job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam, uint32_t seed,
@@ -820,14 +891,9 @@ int job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg,
     j->ctx = ctx; j->scene = scene; j->cfg = *cfg; j->seed = seed;
     j->W = cfg->image_width; j->H = cfg->image_height; j->N = j->W * j->H;
     if ((st = make_camera(*cam, float(j->W) / float(j->H), j->cam))) return st;
-    j->geom.support = cfg->rpf_tile_size;
-    j->geom.step = std::max(1, cfg->rpf_tile_size / 2);
-    j->geom.nx = std::max(2, (j->W + j->geom.step - 1) / j->geom.step + 1);
-    j->geom.ny = std::max(2, (j->H + j->geom.step - 1) / j->geom.step + 1);
-    j->bytes = job_bytes(*cfg);
-    if (ctx->budget && ctx->in_use.load() + j->bytes > ctx->budget)
+    set_geometry(j.get());
+    if (ctx->budget && ctx->in_use.load() + job_bytes(*cfg) > ctx->budget)
         return fail(RPD_ERR_OUT_OF_MEMORY, "job does not fit the context's memory budget");
-    ctx->in_use += j->bytes;
     HIP_TRY(stream_create(&j->stream));
     if ((st = job_alloc(j.get()))) { job_free(j.get()); return st; }
     if ((st = reset_state(j.get()))) { job_free(j.get()); return st; }
@@ -846,14 +912,9 @@ int job_create_denoiser(rpd_context* ctx, const rpd_render_config* cfg, int max_
     j->ctx = ctx; j->scene = nullptr; j->cfg = *cfg; j->seed = 0;
     j->denoiser = true; j->max_spp = max_spp;
     j->W = cfg->image_width; j->H = cfg->image_height; j->N = j->W * j->H;
-    j->geom.support = cfg->rpf_tile_size;
-    j->geom.step = std::max(1, cfg->rpf_tile_size / 2);
-    j->geom.nx = std::max(2, (j->W + j->geom.step - 1) / j->geom.step + 1);
-    j->geom.ny = std::max(2, (j->H + j->geom.step - 1) / j->geom.step + 1);
-    j->bytes = job_bytes(*cfg, max_spp);
-    if (ctx->budget && ctx->in_use.load() + j->bytes > ctx->budget)
+    set_geometry(j.get());
+    if (ctx->budget && ctx->in_use.load() + job_bytes(*cfg, max_spp) > ctx->budget)
         return fail(RPD_ERR_OUT_OF_MEMORY, "job does not fit the context's memory budget");
-    ctx->in_use += j->bytes;
     HIP_TRY(stream_create(&j->stream));
     if ((st = job_alloc(j.get()))) { job_free(j.get()); return st; }
     if ((st = reset_state(j.get()))) { job_free(j.get()); return st; }
@@ -1045,6 +1106,13 @@ int rpd_ctx_set_bvh_builder(rpd_context* ctx, int builder) {
     if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
     if (builder != RPD_BVH_HOST_SAH && builder != RPD_BVH_DEVICE_LBVH) return fail(RPD_ERR_INVALID_ARG, "unknown BVH builder");
     ctx->bvh_builder = builder;
+    return RPD_OK;
+}
+
+int rpd_ctx_set_held_limit(rpd_context* ctx, uint64_t bytes) {
+    if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
+    std::lock_guard<std::mutex> lk(ctx->cache_mu);
+    ctx->held_limit = bytes;
     return RPD_OK;
 }
 
@@ -1332,6 +1400,22 @@ int rpd_job_submit(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* 
     return RPD_OK;
 }
 
+int rpd_job_submit_passes(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam,
+                          uint32_t seed, rpd_job** out) {
+    int st = job_create(ctx, scene, cfg, cam, seed, out);
+    if (st) return st;
+    rpd_job* j = *out;
+    const int dev = ctx->device;
+    j->worker = std::thread([j, dev]() {
+        hipSetDevice(dev);
+        const int s = run_passes(j);
+        set_error_from(j, s);
+        hipStreamSynchronize(j->stream);
+        j->done.store(1);
+    });
+    return RPD_OK;
+}
+
 int rpd_job_poll(rpd_job* j, int* done) {
     if (!j || !done) return fail(RPD_ERR_INVALID_ARG, "null argument");
     *done = j->worker.joinable() ? j->done.load() : 1;
@@ -1518,7 +1602,8 @@ int rpd_job_denoise_records(rpd_job* j) {
     return st ? st : rpd_job_sync(j);
 }
 
-int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu_ms) {
+// Checks a list of jobs for a batch call.
+static int check_batch_list(rpd_context* ctx, rpd_job* const* jobs, int n) {
     if (!ctx || !jobs || n <= 0) return fail(RPD_ERR_INVALID_ARG, "null or empty batch");
     for (int k = 0; k < n; ++k) {
         if (!jobs[k]) return fail(RPD_ERR_INVALID_ARG, "null job in batch");
@@ -1526,22 +1611,23 @@ int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu
         for (int m = 0; m < k; ++m)
             if (jobs[m] == jobs[k]) return fail(RPD_ERR_INVALID_ARG, "job listed twice in batch");
     }
-    std::lock_guard<std::mutex> lk(ctx->batch_mu);
-    HIP_TRY(hipSetDevice(ctx->device));
-    // inputs were produced on the jobs' own streams
-    std::vector<rpd_job*> batched, alone;
-    for (int k = 0; k < n; ++k) {
-        if (jobs[k]->worker.joinable()) jobs[k]->worker.join();
-        HIP_TRY(hipStreamSynchronize(jobs[k]->stream));
-        (capture_records(jobs[k]) ? batched : alone).push_back(jobs[k]);
-    }
+    return RPD_OK;
+}
+
+// The denoise stages of `jobs` as one batch on the context's batch stream
+// (ctx->batch_mu held): with `records`, the fold of each job's loaded pass
+// and the grid gather first (main.cpp:707-717); then features, sensitivity
+// (main.cpp:741-953) and StatMC (main.cpp:957-1162), each one launch over
+// every image.
+static int batch_denoise_stages(rpd_context* ctx, const std::vector<rpd_job*>& jobs, bool records) {
     if (!ctx->b_stream) {
         HIP_TRY(stream_create(&ctx->b_stream));
         HIP_TRY(event_create(&ctx->b_ev[0]));
         HIP_TRY(event_create(&ctx->b_ev[1]));
         HIP_TRY(dev_malloc(&ctx->b_queues, sizeof(unsigned int) * BK_KINDS * BATCH_QUEUES));
     }
-    const int nb = int(batched.size());
+    const int nb = int(jobs.size());
+    if (nb == 0) return RPD_OK;
     if (nb > ctx->b_cap) {
         if (ctx->b_img) HIP_TRY(dev_free(ctx->b_img));
         if (ctx->b_prefix) HIP_TRY(dev_free(ctx->b_prefix));
@@ -1551,67 +1637,122 @@ int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu
         ctx->b_cap = nb;
     }
     hipStream_t s = ctx->b_stream;
-    HIP_TRY(hipEventRecord(ctx->b_ev[0], s));
-    if (nb > 0) {
-        std::vector<ImageDesc> descs(nb);
-        std::vector<int64_t> pf(size_t(BK_KINDS) * (nb + 1), 0);
-        Batch b{};
-        b.tile_w = STATMC_SHAPES[0][0];
-        b.tile_h = STATMC_SHAPES[0][1];
-        for (int k = 0; k < nb; ++k) {
-            descs[k] = batched[k]->desc;
-            b.max_step = std::max(b.max_step, descs[k].g.step);
-            b.max_radius = std::max(b.max_radius, descs[k].fc.radius);
-        }
-        for (int kind = 0; kind < BK_KINDS; ++kind) {
-            for (int k = 0; k < nb; ++k)
-                pf[size_t(kind) * (nb + 1) + k + 1] =
-                    pf[size_t(kind) * (nb + 1) + k] + image_items(kind, descs[k].W, descs[k].H, descs[k].g);
-            b.total[kind] = pf[size_t(kind) * (nb + 1) + nb];
-        }
-        HIP_TRY(hipMemcpyAsync(ctx->b_img, descs.data(), sizeof(ImageDesc) * nb, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(ctx->b_prefix, pf.data(), sizeof(int64_t) * pf.size(), hipMemcpyHostToDevice, s));
-        b.img = ctx->b_img; b.prefix = ctx->b_prefix; b.queues = ctx->b_queues; b.n = nb;
+    std::vector<ImageDesc> descs(nb);
+    std::vector<int64_t> pf(size_t(BK_KINDS) * (nb + 1), 0);
+    Batch b{};
+    b.tile_w = STATMC_SHAPES[0][0];
+    b.tile_h = STATMC_SHAPES[0][1];
+    for (int k = 0; k < nb; ++k) {
+        descs[k] = jobs[k]->desc;
+        b.max_step = std::max(b.max_step, descs[k].g.step);
+        b.max_radius = std::max(b.max_radius, descs[k].fc.radius);
+    }
+    for (int kind = 0; kind < BK_KINDS; ++kind) {
+        for (int k = 0; k < nb; ++k)
+            pf[size_t(kind) * (nb + 1) + k + 1] =
+                pf[size_t(kind) * (nb + 1) + k] + image_items(kind, descs[k].W, descs[k].H, descs[k].g);
+        b.total[kind] = pf[size_t(kind) * (nb + 1) + nb];
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->b_img, descs.data(), sizeof(ImageDesc) * nb, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(ctx->b_prefix, pf.data(), sizeof(int64_t) * pf.size(), hipMemcpyHostToDevice, s));
+    b.img = ctx->b_img; b.prefix = ctx->b_prefix; b.queues = ctx->b_queues; b.n = nb;
+    if (records) {
         launch_fold(b, 1, s);
         launch_batch_queues_reset(b, s);
         launch_grid_gather(b, s);
-        launch_features(b, s);
-        launch_sensitivity(b, s);
-        launch_statmc_prepare(b, s);
-        // the BK_TILE prefix row for a tile shape, copied before the launch that uses it
-        // one host row per launch: each stays alive until the batch has finished
-        std::vector<std::vector<int64_t>> tile_rows;
-        auto set_shape = [&](Batch& bb, int k) -> int {
-            tile_rows.emplace_back(size_t(nb) + 1);
-            std::vector<int64_t>& tile_row = tile_rows.back();
-            bb.tile_w = STATMC_SHAPES[k][0];
-            bb.tile_h = STATMC_SHAPES[k][1];
-            tile_row[0] = 0;
-            for (int m = 0; m < nb; ++m)
-                tile_row[m + 1] = tile_row[m] + image_items(BK_TILE, descs[m].W, descs[m].H, descs[m].g, bb.tile_w,
-                                                            bb.tile_h);
-            bb.total[BK_TILE] = tile_row[nb];
-            HIP_TRY(hipMemcpyAsync(ctx->b_prefix + size_t(BK_TILE) * (nb + 1), tile_row.data(),
-                                   sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice, s));
-            return RPD_OK;
-        };
-        const int st = run_statmc_window(ctx, b, s, true, set_shape);
-        if (st) return st;
-        HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(hipEventRecord(ctx->b_ev[1], s));
-    HIP_TRY(hipEventSynchronize(ctx->b_ev[1]));
-    for (rpd_job* j : batched) {
+    launch_features(b, s);
+    launch_sensitivity(b, s);
+    launch_statmc_prepare(b, s);
+    // the BK_TILE prefix row for a tile shape, copied before the launch that uses it;
+    // one host row per launch, all kept until the stream has drained below
+    std::vector<std::vector<int64_t>> tile_rows;
+    auto set_shape = [&](Batch& bb, int k) -> int {
+        tile_rows.emplace_back(size_t(nb) + 1);
+        std::vector<int64_t>& tile_row = tile_rows.back();
+        bb.tile_w = STATMC_SHAPES[k][0];
+        bb.tile_h = STATMC_SHAPES[k][1];
+        tile_row[0] = 0;
+        for (int m = 0; m < nb; ++m)
+            tile_row[m + 1] = tile_row[m] + image_items(BK_TILE, descs[m].W, descs[m].H, descs[m].g, bb.tile_w,
+                                                        bb.tile_h);
+        bb.total[BK_TILE] = tile_row[nb];
+        HIP_TRY(hipMemcpyAsync(ctx->b_prefix + size_t(BK_TILE) * (nb + 1), tile_row.data(),
+                               sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice, s));
+        return RPD_OK;
+    };
+    const int st = run_statmc_window(ctx, b, s, true, set_shape);
+    if (st) return st;
+    // the host-side descriptor and prefix rows are read by asynchronous copies
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipGetLastError());
+    for (rpd_job* j : jobs) {
         j->features_valid = true;
         j->last_pass_statmc = true;
         j->last_pass_capture = true;
     }
+    return RPD_OK;
+}
+
+int rpd_denoise_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu_ms) {
+    int st = check_batch_list(ctx, jobs, n);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(ctx->batch_mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    // inputs were produced on the jobs' own streams
+    std::vector<rpd_job*> batched, alone;
+    for (int k = 0; k < n; ++k) {
+        if (jobs[k]->worker.joinable()) jobs[k]->worker.join();
+        HIP_TRY(hipStreamSynchronize(jobs[k]->stream));
+        (capture_records(jobs[k]) ? batched : alone).push_back(jobs[k]);
+    }
+    if (!ctx->b_stream && (st = batch_denoise_stages(ctx, {}, true))) return st;  // creates the stream
+    HIP_TRY(hipEventRecord(ctx->b_ev[0], ctx->b_stream));
+    if ((st = batch_denoise_stages(ctx, batched, true))) return st;
+    HIP_TRY(hipEventRecord(ctx->b_ev[1], ctx->b_stream));
+    HIP_TRY(hipEventSynchronize(ctx->b_ev[1]));
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->b_ev[0], ctx->b_ev[1]));
     // images without RPF capture (rpf_shrinkage_scale <= 0) take the per-image path
     for (rpd_job* j : alone) {
-        const int st = rpd_job_denoise_records(j);
-        if (st) return st;
+        if ((st = rpd_job_denoise_records(j))) return st;
+    }
+    if (gpu_ms) *gpu_ms = double(ms);
+    return RPD_OK;
+}
+
+int rpd_render_finish_batch(rpd_context* ctx, rpd_job* const* jobs, int n, double* gpu_ms) {
+    int st = check_batch_list(ctx, jobs, n);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(ctx->batch_mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<rpd_job*> batched, alone;
+    for (int k = 0; k < n; ++k) {
+        rpd_job* j = jobs[k];
+        if (j->worker.joinable()) j->worker.join();
+        HIP_TRY(hipStreamSynchronize(j->stream));
+        if (j->status != RPD_OK) return fail(j->status, j->error);
+        if (!j->tail_pending)
+            return fail(RPD_ERR_STATE, "job has no pending denoise tail (start it with rpd_job_submit_passes)");
+        (capture_records(j) ? batched : alone).push_back(j);
+    }
+    if (!ctx->b_stream && (st = batch_denoise_stages(ctx, {}, false))) return st;  // creates the stream
+    hipStream_t s = ctx->b_stream;
+    HIP_TRY(hipEventRecord(ctx->b_ev[0], s));
+    if ((st = batch_denoise_stages(ctx, batched, false))) return st;
+    for (rpd_job* j : batched) {          // main.cpp:1552-1564: variance denoise, outputs
+        launch_job_variance(j, s);
+        if ((st = finalize_on(j, true, s))) return st;
+        j->tail_pending = false;
+    }
+    HIP_TRY(hipEventRecord(ctx->b_ev[1], s));
+    HIP_TRY(hipEventSynchronize(ctx->b_ev[1]));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->b_ev[0], ctx->b_ev[1]));
+    // images without RPF capture take the per-image tail on their own streams
+    for (rpd_job* j : alone) {
+        if ((st = run_tail(j))) return st;
+        HIP_TRY(hipStreamSynchronize(j->stream));
     }
     if (gpu_ms) *gpu_ms = double(ms);
     return RPD_OK;

@@ -2,7 +2,12 @@
 // (FIFO with a short look-ahead, so a small job can pass a large one that
 // does not fit yet) whenever a slot is free and the job's device footprint
 // fits the memory budget, starts each admitted job on its own stream, and
-// records completion. Built on the public C ABI, plus the engine's
+// records completion. With cobatch_tail, a job's stream runs only its
+// sampling passes; jobs whose passes have finished queue for a tail thread,
+// which runs the denoise tails of everything queued as one batch, one launch
+// per stage over all of them (rpd_render_finish_batch). While a batch runs,
+// the jobs finishing their passes queue for the next one, and the scheduler
+// goes on admitting. Built on the public C ABI, plus the engine's
 // thread-local error setter so a waiter sees the message of a failure that
 // happened on the scheduler thread.
 #include "../../include/rpd_engine.h"
@@ -44,6 +49,7 @@ struct Request {
     int status = 0;
     std::string error;  // rpd_last_error of the failure, taken on the scheduler thread
     bool waited = false;  // a caller is blocked in rpd_server_wait on it
+    bool tail = false;    // started with its passes only: the denoise tail is the scheduler's
     enum State { QUEUED, RUNNING, DONE, CLAIMED } state = QUEUED;
 };
 
@@ -68,8 +74,13 @@ struct rpd_server {
     std::map<uint64_t, std::unique_ptr<Request>> all;
     std::deque<Request*> queue;
     std::vector<Request*> running;
+    std::deque<Request*> tail_q;   // passes finished, tail not yet started
+    int tail_busy = 0;             // jobs in the tail batch being run
+    std::condition_variable cv_tail;
+    std::thread tail_thr;
+    bool stop_tail = false;
     uint64_t next_id = 1, peak = 0;
-    uint64_t submitted = 0, completed = 0, failed = 0;
+    uint64_t submitted = 0, completed = 0, failed = 0, tail_batches = 0, tail_jobs = 0;
     int max_seen = 0;
     std::vector<double> lat, qwait;
     bool stop = false;
@@ -86,8 +97,10 @@ struct rpd_server {
     void start(Request* r, std::unique_lock<std::mutex>& lk) {
         r->t_admit = Clock::now();
         qwait.push_back(ms_between(r->t_submit, r->t_admit));
+        r->tail = cfg.cobatch_tail && r->cfg.use_statmc;  // a job without StatMC has no tail
         lk.unlock();
-        const int st = rpd_job_submit(ctx, r->scene, &r->cfg, &r->cam, r->seed, &r->job);
+        const int st = r->tail ? rpd_job_submit_passes(ctx, r->scene, &r->cfg, &r->cam, r->seed, &r->job)
+                               : rpd_job_submit(ctx, r->scene, &r->cfg, &r->cam, r->seed, &r->job);
         const std::string err = st ? rpd_last_error() : "";
         lk.lock();
         if (st) {
@@ -101,35 +114,48 @@ struct rpd_server {
         } else {
             r->state = Request::RUNNING;
             running.push_back(r);
-            max_seen = std::max(max_seen, int(running.size()));
+            max_seen = std::max(max_seen, in_flight());
             peak = std::max(peak, held());
         }
+    }
+
+    // admitted and not done: sampling, or waiting for or in a tail batch
+    int in_flight() const { return int(running.size() + tail_q.size()) + tail_busy; }
+
+    void finish(Request* r, int status, const std::string& err) {
+        r->status = status;
+        if (status) r->error = err;
+        r->t_done = Clock::now();
+        r->state = Request::DONE;
+        lat.push_back(ms_between(r->t_submit, r->t_done));
+        if (status) ++failed; else ++completed;
+        cv_done.notify_all();
     }
 
     void loop() {
         std::unique_lock<std::mutex> lk(mu);
         while (true) {
-            // completions
+            // completions; jobs whose passes finished queue for the tail thread
             for (size_t k = 0; k < running.size();) {
                 Request* r = running[k];
                 int done = 0;
                 rpd_job_poll(r->job, &done);
                 if (!done) { ++k; continue; }
-                r->status = rpd_job_wait(r->job);  // joins the job's worker; already finished
-                if (r->status) r->error = rpd_last_error();
-                r->t_done = Clock::now();
-                r->state = Request::DONE;
-                lat.push_back(ms_between(r->t_submit, r->t_done));
-                if (r->status) ++failed; else ++completed;
+                const int st = rpd_job_wait(r->job);  // joins the job's worker; already finished
                 running.erase(running.begin() + long(k));
-                cv_done.notify_all();
+                if (!st && r->tail) {
+                    tail_q.push_back(r);
+                    cv_tail.notify_one();
+                } else {
+                    finish(r, st, st ? rpd_last_error() : "");
+                }
             }
             // admission, against the bytes the context's jobs hold now: a finished
             // job keeps its device memory until whoever claimed it releases it,
             // so finished but unreleased jobs count as well as running ones.
             int looked = 0;
             for (auto it = queue.begin(); it != queue.end() && looked < kLookAhead &&
-                                          int(running.size()) < cfg.max_in_flight;) {
+                                          in_flight() < cfg.max_in_flight;) {
                 ++looked;
                 if (held() + (*it)->bytes > budget) { ++it; continue; }
                 Request* r = *it;
@@ -143,7 +169,7 @@ struct rpd_server {
             // one that claims in submission order while later requests finished
             // first) would wait for ever, so that request is admitted over the
             // budget; so is the oldest one once the server is stopping.
-            if (running.empty() && !queue.empty()) {
+            if (in_flight() == 0 && !queue.empty()) {
                 auto it = std::find_if(queue.begin(), queue.end(), [](const Request* r) { return r->waited; });
                 if (it == queue.end() && stop) it = queue.begin();
                 if (it != queue.end()) {
@@ -152,8 +178,31 @@ struct rpd_server {
                     start(r, lk);
                 }
             }
-            if (stop && queue.empty() && running.empty()) break;
+            if (stop && queue.empty() && in_flight() == 0) break;
             cv_sched.wait_for(lk, std::chrono::microseconds(running.empty() ? 2000 : 100));
+        }
+    }
+
+    // Runs everything in the tail queue as one batch, until stopped.
+    void tail_loop() {
+        std::unique_lock<std::mutex> lk(mu);
+        while (true) {
+            cv_tail.wait(lk, [this]() { return !tail_q.empty() || stop_tail; });
+            if (tail_q.empty()) break;
+            std::vector<Request*> batch(tail_q.begin(), tail_q.end());
+            tail_q.clear();
+            tail_busy = int(batch.size());
+            std::vector<rpd_job*> jobs;
+            for (Request* r : batch) jobs.push_back(r->job);
+            lk.unlock();
+            const int st = rpd_render_finish_batch(ctx, jobs.data(), int(jobs.size()), nullptr);
+            const std::string err = st ? rpd_last_error() : "";
+            lk.lock();
+            tail_busy = 0;
+            ++tail_batches;
+            tail_jobs += batch.size();
+            for (Request* r : batch) finish(r, st, err);
+            cv_sched.notify_one();   // slots are free again
         }
     }
 };
@@ -167,16 +216,24 @@ int rpd_server_create(rpd_context* ctx, const rpd_server_config* cfg, rpd_server
     s->ctx = ctx;
     s->cfg = *cfg;
     s->budget = cfg->memory_budget;
-    if (!s->budget) {
-        uint64_t fr = 0, tot = 0;
-        const int st = rpd_ctx_memory(ctx, nullptr, nullptr, &fr, &tot);
-        if (st) return st;
-        s->budget = fr;
-    }
-    int st = rpd_ctx_set_memory_budget(ctx, 0);  // admission is the server's; the context does not refuse
+    uint64_t fr = 0, tot = 0;
+    int st = rpd_ctx_memory(ctx, nullptr, nullptr, &fr, &tot);
+    if (st) return st;
+    if (!s->budget) s->budget = fr;
+    // Released jobs' slabs are kept for the next admitted jobs in what the
+    // budget leaves of the device, less a reserve for scenes and the context's
+    // own buffers, and never less than the context's default; a job that finds
+    // no room empties the cache and allocates again.
+    const uint64_t reserve = uint64_t(16) << 30, floor_cap = uint64_t(32) << 30;
+    const uint64_t spare = tot > s->budget + reserve ? tot - s->budget - reserve : 0;
+    st = rpd_ctx_set_buffer_cache(ctx, std::max(spare, floor_cap), nullptr);
+    if (!st) st = rpd_ctx_set_held_limit(ctx, s->budget);
+    if (st) return st;
+    st = rpd_ctx_set_memory_budget(ctx, 0);  // admission is the server's; the context does not refuse
     if (!st) st = rpd_ctx_set_graph_passes(ctx, cfg->graph_passes);
     if (st) return st;
     rpd_server* raw = s.get();
+    if (cfg->cobatch_tail) s->tail_thr = std::thread([raw]() { raw->tail_loop(); });
     s->sched = std::thread([raw]() { raw->loop(); });
     *out = s.release();
     return RPD_OK;
@@ -247,6 +304,8 @@ int rpd_server_stats_get(rpd_server* s, rpd_server_stats* out) {
     out->latency_max_ms = mx;
     out->queue_p50_ms = percentile(s->qwait, 0.50);
     out->queue_p99_ms = percentile(s->qwait, 0.99);
+    out->tail_batches = s->tail_batches;
+    out->tail_jobs = s->tail_jobs;
     return RPD_OK;
 }
 
@@ -257,7 +316,13 @@ int rpd_server_destroy(rpd_server* s) {
         s->stop = true;
     }
     s->cv_sched.notify_all();
-    if (s->sched.joinable()) s->sched.join();
+    if (s->sched.joinable()) s->sched.join();   // returns once nothing is queued or in flight
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop_tail = true;
+    }
+    s->cv_tail.notify_all();
+    if (s->tail_thr.joinable()) s->tail_thr.join();
     for (auto& kv : s->all)
         if (kv.second->job) rpd_job_release(kv.second->job);
     delete s;

@@ -87,14 +87,16 @@ class Light(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("max_in_flight", C.c_int), ("memory_budget", C.c_uint64), ("graph_passes", C.c_int)]
+    _fields_ = [("max_in_flight", C.c_int), ("memory_budget", C.c_uint64), ("graph_passes", C.c_int),
+                ("cobatch_tail", C.c_int)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
                 ("max_in_flight_seen", C.c_int), ("peak_bytes", C.c_uint64),
                 ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double), ("latency_mean_ms", C.c_double),
-                ("latency_max_ms", C.c_double), ("queue_p50_ms", C.c_double), ("queue_p99_ms", C.c_double)]
+                ("latency_max_ms", C.c_double), ("queue_p50_ms", C.c_double), ("queue_p99_ms", C.c_double),
+                ("tail_batches", C.c_uint64), ("tail_jobs", C.c_uint64)]
 
 
 class ImageMetrics(C.Structure):
@@ -191,6 +193,7 @@ def lib():
             ("rpd_ctx_set_memory_budget", [vp, C.c_uint64]),
             ("rpd_ctx_set_bvh_builder", [vp, C.c_int]),
             ("rpd_ctx_set_buffer_cache", [vp, C.c_uint64, C.POINTER(C.c_uint64)]),
+            ("rpd_ctx_set_held_limit", [vp, C.c_uint64]),
             ("rpd_scene_create", [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.POINTER(vp)]),
             ("rpd_scene_builtin", [vp, C.c_char_p, C.POINTER(vp)]),
             ("rpd_scene_counts", [vp, vp, vp, vp, vp]),
@@ -212,6 +215,8 @@ def lib():
             ("rpd_job_synthesize_records", [vp, C.POINTER(SynthConfig)]),
             ("rpd_job_denoise_records", [vp]),
             ("rpd_denoise_batch", [vp, vp, C.c_int, C.POINTER(C.c_double)]),
+            ("rpd_job_submit_passes", [vp, vp, C.POINTER(RenderConfig), C.POINTER(Camera), C.c_uint32, C.POINTER(vp)]),
+            ("rpd_render_finish_batch", [vp, vp, C.c_int, C.POINTER(C.c_double)]),
             ("rpd_ctx_set_graph_passes", [vp, C.c_int]),
             ("rpd_ctx_set_debug_outputs", [vp, C.c_int]),
             ("rpd_statmc_tile_shapes", [vp, C.c_int, C.POINTER(C.c_int)]),
@@ -341,10 +346,14 @@ class Context:
         return b.value
 
     def set_buffer_cache(self, nbytes):
-        """Bytes of released jobs' device buffers kept for reuse (0 disables); returns the bytes now kept."""
+        """Bytes of released jobs' device slabs kept for reuse (0 disables); returns the bytes now kept."""
         kept = C.c_uint64()
         check(self.L.rpd_ctx_set_buffer_cache(self.h, int(nbytes), C.byref(kept)))
         return kept.value
+
+    def set_held_limit(self, nbytes):
+        """Held bytes up to which a job may take a kept slab larger than it needs (0: no limit)."""
+        check(self.L.rpd_ctx_set_held_limit(self.h, int(nbytes)))
 
     def close(self):
         for s in self._scenes.values():
@@ -521,6 +530,15 @@ def denoise_batch(ctx, jobs):
     return ms.value
 
 
+def render_finish_batch(ctx, jobs):
+    """rpd_render_finish_batch: the denoise tails of jobs submitted with
+    submit="passes", as one batch; returns the batch's device time in ms."""
+    arr = (C.c_void_p * len(jobs))(*[j.h for j in jobs])
+    ms = C.c_double()
+    check(ctx.L.rpd_render_finish_batch(ctx.h, arr, len(jobs), C.byref(ms)))
+    return ms.value
+
+
 def job_bytes(cfg):
     b = C.c_uint64()
     check(lib().rpd_job_bytes(C.byref(cfg), C.byref(b)))
@@ -530,10 +548,10 @@ def job_bytes(cfg):
 class Server:
     """Request loop: continuous admission by slots and HBM footprint (rpd_server_*)."""
 
-    def __init__(self, ctx, max_in_flight=4, memory_budget=0, graph_passes=True):
+    def __init__(self, ctx, max_in_flight=4, memory_budget=0, graph_passes=True, cobatch_tail=False):
         self.ctx = ctx
         self.L = ctx.L
-        c = ServerConfig(max_in_flight, int(memory_budget), int(graph_passes))
+        c = ServerConfig(max_in_flight, int(memory_budget), int(graph_passes), int(cobatch_tail))
         h = C.c_void_p()
         check(self.L.rpd_server_create(ctx.h, C.byref(c), C.byref(h)))
         self.h = h
@@ -564,7 +582,9 @@ class Server:
 
 class Job:
     """One render job's device state; stage methods mirror the reference's functions.
-    denoiser_spp > 0 creates a job that denoises given records instead (no scene)."""
+    submit=True queues the whole pipeline, submit="passes" the sampling passes only
+    (render_finish_batch runs the rest). denoiser_spp > 0 creates a job that
+    denoises given records instead (no scene)."""
 
     def __init__(self, ctx, cfg, cam, seed, scene="cornell_box", submit=False, denoiser_spp=0):
         self.L = ctx.L
@@ -574,7 +594,8 @@ class Job:
         if denoiser_spp > 0:
             check(self.L.rpd_job_create_denoiser(ctx.h, C.byref(cfg), int(denoiser_spp), C.byref(h)))
         else:
-            fn = self.L.rpd_job_submit if submit else self.L.rpd_job_create
+            fn = (self.L.rpd_job_submit_passes if submit == "passes" else
+                  self.L.rpd_job_submit if submit else self.L.rpd_job_create)
             check(fn(ctx.h, ctx.scene(scene), C.byref(cfg), C.byref(cam), seed & 0xFFFFFFFF, C.byref(h)))
         self.h = h
         d = np.zeros(4, np.int32)

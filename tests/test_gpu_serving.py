@@ -2,7 +2,8 @@
 
 A job's result must not depend on how it was launched: kernel by kernel or
 from a captured graph, alone or admitted together with other jobs whose work
-overlaps on the device. Both are compared bit for bit.
+overlaps on the device, with its denoise tail on its own stream or batched
+with other jobs' tails. All are compared bit for bit.
 """
 import numpy as np
 import pytest
@@ -55,29 +56,83 @@ def test_graph_passes_identical_to_eager():
         ctx.close()
 
 
-def test_server_results_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("cobatch", [False, True])
+def test_server_results_same_alone_and_in_the_mix(cobatch):
     ctx = eng.Context(0)
-    srv = eng.Server(ctx, max_in_flight=4, graph_passes=True)
+    plain = []
+    for cfg, cam, seed in _requests():
+        j = eng.Job(ctx, cfg, cam, seed, submit=True)
+        j.wait()
+        plain.append(_outputs(j))
+        j.release()
+    srv = eng.Server(ctx, max_in_flight=4, graph_passes=True, cobatch_tail=cobatch)
     try:
-        alone = []
-        for cfg, cam, seed in _requests():
+        for k, (cfg, cam, seed) in enumerate(_requests()):   # one at a time
             rid = srv.submit(cfg, cam, seed)
             j, _ = srv.wait(rid)
-            alone.append(_outputs(j))
+            _same(plain[k], _outputs(j), f"request {k} alone")
             j.release()
         for rep in range(2):  # everything at once: admitted together, overlapping on the device
             ids = [srv.submit(cfg, cam, seed) for cfg, cam, seed in _requests()]
             for k, rid in enumerate(ids):
                 j, ms = srv.wait(rid)
                 assert ms > 0
-                _same(alone[k], _outputs(j), f"request {k} in the mix (round {rep})")
+                _same(plain[k], _outputs(j), f"request {k} in the mix (round {rep})")
                 j.release()
         st = srv.stats()
         assert st["completed"] == 15 and st["failed"] == 0
         assert st["max_in_flight_seen"] >= 2
         assert st["latency_p99_ms"] >= st["latency_p50_ms"] > 0
+        n_statmc = sum(1 for cfg, _, _ in _requests() if cfg.use_statmc)
+        if cobatch:   # every StatMC job's tail ran in a batch; requests without StatMC have none
+            print(f"tail batches {st['tail_batches']}, jobs {st['tail_jobs']}")
+            assert st["tail_jobs"] == 3 * n_statmc
+            assert 1 <= st["tail_batches"] <= st["tail_jobs"]
+        else:
+            assert st["tail_batches"] == 0 and st["tail_jobs"] == 0
     finally:
         srv.close()
+        ctx.close()
+
+
+def test_render_finish_batch_identical_to_whole_jobs():
+    """Passes on each job's stream, then one tail batch: uneven sizes, radii,
+    adaptive-pass counts, a lens, a job without RPF capture (it takes the
+    per-image tail) and a procedural-scene job; every output equals the whole
+    pipeline's. A job without a pending tail is refused."""
+    ctx = eng.Context(0)
+    try:
+        pc = eng.procedural_config(seed=9, subdivision_level=2, n_spheres=12)
+        ctx.procedural_scene("proc-tail", pc)
+        reqs = []
+        for k, (w, h, spp, passes, ap, radius, shrink) in enumerate(
+                [(64, 48, 4, 3, 0.0, 10, 1.0), (96, 64, 8, 2, 0.14, 5, 1.0), (40, 40, 2, 0, 0.0, 7, 1.0),
+                 (56, 72, 4, 1, 0.0, 3, 0.0)]):
+            rc = ob.config_c0(w, h)
+            rc.spp, rc.adaptive_passes, rc.aperture, rc.focus_distance = spp, passes, ap, 1.05
+            rc.color_window_radius, rc.rpf_shrinkage_scale = radius, shrink
+            reqs.append(("cornell_box", engine_config(rc), engine_camera(rc), 500 + k))
+        cfg = eng.default_config(image_width=72, image_height=40, samples_per_pixel=4, use_statmc=1,
+                                 adaptive_passes=2, adaptive_spp=4, max_depth=8)
+        reqs.append(("proc-tail", cfg, eng.make_camera((0.0, 0.6, 3.4), (0.0, -0.17, -1.0), (0.0, 1.0, 0.0), 40.0,
+                                                      3.4, 0.06), 77))
+        whole = []
+        for scene, cfg, cam, seed in reqs:
+            j = eng.Job(ctx, cfg, cam, seed, scene=scene, submit=True)
+            j.wait()
+            whole.append(_outputs(j))
+            j.release()
+        for rep in range(2):      # the second batch reuses the tuned StatMC shape
+            jobs = [eng.Job(ctx, cfg, cam, seed, scene=scene, submit="passes") for scene, cfg, cam, seed in reqs]
+            ms = eng.render_finish_batch(ctx, jobs)
+            assert ms > 0
+            for k, j in enumerate(jobs):
+                _same(whole[k], _outputs(j), f"request {k} tail in a batch (round {rep})")
+            with pytest.raises(eng.EngineError, match="no pending denoise tail"):
+                eng.render_finish_batch(ctx, jobs[:1])
+            for j in jobs:
+                j.release()
+    finally:
         ctx.close()
 
 
@@ -164,6 +219,40 @@ def test_reused_job_buffers_give_the_same_results():
             j.wait()
             _same(fresh[k], _outputs(j), f"request {k} on reused buffers")
             j.release()
+        assert ctx.set_buffer_cache(0) == 0
+    finally:
+        ctx.close()
+
+
+def test_slab_cache_takes_the_smallest_fitting_slab_within_the_held_limit():
+    """A job's buffers are one slab. A released slab serves a later job that
+    needs at most as much and at least two thirds of it (the job then holds
+    the larger slab); with a held limit at the job's own size it takes only a
+    slab of exactly its size. Beyond the cache cap the oldest slabs are freed."""
+    ctx = eng.Context(0)
+    cam = eng.make_camera()
+    try:
+        big = eng.default_config(image_width=96, image_height=64, samples_per_pixel=24, use_statmc=1)
+        small = eng.default_config(image_width=96, image_height=64, samples_per_pixel=16, use_statmc=1)
+        nb, ns = eng.job_bytes(big), eng.job_bytes(small)
+        assert ns < nb <= ns + ns // 2, (ns, nb)
+        assert ctx.set_buffer_cache(0) == 0
+        assert ctx.set_buffer_cache(1 << 34) == 0
+        j = eng.Job(ctx, big, cam, 1)
+        j.release()
+        assert ctx.set_buffer_cache(1 << 34) == nb
+        j = eng.Job(ctx, small, cam, 2)                 # takes the larger kept slab
+        assert ctx.memory()["in_use"] == nb
+        assert ctx.set_buffer_cache(1 << 34) == 0
+        j.release()
+        ctx.set_held_limit(ns)                          # no room above its own size: a fresh slab
+        j = eng.Job(ctx, small, cam, 3)
+        assert ctx.memory()["in_use"] == ns
+        assert ctx.set_buffer_cache(1 << 34) == nb
+        j.release()
+        ctx.set_held_limit(0)
+        assert ctx.set_buffer_cache(1 << 34) == nb + ns
+        assert ctx.set_buffer_cache(nb) == ns           # over the cap: the oldest (nb) goes first
         assert ctx.set_buffer_cache(0) == 0
     finally:
         ctx.close()
