@@ -1216,9 +1216,14 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
             for (int q = 0; q < 3; ++q) e[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
         }
     }
+    // what shading reads of each triangle, one float4 each
+    if (n_mats > TRI_ATTR_MAX_MATERIALS || n_lights > TRI_ATTR_MAX_LIGHTS)
+        return fail(RPD_ERR_UNSUPPORTED, "more than 65536 materials or 32767 lights");
+    std::vector<float4> attr(static_cast<size_t>(n_tris));
+    for (int i = 0; i < n_tris; ++i) attr[i] = tri_attr_pack(dt[i]);
     // scenes too large for the flat LDS loop get a BVH over triangles and spheres
     BvhHost bvh;
-    const bool use_bvh = trace_needs_bvh(n_tris, n_spheres);
+    const bool use_bvh = trace_needs_bvh(n_tris, n_spheres, n_mats, n_lights);
     bool device_bvh = use_bvh && ctx && ctx->bvh_builder == RPD_BVH_DEVICE_LBVH;
     if (use_bvh && !device_bvh) {
         const int bst = build_bvh4(dt.data(), n_tris, ds.data(), n_spheres, bvh);
@@ -1226,11 +1231,12 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
         if (bst) return fail(RPD_ERR_INVALID_ARG, "scene has no primitives");
     }
     DevTriangle* pt = nullptr; DevSphere* ps = nullptr; DevMaterial* pm = nullptr; DevLight* pl = nullptr;
-    float* pc = nullptr; float4* pp = nullptr;
+    float* pc = nullptr; float4* pp = nullptr; float4* pa = nullptr;
     Bvh4Node* pn = nullptr; float4* pslot = nullptr; int* pprim = nullptr;
     int st;
     if (ctx) {  // a null context keeps the scene on the host (rpd_scene_query only)
         if ((st = dalloc(&pt, dt.size(), sc->owned)) || (st = dalloc(&pp, perm.size(), sc->owned)) ||
+            (st = dalloc(&pa, attr.size(), sc->owned)) ||
             (st = dalloc(&ps, ds.size(), sc->owned)) ||
             (st = dalloc(&pm, dm.size(), sc->owned)) || (st = dalloc(&pl, dl.size(), sc->owned)) ||
             (st = dalloc(&pc, cdf.size(), sc->owned))) {
@@ -1240,6 +1246,7 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
         if (n_tris) {
             HIP_TRY(sync_memcpy(pt, dt.data(), dt.size() * sizeof(DevTriangle), hipMemcpyHostToDevice));
             HIP_TRY(sync_memcpy(pp, perm.data(), perm.size() * sizeof(float4), hipMemcpyHostToDevice));
+            HIP_TRY(sync_memcpy(pa, attr.data(), attr.size() * sizeof(float4), hipMemcpyHostToDevice));
         }
         if (n_spheres) HIP_TRY(sync_memcpy(ps, ds.data(), ds.size() * sizeof(DevSphere), hipMemcpyHostToDevice));
         if (device_bvh) {
@@ -1267,12 +1274,12 @@ int rpd_scene_create(rpd_context* ctx, const rpd_triangle* tris, int n_tris, con
             HIP_TRY(sync_memcpy(pc, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice));
         }
     }
-    sc->dev.tris = pt; sc->dev.n_tris = n_tris; sc->dev.tri_perm = pp;
+    sc->dev.tris = pt; sc->dev.n_tris = n_tris; sc->dev.tri_perm = pp; sc->dev.tri_attr = pa;
     sc->bvh.nodes = pn; sc->bvh.slots = pslot; sc->bvh.slot_prim = pprim;
     sc->bvh_builder = device_bvh ? RPD_BVH_DEVICE_LBVH : RPD_BVH_HOST_SAH;
     sc->h_tris = dt; sc->h_spheres = ds; sc->h_perm = std::move(perm); sc->h_bvh = std::move(bvh);
     sc->dev.spheres = ps; sc->dev.n_spheres = n_spheres;
-    sc->dev.materials = pm;
+    sc->dev.materials = pm; sc->dev.n_materials = n_mats;
     sc->dev.lights = pl; sc->dev.n_lights = n_lights;
     sc->dev.light_cdf = pc;
     for (int k = 0; k < 3; ++k) sc->dev.env[k] = env ? env[k] : 0.0f;

@@ -108,10 +108,15 @@ constexpr int TRACE_CHUNK = 64;
 __device__ __forceinline__ uint64_t ballot64(bool b) { return __ballot(b); }
 __device__ __forceinline__ bool is_zero3(v3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
 
-// Triangle tables up to this many triangles are staged in LDS and looped over
-// (with up to TRACE_FLAT_SPHERES spheres); larger scenes go through the BVH.
-constexpr int TRACE_LDS_TRIS = 256;
+// Scenes of up to TRACE_LDS_TRIS triangles, TRACE_FLAT_SPHERES spheres and
+// TRACE_LDS_MATERIALS materials and lights each are traced by the flat loop,
+// with their triangle tables (the permuted geometry and the shading
+// attributes), materials, lights and light CDF staged in LDS: at most 39.2 KB
+// per 256-lane block, so four blocks fit a CU. Larger scenes go through the
+// BVH and read their tables from global memory.
+constexpr int TRACE_LDS_TRIS = 192;
 constexpr int TRACE_FLAT_SPHERES = 32;
+constexpr int TRACE_LDS_MATERIALS = 64;
 
 // The rays a lane issues in one iteration: all start at `ro` (the camera, or
 // the offset shading point), with their own directions.
@@ -127,14 +132,13 @@ struct LaneRays {
 // then starts a camera ray or shades one bounce, setting q and pend to the
 // queries to run. Returns false once every lane of the wave is out of work.
 template <bool kDepth, bool kLambert>
-__device__ __forceinline__ bool path_step(const TraceParams& p, const int lane, const v3 env, const bool env_on,
+__device__ __forceinline__ bool path_step(const TraceParams& p, const DevScene& S, const int lane, const v3 env, const bool env_on,
                                           int& c_next, int& c_end, const int n_active, const HitRec& h,
                                           const bool occ_l, const bool occ_e, int& st, int& q, int& pend,
                                           int& pix, int& left,
                                           int64_t& slot, Draws& dr, LaneRays& R, v3& T, v3& L, v3& Ld, v3& sp,
                                           v3& sn, v3& swo, v3& cw_l, v3& cw_e, v3& cw_x, float& bpdf, int& smat,
                                           int& depth, uint32_t& flags, uint32_t& light_bits) {
-    const DevScene& S = p.scene;
     const float pdf_env = float(1.0 / (4.0 * RPD_PI));
     // -- resolve the last queries -------------------------------------------
     if (st == ST_WAIT && pend == 0) {
@@ -151,8 +155,8 @@ __device__ __forceinline__ bool path_step(const TraceParams& p, const int lane, 
                     const DevSphere& sp_ = S.spheres[h.index];
                     emitter = sp_.emitter != 0; li = sp_.light; hn = h.sn; hm = sp_.material;
                 } else {
-                    const DevTriangle& t = S.tris[h.index];
-                    emitter = t.emitter != 0; li = t.light; hn = ld3(t.n); hm = t.material;
+                    const TriShade t = tri_attr_unpack(S.tri_attr[h.index]);
+                    emitter = t.emitter; li = t.light; hn = t.n; hm = t.material;
                 }
                 const v3 hp = R.ro + h.t * R.rd_ext;
                 flags |= REC_HIT;
@@ -187,8 +191,8 @@ __device__ __forceinline__ bool path_step(const TraceParams& p, const int lane, 
                         const DevSphere& sp_ = S.spheres[h.index];
                         emitter = sp_.emitter != 0; li = sp_.light; hn = h.sn; hm = sp_.material;
                     } else {
-                        const DevTriangle& t = S.tris[h.index];
-                        emitter = t.emitter != 0; li = t.light; hn = ld3(t.n); hm = t.material;
+                        const TriShade t = tri_attr_unpack(S.tri_attr[h.index]);
+                        emitter = t.emitter; li = t.light; hn = t.n; hm = t.material;
                     }
                 }
                 if (h.hit && emitter) {
@@ -378,11 +382,29 @@ __device__ __forceinline__ bool path_step(const TraceParams& p, const int lane, 
 // mirror code is left out (fewer registers, fewer divergent branches).
 template <bool kBvh, bool kDepth, bool kLambert>
 __global__ void __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) k_trace(TraceParams p) {
-    const DevScene& S = p.scene;
+    // The flat loop's scene: the tables in LDS, the rest as given
+    DevScene S = p.scene;
     extern __shared__ float4 s_dyn[];
     if (!kBvh) {
-        for (int k = threadIdx.x; k < S.n_tris * 9; k += TRACE_BLOCK) s_dyn[k] = S.tri_perm[k];
+        const DevScene& G = p.scene;
+        const int n_perm = G.n_tris * 9;
+        const int n_mat4 = G.n_materials * int(sizeof(DevMaterial) / sizeof(float4));
+        const int n_lig4 = G.n_lights * int(sizeof(DevLight) / sizeof(float4));
+        float4* s_attr = s_dyn + n_perm;
+        float4* s_mat = s_attr + G.n_tris;
+        float4* s_lig = s_mat + n_mat4;
+        float* s_cdf = reinterpret_cast<float*>(s_lig + n_lig4);
+        for (int k = threadIdx.x; k < n_perm; k += TRACE_BLOCK) s_dyn[k] = G.tri_perm[k];
+        for (int k = threadIdx.x; k < G.n_tris; k += TRACE_BLOCK) s_attr[k] = G.tri_attr[k];
+        for (int k = threadIdx.x; k < n_mat4; k += TRACE_BLOCK)
+            s_mat[k] = reinterpret_cast<const float4*>(G.materials)[k];
+        for (int k = threadIdx.x; k < n_lig4; k += TRACE_BLOCK) s_lig[k] = reinterpret_cast<const float4*>(G.lights)[k];
+        for (int k = threadIdx.x; k < G.n_lights; k += TRACE_BLOCK) s_cdf[k] = G.light_cdf[k];
         __syncthreads();
+        S.tri_attr = s_attr;
+        S.materials = reinterpret_cast<const DevMaterial*>(s_mat);
+        S.lights = reinterpret_cast<const DevLight*>(s_lig);
+        S.light_cdf = s_cdf;
     }
     const float4* tp = s_dyn;
     int* stack = reinterpret_cast<int*>(s_dyn) + threadIdx.x;  // entry k at stack[k * TRACE_BLOCK]
@@ -415,7 +437,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_e
     uint32_t n_queries = 0;
 
     while (true) {
-        if (!path_step<kDepth, kLambert>(p, lane, env, env_on, c_next, c_end, n_active, h, occ_l, occ_e, st, q,
+        if (!path_step<kDepth, kLambert>(p, S, lane, env, env_on, c_next, c_end, n_active, h, occ_l, occ_e, st, q,
                                          pend, pix, left, slot, dr, R, T, L, Ld, sp, sn, swo, cw_l, cw_e, cw_x,
                                          bpdf, smat, depth, flags, light_bits)) {
             if (p.query_count) {
@@ -461,11 +483,15 @@ __global__ void __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_e
 }
 int trace_block_size() { return TRACE_BLOCK; }
 
-bool trace_needs_bvh(int n_tris, int n_spheres) { return n_tris > TRACE_LDS_TRIS || n_spheres > TRACE_FLAT_SPHERES; }
+bool trace_needs_bvh(int n_tris, int n_spheres, int n_materials, int n_lights) {
+    return n_tris > TRACE_LDS_TRIS || n_spheres > TRACE_FLAT_SPHERES || n_materials > TRACE_LDS_MATERIALS ||
+           n_lights > TRACE_LDS_MATERIALS;
+}
 
 static size_t trace_lds_bytes(const DevScene& s, const DevBvh& b) {
-    return b.nodes ? size_t(BVH_STACK) * TRACE_BLOCK * sizeof(int)
-                 : std::max<size_t>(16, size_t(s.n_tris) * 9 * sizeof(float4));
+    if (b.nodes) return size_t(BVH_STACK) * TRACE_BLOCK * sizeof(int);
+    return std::max<size_t>(16, size_t(s.n_tris) * 10 * sizeof(float4) + size_t(s.n_materials) * sizeof(DevMaterial) +
+                                    size_t(s.n_lights) * (sizeof(DevLight) + sizeof(float)));
 }
 
 // BVH scenes always take the depth-limited variant (max_depth 0 still means

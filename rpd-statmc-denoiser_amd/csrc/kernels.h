@@ -213,7 +213,8 @@ void launch_tape(const TraceParams& p, hipStream_t st);
 void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st);
 int trace_block_size();
 int trace_blocks_per_cu(const DevScene& s, const DevBvh& b, int max_depth);  // resident trace blocks per CU
-bool trace_needs_bvh(int n_tris, int n_spheres);  // too large for the flat LDS loop: traced through a BVH
+// too large for the flat loop's LDS tables: traced through a BVH
+bool trace_needs_bvh(int n_tris, int n_spheres, int n_materials, int n_lights);
 // per-pixel fold of the current pass's records of every image (Welford statistics when statmc)
 void launch_fold(const Batch& b, int statmc, hipStream_t st);
 void launch_plan_uniform(int n_pixels, int count, int* counts, int* offsets, int* active, int* totals,

@@ -8,6 +8,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #define RPD_EPS 1e-4f         // path_tracer/core/common.h:5
 #define RPD_EPS_ANYHIT 5e-4f  // path_tracer/core/common.h:6
@@ -223,14 +224,40 @@ constexpr int BVH_SLOT_FLOAT4 = 9;   // per primitive slot: triangle (3 axis ord
 struct DevScene {
     const DevTriangle* tris; int n_tris;
     const float4* tri_perm;  // n_tris * 3 entries of 3 float4, see tri_perm_entry
+    const float4* tri_attr;  // per triangle: what shading reads, see tri_attr_pack
     const DevSphere* spheres; int n_spheres;
     const DevMaterial* materials;
     const DevLight* lights; int n_lights;
+    int n_materials;
     const float* light_cdf;  // inclusive cumulative power
     float env[3];
     int env_on;
     int lambert_only;        // every material is Lambertian: the trace kernel's specialised variant applies
 };
+
+// What shading reads of a hit triangle, in one float4: the geometric normal,
+// then the material id (low 16 bits), the light id + 1 (next 15 bits; 0: not
+// linked to a light) and the emitter flag (top bit) as integer bits.
+constexpr int TRI_ATTR_MAX_MATERIALS = 1 << 16;
+constexpr int TRI_ATTR_MAX_LIGHTS = (1 << 15) - 1;
+struct TriShade { v3 n; int material, light; bool emitter; };
+RPD_HD float4 tri_attr_pack(const DevTriangle& t) {
+    const uint32_t bits = uint32_t(t.material) | (uint32_t(t.light + 1) << 16) | (t.emitter ? 0x80000000u : 0u);
+    float4 a;
+    a.x = t.n[0]; a.y = t.n[1]; a.z = t.n[2];
+    memcpy(&a.w, &bits, 4);
+    return a;
+}
+RPD_HD TriShade tri_attr_unpack(float4 a) {
+    TriShade t;
+    t.n = mk3(a.x, a.y, a.z);
+    uint32_t bits;
+    memcpy(&bits, &a.w, 4);
+    t.material = int(bits & 0xffffu);
+    t.light = int((bits >> 16) & 0x7fffu) - 1;
+    t.emitter = (bits >> 31) != 0;
+    return t;
+}
 
 // BVH over a scene's triangles and spheres, for scenes too large for the flat
 // loop. A separate block so that the flat-loop kernel's arguments keep their
