@@ -414,6 +414,10 @@ struct StridedStack {
 // The query in resumable form: init, then step (one node visit with the
 // primitives of its leaf children) until step reports the end, then result.
 // A kernel can interleave other work between the steps of a lane's walk.
+// A step first tests the four child boxes, then the primitives of every leaf
+// child hit in one loop, so the lanes of a wave test primitives together
+// whichever child slots their leaves sit in (a C2 job's trace: 786 -> 704 ms
+// against one primitive loop per child slot).
 struct BvhWalk {
     RayPre rp;
     float ix, iy, iz;
@@ -446,6 +450,9 @@ struct BvhWalk {
         int cn[4];
         float ct[4];
         int nc = 0;
+        // leaf children hit: their first slots and counts (3 bits each), tested
+        // after the box tests in one loop over all their primitives
+        int ls0 = 0, ls1 = 0, ls2 = 0, ls3 = 0, lcnt = 0, nl = 0, ntot = 0;
         for (int k = 0; k < 4; ++k) {
             const int cnt = N.count[k];
             if (cnt < 0) continue;
@@ -456,8 +463,18 @@ struct BvhWalk {
             const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
             if (!(tn <= tf * widen) || !(tn <= cull)) continue;
             if (cnt == 0) { cn[nc] = N.child[k]; ct[nc] = tn; ++nc; continue; }
-            for (int q = 0; q < cnt; ++q) {
-                const int slot = N.child[k] + q;
+            const int c0 = N.child[k];
+            if (nl == 0) ls0 = c0; else if (nl == 1) ls1 = c0; else if (nl == 2) ls2 = c0; else ls3 = c0;
+            lcnt |= cnt << (3 * nl);
+            ++nl;
+            ntot += cnt;
+        }
+        {
+            int leaf = 0, q = 0;
+            for (int i = 0; i < ntot; ++i) {
+                while (q >= ((lcnt >> (3 * leaf)) & 7)) { q = 0; ++leaf; }
+                const int slot = (leaf == 0 ? ls0 : leaf == 1 ? ls1 : leaf == 2 ? ls2 : ls3) + q;
+                ++q;
                 const int prim = B.slot_prim[slot];
                 const float4* e = B.slots + size_t(slot) * BVH_SLOT_FLOAT4;
                 if (prim >= 0) {
