@@ -94,20 +94,30 @@ __device__ __forceinline__ int dep_bin(float u, float v) {  // core/stats.h:165-
     return z * 4 + x;
 }
 
-// Inclusive block scan of one int per thread (GATHER_BLOCK threads).
-__device__ __forceinline__ int gather_scan(int v, int* sh) {
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int o = 1; o < GATHER_BLOCK; o <<= 1) {
-        const int add = t >= o ? sh[t - o] : 0;
-        __syncthreads();
-        sh[t] += add;
-        __syncthreads();
+// Exclusive block scan of one 64-bit value per thread (GATHER_BLOCK threads):
+// each wave scans its 64 values with lane shuffles, the wave totals meet in
+// LDS, and every thread adds those of the waves before it. `total` receives
+// the block's sum. Two barriers; `sh` holds one entry per wave.
+__device__ __forceinline__ int64_t gather_scan(int64_t v, int64_t* sh, int64_t& total) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
     }
-    const int r = sh[t];
+    if (lane == 63) sh[wave] = x;
     __syncthreads();
-    return r;
+    int64_t before = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < GATHER_BLOCK / 64; ++w) {
+        const int64_t tw = sh[w];
+        if (w < wave) before += tw;
+        total += tw;
+    }
+    __syncthreads();    // sh is reused by the next scan
+    return before + x - v;
 }
 
 constexpr int GATHER_BIN_ROW = GATHER_BLOCK / 4 + 1;  // dwords per accumulator row of bin bytes
@@ -117,7 +127,7 @@ struct GatherLds {
     int pix[kMaxPix];
     float w[kMaxPix];
     int pre[kMaxPix + 1];
-    int scan[GATHER_BLOCK];
+    int64_t scan[GATHER_BLOCK / 64];
     // Rows are padded by one element so that the rows one wave reads at the
     // same index (several accumulators' bins, the three pair arrays) start in
     // different LDS banks.
@@ -139,7 +149,7 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds<kMaxPi
     int* s_pix = L.pix;
     float* s_w = L.w;
     int* s_pre = L.pre;
-    int* s_scan = L.scan;
+    int64_t* s_scan = L.scan;
     signed char* c_bin = reinterpret_cast<signed char*>(L.bin4);
 
     const int gx = node % g.nx, gy = node / g.nx;
@@ -189,15 +199,13 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds<kMaxPi
                 if (w > RPD_EPS_SMALL && any_active) c = max(plan.counts[p], 0);
             }
             const bool keep = c > 0;
-            const int kpos = gather_scan(keep ? 1 : 0, s_scan) - (keep ? 1 : 0);
-            const int kcnt = s_scan[GATHER_BLOCK - 1];
-            __syncthreads();
-            const int cpos = gather_scan(c, s_scan) - c;
-            const int ccnt = s_scan[GATHER_BLOCK - 1];
+            // kept pixels (high word) and their sample counts (low word) scanned together
+            int64_t both = 0;
+            const int64_t pos = gather_scan((int64_t(keep ? 1 : 0) << 32) | int64_t(c), s_scan, both);
+            const int kpos = int(pos >> 32), cpos = int(pos & 0xffffffff);
             if (keep) { s_pix[npix + kpos] = p; s_w[npix + kpos] = w; s_pre[npix + kpos] = total + cpos; }
-            npix += kcnt;
-            total += ccnt;
-            __syncthreads();
+            npix += int(both >> 32);
+            total += int(both & 0xffffffff);
         }
         if (tid == 0) s_pre[npix] = total;
         __syncthreads();

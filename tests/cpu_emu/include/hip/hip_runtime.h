@@ -171,6 +171,12 @@ template <class T> inline T __shfl_xor(T v, int mask, int width = 64) {
     std::memcpy(&r, &bits, sizeof(T));
     return r;
 }
+// v of lane (lane - delta); lanes below delta keep their own v (HIP's semantics)
+template <class T> inline T __shfl_up(T v, unsigned delta, int width = 64) {
+    (void)width;
+    const int lane = int(threadIdx.x) & 63;
+    return __shfl(v, lane >= int(delta) ? lane - int(delta) : lane);
+}
 // v of lane `lane` (a wave-uniform index, as the device instruction requires)
 inline int __builtin_amdgcn_readlane(int v, int lane) { return __shfl(v, lane); }
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
