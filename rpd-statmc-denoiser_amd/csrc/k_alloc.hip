@@ -499,25 +499,35 @@ __global__ void __launch_bounds__(64) k_alloc_sweeps(int n, AllocConfig ac, cons
 }
 
 // shares, floors and residuals (main.cpp:1403-1408, 1414-1423). The floors are
-// summed per wave and added to the assigned count with one atomic per wave
-// (an integer sum, the same in any order). Every lane of the grid reaches the
-// shuffles: lanes past n add 0, and the inactive case returns for all lanes.
-__global__ void k_alloc_floor(int n, const float* imp, int* counts, AllocBufs a, float* resid) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!a.st->active) {
-        if (i < n) resid[i] = 0.0f;
-        return;
-    }
+// added to the assigned count as an integer sum, the same in any order: each
+// thread sums its pixels (a grid-stride loop over ALLOC_FLOOR_BLOCKS blocks),
+// the block sums its threads, and one atomic per block adds that. One atomic
+// per wave (16,384 on 1 Mpx, all on one address) made this kernel 190 us.
+constexpr int ALLOC_FLOOR_BLOCKS = 512;
+__global__ void __launch_bounds__(256) k_alloc_floor(int n, const float* imp, int* counts, AllocBufs a,
+                                                     float* resid) {
+    __shared__ int s_sum[256 / 64];
+    const bool active = a.st->active != 0;
+    const bool final_assign = a.st->final_assign != 0;
+    const double rem_imp = a.st->rem_imp, remaining = double(a.st->remaining);
     int add = 0;
-    if (i < n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (!active) { resid[i] = 0.0f; continue; }
         float d = a.desired[i];
-        if (a.st->final_assign && !a.clipped[i]) d = float(double(imp[i]) / a.st->rem_imp * double(a.st->remaining));
-        add = int(floorf(d));
-        counts[i] += add;
-        resid[i] = fmaxf(0.0f, d - float(add));
+        if (final_assign && !a.clipped[i]) d = float(double(imp[i]) / rem_imp * remaining);
+        const int f = int(floorf(d));
+        counts[i] += f;
+        resid[i] = fmaxf(0.0f, d - float(f));
+        add += f;
     }
     for (int o = 32; o > 0; o >>= 1) add += __shfl_xor(add, o);
-    if ((threadIdx.x & 63) == 0 && add) atomicAdd(&a.st->assigned, add);
+    if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = add;
+    __syncthreads();
+    if (threadIdx.x == 0 && active) {
+        int sum = 0;
+        for (int w = 0; w < 256 / 64; ++w) sum += s_sum[w];
+        if (sum) atomicAdd(&a.st->assigned, sum);
+    }
 }
 
 // systematic residual sampling (main.cpp:1425-1438), counted per pixel
@@ -625,7 +635,8 @@ void launch_alloc_counts(int n, const AllocConfig& ac, const float* imp, int* co
     hipLaunchKernelGGL(k_alloc_chunk_minmax, dim3((nch + 3) / 4), dim3(256), 0, st, n, imp, c.a);
     hipLaunchKernelGGL(k_alloc_super_minmax, dim3((nsc + 3) / 4), dim3(256), 0, st, n, c.a);
     hipLaunchKernelGGL(k_alloc_sweeps, dim3(1), dim3(64), 0, st, n, ac, imp, c.a);
-    hipLaunchKernelGGL(k_alloc_floor, g, b, 0, st, n, imp, counts, c.a, c.resid);
+    hipLaunchKernelGGL(k_alloc_floor, dim3(std::min<int>((n + 255) / 256, ALLOC_FLOOR_BLOCKS)), b, 0, st, n, imp,
+                       counts, c.a, c.resid);
     // residual prefix sums as the reference's running `cumulative` forms them
     c.a.seq.x = c.resid;
     seq_sum_launch(c.a.seq, true, st);
