@@ -124,6 +124,12 @@ int rpd_ctx_set_buffer_cache(rpd_context* ctx, uint64_t bytes, uint64_t* cached)
    bytes held by jobs stay within `bytes` (0, the default: no limit). The
    serving loop sets its budget here, so that admission by footprint holds. */
 int rpd_ctx_set_held_limit(rpd_context* ctx, uint64_t bytes);
+/* Extension: before each pass the engine writes the first
+   min(624, words_per_sample * count + margin) words (rounded up to 4) of a
+   pixel's MT19937 state after its first twist, count being the pixel's samples
+   in the pass; a pixel that draws more writes the rest during the trace.
+   Draws are the same for any setting (defaults 48 and 64). */
+int rpd_ctx_set_tape_prefix(rpd_context* ctx, int words_per_sample, int margin);
 /* How scenes created next build their BVH (only scenes too large for the flat
    loop get one; the reference builds a binary median-split BVH on the host,
    core/ray_tracer.h:49-226): RPD_BVH_HOST_SAH (default) bins a surface-area

@@ -148,6 +148,11 @@ struct rpd_context {
     std::map<uint64_t, int> tuned;
     int forced_shape = -1;
     bool debug_outputs = false;  // jobs created next keep the StatMC and allocation debug planes
+    // tape words written before a pass per sample and beyond (tape_prefix_words);
+    // 48 per sample covers a cornell_box sample of 7 bounces (C1 per job: 49.7 ms
+    // with whole tapes, 48.7 at 64/96, 48.4 at 48/64, 59.2 at 32/32, where the
+    // trace extends too many tapes itself)
+    int tape_per_sample = 48, tape_margin = 64;
 };
 
 struct rpd_scene {
@@ -342,6 +347,8 @@ TraceParams trace_params(rpd_job* j, int statmc) {
     p.tape = j->tape;
     p.max_depth = j->cfg.max_depth;
     p.bvh = j->scene->bvh;
+    p.tape_per_sample = j->ctx->tape_per_sample;
+    p.tape_margin = j->ctx->tape_margin;
     p.work_counter = j->work;
     p.query_count = j->query_count;
     p.rec = j->rec;
@@ -790,7 +797,7 @@ void job_layout(rpd_job* j, SlabLayout& L) {
     A(j->scan_scratch, (size_t)plan_scan_scratch_elems(j->N));
     A(j->query_count, 2);
     // a denoiser job draws no random numbers: one placeholder word instead of the tapes
-    const size_t tape_words = j->denoiser ? 1 : ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_N;
+    const size_t tape_words = j->denoiser ? 1 : ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_TAPE_WORDS;
     A(j->tape, tape_words);
     const int per_px = j->denoiser ? j->max_spp : std::max(j->cfg.samples_per_pixel, j->cfg.adaptive_spp + 1);
     const int64_t cap = int64_t(per_px) * int64_t(N);
@@ -1106,6 +1113,14 @@ int rpd_ctx_set_bvh_builder(rpd_context* ctx, int builder) {
     if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
     if (builder != RPD_BVH_HOST_SAH && builder != RPD_BVH_DEVICE_LBVH) return fail(RPD_ERR_INVALID_ARG, "unknown BVH builder");
     ctx->bvh_builder = builder;
+    return RPD_OK;
+}
+
+int rpd_ctx_set_tape_prefix(rpd_context* ctx, int words_per_sample, int margin) {
+    if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
+    if (words_per_sample < 0 || margin < 0) return fail(RPD_ERR_INVALID_ARG, "tape prefix sizes must be non-negative");
+    ctx->tape_per_sample = words_per_sample;
+    ctx->tape_margin = margin;
     return RPD_OK;
 }
 

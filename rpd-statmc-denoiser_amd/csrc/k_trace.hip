@@ -19,11 +19,10 @@ namespace rpd {
 // earlier. Words are written four at a time (one uint4 per lane, so a wave
 // writes one contiguous KiB) and read back the same way: the chunk holding
 // words 4c-227 .. 4c-224 is read one chunk ahead of its use, so the read-back
-// latency overlaps the next four words' work.
-__device__ __forceinline__ uint32_t mt_seed_step(uint32_t prev, uint32_t k) {
-    return 1812433253u * (prev ^ (prev >> 30)) + k;
-}
-
+// latency overlaps the next four words' work. Only the pixel's prefix is
+// written (tape_prefix_words of its sample count): an adaptive pass's pixel of
+// a few samples draws a small part of the 624 words, and the trace extends the
+// rare tape drawn past its prefix.
 __device__ __forceinline__ uint32_t u4_at(const uint4& v, int i) {
     return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
 }
@@ -35,6 +34,8 @@ __global__ void __launch_bounds__(256) k_tape(TraceParams p) {
     const int n_before = p.seed_n ? p.seed_n[pix] : 0;
     const uint32_t s = pixel_sample_seed(p.base_seed, pix, n_before);
     uint32_t* t = p.tape + tape_base(a);
+    const int n_chunks = tape_prefix_words(p.plan.counts[pix], p.tape_per_sample, p.tape_margin) / 4;
+    if (n_chunks < MT_N / 4) t[tape_word(MT_TAPE_SEED)] = s;   // for the trace to extend the prefix
     uint32_t lo = s;               // x[k]
     uint32_t far = s;              // x[k + 397] while k + 397 < 624
     for (uint32_t k = 1; k <= (uint32_t)MT_M; ++k) far = mt_seed_step(far, k);
@@ -44,7 +45,7 @@ __global__ void __launch_bounds__(256) k_tape(TraceParams p) {
     // and m56 = chunk c-56 (word k = 4c+i reads word 4(c-57)+i+1); chunk c-54 is
     // requested at the end of iteration c, after the iteration that wrote it.
     uint4 m57 = make_uint4(0, 0, 0, 0), m56 = make_uint4(0, 0, 0, 0), m55 = make_uint4(0, 0, 0, 0);
-    for (int c = 0; c < MT_N / 4; ++c) {
+    for (int c = 0; c < n_chunks; ++c) {
         uint32_t w4[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -257,7 +258,7 @@ __device__ __forceinline__ bool path_step(const TraceParams& p, const DevScene& 
             pix = p.plan.active[a];
             left = p.plan.counts[pix];
             slot = p.plan.offsets[pix];
-            dr.reset(p.tape + tape_base(a));
+            dr.reset(p.tape + tape_base(a), tape_prefix_words(left, p.tape_per_sample, p.tape_margin));
             st = ST_CAMERA;
         }
         const int taken = min(__popcll(need), avail);
@@ -423,7 +424,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_e
     int pix = -1, left = 0;
     int64_t slot = 0;
     Draws dr;
-    dr.tape = nullptr; dr.pos = 0; dr.cache = make_uint4(0, 0, 0, 0);
+    dr.tape = nullptr; dr.pos = 0; dr.limit = MT_N; dr.cache = make_uint4(0, 0, 0, 0);
     LaneRays R;
     R.ro = mk3(0, 0, 0); R.rd_ext = R.ro; R.rd_l = R.ro; R.rd_e = R.ro; R.t_l = 0.0f;
     v3 T = mk3(1, 1, 1), L = mk3(0, 0, 0), Ld = mk3(0, 0, 0);

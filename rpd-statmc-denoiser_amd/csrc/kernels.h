@@ -55,13 +55,14 @@ struct TraceParams {
     PassPlan plan;
     const int* seed_n;       // per-pixel sample offset used for seeding (finite samples so far), or null for 0
     uint32_t base_seed;
-    uint32_t* tape;          // ceil(n/64)*64 tapes of 624 words, layout of tape_base/tape_word
+    uint32_t* tape;          // ceil(n/64)*64 tapes of 624 words and a seed, layout of tape_base/tape_word
     int* work_counter;       // zero before launch
     unsigned long long* query_count;  // ray queries issued (closest + occlusion), accumulated; may be null
     Records rec;
     // extensions, kept last so that the fields above keep their offsets
     int max_depth;           // shading points per path, 0 = unbounded
     DevBvh bvh;              // nodes null: the flat loop over scene.tri_perm
+    int tape_per_sample, tape_margin;  // tape words written per sample and beyond (tape_prefix_words)
 };
 
 // Per-pixel accumulators (structure of arrays, n_pixels each unless noted)

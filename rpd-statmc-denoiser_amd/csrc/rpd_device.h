@@ -117,14 +117,37 @@ __device__ __forceinline__ float mt_to_unit(uint32_t u) {
     return r >= 1.0f ? 0x1.fffffep-1f : r;
 }
 // Tapes are stored in groups of 64: word w of tape a lives at
-//   (a / 64) * 624 * 64 + (w / 4) * 256 + (a % 64) * 4 + (w % 4)
+//   (a / 64) * 628 * 64 + (w / 4) * 256 + (a % 64) * 4 + (w % 4)
 // so 64 consecutive tapes write one contiguous KiB per 4 words when they are
 // generated together, and one tape still reads 4 consecutive words as a uint4.
+// Word 624 (MT_TAPE_SEED) holds the tape's seed, for extending a prefix.
 constexpr int MT_GROUP = 64;
+constexpr int MT_TAPE_SEED = MT_N;
+constexpr int MT_TAPE_WORDS = MT_N + 4;
 __host__ __device__ __forceinline__ size_t tape_base(size_t a) {
-    return (a / MT_GROUP) * size_t(MT_N * MT_GROUP) + (a % MT_GROUP) * 4;
+    return (a / MT_GROUP) * size_t(MT_TAPE_WORDS * MT_GROUP) + (a % MT_GROUP) * 4;
 }
 __device__ __forceinline__ int tape_word(int w) { return (w >> 2) * (4 * MT_GROUP) + (w & 3); }
+
+__host__ __device__ __forceinline__ uint32_t mt_seed_step(uint32_t prev, uint32_t k) {
+    return 1812433253u * (prev ^ (prev >> 30)) + k;
+}
+
+// The words of a pixel's tape written before the trace: room for `count`
+// samples of per_sample draws plus margin words, in whole 4-word chunks, at
+// most the full 624. A lane that draws past them writes the rest itself
+// (mt_tape_extend); the words are the same either way.
+__host__ __device__ __forceinline__ int tape_prefix_words(int count, int per_sample, int margin) {
+    const long long w = (long long)per_sample * count + margin;
+    return w >= MT_N ? MT_N : int((w + 3) / 4 * 4);
+}
+
+// Writes words [from, 624) of the tape after its first twist, words [0, from)
+// and the seed (word MT_TAPE_SEED) being there already (from a multiple of 4): the seeded state
+// is walked by two cursors (x[k] and, while k < 227, x[k+397]) and words from
+// 227 on mix in the tape word 227 back. Rare (a pixel whose samples drew more
+// than its prefix), so written for size, not speed.
+__device__ __noinline__ void mt_tape_extend(uint32_t* __restrict__ s, int from);
 
 // Regenerates a tape in place (the standard twist over all 624 words) and
 // returns its first four words. Kept out of line: a draw site needs it once
@@ -144,18 +167,42 @@ __device__ __noinline__ uint4 mt_twist_reload(uint32_t* __restrict__ s) {
     return *reinterpret_cast<const uint4*>(s);
 }
 
+__device__ __noinline__ void mt_tape_extend(uint32_t* __restrict__ s, int from) {
+    constexpr int kBack = MT_N - MT_M;  // 227
+    uint32_t lo = s[tape_word(MT_TAPE_SEED)];
+    for (int k = 1; k <= from; ++k) lo = mt_seed_step(lo, uint32_t(k));           // x[from]
+    uint32_t far = lo;
+    if (from < kBack)
+        for (int k = from + 1; k <= from + MT_M; ++k) far = mt_seed_step(far, uint32_t(k));  // x[from + 397]
+    for (int k = from; k < MT_N - 1; ++k) {
+        const uint32_t x1 = mt_seed_step(lo, uint32_t(k + 1));
+        const uint32_t back = k < kBack ? far : s[tape_word(k - kBack)];
+        s[tape_word(k)] = mt_mix(lo, x1, back);
+        lo = x1;
+        if (k + 1 < kBack) far = mt_seed_step(far, uint32_t(k + 1 + MT_M));
+    }
+    s[tape_word(MT_N - 1)] = mt_mix(lo, s[tape_word(0)], s[tape_word(MT_M - 1)]);
+}
+
 // Per-lane draw cursor over a tape with a 4-word register cache.
 struct Draws {
     uint32_t* tape;  // tape_base of this pixel's tape
     int pos;         // next word index in [0, 624]
+    int limit;       // words of the current state written so far (a multiple of 4)
     uint4 cache;     // words [pos & ~3, pos | 3]
-    __device__ __forceinline__ void reset(uint32_t* t) { tape = t; pos = 0; }
+    __device__ __forceinline__ void reset(uint32_t* t, int prefix) { tape = t; pos = 0; limit = prefix; }
     __device__ __forceinline__ float next() {
-        if (pos == MT_N) {
-            cache = mt_twist_reload(tape);
-            pos = 0;
-        } else if ((pos & 3) == 0) {
-            cache = *reinterpret_cast<const uint4*>(tape + tape_word(pos));
+        if ((pos & 3) == 0) {
+            if (pos == limit && limit < MT_N) {   // past the prefix: the lane writes the rest
+                mt_tape_extend(tape, limit);
+                limit = MT_N;
+            }
+            if (pos == MT_N) {
+                cache = mt_twist_reload(tape);
+                pos = 0;
+            } else {
+                cache = *reinterpret_cast<const uint4*>(tape + tape_word(pos));
+            }
         }
         const int k = pos & 3;
         const uint32_t w = k == 0 ? cache.x : (k == 1 ? cache.y : (k == 2 ? cache.z : cache.w));

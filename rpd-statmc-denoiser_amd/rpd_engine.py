@@ -194,6 +194,7 @@ def lib():
             ("rpd_ctx_set_bvh_builder", [vp, C.c_int]),
             ("rpd_ctx_set_buffer_cache", [vp, C.c_uint64, C.POINTER(C.c_uint64)]),
             ("rpd_ctx_set_held_limit", [vp, C.c_uint64]),
+            ("rpd_ctx_set_tape_prefix", [vp, C.c_int, C.c_int]),
             ("rpd_scene_create", [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.POINTER(vp)]),
             ("rpd_scene_builtin", [vp, C.c_char_p, C.POINTER(vp)]),
             ("rpd_scene_counts", [vp, vp, vp, vp, vp]),
@@ -350,6 +351,10 @@ class Context:
         kept = C.c_uint64()
         check(self.L.rpd_ctx_set_buffer_cache(self.h, int(nbytes), C.byref(kept)))
         return kept.value
+
+    def set_tape_prefix(self, words_per_sample, margin):
+        """Tape words written before a pass: per sample of the pixel, and beyond (defaults 48, 64)."""
+        check(self.L.rpd_ctx_set_tape_prefix(self.h, int(words_per_sample), int(margin)))
 
     def set_held_limit(self, nbytes):
         """Held bytes up to which a job may take a kept slab larger than it needs (0: no limit)."""

@@ -59,6 +59,31 @@ def test_trace_records_past_the_first_twist_match_oracle(engine_ctx):
     check_records(orec, grec, BUDGET, "32 spp:")
 
 
+@pytest.mark.parametrize("per_sample,margin", [(48, 64), (4, 8), (0, 0), (0, 624)])
+def test_trace_records_with_short_tape_prefixes_match_oracle(engine_ctx, per_sample, margin):
+    """Tapes written only in part before the pass (48 words per sample and 64
+    more, the default; 4 and 8, so that nearly every pixel's lane extends its
+    own tape; none at all; and the whole 624 words) give the same draws, on
+    uneven counts of 1 to 12 samples per pixel and a second pass seeded past
+    the first one's samples."""
+    rc = ob.config_c0(24, 20)
+    rc.spp = 12                       # record room for up to 12 samples per pixel
+    rng = np.random.default_rng(per_sample + margin)
+    counts = rng.integers(1, 13, rc.width * rc.height).astype(np.int32)
+    oj = _pair(rc)
+    gj = _gpu_job(engine_ctx, rc)
+    engine_ctx.set_tape_prefix(per_sample, margin)
+    try:
+        oj.run_pass(np.full(rc.width * rc.height, 2, np.int32), statmc=True)
+        gj.run_pass(np.full(rc.width * rc.height, 2, np.int32), statmc=True)
+        oj.run_pass(counts, statmc=True)
+        gj.run_pass(counts, statmc=True)
+        check_records(oj.get("records"), gj.get("records"), BUDGET, f"prefix {per_sample}/{margin}:")
+    finally:
+        engine_ctx.set_tape_prefix(48, 64)
+        gj.release()
+
+
 def test_trace_uniform_render_matches_oracle(engine_ctx):
     rc = ob.config_c0(24, 20)
     rc.use_statmc = 0
