@@ -456,6 +456,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-quality", action="store_true", help="skip the relMSE against a high-spp reference")
     ap.add_argument("--size", type=int, default=WORKLOAD["width"])
+    ap.add_argument("--trace-wavefront", type=float, default=None,
+                    help="wavefront trace rounds per sample for flat scenes (0: path-regeneration kernel only; "
+                         "default: the engine's default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy (counter calibration) and exit")
     ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4"],
@@ -484,6 +487,8 @@ def main():
         sys.exit(2)
     w = h = args.size
     ctx = eng.Context(0)
+    if args.trace_wavefront is not None:
+        ctx.set_trace_wavefront(args.trace_wavefront)
     cfg, cam = job_config(w, h), job_camera()
     for _ in range(args.warmup):
         j = eng.Job(ctx, cfg, cam, WORKLOAD["seed"], submit=True)

@@ -383,6 +383,19 @@ int rpd_ctx_set_debug_outputs(rpd_context* ctx, int on);
    sampling pass) as a hipGraph on its first pass and replay it for every
    pass; results are identical to launching the kernels one by one. */
 int rpd_ctx_set_graph_passes(rpd_context* ctx, int on);
+/* Extension (no counterpart in the reference, whose sample loop is
+   main.cpp:649-737 and 1451-1538 around shade_mis, main.cpp:394-533): passes
+   over scenes traced without a BVH (the flat triangle loop) run as a wavefront
+   trace of ceil(rounds_per_sample * samples per pixel) rounds, each a shading
+   launch over path slots held in device memory (one bounce or camera ray per
+   slot, its queries listed) and an intersection launch over the listed
+   queries; the path-regeneration kernel then finishes the slots still
+   running. Records are the same as with the path-regeneration kernel alone,
+   which 0 (the default) selects. rounds_per_sample must lie in [0, 64]. It
+   applies to jobs created after the call: their slabs hold the path slots
+   (51 state words per pixel, up to 4M slots, plus 3 query words and the round
+   counters), which rpd_job_bytes does not count. */
+int rpd_ctx_set_trace_wavefront(rpd_context* ctx, float rounds_per_sample);
 /* Device bytes a job of this configuration needs (its slab size; admission by
    HBM footprint). A job that takes a larger kept slab holds that slab's size. */
 int rpd_job_bytes(const rpd_render_config* cfg, uint64_t* bytes);

@@ -153,6 +153,10 @@ struct rpd_context {
     // with whole tapes, 48.7 at 64/96, 48.4 at 48/64, 59.2 at 32/32, where the
     // trace extends too many tapes itself)
     int tape_per_sample = 48, tape_margin = 64;
+    // flat-scene passes run as a wavefront trace of ceil(wf_rounds * samples
+    // per slot) shading/intersection rounds before the path-regeneration
+    // kernel finishes them; 0 keeps the path-regeneration kernel alone
+    float wf_rounds = 0.0f;
 };
 
 struct rpd_scene {
@@ -195,6 +199,7 @@ struct rpd_job {
     int64_t* scan_scratch = nullptr;
     uint32_t* tape = nullptr;
     Records rec{};
+    WavefrontState wf{};     // wavefront trace slots (flat scenes), or slots 0
     double* grid = nullptr;
     float *tiles = nullptr, *node_scratch = nullptr, *filter_scratch = nullptr;
     void* alloc_scratch = nullptr;
@@ -203,6 +208,7 @@ struct rpd_job {
     int64_t* b1_prefix = nullptr;
     unsigned int* b1_queues = nullptr;
     bool denoiser = false;   // created for denoising records: no scene, camera or tape
+    bool wf_on = false;      // the slab holds wavefront trace slots (the context had it on at creation)
     bool capturing = false;  // the stream is being captured into a graph: no per-stage timing events
     int max_spp = 0;         // record capacity per pixel of a denoiser job
     // this job as an image of a batch (device copies for the batch of one)
@@ -352,7 +358,18 @@ TraceParams trace_params(rpd_job* j, int statmc) {
     p.work_counter = j->work;
     p.query_count = j->query_count;
     p.rec = j->rec;
+    p.wf = j->wf;
     return p;
+}
+
+// Wavefront rounds for a pass of this job, or 0 for the path-regeneration
+// kernel alone: flat scenes with slots, rounds for the pass's mean samples
+// per pixel (the base pass's count, or the adaptive budget per pixel).
+int wavefront_rounds(const rpd_job* j, int stage) {
+    if (j->ctx->wf_rounds <= 0.0f || j->scene->bvh.nodes || j->wf.slots <= 0) return 0;
+    const int per_px = stage == 0 ? j->cfg.samples_per_pixel : std::max(1, j->cfg.adaptive_spp);
+    const double r = std::ceil(double(j->ctx->wf_rounds) * double(per_px) * double(j->N) / double(j->wf.slots));
+    return int(std::min<double>(r, WF_MAX_ITERS));
 }
 
 int trace_blocks(rpd_job* j) {
@@ -374,7 +391,9 @@ int pass_after_plan(rpd_job* j, int statmc, int stage) {
     launch_tape(p, j->stream);
     {
         StageTimer tk(j, 7);
-        launch_trace(p, trace_blocks(j), j->stream);
+        const int rounds = wavefront_rounds(j, stage);
+        if (rounds > 0) launch_trace_wavefront(p, rounds, trace_blocks(j), j->stream);
+        else launch_trace(p, trace_blocks(j), j->stream);
     }
     launch_fold(j->batch1, statmc, j->stream);
     if (p.capture) {
@@ -814,6 +833,17 @@ void job_layout(rpd_job* j, SlabLayout& L) {
     A(j->b1_img, 1);
     A(j->b1_prefix, BK_KINDS * 2);
     A(j->b1_queues, BK_KINDS * BATCH_QUEUES);
+    // wavefront trace slots: render jobs created while the context has the
+    // wavefront trace on, whose tapes an unsigned 32-bit word offset can
+    // address (the trace stores that offset per slot)
+    if (j->wf_on && !j->denoiser && tape_words < (size_t(1) << 32)) {
+        const int slots = int(std::min<size_t>(N, size_t(WF_MAX_SLOTS)));
+        j->wf.slots = slots;
+        A(j->wf.state, size_t(WF_WORDS) * slots);
+        A(j->wf.queue, size_t(wf_queue_words(slots)));
+        A(j->wf.ctrl, size_t(WF_CTRL_WORDS));
+        A(j->wf.cursor, size_t(2) * wf_groups(slots));
+    }
 #undef A
 }
 
@@ -853,13 +883,15 @@ void set_geometry(rpd_job* j) {
 }
 
 // Device bytes of a job (its slab, without debug planes); a denoiser job
-// (max_spp > 0) has no tape and room for max_spp records per pixel.
-size_t job_bytes(const rpd_render_config& c, int max_spp = 0) {
+// (max_spp > 0) has no tape and room for max_spp records per pixel; wf: with
+// wavefront trace slots.
+size_t job_bytes(const rpd_render_config& c, int max_spp = 0, bool wf = false) {
     rpd_job t;
     t.cfg = c;
     t.W = c.image_width; t.H = c.image_height; t.N = t.W * t.H;
     t.denoiser = max_spp > 0;
     t.max_spp = max_spp;
+    t.wf_on = wf;
     set_geometry(&t);
     SlabLayout L;
     job_layout(&t, L);
@@ -899,7 +931,8 @@ int job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg,
     j->W = cfg->image_width; j->H = cfg->image_height; j->N = j->W * j->H;
     if ((st = make_camera(*cam, float(j->W) / float(j->H), j->cam))) return st;
     set_geometry(j.get());
-    if (ctx->budget && ctx->in_use.load() + job_bytes(*cfg) > ctx->budget)
+    j->wf_on = ctx->wf_rounds > 0.0f && !scene->bvh.nodes;
+    if (ctx->budget && ctx->in_use.load() + job_bytes(*cfg, 0, j->wf_on) > ctx->budget)
         return fail(RPD_ERR_OUT_OF_MEMORY, "job does not fit the context's memory budget");
     HIP_TRY(stream_create(&j->stream));
     if ((st = job_alloc(j.get()))) { job_free(j.get()); return st; }
@@ -1817,6 +1850,14 @@ int rpd_ctx_statmc_tile(rpd_context* ctx, int radius, int64_t pixels, int* tile_
 int rpd_ctx_set_debug_outputs(rpd_context* ctx, int on) {
     if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
     ctx->debug_outputs = on != 0;
+    return RPD_OK;
+}
+
+int rpd_ctx_set_trace_wavefront(rpd_context* ctx, float rounds_per_sample) {
+    if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
+    if (!(rounds_per_sample >= 0.0f) || rounds_per_sample > 64.0f)
+        return fail(RPD_ERR_INVALID_ARG, "rounds_per_sample must lie in [0, 64]");
+    ctx->wf_rounds = rounds_per_sample;
     return RPD_OK;
 }
 

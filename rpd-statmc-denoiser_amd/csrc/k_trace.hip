@@ -376,12 +376,126 @@ __device__ __forceinline__ bool path_step(const TraceParams& p, const DevScene& 
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Path slots of the wavefront trace. A slot holds one path lane's state
+// between kernels: the words path_step keeps from one call to the next (the
+// shading point itself is formed and used within one call), the rays of its
+// queries and their results. Only a lane waiting on queries (ST_WAIT) has
+// state beyond its ST word, and of that only what its issued queries need.
+enum : int {
+    WF_ST = 0, WF_Q, WF_PIX, WF_LEFT, WF_TAPE, WF_POS, WF_LIMIT, WF_SLOT, WF_FLAGS, WF_LBITS, WF_DEPTH, WF_SMAT,
+    WF_BPDF, WF_T = 13, WF_L = 16, WF_LD = 19, WF_CWL = 22, WF_CWE = 25, WF_CWX = 28,
+    WF_RO = 31, WF_RDX = 34, WF_RDL = 37, WF_TL = 40, WF_RDE = 41,
+    WF_HCODE = 44, WF_HT = 45, WF_HSN = 46, WF_OCCL = 49, WF_OCCE = 50, WF_USED = 51
+};
+static_assert(WF_USED <= WF_WORDS, "wavefront slot words");
+__device__ __forceinline__ int* wf_count(const WavefrontState& w, int iter) { return w.ctrl + iter * WF_LISTS; }
+__device__ __forceinline__ int* wf_live(const WavefrontState& w, int iter) {
+    return w.ctrl + (WF_MAX_ITERS + iter) * WF_LISTS;
+}
+
+struct WfRef {
+    const WavefrontState& w;
+    int s;
+    __device__ __forceinline__ uint32_t u(int k) const { return w.state[size_t(k) * w.slots + s]; }
+    __device__ __forceinline__ float f(int k) const { return __uint_as_float(u(k)); }
+    __device__ __forceinline__ v3 f3(int k) const { return mk3(f(k), f(k + 1), f(k + 2)); }
+    __device__ __forceinline__ void put(int k, uint32_t v) const { w.state[size_t(k) * w.slots + s] = v; }
+    __device__ __forceinline__ void putf(int k, float v) const { put(k, __float_as_uint(v)); }
+    __device__ __forceinline__ void put3(int k, v3 v) const { putf(k, v.x); putf(k + 1, v.y); putf(k + 2, v.z); }
+};
+
+// Loads slot s into the lane's path variables. A waiting lane's queries have
+// all been answered (pend = 0), so path_step resolves them first.
+template <bool kDepth>
+__device__ __forceinline__ void wf_load(const TraceParams& p, const WfRef& W, HitRec& h, bool& occ_l, bool& occ_e,
+                                        int& st, int& q, int& pend, int& pix, int& left, int64_t& slot, Draws& dr,
+                                        LaneRays& R, v3& T, v3& L, v3& Ld, v3& cw_l, v3& cw_e, v3& cw_x,
+                                        float& bpdf, int& smat, int& depth, uint32_t& flags, uint32_t& light_bits) {
+    st = int(W.u(WF_ST));
+    pend = 0;
+    if (st != ST_WAIT) return;
+    q = int(W.u(WF_Q));
+    pix = int(W.u(WF_PIX));
+    left = int(W.u(WF_LEFT));
+    slot = int64_t(W.u(WF_SLOT));
+    flags = W.u(WF_FLAGS);
+    light_bits = W.u(WF_LBITS);
+    if (kDepth) depth = int(W.u(WF_DEPTH));
+    dr.tape = p.tape + W.u(WF_TAPE);
+    dr.pos = int(W.u(WF_POS));
+    dr.limit = int(W.u(WF_LIMIT));
+    if (dr.pos & 3) dr.cache = *reinterpret_cast<const uint4*>(dr.tape + tape_word(dr.pos & ~3));
+    T = W.f3(WF_T);
+    L = W.f3(WF_L);
+    if (!(q & Q_PRIMARY)) {
+        Ld = W.f3(WF_LD);
+        smat = int(W.u(WF_SMAT));
+    }
+    if (q & Q_EXT) {
+        R.ro = W.f3(WF_RO);
+        R.rd_ext = W.f3(WF_RDX);
+        const int code = int(W.u(WF_HCODE));
+        h.hit = code >= 0 ? 1 : 0;
+        h.sphere = code >= 0 ? (code & 1) : 0;
+        h.index = code >= 0 ? (code >> 1) : -1;
+        h.t = W.f(WF_HT);
+        if (h.sphere) h.sn = W.f3(WF_HSN);
+        if (!(q & Q_PRIMARY)) {
+            cw_x = W.f3(WF_CWX);
+            bpdf = W.f(WF_BPDF);
+        }
+    }
+    if (q & Q_SHADOW_L) { cw_l = W.f3(WF_CWL); occ_l = W.u(WF_OCCL) != 0; }
+    if (q & Q_SHADOW_E) { cw_e = W.f3(WF_CWE); occ_e = W.u(WF_OCCE) != 0; }
+}
+
+// Stores what wf_load reads back and the rays of the issued queries.
+template <bool kDepth>
+__device__ __forceinline__ void wf_store(const TraceParams& p, const WfRef& W, int st, int q, int pix, int left,
+                                         int64_t slot, const Draws& dr, const LaneRays& R, v3 T, v3 L, v3 Ld,
+                                         v3 cw_l, v3 cw_e, v3 cw_x, float bpdf, int smat, int depth, uint32_t flags,
+                                         uint32_t light_bits) {
+    W.put(WF_ST, uint32_t(st));
+    if (st != ST_WAIT) return;
+    W.put(WF_Q, uint32_t(q));
+    W.put(WF_PIX, uint32_t(pix));
+    W.put(WF_LEFT, uint32_t(left));
+    W.put(WF_SLOT, uint32_t(slot));
+    W.put(WF_FLAGS, flags);
+    W.put(WF_LBITS, light_bits);
+    if (kDepth) W.put(WF_DEPTH, uint32_t(depth));
+    W.put(WF_TAPE, uint32_t(dr.tape - p.tape));
+    W.put(WF_POS, uint32_t(dr.pos));
+    W.put(WF_LIMIT, uint32_t(dr.limit));
+    W.put3(WF_T, T);
+    W.put3(WF_L, L);
+    if (!(q & Q_PRIMARY)) {
+        W.put3(WF_LD, Ld);
+        W.put(WF_SMAT, uint32_t(smat));
+    }
+    if (q & Q_EXT) {
+        W.put3(WF_RO, R.ro);
+        W.put3(WF_RDX, R.rd_ext);
+        if (!(q & Q_PRIMARY)) {
+            W.put3(WF_CWX, cw_x);
+            W.putf(WF_BPDF, bpdf);
+        }
+    } else if (q & (Q_SHADOW_L | Q_SHADOW_E)) {
+        W.put3(WF_RO, R.ro);
+    }
+    if (q & Q_SHADOW_L) { W.put3(WF_CWL, cw_l); W.put3(WF_RDL, R.rd_l); W.putf(WF_TL, R.t_l); }
+    if (q & Q_SHADOW_E) { W.put3(WF_CWE, cw_e); W.put3(WF_RDE, R.rd_e); }
+}
+
 // kBvh: the scene's BVH with a per-lane traversal stack in LDS; otherwise the
 // triangle table staged in LDS and looped over. kDepth: paths stop after
 // p.max_depth shading points; without it the loop carries no depth count.
 // kLambert: every material of the scene is Lambertian, so the microfacet and
 // mirror code is left out (fewer registers, fewer divergent branches).
-template <bool kBvh, bool kDepth, bool kLambert>
+// kResume: the lanes first take over the path slots a wavefront trace left
+// running (lane = slot), and the hand-out cursors of their 64-slot groups.
+template <bool kBvh, bool kDepth, bool kLambert, bool kResume = false>
 __global__ void __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) k_trace(TraceParams p) {
     // The flat loop's scene: the tables in LDS, the rest as given
     DevScene S = p.scene;
@@ -436,6 +550,18 @@ __global__ void __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_e
     HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
     bool occ_l = false, occ_e = false;
     uint32_t n_queries = 0;
+    if (kResume) {
+        const int s = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+        if ((s >> 6) < wf_groups(p.wf.slots)) {
+            c_next = p.wf.cursor[2 * (s >> 6)];
+            c_end = p.wf.cursor[2 * (s >> 6) + 1];
+        }
+        if (s < p.wf.slots) {
+            const WfRef W{p.wf, s};
+            wf_load<kDepth>(p, W, h, occ_l, occ_e, st, q, pend, pix, left, slot, dr, R, T, L, Ld, cw_l, cw_e, cw_x,
+                            bpdf, smat, depth, flags, light_bits);
+        }
+    }
 
     while (true) {
         if (!path_step<kDepth, kLambert>(p, S, lane, env, env_on, c_next, c_end, n_active, h, occ_l, occ_e, st, q,
@@ -529,6 +655,201 @@ void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st) {
     const size_t lds = trace_lds_bytes(p.scene, p.bvh);
     if (p.scene.lambert_only) launch_trace_m<true>(p, n_blocks, lds, st);
     else launch_trace_m<false>(p, n_blocks, lds, st);
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront trace of a flat scene. The path-regeneration kernel keeps each
+// lane's whole path state in registers, so every loop iteration runs the
+// shading code and the triangle loop at the occupancy the shading code
+// allows, and each code block of an iteration with only the lanes that need
+// it. Here the two halves are separate kernels over path slots held in HBM:
+//
+//  * k_wf_shade: one lane per slot calls path_step once (resolve the answered
+//    queries, write a finished sample, hand out pixels, start a camera ray or
+//    shade one bounce) and lists the slot's new queries;
+//  * k_wf_isect: one lane per listed query runs the triangle loop, at the
+//    register count of the loop alone.
+//
+// The same path_step, tapes and hand-out rules give the same records as the
+// path-regeneration kernel. A fixed number of rounds is launched (the host
+// cannot see when a pass ends); k_trace<..., kResume> then takes over the
+// slots still running, so a pass ends however long its longest pixel is.
+__global__ void k_wf_begin(WavefrontState w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < w.slots) w.state[size_t(WF_ST) * w.slots + i] = uint32_t(ST_NEED);
+    if (i < 2 * wf_groups(w.slots)) w.cursor[i] = 0;
+    if (i < 2 * WF_MAX_ITERS * WF_LISTS) w.ctrl[i] = 0;
+}
+
+// True when some slot was still running after round `iter - 1` (or iter is
+// the first round). Wave-uniform.
+__device__ __forceinline__ bool wf_running(const WavefrontState& w, int iter, int lane) {
+    if (iter == 0) return true;
+    return ballot64(wf_live(w, iter - 1)[lane] != 0) != 0;
+}
+
+template <bool kDepth, bool kLambert>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_wf_shade(TraceParams p, int iter) {
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    const int g = s >> 6;
+    if (!wf_running(p.wf, iter, lane)) return;   // the same for every wave of the launch
+    DevScene S = p.scene;
+    {
+        extern __shared__ float4 s_dyn[];
+        const DevScene& G = p.scene;
+        const int n_mat4 = G.n_materials * int(sizeof(DevMaterial) / sizeof(float4));
+        const int n_lig4 = G.n_lights * int(sizeof(DevLight) / sizeof(float4));
+        float4* s_attr = s_dyn;
+        float4* s_mat = s_attr + G.n_tris;
+        float4* s_lig = s_mat + n_mat4;
+        float* s_cdf = reinterpret_cast<float*>(s_lig + n_lig4);
+        for (int k = threadIdx.x; k < G.n_tris; k += TRACE_BLOCK) s_attr[k] = G.tri_attr[k];
+        for (int k = threadIdx.x; k < n_mat4; k += TRACE_BLOCK)
+            s_mat[k] = reinterpret_cast<const float4*>(G.materials)[k];
+        for (int k = threadIdx.x; k < n_lig4; k += TRACE_BLOCK) s_lig[k] = reinterpret_cast<const float4*>(G.lights)[k];
+        for (int k = threadIdx.x; k < G.n_lights; k += TRACE_BLOCK) s_cdf[k] = G.light_cdf[k];
+        __syncthreads();
+        S.tri_attr = s_attr;
+        S.materials = reinterpret_cast<const DevMaterial*>(s_mat);
+        S.lights = reinterpret_cast<const DevLight*>(s_lig);
+        S.light_cdf = s_cdf;
+    }
+    if ((g << 6) >= p.wf.slots) return;          // a wave past the last slot
+    const v3 env = mk3(S.env[0], S.env[1], S.env[2]);
+    const bool env_on = maxc3(env) > 0.0f;
+    const int n_active = p.plan.totals[1];
+    int c_next = p.wf.cursor[2 * g], c_end = p.wf.cursor[2 * g + 1];
+
+    int st = ST_DEAD, q = 0, pend = 0;
+    int pix = -1, left = 0;
+    int64_t slot = 0;
+    Draws dr;
+    dr.tape = p.tape; dr.pos = 0; dr.limit = MT_N; dr.cache = make_uint4(0, 0, 0, 0);
+    LaneRays R;
+    R.ro = mk3(0, 0, 0); R.rd_ext = R.ro; R.rd_l = R.ro; R.rd_e = R.ro; R.t_l = 0.0f;
+    v3 T = mk3(1, 1, 1), L = mk3(0, 0, 0), Ld = mk3(0, 0, 0);
+    v3 sp = mk3(0, 0, 0), sn = mk3(0, 0, 0), swo = mk3(0, 0, 0);
+    v3 cw_l = mk3(0, 0, 0), cw_e = mk3(0, 0, 0), cw_x = mk3(0, 0, 0);
+    float bpdf = 0.0f;
+    int smat = 0, depth = 0;
+    uint32_t flags = 0, light_bits = 255u;
+    HitRec h; h.hit = 0; h.sphere = 0; h.index = -1; h.t = 0.0f; h.sn = mk3(0, 0, 0);
+    bool occ_l = false, occ_e = false;
+    const bool valid = s < p.wf.slots;
+    const WfRef W{p.wf, s};
+    if (valid)
+        wf_load<kDepth>(p, W, h, occ_l, occ_e, st, q, pend, pix, left, slot, dr, R, T, L, Ld, cw_l, cw_e, cw_x, bpdf,
+                        smat, depth, flags, light_bits);
+    path_step<kDepth, kLambert>(p, S, lane, env, env_on, c_next, c_end, n_active, h, occ_l, occ_e, st, q, pend, pix,
+                                left, slot, dr, R, T, L, Ld, sp, sn, swo, cw_l, cw_e, cw_x, bpdf, smat, depth, flags,
+                                light_bits);
+    if (valid)
+        wf_store<kDepth>(p, W, st, q, pix, left, slot, dr, R, T, L, Ld, cw_l, cw_e, cw_x, bpdf, smat, depth, flags,
+                         light_bits);
+    if (lane == 0) { p.wf.cursor[2 * g] = c_next; p.wf.cursor[2 * g + 1] = c_end; }
+
+    // list the queries: this group's sub-list, one counter update per wave
+    const int list = g % WF_LISTS;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t mx = ballot64(pend & Q_EXT), ml = ballot64(pend & Q_SHADOW_L), me = ballot64(pend & Q_SHADOW_E);
+    const int nx = __popcll(mx), nl = __popcll(ml), n = nx + nl + __popcll(me);
+    if (n) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(wf_count(p.wf, iter) + list, n);
+        base = __shfl(base, 0);
+        int* out = p.wf.queue + size_t(list) * wf_list_cap(p.wf.slots) + base;
+        if (pend & Q_EXT) out[__popcll(mx & below)] = s * 4 + 1;
+        if (pend & Q_SHADOW_L) out[nx + __popcll(ml & below)] = s * 4 + 2;
+        if (pend & Q_SHADOW_E) out[nx + nl + __popcll(me & below)] = s * 4 + 3;
+    }
+    if (lane == 0 && ballot64(st != ST_DEAD) != 0) wf_live(p.wf, iter)[list] = 1;
+}
+
+constexpr int WF_ISECT_BLOCK = 256;
+__global__ void __launch_bounds__(WF_ISECT_BLOCK) k_wf_isect(TraceParams p, int iter) {
+    __shared__ int s_count[WF_LISTS];
+    extern __shared__ float4 s_dyn[];
+    const int* cnt = wf_count(p.wf, iter);
+    if (threadIdx.x < WF_LISTS) s_count[threadIdx.x] = cnt[threadIdx.x];
+    const DevScene& S = p.scene;
+    const int n_perm = S.n_tris * 9;
+    for (int k = threadIdx.x; k < n_perm; k += WF_ISECT_BLOCK) s_dyn[k] = S.tri_perm[k];
+    __syncthreads();
+    const int cap = wf_list_cap(p.wf.slots);
+    const float inf = __builtin_inff();
+    uint32_t n_queries = 0;
+    const int64_t total = int64_t(WF_LISTS) * cap;
+    for (int64_t w = int64_t(blockIdx.x) * WF_ISECT_BLOCK + threadIdx.x; w < total;
+         w += int64_t(gridDim.x) * WF_ISECT_BLOCK) {
+        const int list = int(w / cap), e = int(w - int64_t(list) * cap);
+        if (e >= s_count[list]) continue;
+        const int code = p.wf.queue[w];
+        const int kind = code & 3;
+        const WfRef W{p.wf, code >> 2};
+        const v3 ro = W.f3(WF_RO);
+        if (kind == 1) {
+            const HitRec h = trace_query(S, s_dyn, ro, W.f3(WF_RDX), inf, false);
+            W.put(WF_HCODE, h.hit ? uint32_t(h.index * 2 + h.sphere) : 0xffffffffu);
+            W.putf(WF_HT, h.t);
+            if (h.sphere) W.put3(WF_HSN, h.sn);
+        } else if (kind == 2) {
+            W.put(WF_OCCL, trace_query(S, s_dyn, ro, W.f3(WF_RDL), W.f(WF_TL), true).hit ? 1u : 0u);
+        } else {
+            W.put(WF_OCCE, trace_query(S, s_dyn, ro, W.f3(WF_RDE), inf, true).hit ? 1u : 0u);
+        }
+        ++n_queries;
+    }
+    if (p.query_count) {
+        unsigned long long tot = n_queries;
+        for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+        if ((threadIdx.x & 63) == 0 && tot) atomicAdd(p.query_count, tot);
+    }
+}
+
+static size_t wf_shade_lds_bytes(const DevScene& s) {
+    return std::max<size_t>(16, size_t(s.n_tris) * sizeof(float4) + size_t(s.n_materials) * sizeof(DevMaterial) +
+                                    size_t(s.n_lights) * (sizeof(DevLight) + sizeof(float)));
+}
+
+template <bool kLambert>
+static void launch_wavefront_m(const TraceParams& p, int iterations, int finish_blocks, hipStream_t st) {
+    const WavefrontState& w = p.wf;
+    const int begin_n = std::max(w.slots, 2 * WF_MAX_ITERS * WF_LISTS);
+    hipLaunchKernelGGL(k_wf_begin, dim3((begin_n + 255) / 256), dim3(256), 0, st, w);
+    const int shade_blocks = (w.slots + TRACE_BLOCK - 1) / TRACE_BLOCK;
+    const size_t shade_lds = wf_shade_lds_bytes(p.scene);
+    const size_t isect_lds = std::max<size_t>(16, size_t(p.scene.n_tris) * 9 * sizeof(float4));
+    static int isect_blocks = [] {
+        int nb = 0, dev = 0, cus = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_wf_isect),
+                                                         WF_ISECT_BLOCK, 9 * TRACE_LDS_TRIS * sizeof(float4)) !=
+                hipSuccess || nb <= 0)
+            nb = 1;
+        return nb * std::max(1, cus);
+    }();
+    const int n_iter = std::min(iterations, WF_MAX_ITERS);
+    for (int i = 0; i < n_iter; ++i) {
+        if (p.max_depth > 0)
+            hipLaunchKernelGGL((k_wf_shade<true, kLambert>), dim3(shade_blocks), dim3(TRACE_BLOCK), shade_lds, st, p, i);
+        else
+            hipLaunchKernelGGL((k_wf_shade<false, kLambert>), dim3(shade_blocks), dim3(TRACE_BLOCK), shade_lds, st, p,
+                               i);
+        hipLaunchKernelGGL(k_wf_isect, dim3(isect_blocks), dim3(WF_ISECT_BLOCK), isect_lds, st, p, i);
+    }
+    const int fb = std::max(finish_blocks, shade_blocks);
+    const size_t lds = trace_lds_bytes(p.scene, p.bvh);
+    if (p.max_depth > 0)
+        hipLaunchKernelGGL((k_trace<false, true, kLambert, true>), dim3(fb), dim3(TRACE_BLOCK), lds, st, p);
+    else
+        hipLaunchKernelGGL((k_trace<false, false, kLambert, true>), dim3(fb), dim3(TRACE_BLOCK), lds, st, p);
+}
+
+void launch_trace_wavefront(const TraceParams& p, int iterations, int finish_blocks, hipStream_t st) {
+    if (p.scene.lambert_only) launch_wavefront_m<true>(p, iterations, finish_blocks, st);
+    else launch_wavefront_m<false>(p, iterations, finish_blocks, st);
 }
 
 // ---------------------------------------------------------------------------

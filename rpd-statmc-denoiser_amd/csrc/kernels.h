@@ -47,6 +47,33 @@ struct PassPlan {            // which pixels a pass samples and where their slot
     int n_pixels;
 };
 
+// Wavefront trace (flat scenes): every path slot's lane state between the
+// shading and intersection kernels, as WF_WORDS planes of `slots` words
+// (word k of slot s at state[k * slots + s]); the query lists of one round
+// (slot * 4 + query kind, at most 3 per slot); per-round counters (for round
+// i, ctrl[i * WF_LISTS + l] counts the queries of list l and
+// ctrl[(WF_MAX_ITERS + i) * WF_LISTS + l] is set while a slot of list l still
+// runs); and each 64-slot group's pixel hand-out cursor.
+constexpr int WF_WORDS = 51;
+constexpr int WF_MAX_ITERS = 512;
+struct WavefrontState {
+    uint32_t* state;
+    int* queue;
+    int* ctrl;
+    int* cursor;
+    int slots;
+};
+// Query lists: 64 sub-lists, a 64-slot group g appending to sub-list g % 64,
+// so that no more than a 64th of the waves add to any one counter.
+constexpr int WF_LISTS = 64;
+constexpr int WF_MAX_SLOTS = 1 << 22;
+__host__ __device__ __forceinline__ int wf_groups(int slots) { return (slots + 63) / 64; }
+__host__ __device__ __forceinline__ int wf_list_cap(int slots) {
+    return 3 * 64 * ((wf_groups(slots) + WF_LISTS - 1) / WF_LISTS);
+}
+__host__ __device__ __forceinline__ int64_t wf_queue_words(int slots) { return int64_t(WF_LISTS) * wf_list_cap(slots); }
+constexpr int WF_CTRL_WORDS = 2 * WF_MAX_ITERS * WF_LISTS;
+
 struct TraceParams {
     DevScene scene;
     Camera cam;
@@ -63,6 +90,7 @@ struct TraceParams {
     int max_depth;           // shading points per path, 0 = unbounded
     DevBvh bvh;              // nodes null: the flat loop over scene.tri_perm
     int tape_per_sample, tape_margin;  // tape words written per sample and beyond (tape_prefix_words)
+    WavefrontState wf;       // path slots of the wavefront trace (flat scenes), or null state
 };
 
 // Per-pixel accumulators (structure of arrays, n_pixels each unless noted)
@@ -212,6 +240,11 @@ hipError_t sort_pairs_u32(const uint32_t* keys_in, uint32_t* keys_out, const int
 // ---- launch wrappers (each enqueues on `st`) ----
 void launch_tape(const TraceParams& p, hipStream_t st);
 void launch_trace(const TraceParams& p, int n_blocks, hipStream_t st);
+// Wavefront trace of a flat scene: `iterations` rounds of shading (one bounce
+// or camera ray per path slot, queries listed) and intersection (one lane per
+// listed query), then the path-regeneration kernel resumes the slots still
+// running and finishes the pass. Same records as launch_trace.
+void launch_trace_wavefront(const TraceParams& p, int iterations, int finish_blocks, hipStream_t st);
 int trace_block_size();
 int trace_blocks_per_cu(const DevScene& s, const DevBvh& b, int max_depth);  // resident trace blocks per CU
 // too large for the flat loop's LDS tables: traced through a BVH

@@ -219,6 +219,7 @@ def lib():
             ("rpd_job_submit_passes", [vp, vp, C.POINTER(RenderConfig), C.POINTER(Camera), C.c_uint32, C.POINTER(vp)]),
             ("rpd_render_finish_batch", [vp, vp, C.c_int, C.POINTER(C.c_double)]),
             ("rpd_ctx_set_graph_passes", [vp, C.c_int]),
+            ("rpd_ctx_set_trace_wavefront", [vp, C.c_float]),
             ("rpd_ctx_set_debug_outputs", [vp, C.c_int]),
             ("rpd_statmc_tile_shapes", [vp, C.c_int, C.POINTER(C.c_int)]),
             ("rpd_ctx_set_statmc_tile", [vp, C.c_int, C.c_int]),
@@ -321,6 +322,11 @@ class Context:
 
     def set_graph_passes(self, on):
         check(self.L.rpd_ctx_set_graph_passes(self.h, int(on)))
+
+    def set_trace_wavefront(self, rounds_per_sample):
+        """Wavefront trace for flat scenes: rounds per sample of a pass before the path-regeneration
+        kernel finishes it (0: the path-regeneration kernel alone)."""
+        check(self.L.rpd_ctx_set_trace_wavefront(self.h, float(rounds_per_sample)))
 
     def memory(self):
         v = [C.c_uint64() for _ in range(4)]
