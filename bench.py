@@ -7,10 +7,12 @@ StatMC with colour window radius 10, 3 variance-smoothing iterations and a
 thin lens (aperture 0.14, focus 1.05), seed 1234, FP32 sample records.
 
 Prints one JSON line. `value` is whole-job throughput in Msamples/s (path
-samples traced per second of job time, all device state resident); the line
-also carries denoised MP/s, the dominant kernel's roofline (HIP-event timed
-inside the engine on the job's stream) and the CPU oracle timed on a bounded
-sample of the same workload on this host.
+samples traced per second of job time, all device state resident), one job
+after another; the line also carries denoised MP/s, the same jobs through the
+serving loop (several in flight: throughput, p50/p99 latency and image
+checksums against the one-by-one images), the dominant kernel's roofline
+(HIP-event timed inside the engine on the job's stream) and the CPU oracle
+timed on a bounded sample of the same workload on this host.
 """
 import argparse
 import json
@@ -366,6 +368,65 @@ def c4_stream(ctx, reqs, rate_scale, max_in_flight, cobatch_tail=False):
     return wall, sum(samples), st, sums
 
 
+SERVED_C1 = dict(jobs=16, max_in_flight=6, sustainable_fraction=0.8, seed=21)
+
+
+def served_c1(ctx, cfg, cam, plain_ms, plain_sum, plain_samples):
+    """The full-size job through the serving loop (rpd_server): 16 requests of
+    the same job (seed 1234), first all at once (the saturated throughput, 6 in
+    flight), then as Poisson arrivals at 0.8 of that throughput (the latency of
+    a loop that keeps up). Every served image must equal the one-by-one image
+    (radiance checksum), so relMSE is the same in both modes."""
+    import threading
+
+    def stream(arrivals):
+        srv = eng.Server(ctx, max_in_flight=SERVED_C1["max_in_flight"], graph_passes=False)
+        ids = [None] * len(arrivals)
+
+        def submitter(t0):
+            for k, at in enumerate(arrivals):
+                delay = t0 + at - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids[k] = srv.submit(cfg, cam, WORKLOAD["seed"])
+
+        t0 = time.perf_counter()
+        th = threading.Thread(target=submitter, args=(t0,))
+        th.start()
+        sums, samples = [], 0
+        for k in range(len(arrivals)):
+            while ids[k] is None:
+                time.sleep(0.0005)
+            j, _ = srv.wait(ids[k])
+            sums.append(float(np.asarray(j.get("radiance"), np.float64).sum()))
+            samples += int(j.get("sample_count").sum())
+            j.release()
+        wall = time.perf_counter() - t0
+        th.join()
+        st = srv.stats()
+        srv.close()
+        return wall, samples, sums, st
+
+    n = SERVED_C1["jobs"]
+    sat_wall, sat_samples, sat_sums, sat = stream([0.0] * n)
+    rate = n / sat_wall
+    rng = np.random.default_rng(SERVED_C1["seed"])
+    arrivals = list(np.cumsum(rng.exponential(1.0 / (SERVED_C1["sustainable_fraction"] * rate), n)))
+    sus_wall, sus_samples, sus_sums, sus = stream(arrivals)
+    lat = lambda st: {"p50": st["latency_p50_ms"], "p99": st["latency_p99_ms"], "max": st["latency_max_ms"],
+                      "queue_p99": st["queue_p99_ms"], "max_in_flight_seen": st["max_in_flight_seen"]}
+    return {
+        "jobs": n, "max_in_flight": SERVED_C1["max_in_flight"],
+        "jobs_per_s": rate, "msamples_per_s": sat_samples / sat_wall / 1e6, "ms_per_job": sat_wall * 1e3 / n,
+        "over_plain": plain_ms / (sat_wall * 1e3 / n),
+        "latency_ms_all_at_once": lat(sat),
+        "sustainable": {"arrival_jobs_per_s": SERVED_C1["sustainable_fraction"] * rate, "latency_ms": lat(sus)},
+        "plain_latency_ms": plain_ms,
+        "images_equal_to_plain": all(v == plain_sum for v in sat_sums + sus_sums),
+        "samples_equal_to_plain": sat_samples == n * plain_samples and sus_samples == n * plain_samples,
+    }
+
+
 def run_c4(args):
     """C4: one fixed request stream four ways, on one context:
       * plain: the per-image GPU path, one whole job after another
@@ -455,6 +516,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-quality", action="store_true", help="skip the relMSE against a high-spp reference")
+    ap.add_argument("--no-served", action="store_true", help="skip the same jobs through the serving loop")
     ap.add_argument("--size", type=int, default=WORKLOAD["width"])
     ap.add_argument("--trace-wavefront", type=float, default=None,
                     help="wavefront trace rounds per sample for flat scenes (0: path-regeneration kernel only; "
@@ -506,8 +568,13 @@ def main():
         cnt = j.counters()
         samples += cnt["samples"]
         queries += cnt["queries"]
-        j.release()
+        if k < args.steps - 1:
+            j.release()
     wall = time.perf_counter() - t0
+    # the last job's image, read after the timed region (for the served comparison)
+    plain_sum = float(np.asarray(j.get("radiance"), np.float64).sum())
+    plain_samples = int(j.get("sample_count").sum())
+    j.release()
     ms_per_step = wall * 1e3 / args.steps
     n_px = w * h
     # denoised MP/s over the StatMC-side stages (sensitivity, StatMC, variance, allocation)
@@ -551,6 +618,8 @@ def main():
             # the same kernel against the HBM roof: counter bytes per launch over its launch time
             result["roofline"]["hbm_gbs"] = traffic / avg_launch_s / 1e9
             result["roofline"]["hbm_frac"] = traffic / avg_launch_s / 1e9 / HBM_PEAK_GBS
+    if not args.no_served:
+        result["served"] = served_c1(ctx, cfg, cam, ms_per_step, plain_sum, plain_samples)
     if not args.no_quality:
         result["quality"] = c1_quality(ctx, cfg, cam, w, h)
     if not args.no_cpu_baseline:

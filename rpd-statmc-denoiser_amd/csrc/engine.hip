@@ -157,7 +157,39 @@ struct rpd_context {
     // per slot) shading/intersection rounds before the path-regeneration
     // kernel finishes them; 0 keeps the path-regeneration kernel alone
     float wf_rounds = 0.0f;
+    // streams of released jobs, for the next jobs: creating a stream costs
+    // host time, and with more hardware queues than the default (the
+    // GPU_MAX_HW_QUEUES environment variable) a new stream can also set up a
+    // new queue, a few milliseconds
+    std::mutex streams_mu;
+    std::vector<hipStream_t> streams;
 };
+
+namespace {
+constexpr size_t STREAM_POOL_MAX = 64;
+hipError_t job_stream_take(rpd_context* c, hipStream_t* s) {
+    {
+        std::lock_guard<std::mutex> lk(c->streams_mu);
+        if (!c->streams.empty()) {
+            *s = c->streams.back();
+            c->streams.pop_back();
+            return hipSuccess;
+        }
+    }
+    return stream_create(s);
+}
+// s is idle (its job synchronised it)
+void job_stream_give(rpd_context* c, hipStream_t s) {
+    {
+        std::lock_guard<std::mutex> lk(c->streams_mu);
+        if (c->streams.size() < STREAM_POOL_MAX) {
+            c->streams.push_back(s);
+            return;
+        }
+    }
+    stream_destroy(s);
+}
+}  // namespace
 
 struct rpd_scene {
     rpd_context* ctx = nullptr;
@@ -914,7 +946,7 @@ void job_free(rpd_job* j) {
     }
     j->slab = nullptr;
     j->bytes = 0;
-    if (j->stream) stream_destroy(j->stream);
+    if (j->stream) job_stream_give(j->ctx, j->stream);
     j->stream = nullptr;
 }
 
@@ -934,7 +966,7 @@ int job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg,
     j->wf_on = ctx->wf_rounds > 0.0f && !scene->bvh.nodes;
     if (ctx->budget && ctx->in_use.load() + job_bytes(*cfg, 0, j->wf_on) > ctx->budget)
         return fail(RPD_ERR_OUT_OF_MEMORY, "job does not fit the context's memory budget");
-    HIP_TRY(stream_create(&j->stream));
+    HIP_TRY(job_stream_take(ctx, &j->stream));
     if ((st = job_alloc(j.get()))) { job_free(j.get()); return st; }
     if ((st = reset_state(j.get()))) { job_free(j.get()); return st; }
     *out = j.release();
@@ -955,7 +987,7 @@ int job_create_denoiser(rpd_context* ctx, const rpd_render_config* cfg, int max_
     set_geometry(j.get());
     if (ctx->budget && ctx->in_use.load() + job_bytes(*cfg, max_spp) > ctx->budget)
         return fail(RPD_ERR_OUT_OF_MEMORY, "job does not fit the context's memory budget");
-    HIP_TRY(stream_create(&j->stream));
+    HIP_TRY(job_stream_take(ctx, &j->stream));
     if ((st = job_alloc(j.get()))) { job_free(j.get()); return st; }
     if ((st = reset_state(j.get()))) { job_free(j.get()); return st; }
     *out = j.release();
@@ -1133,6 +1165,8 @@ int rpd_ctx_destroy(rpd_context* ctx) {
     if (ctx->f_stream) hipStreamSynchronize(ctx->f_stream);
     if (ctx->f_buf) dev_free(ctx->f_buf);
     if (ctx->f_stream) stream_destroy(ctx->f_stream);
+    for (hipStream_t st : ctx->streams) stream_destroy(st);
+    ctx->streams.clear();
     cache_trim(ctx, 0);
     delete ctx;
     return RPD_OK;
