@@ -10,7 +10,7 @@ ROOT=$(cd "$(dirname "$0")/.." && pwd)
 OUT=${1:-$ROOT/gpurun_out/profile}
 mkdir -p "$OUT"
 cd /tmp && export TMPDIR=/tmp
-BENCH="python3 $ROOT/bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-quality"
+BENCH="python3 $ROOT/bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-quality --no-served"
 echo "profile: kernel trace"
 timeout -k 10 300 rocprofv3 --kernel-trace --stats --output-format csv -d "$OUT/trace" -o run -- $BENCH > "$OUT/trace.log" 2>&1 || exit $?
 for c in FETCH_SIZE WRITE_SIZE; do

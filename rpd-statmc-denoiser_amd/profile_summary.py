@@ -12,7 +12,7 @@ import os
 import sys
 from collections import defaultdict
 
-CMD = "rpd-statmc-denoiser_amd/profile_kernels.sh (rocprofv3 on: python3 bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-quality)"
+CMD = "rpd-statmc-denoiser_amd/profile_kernels.sh (rocprofv3 on: python3 bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-quality --no-served)"
 COPY_BYTES = 1 << 30
 
 
