@@ -40,14 +40,22 @@ __global__ void k_importance(int n, PixelBuffers pb, const float* vden, float sm
 }
 
 // main.cpp:1331-1368: edge-aware smoothing
-__global__ void k_importance_smooth(int W, int H, int R, PixelBuffers pb, Features f, const float* src, float* dst) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = W * H;
-    if (i >= n) return;
-    if (pb.hit_count[i] == 0) { dst[i] = src[i]; return; }
-    const int x = i % W, y = i / W;
-    const v3 ni = mk3(f.normal[i], f.normal[n + i], f.normal[2 * n + i]);
-    const float di = f.depth[i];
+__global__ void __launch_bounds__(SMOOTH_TW * SMOOTH_TH) k_importance_smooth(int W, int H, int R, PixelBuffers pb,
+                                                                             Features f, const float* src,
+                                                                             float* dst, int staged) {
+    extern __shared__ float4 s_tile[];
+    const SmoothSrc S = smooth_stage(W, H, R, pb, f, src, staged ? s_tile : nullptr);
+    const int x = int(blockIdx.x) * SMOOTH_TW + int(threadIdx.x) % SMOOTH_TW;
+    const int y = int(blockIdx.y) * SMOOTH_TH + int(threadIdx.x) / SMOOTH_TW;
+    if (x >= W || y >= H) return;
+    const int i = y * W + x;
+    float4 gi;
+    float vsrc;
+    int hi;
+    S.get(x, y, gi, vsrc, hi);
+    if (hi == 0) { dst[i] = vsrc; return; }
+    const v3 ni = mk3(gi.x, gi.y, gi.z);
+    const float di = gi.w;
     double acc = 0.0, ws = 0.0;
     for (int dy = -R; dy <= R; ++dy) {
         const int yy = y + dy;
@@ -55,18 +63,20 @@ __global__ void k_importance_smooth(int W, int H, int R, PixelBuffers pb, Featur
         for (int dx = -R; dx <= R; ++dx) {
             const int xx = x + dx;
             if (xx < 0 || xx >= W) continue;
-            const int q = yy * W + xx;
-            if (pb.hit_count[q] == 0) continue;
+            float4 gq;
+            float vq;
+            int hq;
+            S.get(xx, yy, gq, vq, hq);
+            if (hq == 0) continue;
             const float sw = 1.0f / (1.0f + float(dx * dx + dy * dy));
-            const float nw = clampf((dot3(ni, mk3(f.normal[q], f.normal[n + q], f.normal[2 * n + q])) + 1.0f) * 0.5f,
-                                    0.0f, 1.0f);
-            const float dw = expf_cr(-4.0f * relative_depth_delta(di, f.depth[q]));
+            const float nw = clampf((dot3(ni, mk3(gq.x, gq.y, gq.z)) + 1.0f) * 0.5f, 0.0f, 1.0f);
+            const float dw = expf_cr(-4.0f * relative_depth_delta(di, gq.w));
             const double w = double(sw) * double(nw) * double(dw);
-            acc += w * double(src[q]);
+            acc += w * double(vq);
             ws += w;
         }
     }
-    dst[i] = (ws > double(RPD_EPS_SMALL)) ? float(acc / ws) : src[i];
+    dst[i] = (ws > double(RPD_EPS_SMALL)) ? float(acc / ws) : vsrc;
 }
 
 // ---------------------------------------------------------------------------
@@ -618,7 +628,10 @@ void launch_importance(int W, int H, const AllocConfig& ac, const PixelBuffers& 
     if (ac.adaptive_spp <= 0) return;
     hipLaunchKernelGGL(k_importance, grid, dim3(256), 0, st, n, pb, vden, ac.sigma_max, imp, ac.dbg_variance_source);
     if (ac.radius > 0) {
-        hipLaunchKernelGGL(k_importance_smooth, grid, dim3(256), 0, st, W, H, ac.radius, pb, f, imp, c.tmp);
+        const dim3 tiles((W + SMOOTH_TW - 1) / SMOOTH_TW, (H + SMOOTH_TH - 1) / SMOOTH_TH);
+        const size_t lds = smooth_lds_bytes(ac.radius);
+        hipLaunchKernelGGL(k_importance_smooth, tiles, dim3(SMOOTH_TW * SMOOTH_TH), lds, st, W, H, ac.radius, pb, f,
+                           imp, c.tmp, lds > 0 ? 1 : 0);
         hipMemcpyAsync(imp, c.tmp, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st);
     }
 }

@@ -434,29 +434,39 @@ __global__ void k_var_init(int n, PixelBuffers pb, float maxv, float* cur) {
     if (!isfinite(v)) v = RPD_EPS_SMALL;
     cur[i] = clampf(v, RPD_EPS_SMALL, maxv);
 }
-__global__ void k_var_smooth(int W, int H, VarianceConfig vc, PixelBuffers pb, Features f, const float* src,
-                             float* dst) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = W * H;
-    if (i >= n) return;
-    if (pb.hit_count[i] == 0) { dst[i] = src[i]; return; }
-    const int x = i % W, y = i / W;
-    const v3 ni = mk3(f.normal[i], f.normal[n + i], f.normal[2 * n + i]);
-    const float di = f.depth[i];
-    const float vi = fmaxf(RPD_EPS_SMALL, src[i]);
-    float ws = 0.0f, wsum = 0.0f;
+__global__ void __launch_bounds__(SMOOTH_TW * SMOOTH_TH) k_var_smooth(int W, int H, VarianceConfig vc,
+                                                                      PixelBuffers pb, Features f, const float* src,
+                                                                      float* dst, int staged) {
+    extern __shared__ float4 s_tile[];
     const int R = vc.radius;
+    const SmoothSrc S = smooth_stage(W, H, R, pb, f, src, staged ? s_tile : nullptr);
+    const int x = int(blockIdx.x) * SMOOTH_TW + int(threadIdx.x) % SMOOTH_TW;
+    const int y = int(blockIdx.y) * SMOOTH_TH + int(threadIdx.x) / SMOOTH_TW;
+    if (x >= W || y >= H) return;
+    const int i = y * W + x;
+    float4 gi;
+    float vsrc;
+    int hi;
+    S.get(x, y, gi, vsrc, hi);
+    if (hi == 0) { dst[i] = vsrc; return; }
+    const v3 ni = mk3(gi.x, gi.y, gi.z);
+    const float di = gi.w;
+    const float vi = fmaxf(RPD_EPS_SMALL, vsrc);
+    float ws = 0.0f, wsum = 0.0f;
     for (int dy = -R; dy <= R; ++dy) {
         const int yy = y + dy;
         if (yy < 0 || yy >= H) continue;
         for (int dx = -R; dx <= R; ++dx) {
             const int xx = x + dx;
             if (xx < 0 || xx >= W || (dx == 0 && dy == 0)) continue;
-            const int q = yy * W + xx;
-            if (pb.hit_count[q] == 0) continue;
-            if (dot3(ni, mk3(f.normal[q], f.normal[n + q], f.normal[2 * n + q])) < vc.normal_threshold) continue;
-            if (relative_depth_delta(di, f.depth[q]) > vc.depth_threshold) continue;
-            const float vj = fmaxf(RPD_EPS_SMALL, src[q]);
+            float4 gq;
+            float vq;
+            int hq;
+            S.get(xx, yy, gq, vq, hq);
+            if (hq == 0) continue;
+            if (dot3(ni, mk3(gq.x, gq.y, gq.z)) < vc.normal_threshold) continue;
+            if (relative_depth_delta(di, gq.w) > vc.depth_threshold) continue;
+            const float vj = fmaxf(RPD_EPS_SMALL, vq);
             const float rd = fabsf(vi - vj) / fmaxf(RPD_EPS_SMALL, sqrtf(vi * vj));
             if (rd > vc.compat_sigma) continue;
             const float wj = 1.0f / (1.0f + float(dx * dx + dy * dy));
@@ -489,8 +499,11 @@ void launch_variance(int W, int H, const VarianceConfig& vc, const PixelBuffers&
     float* cur = out;
     float* nxt = tmp;
     hipLaunchKernelGGL(k_var_init, grid, dim3(256), 0, st, n, pb, maxv, cur);
+    const dim3 tiles((W + SMOOTH_TW - 1) / SMOOTH_TW, (H + SMOOTH_TH - 1) / SMOOTH_TH);
+    const size_t lds = smooth_lds_bytes(vc.radius);
     for (int it = 0; it < vc.iterations; ++it) {
-        hipLaunchKernelGGL(k_var_smooth, grid, dim3(256), 0, st, W, H, vc, pb, f, cur, nxt);
+        hipLaunchKernelGGL(k_var_smooth, tiles, dim3(SMOOTH_TW * SMOOTH_TH), lds, st, W, H, vc, pb, f, cur, nxt,
+                           lds > 0 ? 1 : 0);
         float* t = cur; cur = nxt; nxt = t;
     }
     hipLaunchKernelGGL(k_var_final, grid, dim3(256), 0, st, n, pb, maxv, cur, out);

@@ -149,6 +149,76 @@ struct VarianceConfig {
     float normal_threshold, depth_threshold, compat_sigma, shrinkage_k, sigma_max;
 };
 
+// ---- edge-aware smoothing windows --------------------------------------------
+// Variance smoothing (main.cpp:1166-1268) and importance smoothing
+// (main.cpp:1331-1368) filter a per-pixel field over a small window, gated or
+// weighted by the neighbours' normals, depths and primary hits. A block
+// filters an SMOOTH_TW x SMOOTH_TH tile and first stages, for the tile and its
+// halo of `radius` pixels, the unit normal and depth (one float4) and the
+// field value with the hit count (one float2) in LDS, so a pixel is read from
+// HBM about once per block instead of once per neighbour. Radii above
+// SMOOTH_MAX_R read the neighbours from global memory instead.
+constexpr int SMOOTH_TW = 32, SMOOTH_TH = 8;
+constexpr int SMOOTH_MAX_R = 8;
+inline size_t smooth_lds_bytes(int radius) {
+    return radius > SMOOTH_MAX_R ? 0
+                                 : size_t(SMOOTH_TW + 2 * radius) * size_t(SMOOTH_TH + 2 * radius) *
+                                       (sizeof(float4) + sizeof(float2));
+}
+// A pixel's guide features and field value, from the staged tile or from
+// global memory (ld == nullptr).
+struct SmoothSrc {
+    const float4* nd;   // staged (normal xyz, depth), or null
+    const float2* vh;   // staged (value, hit count bits)
+    int span, x0, y0;
+    int W, n;
+    const float* nrm;   // 3 planes
+    const float* dep;
+    const int* hits;
+    const float* val;
+    __device__ __forceinline__ void get(int x, int y, float4& g, float& v, int& hit) const {
+        if (nd) {
+            const int k = (y - y0) * span + (x - x0);
+            g = nd[k];
+            const float2 t = vh[k];
+            v = t.x;
+            hit = __float_as_int(t.y);
+        } else {
+            const int q = y * W + x;
+            g = make_float4(nrm[q], nrm[n + q], nrm[2 * n + q], dep[q]);
+            v = val[q];
+            hit = hits[q];
+        }
+    }
+};
+// Called by every thread of the block (it holds a barrier); s: dynamic LDS of
+// smooth_lds_bytes(R) bytes, or null for the global-memory form.
+__device__ __forceinline__ SmoothSrc smooth_stage(int W, int H, int R, const PixelBuffers& pb, const Features& f,
+                                                  const float* src, float4* s) {
+    SmoothSrc o;
+    o.W = W; o.n = W * H;
+    o.nrm = f.normal; o.dep = f.depth; o.hits = pb.hit_count; o.val = src;
+    o.span = SMOOTH_TW + 2 * R;
+    o.x0 = int(blockIdx.x) * SMOOTH_TW - R;
+    o.y0 = int(blockIdx.y) * SMOOTH_TH - R;
+    o.nd = nullptr; o.vh = nullptr;
+    if (!s) return o;
+    const int sh = SMOOTH_TH + 2 * R, cnt = o.span * sh;
+    float2* vh = reinterpret_cast<float2*>(s + cnt);
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+        const int xx = o.x0 + k % o.span, yy = o.y0 + k / o.span;
+        if (xx >= 0 && xx < W && yy >= 0 && yy < H) {
+            const int q = yy * W + xx;
+            s[k] = make_float4(f.normal[q], f.normal[o.n + q], f.normal[2 * o.n + q], f.depth[q]);
+            vh[k] = make_float2(src[q], __int_as_float(pb.hit_count[q]));
+        }
+    }
+    __syncthreads();
+    o.nd = s;
+    o.vh = vh;
+    return o;
+}
+
 // ---- batches of images ------------------------------------------------------
 // The denoising stages (per-pixel fold, features, grid gather, sensitivity,
 // StatMC) run over a batch of images in one launch per stage; a single image

@@ -176,9 +176,12 @@ def test_statmc_stage_identical_inputs(engine_ctx, radius, alpha):
         np.testing.assert_array_equal(oj.get(name), gj.get(name))
 
 
-def test_variance_stage_identical_inputs(engine_ctx):
+@pytest.mark.parametrize("radius", [3, 10])
+def test_variance_stage_identical_inputs(engine_ctx, radius):
+    """Bit-exact from identical statistics; radius 3 filters from the tile
+    staged in LDS, radius 10 (past its halo limit) from global memory."""
     rc = ob.config_c0(48, 40)
-    rc.var_window_radius = 3
+    rc.var_window_radius = radius
     rc.var_iterations = 6
     rc.var_compat_sigma = 3.0
     rc.var_shrinkage_k = 0.7
@@ -190,7 +193,7 @@ def test_variance_stage_identical_inputs(engine_ctx):
     np.testing.assert_array_equal(oj.get("variance_denoised"), gj.get("variance_denoised"))
 
 
-@pytest.mark.parametrize("base_samples,radius,spp", [(0, 1, 4), (2, 1, 4), (1, 0, 4), (0, 2, 1)])
+@pytest.mark.parametrize("base_samples,radius,spp", [(0, 1, 4), (2, 1, 4), (1, 0, 4), (0, 2, 1), (0, 9, 4)])
 def test_allocation_from_state(engine_ctx, base_samples, radius, spp):
     """compute_adaptive_sample_counts from identical statistics: importance
     agrees to one float ulp (the smoothing's expf is correctly rounded here and
