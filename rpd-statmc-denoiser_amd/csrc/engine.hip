@@ -735,6 +735,15 @@ ImageDesc make_desc(const rpd_job* j) {
     d.fc.rpd_scale = j->cfg.rpf_shrinkage_scale;
     d.conf_n0 = j->cfg.rpf_confidence_samples;
     d.dbg = j->dbg;
+    // grid accumulators this job's samples can reach: a render job records no
+    // environment sample without an environment and no lens sample without an
+    // aperture (k_trace); records given by the caller may carry any
+    d.acc_mask = GRID_ACC_ALL;
+    if (j->scene && !j->denoiser) {
+        const float* e = j->scene->dev.env;
+        if (!(e[0] > 0.0f || e[1] > 0.0f || e[2] > 0.0f)) d.acc_mask &= ~((1u << 7) | (1u << 8));
+        if (!(j->cam.lens_radius > 0.0f)) d.acc_mask &= ~((1u << 3) | (1u << 4));
+    }
     return d;
 }
 

@@ -65,8 +65,9 @@ void launch_batch_queues_reset(const Batch& b, hipStream_t st) {
 // chunk sequentially in FP64, so every sum is formed in sample order without
 // atomics. The pass totals are then added to the node.
 //
-// Threads 0-207 are the bin sums (accumulator tid / 16, bin tid % 16) and
-// 208-233 the totals. Every thread runs the same loop: it reads its own pair
+// The first threads hold the bin sums of the accumulators the image's samples
+// can reach, the next ones their totals (at most 220 threads for all 13
+// accumulators). Every such thread runs the same loop: it reads its own pair
 // of terms ((w, w*y) for a bin, (w, w*w) or (w*y, w*y*y) for the totals),
 // compares the contribution's bin with its key (its own bin, or -1, "no
 // bin", for the totals) and picks m from two per-thread values by the result
@@ -135,8 +136,8 @@ struct GatherLds {
     uint32_t bin4[GRID_ACC][GATHER_BIN_ROW];
     int64_t item;
 };
-constexpr int GATHER_BIN_THREADS = GRID_ACC * 16;                      // 208
-constexpr int GATHER_SUM_THREADS = GATHER_BIN_THREADS + 2 * GRID_ACC;  // 234
+// every accumulator's bins and total pairs have a thread of the block
+static_assert((GRID_ACC - 1) * 16 + grid_acc_bins(11) + 2 * GRID_ACC <= GATHER_BLOCK, "gather sum threads");
 
 // One node of one image; called by all threads of the block.
 template <int kMaxPix>
@@ -160,12 +161,29 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds<kMaxPi
     const int nwin = (wx > 0 && wy > 0) ? wx * wy : 0;
     const bool any_active = plan.totals[1] > 0;
 
-    // this thread's sum: bin (acc, bsel) for tid < 208; totals of accumulator
-    // acc for 208 <= tid < 234 (bsel 16: w and w*w, bsel 17: w*y and w*y*y)
-    const bool bin_t = tid < GATHER_BIN_THREADS;
-    const int acc = bin_t ? tid / 16 : (tid - GATHER_BIN_THREADS) >> 1;
-    const int bsel = bin_t ? tid % 16 : 16 + ((tid - GATHER_BIN_THREADS) & 1);
-    const bool active_t = tid < GATHER_SUM_THREADS && (bsel >= 16 || bsel < grid_acc_bins(acc));
+    // This thread's sum: a bin (acc, bsel < 16) or a total pair of accumulator
+    // acc (bsel 16: w and w*w, bsel 17: w*y and w*y*y). Only the accumulators
+    // the image's samples can reach (acc_mask; the environment's when the
+    // scene has an environment, the lens's when the camera has an aperture)
+    // get threads: first every such accumulator's bins, then their totals, so
+    // that the threads past the last sum form whole waves that skip the sums.
+    // An accumulator no sample reaches keeps the zeros it starts with.
+    int acc = -1, bsel = 0;
+    {
+        int t = tid;
+        for (int k = 0; k < GRID_ACC && acc < 0; ++k) {
+            if (!((d.acc_mask >> k) & 1u)) continue;
+            if (t < grid_acc_bins(k)) { acc = k; bsel = t; }
+            else t -= grid_acc_bins(k);
+        }
+        for (int k = 0; k < GRID_ACC && acc < 0; ++k) {
+            if (!((d.acc_mask >> k) & 1u)) continue;
+            if (t < 2) { acc = k; bsel = 16 + t; }
+            else t -= 2;
+        }
+    }
+    const bool active_t = acc >= 0;
+    const bool bin_t = bsel < 16;
     double a0 = 0.0, a1 = 0.0;
 
     const float half = 0.5f * float(g.support);
@@ -264,7 +282,7 @@ __device__ void gather_node(const ImageDesc& d, const int node, GatherLds<kMaxPi
             for (int k = 0; k < GRID_ACC; ++k) c_bin[k * 4 * GATHER_BIN_ROW + tid] = bins[k];
             __syncthreads();
             const int nquad = (min(GATHER_BLOCK, total - base) + 3) >> 2;
-            if (tid < GATHER_SUM_THREADS) {
+            if (active_t) {
                 const double2* src = L.pr[bin_t ? 0 : bsel - 15];
                 const int key = bin_t ? bsel : -1;
                 const double m_eq = bin_t ? 1.0 : -0.0, m_ne = bin_t ? -0.0 : 1.0;

@@ -244,7 +244,9 @@ struct ImageDesc {
     float conf_n0;
     float* dbg;              // null, or 3 planes the window filter fills per pixel with a primary hit:
                              // accepted neighbours, weight sum, effective neighbours (main.cpp:991-1108)
+    uint32_t acc_mask;       // bit k: grid accumulator k can receive samples (GRID_ACC_ALL: every one)
 };
+constexpr uint32_t GRID_ACC_ALL = (1u << GRID_ACC) - 1u;
 enum : int { BK_PIXEL = 0, BK_NODE = 1, BK_TILE = 2, BK_KINDS = 3 };
 constexpr int BATCH_QUEUES = 8;     // one work range per XCD for persistent kernels
 // StatMC window tile shapes (width, height): STATMC_SHAPES[k], k < STATMC_SHAPE_COUNT;
