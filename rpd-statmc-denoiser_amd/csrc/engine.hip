@@ -415,6 +415,9 @@ int trace_blocks(rpd_job* j) {
     return std::max(1, std::min(by_work, per_cu * std::max(1, j->ctx->num_cus)));
 }
 
+// A render plan's active list in the order the trace hands pixels out in.
+void order_plan(rpd_job* j) { launch_plan_tile_order(j->W, j->H, j->counts, j->active, j->scan_scratch, j->stream); }
+
 // Sampling pass with the plan already in counts/offsets/active/totals.
 int pass_after_plan(rpd_job* j, int statmc, int stage) {
     StageTimer tm(j, stage);
@@ -568,6 +571,7 @@ int do_allocation(rpd_job* j) {
     launch_importance(j->W, j->H, ac, j->pb, j->feat, j->vden, j->imp, j->alloc_scratch, j->stream);
     launch_alloc_counts(j->N, ac, j->imp, j->counts, j->alloc_scratch, j->stream);
     launch_plan_scan(j->N, j->counts, j->offsets, j->active, j->totals, j->scan_scratch, j->rec.capacity, j->stream);
+    order_plan(j);
     HIP_TRY(hipGetLastError());
     return RPD_OK;
 }
@@ -672,10 +676,12 @@ int run_passes(rpd_job* j) {
     const bool statmc = j->cfg.use_statmc != 0;
     if (!statmc) {
         launch_plan_uniform(j->N, j->cfg.samples_per_pixel, j->counts, j->offsets, j->active, j->totals, j->stream);
+        order_plan(j);
         if ((st = pass_after_plan(j, 0, 0))) return st;
         return do_finalize(j, false);
     }
     launch_plan_uniform(j->N, j->cfg.samples_per_pixel, j->counts, j->offsets, j->active, j->totals, j->stream);
+    order_plan(j);
     if ((st = pass_after_plan(j, 1, 0))) return st;
     const bool passes = j->cfg.adaptive_passes > 0 && j->cfg.adaptive_spp > 0;
     if (!passes && (st = do_recompute_sensitivity(j))) return st;
@@ -854,7 +860,7 @@ void job_layout(rpd_job* j, SlabLayout& L) {
     int* plan = nullptr; A(plan, 3 * N + 8);
     j->counts = plan; j->offsets = plan + N; j->active = plan + 2 * N; j->totals = plan + 3 * N;
     j->work = plan + 3 * N + 4;
-    A(j->scan_scratch, (size_t)plan_scan_scratch_elems(j->N));
+    A(j->scan_scratch, (size_t)std::max(plan_scan_scratch_elems(j->N), plan_tile_scratch_elems(j->W, j->H)));
     A(j->query_count, 2);
     // a denoiser job draws no random numbers: one placeholder word instead of the tapes
     const size_t tape_words = j->denoiser ? 1 : ((N + MT_GROUP - 1) / MT_GROUP) * MT_GROUP * MT_TAPE_WORDS;
@@ -1558,6 +1564,7 @@ int rpd_job_run_pass(rpd_job* j, const int* counts, int statmc) {
         HIP_TRY(hipMemcpyAsync(j->counts, counts, sizeof(int) * j->N, hipMemcpyHostToDevice, j->stream));
         launch_plan_scan(j->N, j->counts, j->offsets, j->active, j->totals, j->scan_scratch, j->rec.capacity, j->stream);
     }
+    order_plan(j);
     const int st = pass_after_plan(j, statmc ? 1 : 0, counts ? 1 : 0);
     if (st) return st;
     return rpd_job_sync(j);

@@ -936,6 +936,66 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(int n, const int* cou
     }
 }
 int64_t plan_scan_scratch_elems(int n) { return (n + SCAN_BLOCK - 1) / SCAN_BLOCK + 1; }
+
+// Tile order. The trace hands out the active list in order, 64 pixels to a
+// wave at a time; in row order those are a 64x1 strip, in tile order an 8x8
+// tile, whose camera rays, first hits and light directions lie closer
+// together (the BVH walk diverges less across the lanes, the shading less
+// across materials). This rewrites the active list of a plan in tile order
+// (8x8 tiles row by row, pixels row by row within a tile): the same pixels,
+// counts, offsets and records, handed out in another order.
+constexpr int PLAN_TILE = 8;
+__host__ __device__ inline int64_t plan_tile_positions(int W, int H) {
+    return int64_t((W + PLAN_TILE - 1) / PLAN_TILE) * int64_t((H + PLAN_TILE - 1) / PLAN_TILE) * PLAN_TILE *
+           PLAN_TILE;
+}
+__device__ __forceinline__ int tile_pixel(int64_t k, int W, int H) {
+    const int tiles_x = (W + PLAN_TILE - 1) / PLAN_TILE;
+    const int64_t t = k / (PLAN_TILE * PLAN_TILE);
+    const int w = int(k % (PLAN_TILE * PLAN_TILE));
+    const int x = int(t % tiles_x) * PLAN_TILE + w % PLAN_TILE, y = int(t / tiles_x) * PLAN_TILE + w / PLAN_TILE;
+    return (x < W && y < H) ? y * W + x : -1;
+}
+__global__ void __launch_bounds__(SCAN_BLOCK) k_tord_blocks(int W, int H, int64_t K, const int* counts,
+                                                            int64_t* block_sums) {
+    __shared__ int64_t sh[SCAN_BLOCK];
+    const int64_t k = int64_t(blockIdx.x) * SCAN_BLOCK + threadIdx.x;
+    const int p = k < K ? tile_pixel(k, W, H) : -1;
+    const int64_t v = (p >= 0 && counts[p] > 0) ? 1 : 0;
+    const int64_t s = block_incl_scan(v, sh);
+    if (threadIdx.x == SCAN_BLOCK - 1) block_sums[blockIdx.x] = s;
+}
+__global__ void __launch_bounds__(SCAN_BLOCK) k_tord_totals(int nb, int64_t* block_sums) {
+    __shared__ int64_t sh[SCAN_BLOCK];
+    int64_t carry = 0;
+    for (int base = 0; base < nb; base += SCAN_BLOCK) {
+        const int i = base + threadIdx.x;
+        const int64_t v = i < nb ? block_sums[i] : 0;
+        const int64_t s = block_incl_scan(v, sh);
+        if (i < nb) block_sums[i] = carry + s - v;  // exclusive
+        __syncthreads();
+        carry += sh[SCAN_BLOCK - 1];
+        __syncthreads();
+    }
+}
+__global__ void __launch_bounds__(SCAN_BLOCK) k_tord_apply(int W, int H, int64_t K, const int* counts,
+                                                           const int64_t* block_sums, int* active) {
+    __shared__ int64_t sh[SCAN_BLOCK];
+    const int64_t k = int64_t(blockIdx.x) * SCAN_BLOCK + threadIdx.x;
+    const int p = k < K ? tile_pixel(k, W, H) : -1;
+    const bool on = p >= 0 && counts[p] > 0;
+    const int64_t v = on ? 1 : 0;
+    const int64_t s = block_incl_scan(v, sh) - v + block_sums[blockIdx.x];
+    if (on) active[s] = p;
+}
+int64_t plan_tile_scratch_elems(int W, int H) { return (plan_tile_positions(W, H) + SCAN_BLOCK - 1) / SCAN_BLOCK + 1; }
+void launch_plan_tile_order(int W, int H, const int* counts, int* active, int64_t* scratch, hipStream_t st) {
+    const int64_t K = plan_tile_positions(W, H);
+    const int nb = int((K + SCAN_BLOCK - 1) / SCAN_BLOCK);
+    hipLaunchKernelGGL(k_tord_blocks, dim3(nb), dim3(SCAN_BLOCK), 0, st, W, H, K, counts, scratch);
+    hipLaunchKernelGGL(k_tord_totals, dim3(1), dim3(SCAN_BLOCK), 0, st, nb, scratch);
+    hipLaunchKernelGGL(k_tord_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, W, H, K, counts, scratch, active);
+}
 void launch_plan_scan(int n, const int* counts, int* offsets, int* active, int* totals, int64_t* scratch,
                       int64_t capacity, hipStream_t st) {
     const int nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
@@ -960,11 +1020,13 @@ __global__ void __launch_bounds__(256) k_fold(Batch b, int statmc) {
     const PassPlan& plan = d.plan;
     const Records& rec = d.rec;
     const PixelBuffers& pb = d.pb;
-    const int a = int(la);
-    if (a >= plan.totals[1]) return;
-    const int i = plan.active[a];
+    // one lane per pixel in pixel order (the active list is in tile order); a
+    // pass over the record capacity has no active pixels
+    const int i = int(la);
+    if (plan.totals[1] <= 0) return;
     const int N = plan.n_pixels;
     const int cnt = plan.counts[i];
+    if (cnt <= 0) return;
     const int64_t off = plan.offsets[i];
     float rr = pb.radiance[i], rg = pb.radiance[N + i], rb = pb.radiance[2 * N + i];
     float ar = pb.albedo[i], ag = pb.albedo[N + i], ab = pb.albedo[2 * N + i];

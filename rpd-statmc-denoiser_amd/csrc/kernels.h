@@ -42,7 +42,7 @@ struct Camera {
 struct PassPlan {            // which pixels a pass samples and where their slots are
     const int* counts;       // samples per pixel
     const int* offsets;      // first slot per pixel (exclusive scan of counts)
-    const int* active;       // compacted pixel ids with counts > 0
+    const int* active;       // pixel ids with counts > 0 (render passes: in 8x8 tile order)
     const int* totals;       // device: [0] samples in the pass, [1] active pixels (0 if over capacity)
     int n_pixels;
 };
@@ -331,6 +331,11 @@ void launch_plan_scan(int n_pixels, const int* counts, int* offsets, int* active
                       int64_t* scratch, int64_t capacity, hipStream_t st);
 void launch_fill_i32(int* p, int value, int64_t n, hipStream_t st);
 int64_t plan_scan_scratch_elems(int n_pixels);
+// rewrites the active list of a plan (counts set) in 8x8 tile order, the order
+// the trace hands pixels out in (scratch: plan_tile_scratch_elems entries)
+void launch_plan_tile_order(int width, int height, const int* counts, int* active, int64_t* scratch,
+                            hipStream_t st);
+int64_t plan_tile_scratch_elems(int width, int height);
 
 void launch_features(const Batch& b, hipStream_t st);
 void launch_grid_gather(const Batch& b, hipStream_t st);
