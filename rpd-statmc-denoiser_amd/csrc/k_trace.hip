@@ -1020,13 +1020,14 @@ __global__ void __launch_bounds__(256) k_fold(Batch b, int statmc) {
     const PassPlan& plan = d.plan;
     const Records& rec = d.rec;
     const PixelBuffers& pb = d.pb;
-    // one lane per pixel in pixel order (the active list is in tile order); a
-    // pass over the record capacity has no active pixels
-    const int i = int(la);
-    if (plan.totals[1] <= 0) return;
+    // one lane per active pixel, in the active list's order (walking all
+    // pixels and skipping the inactive ones measured slower: the C3 batch
+    // 447 -> 476 ms)
+    const int a = int(la);
+    if (a >= plan.totals[1]) return;
+    const int i = plan.active[a];
     const int N = plan.n_pixels;
     const int cnt = plan.counts[i];
-    if (cnt <= 0) return;
     const int64_t off = plan.offsets[i];
     float rr = pb.radiance[i], rg = pb.radiance[N + i], rb = pb.radiance[2 * N + i];
     float ar = pb.albedo[i], ag = pb.albedo[N + i], ab = pb.albedo[2 * N + i];
