@@ -22,7 +22,9 @@ namespace rpd {
 // latency overlaps the next four words' work. Only the pixel's prefix is
 // written (tape_prefix_words of its sample count): an adaptive pass's pixel of
 // a few samples draws a small part of the 624 words, and the trace extends the
-// rare tape drawn past its prefix.
+// rare tape drawn past its prefix. A whole tape also gets the next state's
+// first MT_S2 words, so that a pixel drawing a little more than 624 numbers in
+// a pass (about 4% of them in a 16-sample pass) needs no twist of its own.
 __device__ __forceinline__ uint32_t u4_at(const uint4& v, int i) {
     return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
 }
@@ -35,7 +37,7 @@ __global__ void __launch_bounds__(256) k_tape(TraceParams p) {
     const uint32_t s = pixel_sample_seed(p.base_seed, pix, n_before);
     uint32_t* t = p.tape + tape_base(a);
     const int n_chunks = tape_prefix_words(p.plan.counts[pix], p.tape_per_sample, p.tape_margin) / 4;
-    if (n_chunks < MT_N / 4) t[tape_word(MT_TAPE_SEED)] = s;   // for the trace to extend the prefix
+    const bool whole = n_chunks == MT_N / 4;
     uint32_t lo = s;               // x[k]
     uint32_t far = s;              // x[k + 397] while k + 397 < 624
     for (uint32_t k = 1; k <= (uint32_t)MT_M; ++k) far = mt_seed_step(far, k);
@@ -45,6 +47,12 @@ __global__ void __launch_bounds__(256) k_tape(TraceParams p) {
     // and m56 = chunk c-56 (word k = 4c+i reads word 4(c-57)+i+1); chunk c-54 is
     // requested at the end of iteration c, after the iteration that wrote it.
     uint4 m57 = make_uint4(0, 0, 0, 0), m56 = make_uint4(0, 0, 0, 0), m55 = make_uint4(0, 0, 0, 0);
+    // The next state's first MT_S2 words, for a whole tape: its word k mixes
+    // words k, k+1 and k+397 of this state, so chunk m of it is formed when
+    // this state's chunk m+100 is, from chunks m and m+1 (read back) and
+    // m+99 and m+100 (in registers).
+    constexpr int kS2From = MT_M / 4 + 1;                        // 100: chunk of word 397 + 3
+    uint4 prev = make_uint4(0, 0, 0, 0), s2a = make_uint4(0, 0, 0, 0);
     for (int c = 0; c < n_chunks; ++c) {
         uint32_t w4[4];
 #pragma unroll
@@ -65,11 +73,25 @@ __global__ void __launch_bounds__(256) k_tape(TraceParams p) {
             }
         }
         *reinterpret_cast<uint4*>(t + tape_word(4 * c)) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+        if (whole && c >= kS2From && c < kS2From + MT_S2 / 4) {
+            const int m = c - kS2From;
+            if (m == 0) s2a = *reinterpret_cast<const uint4*>(t + tape_word(0));
+            const uint4 s2b = *reinterpret_cast<const uint4*>(t + tape_word(4 * m + 4));
+            const uint32_t x2 = mt_mix(s2a.x, s2a.y, prev.y), y2 = mt_mix(s2a.y, s2a.z, prev.z);
+            const uint32_t z2 = mt_mix(s2a.z, s2a.w, prev.w), w2 = mt_mix(s2a.w, s2b.x, w4[0]);
+            *reinterpret_cast<uint4*>(t + tape_word(MT_S2_AT + 4 * m)) = make_uint4(x2, y2, z2, w2);
+            s2a = s2b;
+        }
+        prev = make_uint4(w4[0], w4[1], w4[2], w4[3]);
         m57 = m56;
         m56 = m55;
         const int ahead = c - 54;
         if (ahead >= 0 && ahead <= kLastBackChunk)
             m55 = *reinterpret_cast<const uint4*>(t + tape_word(4 * ahead));
+    }
+    if (!whole) {   // where the trace goes on if it draws past the prefix
+        t[tape_word(MT_TAPE_CURSOR)] = lo;
+        t[tape_word(MT_TAPE_CURSOR + 1)] = far;
     }
 }
 
@@ -425,7 +447,7 @@ __device__ __forceinline__ void wf_load(const TraceParams& p, const WfRef& W, Hi
     dr.tape = p.tape + W.u(WF_TAPE);
     dr.pos = int(W.u(WF_POS));
     dr.limit = int(W.u(WF_LIMIT));
-    if (dr.pos & 3) dr.cache = *reinterpret_cast<const uint4*>(dr.tape + tape_word(dr.pos & ~3));
+    if (dr.pos & 3) dr.cache = *reinterpret_cast<const uint4*>(dr.tape + tape_pos_word(dr.pos & ~3));
     T = W.f3(WF_T);
     L = W.f3(WF_L);
     if (!(q & Q_PRIMARY)) {

@@ -117,17 +117,26 @@ __device__ __forceinline__ float mt_to_unit(uint32_t u) {
     return r >= 1.0f ? 0x1.fffffep-1f : r;
 }
 // Tapes are stored in groups of 64: word w of tape a lives at
-//   (a / 64) * 628 * 64 + (w / 4) * 256 + (a % 64) * 4 + (w % 4)
+//   (a / 64) * 756 * 64 + (w / 4) * 256 + (a % 64) * 4 + (w % 4)
 // so 64 consecutive tapes write one contiguous KiB per 4 words when they are
 // generated together, and one tape still reads 4 consecutive words as a uint4.
-// Word 624 (MT_TAPE_SEED) holds the tape's seed, for extending a prefix.
+// Words 0-623: the state after the first twist (or its first `prefix` words).
+// Words 624-625 (MT_TAPE_CURSOR) of a prefix: the seeding recurrence's x[prefix]
+// and x[prefix + 397], from which the trace extends the prefix. Words 628-755
+// of a whole tape: the first MT_S2 words of the state after the second twist,
+// so that a pixel drawing a little past word 623 needs no twist of its own.
 constexpr int MT_GROUP = 64;
-constexpr int MT_TAPE_SEED = MT_N;
-constexpr int MT_TAPE_WORDS = MT_N + 4;
+constexpr int MT_TAPE_CURSOR = MT_N;
+constexpr int MT_S2 = 128;
+constexpr int MT_S2_AT = MT_N + 4;
+constexpr int MT_TAPE_WORDS = MT_S2_AT + MT_S2;
 __host__ __device__ __forceinline__ size_t tape_base(size_t a) {
     return (a / MT_GROUP) * size_t(MT_TAPE_WORDS * MT_GROUP) + (a % MT_GROUP) * 4;
 }
 __device__ __forceinline__ int tape_word(int w) { return (w >> 2) * (4 * MT_GROUP) + (w & 3); }
+// Tape word of draw position pos: 0-623 the current state, 624-751 the next
+// state's prefix (tape words 628-755).
+__device__ __forceinline__ int tape_pos_word(int pos) { return tape_word(pos < MT_N ? pos : pos + (MT_S2_AT - MT_N)); }
 
 __host__ __device__ __forceinline__ uint32_t mt_seed_step(uint32_t prev, uint32_t k) {
     return 1812433253u * (prev ^ (prev >> 30)) + k;
@@ -141,19 +150,19 @@ __host__ __device__ __forceinline__ int tape_prefix_words(int count, int per_sam
     const long long w = (long long)per_sample * count + margin;
     return w >= MT_N ? MT_N : int((w + 3) / 4 * 4);
 }
+// Draw positions readable from a tape with that prefix: a whole tape also
+// holds the next state's first MT_S2 words.
+__host__ __device__ __forceinline__ int tape_draw_limit(int prefix) { return prefix >= MT_N ? MT_N + MT_S2 : prefix; }
 
 // Writes words [from, 624) of the tape after its first twist, words [0, from)
-// and the seed (word MT_TAPE_SEED) being there already (from a multiple of 4): the seeded state
-// is walked by two cursors (x[k] and, while k < 227, x[k+397]) and words from
-// 227 on mix in the tape word 227 back. Rare (a pixel whose samples drew more
-// than its prefix), so written for size, not speed.
+// and the seeding cursors x[from], x[from + 397] (MT_TAPE_CURSOR) being there
+// already (from a multiple of 4): the seeded state is walked on from the two
+// cursors (x[k] and, while k < 227, x[k+397]) and words from 227 on mix in the
+// tape word 227 back. Rare (a pixel whose samples drew more than its prefix),
+// so written for size, not speed.
 __device__ __noinline__ void mt_tape_extend(uint32_t* __restrict__ s, int from);
 
-// Regenerates a tape in place (the standard twist over all 624 words) and
-// returns its first four words. Kept out of line: a draw site needs it once
-// per 624 draws, and inlining it at every draw site would multiply the trace
-// kernel's code size past the instruction cache.
-__device__ __noinline__ uint4 mt_twist_reload(uint32_t* __restrict__ s);
+// The standard twist over all 624 words, in place.
 __device__ inline void mt_twist(uint32_t* __restrict__ s) {
     for (int k = 0; k < MT_N - MT_M; ++k)
         s[tape_word(k)] = mt_mix(s[tape_word(k)], s[tape_word(k + 1)], s[tape_word(k + MT_M)]);
@@ -162,18 +171,10 @@ __device__ inline void mt_twist(uint32_t* __restrict__ s) {
     s[tape_word(MT_N - 1)] = mt_mix(s[tape_word(MT_N - 1)], s[tape_word(0)], s[tape_word(MT_M - 1)]);
 }
 
-__device__ __noinline__ uint4 mt_twist_reload(uint32_t* __restrict__ s) {
-    mt_twist(s);
-    return *reinterpret_cast<const uint4*>(s);
-}
-
 __device__ __noinline__ void mt_tape_extend(uint32_t* __restrict__ s, int from) {
     constexpr int kBack = MT_N - MT_M;  // 227
-    uint32_t lo = s[tape_word(MT_TAPE_SEED)];
-    for (int k = 1; k <= from; ++k) lo = mt_seed_step(lo, uint32_t(k));           // x[from]
-    uint32_t far = lo;
-    if (from < kBack)
-        for (int k = from + 1; k <= from + MT_M; ++k) far = mt_seed_step(far, uint32_t(k));  // x[from + 397]
+    uint32_t lo = s[tape_word(MT_TAPE_CURSOR)];        // x[from]
+    uint32_t far = s[tape_word(MT_TAPE_CURSOR + 1)];   // x[from + 397], used while k < 227
     for (int k = from; k < MT_N - 1; ++k) {
         const uint32_t x1 = mt_seed_step(lo, uint32_t(k + 1));
         const uint32_t back = k < kBack ? far : s[tape_word(k - kBack)];
@@ -184,24 +185,48 @@ __device__ __noinline__ void mt_tape_extend(uint32_t* __restrict__ s, int from) 
     s[tape_word(MT_N - 1)] = mt_mix(lo, s[tape_word(0)], s[tape_word(MT_M - 1)]);
 }
 
+// The rare refill of a draw cursor at the end of the draws its tape holds
+// (pos == limit, out of line):
+//  * pos < 624, past the prefix k_tape wrote: the lane writes the rest of the
+//    state, limit becomes 624;
+//  * pos == 624 of a tape without the next state's prefix (extended, or
+//    twisted here before): the tape is twisted in place and the draws go on
+//    from its word 0;
+//  * pos == 624 + MT_S2, past the next state's prefix: the tape (still the
+//    first state) is twisted in place and the draws go on from its word
+//    MT_S2; the prefix is gone, limit becomes 624.
+struct TapeRefill { uint4 cache; int pos; int limit; };
+__device__ __noinline__ TapeRefill mt_refill_slow(uint32_t* __restrict__ tape, int pos, int limit) {
+    TapeRefill r;
+    r.pos = pos;
+    r.limit = MT_N;
+    if (pos < MT_N) {
+        mt_tape_extend(tape, pos);
+        r.cache = *reinterpret_cast<const uint4*>(tape + tape_word(pos));
+    } else {
+        mt_twist(tape);
+        r.pos = pos - MT_N;
+        r.cache = *reinterpret_cast<const uint4*>(tape + tape_word(r.pos));
+    }
+    return r;
+}
+
 // Per-lane draw cursor over a tape with a 4-word register cache.
 struct Draws {
     uint32_t* tape;  // tape_base of this pixel's tape
-    int pos;         // next word index in [0, 624]
-    int limit;       // words of the current state written so far (a multiple of 4)
+    int pos;         // next draw position in [0, limit] (tape_pos_word)
+    int limit;       // draw positions the tape holds (a multiple of 4; tape_draw_limit)
     uint4 cache;     // words [pos & ~3, pos | 3]
-    __device__ __forceinline__ void reset(uint32_t* t, int prefix) { tape = t; pos = 0; limit = prefix; }
+    __device__ __forceinline__ void reset(uint32_t* t, int prefix) { tape = t; pos = 0; limit = tape_draw_limit(prefix); }
     __device__ __forceinline__ float next() {
         if ((pos & 3) == 0) {
-            if (pos == limit && limit < MT_N) {   // past the prefix: the lane writes the rest
-                mt_tape_extend(tape, limit);
-                limit = MT_N;
-            }
-            if (pos == MT_N) {
-                cache = mt_twist_reload(tape);
-                pos = 0;
+            if (pos >= limit) {   // pos == limit: extend the tape or twist it (rare)
+                const TapeRefill r = mt_refill_slow(tape, pos, limit);
+                cache = r.cache;
+                pos = r.pos;
+                limit = r.limit;
             } else {
-                cache = *reinterpret_cast<const uint4*>(tape + tape_word(pos));
+                cache = *reinterpret_cast<const uint4*>(tape + tape_pos_word(pos));
             }
         }
         const int k = pos & 3;

@@ -59,6 +59,22 @@ def test_trace_records_past_the_first_twist_match_oracle(engine_ctx):
     check_records(orec, grec, BUDGET, "32 spp:")
 
 
+def test_trace_records_past_the_next_state_prefix_match_oracle(engine_ctx):
+    """64 samples per pixel draw about 1,400 numbers per pixel: past word 623
+    into the next state's first 128 words that the tape kernel wrote, past
+    those (the tape is then twisted in place and the draws go on from word
+    128), and past the end of that state (twisted again, from word 0)."""
+    rc = ob.config_c0(12, 10)
+    rc.spp = 64
+    oj = _pair(rc)
+    oj.run_pass(np.full(12 * 10, rc.spp, np.int32), statmc=True)
+    gj = _gpu_job(engine_ctx, rc)
+    gj.run_pass(None, statmc=True)
+    orec, grec = oj.get("records"), gj.get("records")
+    assert len(orec) == 12 * 10 * 64
+    check_records(orec, grec, BUDGET, "64 spp:")
+
+
 @pytest.mark.parametrize("per_sample,margin", [(48, 64), (4, 8), (0, 0), (0, 624)])
 def test_trace_records_with_short_tape_prefixes_match_oracle(engine_ctx, per_sample, margin):
     """Tapes written only in part before the pass (48 words per sample and 64
