@@ -139,11 +139,21 @@ C2 = dict(width=1920, height=1080, spp=8, adaptive_spp=8, adaptive_passes=4, max
           focus_distance=3.4, fov=40.0, cam_pos=(0.0, 0.6, 3.4), cam_dir=(0.0, -0.17, -1.0), seed=1234)
 
 
+def new_context(args):
+    """The engine context with the trace and gather settings asked for on the command line."""
+    ctx = eng.Context(0)
+    if getattr(args, "trace_wavefront", None) is not None:
+        ctx.set_trace_wavefront(args.trace_wavefront)
+    if getattr(args, "gather_overlap", None) is not None:
+        ctx.set_gather_overlap(args.gather_overlap)
+    return ctx
+
+
 def run_c2(args):
     """The procedural scene (seed 7: 1,310,720-triangle displaced icosphere, 256 GGX
     spheres, 4 area lights, constant environment) at 1920x1080, 8 spp + 4 adaptive
     passes of 8 spp, max depth 8 with Russian roulette, thin lens: one job per step."""
-    ctx = eng.Context(0)
+    ctx = new_context(args)
     pc = eng.procedural_config()
     t0 = time.perf_counter()
     ctx.procedural_scene("c2", pc)
@@ -219,7 +229,7 @@ def run_c3(args):
     rng = np.random.default_rng(C3["seed"])
     sides = rng.integers(C3["side"][0], C3["side"][1] + 1, C3["images"])
     spps = rng.integers(C3["spp"][0], C3["spp"][1] + 1, C3["images"])
-    ctx = eng.Context(0)
+    ctx = new_context(args)
     cfg0 = eng.default_config(image_width=8, image_height=8, use_statmc=1)
     radius = cfg0.color_window_radius
     ctx.set_memory_budget(int(C3["budget_gb"] * 1e9))
@@ -439,7 +449,7 @@ def run_c4(args):
     Served jobs equal the plain ones bit for bit (a job's result does not
     depend on what else runs), so the images, and their relMSE, are the same;
     the checksums of every image are compared."""
-    ctx = eng.Context(0)
+    ctx = new_context(args)
     ctx.procedural_scene("c2", eng.procedural_config())
     reqs = c4_requests()
     # warm-up: one small job per scene (code objects, graphs, allocator, StatMC tile tuning)
@@ -479,6 +489,7 @@ def run_c4(args):
         return {"p50": st["latency_p50_ms"], "p99": st["latency_p99_ms"], "mean": st["latency_mean_ms"],
                 "max": st["latency_max_ms"], "queue_p50": st["queue_p50_ms"], "queue_p99": st["queue_p99_ms"],
                 "max_in_flight_seen": st["max_in_flight_seen"], "peak_gb": st["peak_bytes"] / 1e9,
+                "admitted_over_budget": st["over_budget"],
                 "failed": st["failed"]}
     result = {
         "metric": "Msamples/s & denoised MP/s; p50/p99 job latency vs plain at equal relMSE",
@@ -521,6 +532,9 @@ def main():
     ap.add_argument("--trace-wavefront", type=float, default=None,
                     help="wavefront trace rounds per sample for flat scenes (0: path-regeneration kernel only; "
                          "default: the engine's default)")
+    ap.add_argument("--gather-overlap", type=int, choices=[0, 1], default=None,
+                    help="grid gather of a pass beside the next pass's trace (1) or in line (0); "
+                         "default: the engine's default")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy (counter calibration) and exit")
     ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4"],
@@ -548,9 +562,7 @@ def main():
               "MI355X; sharding it over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
     w = h = args.size
-    ctx = eng.Context(0)
-    if args.trace_wavefront is not None:
-        ctx.set_trace_wavefront(args.trace_wavefront)
+    ctx = new_context(args)
     cfg, cam = job_config(w, h), job_camera()
     for _ in range(args.warmup):
         j = eng.Job(ctx, cfg, cam, WORKLOAD["seed"], submit=True)

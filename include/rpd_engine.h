@@ -114,8 +114,8 @@ const char* rpd_last_error(void);
 int rpd_ctx_set_memory_budget(rpd_context* ctx, uint64_t bytes);
 /* Extension (the reference allocates its buffers once per process): a job's
    device buffers live in one slab, sized in classes of eight per doubling.
-   Slabs of released jobs are kept, up to `bytes` (default 32 GiB; the oldest
-   are freed first), and a later job takes the smallest kept slab that holds
+   Slabs of released jobs are kept, up to `bytes` (default: a quarter of the
+   device's memory, at most 64 GiB; the oldest are freed first), and a later job takes the smallest kept slab that holds
    it and is at most half as large again; kept slabs do not count as held by
    jobs. A device allocation that fails empties the cache and is tried again.
    0 frees and disables it. `cached` (may be NULL) receives the bytes kept. */
@@ -396,8 +396,22 @@ int rpd_ctx_set_graph_passes(rpd_context* ctx, int on);
    (51 state words per pixel, up to 4M slots, plus 3 query words and the round
    counters), which rpd_job_bytes does not count. */
 int rpd_ctx_set_trace_wavefront(rpd_context* ctx, float rounds_per_sample);
+/* Extension (no counterpart in the reference, whose passes splat into the RPF
+   grid inside the sample loop, main.cpp:329-388 and 1451-1538): with on != 0
+   (the default), jobs created after the call over scenes traced without a BVH
+   (the flat triangle loop) whose adaptive passes gather into the grid (StatMC
+   on, adaptive passes, rpf_shrinkage_scale > 0) run each
+   pass's grid gather, the last excepted, on a second stream beside the next
+   pass's trace, the passes alternating between two plan and record sets; the
+   second set holds (adaptive_spp + 1) samples per pixel. Outputs are the same
+   bits as with every gather in line, which 0 selects (and graph passes
+   always use). */
+int rpd_ctx_set_gather_overlap(rpd_context* ctx, int on);
 /* Device bytes a job of this configuration needs (its slab size; admission by
-   HBM footprint). A job that takes a larger kept slab holds that slab's size. */
+   HBM footprint), the second plan and record set of the gather overlap
+   included wherever its configuration allows it (an upper bound for scenes
+   traced with a BVH). A job that takes a larger kept slab holds that slab's
+   size. */
 int rpd_job_bytes(const rpd_render_config* cfg, uint64_t* bytes);
 /* Bytes held by the context's jobs, its budget (0 = none) and the device's free and total memory. */
 int rpd_ctx_memory(rpd_context* ctx, uint64_t* in_use, uint64_t* budget, uint64_t* device_free,
@@ -427,6 +441,9 @@ typedef struct rpd_server_stats {
     double latency_p50_ms, latency_p99_ms, latency_mean_ms, latency_max_ms;  /* submit -> done */
     double queue_p50_ms, queue_p99_ms;                                        /* submit -> admitted */
     uint64_t tail_batches, tail_jobs;  /* cobatch_tail: denoise-tail batches run, and the jobs in them */
+    uint64_t over_budget;  /* requests admitted beyond the budget: nothing ran, nothing queued fitted
+                              (finished jobs not yet released held the memory), and a caller waited
+                              for the request, or the server was stopping */
 } rpd_server_stats;
 /* Also sets the context's slab cache (rpd_ctx_set_buffer_cache) to what the
    budget leaves of the device less 16 GiB, and at least 32 GiB, and its held

@@ -157,6 +157,9 @@ struct rpd_context {
     // per slot) shading/intersection rounds before the path-regeneration
     // kernel finishes them; 0 keeps the path-regeneration kernel alone
     float wf_rounds = 0.0f;
+    // jobs created next run a pass's grid gather beside the next pass's trace
+    // (second plan and record set, second stream)
+    bool gather_overlap = true;
     // streams of released jobs, for the next jobs: creating a stream costs
     // host time, and with more hardware queues than the default (the
     // GPU_MAX_HW_QUEUES environment variable) a new stream can also set up a
@@ -241,6 +244,22 @@ struct rpd_job {
     unsigned int* b1_queues = nullptr;
     bool denoiser = false;   // created for denoising records: no scene, camera or tape
     bool wf_on = false;      // the slab holds wavefront trace slots (the context had it on at creation)
+    // Gather overlap: the slab holds a second plan and record set, and the
+    // grid gather of a pass runs on `stream2` beside the next pass's trace.
+    // counts/offsets/active/totals, rec and batch1 are those of set cur_set.
+    bool ov_on = false;
+    int cur_set = 0;
+    int* plan_sets[2] = {nullptr, nullptr};  // counts, offsets, active, totals: 3N + 8 ints each
+    Records rec_sets[2]{};
+    Batch batch_sets[2]{};
+    ImageDesc* b2_img = nullptr;             // device copies of set 1's batch of one
+    int64_t* b2_prefix = nullptr;
+    unsigned int* b2_queues = nullptr;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_go = nullptr;              // stream: the next pass's trace may start
+    hipEvent_t ev_gathered[2] = {nullptr, nullptr};  // stream2: the gather of set k's records ended
+    bool gathered_pending[2] = {false, false};       // ev_gathered[k] recorded and not yet waited on
+    int gather_waiting = -1;                 // set whose records wait for their gather, or -1
     bool capturing = false;  // the stream is being captured into a graph: no per-stage timing events
     int max_spp = 0;         // record capacity per pixel of a denoiser job
     // this job as an image of a batch (device copies for the batch of one)
@@ -268,14 +287,15 @@ namespace {
 struct StageTimer {
     rpd_job* j;
     int stage;
+    hipStream_t on;          // the stream the stage's work is queued on
     hipEvent_t a = nullptr;
-    StageTimer(rpd_job* job, int s) : j(job), stage(s) {
-        if (!j->capturing && event_create(&a) == hipSuccess) hipEventRecord(a, j->stream);
+    StageTimer(rpd_job* job, int s, hipStream_t stream = nullptr) : j(job), stage(s), on(stream ? stream : job->stream) {
+        if (!j->capturing && event_create(&a) == hipSuccess) hipEventRecord(a, on);
     }
     ~StageTimer() {
         hipEvent_t b = nullptr;
         if (a && !j->capturing && event_create(&b) == hipSuccess) {
-            hipEventRecord(b, j->stream);
+            hipEventRecord(b, on);
             j->spans.push_back({stage, a, b});
         }
     }
@@ -418,12 +438,90 @@ int trace_blocks(rpd_job* j) {
 // A render plan's active list in the order the trace hands pixels out in.
 void order_plan(rpd_job* j) { launch_plan_tile_order(j->W, j->H, j->counts, j->active, j->scan_scratch, j->stream); }
 
-// Sampling pass with the plan already in counts/offsets/active/totals.
-int pass_after_plan(rpd_job* j, int statmc, int stage) {
+// ---- gather overlap -----------------------------------------------------------
+// A job whose passes gather into the RPF grid runs the gather of pass k on a
+// second stream, started once pass k+1's tape kernel has run, so it shares the
+// device with pass k+1's trace. The trace holds every compute unit while
+// pixels are left to hand out (its waves take all but 32 of each SIMD's 512
+// VGPRs); when the active list runs out, its waves end one by one while the
+// lanes still holding a pixel finish their samples in sequence, and the
+// gather's blocks take the units they leave. Passes alternate between two plan
+// and record sets, so pass k+1 plans and traces into the set pass k's gather
+// is not reading; gathers keep their pass order on the second stream, and the
+// last pass gathers on the job's stream after all of them, so the grid's sums
+// are the same as with every gather in line.
+
+// Makes plan and record set s current (counts, offsets, active, totals, rec,
+// batch1); the job's stream first waits for a gather still reading set s.
+int use_set(rpd_job* j, int s) {
+    if (j->gathered_pending[s]) {
+        HIP_TRY(hipStreamWaitEvent(j->stream, j->ev_gathered[s], 0));
+        j->gathered_pending[s] = false;
+    }
+    const size_t N = j->N;
+    int* q = j->plan_sets[s];
+    j->counts = q; j->offsets = q + N; j->active = q + 2 * N; j->totals = q + 3 * N;
+    j->rec = j->rec_sets[s];
+    j->batch1 = j->batch_sets[s];
+    j->cur_set = s;
+    return RPD_OK;
+}
+
+// The job's stream waits for every gather queued on the second stream.
+int wait_gathers(rpd_job* j) {
+    for (int s = 0; s < 2; ++s)
+        if (j->gathered_pending[s]) {
+            HIP_TRY(hipStreamWaitEvent(j->stream, j->ev_gathered[s], 0));
+            j->gathered_pending[s] = false;
+        }
+    return RPD_OK;
+}
+
+int sync_event_create(hipEvent_t* e) {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    return RPD_OK;
+}
+
+int ensure_overlap_stream(rpd_job* j) {
+    if (!j->stream2) HIP_TRY(job_stream_take(j->ctx, &j->stream2));
+    int st = RPD_OK;
+    if (!j->ev_go && (st = sync_event_create(&j->ev_go))) return st;
+    for (auto& e : j->ev_gathered)
+        if (!e && (st = sync_event_create(&e))) return st;
+    return RPD_OK;
+}
+
+// Launches the gather of the set that waits for it on the second stream,
+// after the work queued so far on the job's stream (the next pass's plan and
+// tape kernel).
+int launch_waiting_gather(rpd_job* j) {
+    const int k = j->gather_waiting;
+    j->gather_waiting = -1;
+    HIP_TRY(hipEventRecord(j->ev_go, j->stream));
+    HIP_TRY(hipStreamWaitEvent(j->stream2, j->ev_go, 0));
+    {
+        StageTimer tg(j, 10, j->stream2);
+        launch_batch_queues_reset(j->batch_sets[k], j->stream2);
+        launch_grid_gather(j->batch_sets[k], j->stream2);
+    }
+    HIP_TRY(hipEventRecord(j->ev_gathered[k], j->stream2));
+    j->gathered_pending[k] = true;
+    HIP_TRY(hipGetLastError());
+    return RPD_OK;
+}
+
+// Sampling pass with the plan already in counts/offsets/active/totals. With
+// defer_gather the pass's grid gather waits for the next pass (gather overlap).
+int pass_after_plan(rpd_job* j, int statmc, int stage, bool defer_gather = false) {
     StageTimer tm(j, stage);
     const TraceParams p = trace_params(j, statmc);
     launch_fill_i32(j->work, 0, 1, j->stream);
     launch_tape(p, j->stream);
+    if (j->gather_waiting >= 0) {
+        const int st = launch_waiting_gather(j);
+        if (st) return st;
+    }
     {
         StageTimer tk(j, 7);
         const int rounds = wavefront_rounds(j, stage);
@@ -431,7 +529,11 @@ int pass_after_plan(rpd_job* j, int statmc, int stage) {
         else launch_trace(p, trace_blocks(j), j->stream);
     }
     launch_fold(j->batch1, statmc, j->stream);
-    if (p.capture) {
+    if (p.capture && defer_gather) {
+        j->gather_waiting = j->cur_set;
+    } else if (p.capture) {
+        const int st = wait_gathers(j);   // the grid's sums follow the pass order
+        if (st) return st;
         StageTimer tg(j, 10);
         launch_batch_queues_reset(j->batch1, j->stream);
         launch_grid_gather(j->batch1, j->stream);
@@ -599,6 +701,10 @@ int do_finalize(rpd_job* j, bool statmc) { return finalize_on(j, statmc, j->stre
 
 int reset_state(rpd_job* j) {
     const size_t N = j->N;
+    int st = wait_gathers(j);
+    if (!st) st = use_set(j, 0);
+    if (st) return st;
+    j->gather_waiting = -1;
     // zero every accumulator (buffers.init, pixel_stats.assign, rpf_grid.reset)
     HIP_TRY(hipMemsetAsync(j->pb.radiance, 0, sizeof(float) * 13 * N, j->stream));
     HIP_TRY(hipMemsetAsync(j->pb.hit_count, 0, sizeof(int) * 2 * N, j->stream));
@@ -682,16 +788,19 @@ int run_passes(rpd_job* j) {
     }
     launch_plan_uniform(j->N, j->cfg.samples_per_pixel, j->counts, j->offsets, j->active, j->totals, j->stream);
     order_plan(j);
-    if ((st = pass_after_plan(j, 1, 0))) return st;
     const bool passes = j->cfg.adaptive_passes > 0 && j->cfg.adaptive_spp > 0;
+    const bool overlap = passes && j->ov_on && !j->ctx->graph_passes;
+    if (overlap && (st = ensure_overlap_stream(j))) return st;
+    if ((st = pass_after_plan(j, 1, 0, overlap))) return st;
     if (!passes && (st = do_recompute_sensitivity(j))) return st;
     if (passes && j->ctx->graph_passes) {
         if ((st = run_adaptive_passes_graph(j, j->cfg.adaptive_passes))) return st;
     } else {
         for (int i = 0; passes && i < j->cfg.adaptive_passes; ++i) {
             if ((st = do_variance(j))) return st;
+            if (overlap && (st = use_set(j, 1 - j->cur_set))) return st;
             if ((st = do_allocation(j))) return st;
-            if ((st = pass_after_plan(j, 1, 1))) return st;
+            if ((st = pass_after_plan(j, 1, 1, overlap && i + 1 < j->cfg.adaptive_passes))) return st;
         }
     }
     j->tail_pending = true;
@@ -870,6 +979,21 @@ void job_layout(rpd_job* j, SlabLayout& L) {
     j->rec.capacity = cap;
     A(j->rec.r0, cap); A(j->rec.r1, cap); A(j->rec.r2, cap); A(j->rec.r3, cap);
     A(j->rec.r4, cap); A(j->rec.r5, cap); A(j->rec.r6, cap);
+    j->plan_sets[0] = plan;
+    j->rec_sets[0] = j->rec;
+    if (j->ov_on) {
+        // the second set holds adaptive passes only (at most adaptive_spp * N samples)
+        int* plan2 = nullptr; A(plan2, 3 * N + 8);
+        j->plan_sets[1] = plan2;
+        Records& r = j->rec_sets[1];
+        const int64_t cap2 = int64_t(j->cfg.adaptive_spp + 1) * int64_t(N);
+        r.capacity = cap2;
+        A(r.r0, cap2); A(r.r1, cap2); A(r.r2, cap2); A(r.r3, cap2);
+        A(r.r4, cap2); A(r.r5, cap2); A(r.r6, cap2);
+        A(j->b2_img, 1);
+        A(j->b2_prefix, BK_KINDS * 2);
+        A(j->b2_queues, BK_KINDS * BATCH_QUEUES);
+    }
     A(j->grid, nn * GRID_NODE_DOUBLES);
     A(j->tiles, nn * SF_COUNT);
     A(j->node_scratch, nn * 33);
@@ -919,7 +1043,27 @@ int job_alloc(rpd_job* j) {
     L.base = static_cast<char*>(q);
     L.off = 0;
     job_layout(j, L);
-    return build_batch1(j, j->b1_img, j->b1_prefix, j->b1_queues);
+    const int st = build_batch1(j, j->b1_img, j->b1_prefix, j->b1_queues);
+    if (st) return st;
+    j->batch_sets[0] = j->batch1;
+    j->cur_set = 0;
+    if (j->ov_on) {
+        // set 1's batch of one: the same image with the second plan and records
+        ImageDesc d = j->desc;
+        const size_t N = j->N;
+        int* p2 = j->plan_sets[1];
+        d.plan.counts = p2; d.plan.offsets = p2 + N; d.plan.active = p2 + 2 * N; d.plan.totals = p2 + 3 * N;
+        d.rec = j->rec_sets[1];
+        int64_t pf[BK_KINDS * 2];
+        for (int k = 0; k < BK_KINDS; ++k) { pf[2 * k] = 0; pf[2 * k + 1] = j->batch1.total[k]; }
+        HIP_TRY(sync_memcpy(j->b2_img, &d, sizeof(ImageDesc), hipMemcpyHostToDevice));
+        HIP_TRY(sync_memcpy(j->b2_prefix, pf, sizeof(pf), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemset(j->b2_queues, 0, sizeof(unsigned int) * BK_KINDS * BATCH_QUEUES));
+        Batch b = j->batch1;
+        b.img = j->b2_img; b.prefix = j->b2_prefix; b.queues = j->b2_queues;
+        j->batch_sets[1] = b;
+    }
+    return RPD_OK;
 }
 
 void set_geometry(rpd_job* j) {
@@ -929,16 +1073,23 @@ void set_geometry(rpd_job* j) {
     j->geom.ny = std::max(2, (j->H + j->geom.step - 1) / j->geom.step + 1);
 }
 
+// Jobs whose adaptive passes gather into the RPF grid (StatMC on, adaptive
+// passes, records captured) can overlap those gathers with the next trace.
+bool gather_overlap_applies(const rpd_render_config& c) {
+    return c.use_statmc != 0 && c.adaptive_passes > 0 && c.adaptive_spp > 0 && c.rpf_shrinkage_scale > 0.0f;
+}
+
 // Device bytes of a job (its slab, without debug planes); a denoiser job
 // (max_spp > 0) has no tape and room for max_spp records per pixel; wf: with
 // wavefront trace slots.
-size_t job_bytes(const rpd_render_config& c, int max_spp = 0, bool wf = false) {
+size_t job_bytes(const rpd_render_config& c, int max_spp = 0, bool wf = false, bool ov = false) {
     rpd_job t;
     t.cfg = c;
     t.W = c.image_width; t.H = c.image_height; t.N = t.W * t.H;
     t.denoiser = max_spp > 0;
     t.max_spp = max_spp;
     t.wf_on = wf;
+    t.ov_on = ov && max_spp <= 0;
     set_geometry(&t);
     SlabLayout L;
     job_layout(&t, L);
@@ -947,6 +1098,7 @@ size_t job_bytes(const rpd_render_config& c, int max_spp = 0, bool wf = false) {
 
 void job_free(rpd_job* j) {
     if (j->worker.joinable()) j->worker.join();
+    if (j->stream2) hipStreamSynchronize(j->stream2);
     if (j->stream) hipStreamSynchronize(j->stream);
     for (auto& s : j->spans) { event_destroy(s.a); event_destroy(s.b); }
     j->spans.clear();
@@ -963,6 +1115,12 @@ void job_free(rpd_job* j) {
     j->bytes = 0;
     if (j->stream) job_stream_give(j->ctx, j->stream);
     j->stream = nullptr;
+    if (j->stream2) job_stream_give(j->ctx, j->stream2);
+    j->stream2 = nullptr;
+    for (hipEvent_t* e : {&j->ev_go, &j->ev_gathered[0], &j->ev_gathered[1]}) {
+        if (*e) event_destroy(*e);
+        *e = nullptr;
+    }
 }
 
 int job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam, uint32_t seed,
@@ -979,7 +1137,11 @@ int job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg,
     if ((st = make_camera(*cam, float(j->W) / float(j->H), j->cam))) return st;
     set_geometry(j.get());
     j->wf_on = ctx->wf_rounds > 0.0f && !scene->bvh.nodes;
-    if (ctx->budget && ctx->in_use.load() + job_bytes(*cfg, 0, j->wf_on) > ctx->budget)
+    // BVH scenes keep their gathers in line: beside the BVH walk (168 VGPRs,
+    // 48 KB of LDS stack per block) the gather slowed the trace by more than
+    // it hid (C2 job 782 -> 789 ms)
+    j->ov_on = ctx->gather_overlap && gather_overlap_applies(*cfg) && !scene->bvh.nodes;
+    if (ctx->budget && ctx->in_use.load() + job_bytes(*cfg, 0, j->wf_on, j->ov_on) > ctx->budget)
         return fail(RPD_ERR_OUT_OF_MEMORY, "job does not fit the context's memory budget");
     HIP_TRY(job_stream_take(ctx, &j->stream));
     if ((st = job_alloc(j.get()))) { job_free(j.get()); return st; }
@@ -1157,6 +1319,10 @@ int rpd_ctx_create(int device, rpd_context** out) {
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     c->num_cus = prop.multiProcessorCount;
+    // keep released slabs up to a quarter of the device, at most 64 GiB: the
+    // largest jobs (2048^2 at 32 spp) take about 30 GB each, and a cap below two
+    // of them frees and re-allocates slabs between jobs of mixed sizes
+    c->cache_cap = std::min<uint64_t>(uint64_t(64) << 30, uint64_t(prop.totalGlobalMem) / 4);
     // set-up of the allocation-free metrics call: this device's scratch array
     // and the stream it runs on, so that the call itself creates nothing
     HIP_TRY(hipGetSymbolAddress(reinterpret_cast<void**>(&c->scratch), HIP_SYMBOL(scratch)));
@@ -1556,6 +1722,7 @@ int rpd_job_sync(rpd_job* j) {
 int rpd_job_run_pass(rpd_job* j, const int* counts, int statmc) {
     if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
     if (!j->scene) return fail(RPD_ERR_STATE, "job has no scene (it was created for denoising records)");
+    if (int st0 = use_set(j, 0)) return st0;
     if (!counts) {
         launch_plan_uniform(j->N, j->cfg.samples_per_pixel, j->counts, j->offsets, j->active, j->totals, j->stream);
     } else {
@@ -1573,7 +1740,8 @@ int rpd_job_run_pass(rpd_job* j, const int* counts, int statmc) {
 int rpd_job_run_adaptive_pass(rpd_job* j) {
     if (!j) return fail(RPD_ERR_INVALID_ARG, "null job");
     if (!j->scene) return fail(RPD_ERR_STATE, "job has no scene (it was created for denoising records)");
-    int st = do_allocation(j);
+    int st = use_set(j, 0);
+    if (!st) st = do_allocation(j);
     if (!st) st = pass_after_plan(j, 1, 1);
     if (st) return st;
     return rpd_job_sync(j);
@@ -1911,6 +2079,12 @@ int rpd_ctx_set_trace_wavefront(rpd_context* ctx, float rounds_per_sample) {
     return RPD_OK;
 }
 
+int rpd_ctx_set_gather_overlap(rpd_context* ctx, int on) {
+    if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
+    ctx->gather_overlap = on != 0;
+    return RPD_OK;
+}
+
 int rpd_ctx_set_graph_passes(rpd_context* ctx, int on) {
     if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
     ctx->graph_passes = on != 0;
@@ -1933,7 +2107,7 @@ int rpd_job_bytes(const rpd_render_config* cfg, uint64_t* bytes) {
     if (!cfg || !bytes) return fail(RPD_ERR_INVALID_ARG, "null argument");
     const int st = validate(*cfg);
     if (st) return st;
-    *bytes = job_bytes(*cfg);
+    *bytes = job_bytes(*cfg, 0, false, gather_overlap_applies(*cfg));
     return RPD_OK;
 }
 

@@ -80,6 +80,7 @@ struct rpd_server {
     std::thread tail_thr;
     bool stop_tail = false;
     uint64_t next_id = 1, peak = 0;
+    uint64_t over_budget = 0;  // requests admitted beyond the budget (see the scheduler)
     uint64_t submitted = 0, completed = 0, failed = 0, tail_batches = 0, tail_jobs = 0;
     int max_seen = 0;
     std::vector<double> lat, qwait;
@@ -175,6 +176,7 @@ struct rpd_server {
                 if (it != queue.end()) {
                     Request* r = *it;
                     queue.erase(it);
+                    if (held() + r->bytes > budget) ++over_budget;
                     start(r, lk);
                 }
             }
@@ -296,6 +298,7 @@ int rpd_server_stats_get(rpd_server* s, rpd_server_stats* out) {
     out->failed = s->failed;
     out->max_in_flight_seen = s->max_seen;
     out->peak_bytes = s->peak;
+    out->over_budget = s->over_budget;
     out->latency_p50_ms = percentile(s->lat, 0.50);
     out->latency_p99_ms = percentile(s->lat, 0.99);
     double sum = 0.0, mx = 0.0;

@@ -96,7 +96,7 @@ class ServerStats(C.Structure):
                 ("max_in_flight_seen", C.c_int), ("peak_bytes", C.c_uint64),
                 ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double), ("latency_mean_ms", C.c_double),
                 ("latency_max_ms", C.c_double), ("queue_p50_ms", C.c_double), ("queue_p99_ms", C.c_double),
-                ("tail_batches", C.c_uint64), ("tail_jobs", C.c_uint64)]
+                ("tail_batches", C.c_uint64), ("tail_jobs", C.c_uint64), ("over_budget", C.c_uint64)]
 
 
 class ImageMetrics(C.Structure):
@@ -220,6 +220,7 @@ def lib():
             ("rpd_render_finish_batch", [vp, vp, C.c_int, C.POINTER(C.c_double)]),
             ("rpd_ctx_set_graph_passes", [vp, C.c_int]),
             ("rpd_ctx_set_trace_wavefront", [vp, C.c_float]),
+            ("rpd_ctx_set_gather_overlap", [vp, C.c_int]),
             ("rpd_ctx_set_debug_outputs", [vp, C.c_int]),
             ("rpd_statmc_tile_shapes", [vp, C.c_int, C.POINTER(C.c_int)]),
             ("rpd_ctx_set_statmc_tile", [vp, C.c_int, C.c_int]),
@@ -322,6 +323,11 @@ class Context:
 
     def set_graph_passes(self, on):
         check(self.L.rpd_ctx_set_graph_passes(self.h, int(on)))
+
+    def set_gather_overlap(self, on):
+        """Jobs created next gather a pass into the RPF grid beside the next
+        pass's trace (on, the default) or in line (off); same outputs."""
+        check(self.L.rpd_ctx_set_gather_overlap(self.h, int(on)))
 
     def set_trace_wavefront(self, rounds_per_sample):
         """Wavefront trace for flat scenes: rounds per sample of a pass before the path-regeneration

@@ -415,7 +415,12 @@ hipError_t hipStreamSynchronize(hipStream_t s) {
 }
 hipError_t hipDeviceSynchronize() { return hipSuccess; }
 hipError_t hipEventCreate(hipEvent_t* e) { EMU_NO_CAPTURE(); *e = new ihipEvent_t(); (*e)->t = std::chrono::steady_clock::now(); return hipSuccess; }
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
 hipError_t hipEventDestroy(hipEvent_t e) { EMU_NO_CAPTURE(); delete e; return hipSuccess; }
+// Every launch runs when it is called, so work queued before the event is done.
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t, unsigned) {
+    return (s && s->capturing) ? hipErrorStreamCaptureUnsupported : hipSuccess;
+}
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) {
     if (s && s->capturing) return hipErrorStreamCaptureUnsupported;
     e->t = std::chrono::steady_clock::now();
