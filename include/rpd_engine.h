@@ -367,7 +367,8 @@ int rpd_scene_bvh_builder(const rpd_scene* scene, int* builder);
    thread per pixel over tiles of one of the shapes rpd_statmc_tile_shapes
    lists ((w, h) pairs; *count receives how many there are). By default a
    context tunes: the first StatMC launch of each (window radius, image-size
-   class) times every shape on its own images and keeps the fastest.
+   class) times every shape twice on its own images (in order, then in
+   reverse) and keeps the one with the fastest run.
    rpd_ctx_set_statmc_tile fixes one shape instead; 0, 0 returns to tuning.
    Every shape gives the same bits. rpd_ctx_statmc_tile reports the shape used
    for a radius and pixel count (0, 0: not tuned yet). */

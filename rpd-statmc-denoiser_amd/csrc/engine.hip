@@ -586,24 +586,35 @@ int run_statmc_window(rpd_context* ctx, Batch& b, hipStream_t st, bool can_time,
         if (it != ctx->tuned.end()) shape = it->second;
     }
     if (shape < 0 && can_time) {
-        hipEvent_t ev[STATMC_SHAPE_COUNT + 1];
+        // every shape twice, first in order and then in reverse (the first
+        // launch finds colder caches), each timed by the faster of its two runs
+        constexpr int kRuns = 2 * STATMC_SHAPE_COUNT;
+        auto shape_of_run = [](int r) { return r < STATMC_SHAPE_COUNT ? r : kRuns - 1 - r; };
+        hipEvent_t ev[kRuns + 1];
         for (auto& e : ev) HIP_TRY(event_create(&e));
         int st2 = RPD_OK;
         HIP_TRY(hipEventRecord(ev[0], st));
-        for (int k = 0; k < STATMC_SHAPE_COUNT && !st2; ++k) {
-            st2 = set_shape(b, k);
+        for (int r = 0; r < kRuns && !st2; ++r) {
+            st2 = set_shape(b, shape_of_run(r));
             launch_batch_queues_reset(b, st);
             launch_statmc_window(b, st);
-            if (!st2 && hipEventRecord(ev[k + 1], st) != hipSuccess) st2 = fail(RPD_ERR_DEVICE, "hipEventRecord failed");
+            if (!st2 && hipEventRecord(ev[r + 1], st) != hipSuccess) st2 = fail(RPD_ERR_DEVICE, "hipEventRecord failed");
         }
         int best = 0;
         float best_ms = 0.0f;
-        if (!st2 && hipEventSynchronize(ev[STATMC_SHAPE_COUNT]) == hipSuccess) {
-            for (int k = 0; k < STATMC_SHAPE_COUNT; ++k) {
+        if (!st2 && hipEventSynchronize(ev[kRuns]) == hipSuccess) {
+            float shape_ms[STATMC_SHAPE_COUNT];
+            bool timed[STATMC_SHAPE_COUNT] = {};
+            for (int r = 0; r < kRuns; ++r) {
                 float ms = 0.0f;
-                if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) continue;
-                if (k == 0 || ms < best_ms) { best = k; best_ms = ms; }
+                if (hipEventElapsedTime(&ms, ev[r], ev[r + 1]) != hipSuccess) continue;
+                const int k = shape_of_run(r);
+                if (!timed[k] || ms < shape_ms[k]) shape_ms[k] = ms;
+                timed[k] = true;
             }
+            bool any = false;
+            for (int k = 0; k < STATMC_SHAPE_COUNT; ++k)
+                if (timed[k] && (!any || shape_ms[k] < best_ms)) { best = k; best_ms = shape_ms[k]; any = true; }
             std::lock_guard<std::mutex> lk(ctx->tune_mu);
             ctx->tuned.emplace(key, best);
         }

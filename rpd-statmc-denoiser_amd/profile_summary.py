@@ -45,9 +45,9 @@ def main():
         f.write(f"Command: `{CMD}`, step 1 (`--kernel-trace --stats`).\n\n")
         shapes = [short(r["Name"]) for r in stats if short(r["Name"]).startswith("rpd::k_statmc<")]
         if len(shapes) > 1:
-            f.write(f"The run is the process's first job, so the StatMC window kernel runs once in each of its "
-                    f"{len(shapes)} tile shapes (the context times them and keeps the fastest for this radius and "
-                    f"image size); a later job launches only the fastest.\n\n")
+            f.write(f"The run is the process's first job, so the StatMC window kernel runs twice in each of its "
+                    f"{len(shapes)} tile shapes (the context times them, in order and then in reverse, and keeps "
+                    f"the fastest for this radius and image size); a later job launches only the fastest.\n\n")
         f.write("| kernel | calls | total ms | avg us | % |\n|---|---|---|---|---|\n")
         for r in stats[:20]:
             f.write(f"| {short(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
