@@ -436,7 +436,10 @@ int trace_blocks(rpd_job* j) {
 }
 
 // A render plan's active list in the order the trace hands pixels out in.
-void order_plan(rpd_job* j) { launch_plan_tile_order(j->W, j->H, j->counts, j->active, j->scan_scratch, j->stream); }
+// long_from > 0: pixels of at least that many samples are handed out first.
+void order_plan(rpd_job* j, int long_from = 0) {
+    launch_plan_tile_order(j->W, j->H, j->counts, j->active, j->scan_scratch, long_from, j->stream);
+}
 
 // ---- gather overlap -----------------------------------------------------------
 // A job whose passes gather into the RPF grid runs the gather of pass k on a
@@ -684,7 +687,9 @@ int do_allocation(rpd_job* j) {
     launch_importance(j->W, j->H, ac, j->pb, j->feat, j->vden, j->imp, j->alloc_scratch, j->stream);
     launch_alloc_counts(j->N, ac, j->imp, j->counts, j->alloc_scratch, j->stream);
     launch_plan_scan(j->N, j->counts, j->offsets, j->active, j->totals, j->scan_scratch, j->rec.capacity, j->stream);
-    order_plan(j);
+    // an adaptive pass gives some pixels up to eight times the mean: those
+    // with twice the mean or more go first
+    order_plan(j, 2 * std::max(1, j->cfg.adaptive_spp));
     HIP_TRY(hipGetLastError());
     return RPD_OK;
 }

@@ -131,6 +131,19 @@ constexpr int TRACE_CHUNK = 64;
 __device__ __forceinline__ uint64_t ballot64(bool b) { return __ballot(b); }
 __device__ __forceinline__ bool is_zero3(v3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
 
+// Issue priority of a wave on its SIMD, from the most samples one of its lanes
+// still has to trace. A pixel's samples run one after another on one lane, so
+// once the active list is used up the pass ends with the wave that holds the
+// longest remaining chain; with a higher priority its instructions issue ahead
+// of those of waves with shorter chains, which fill the cycles it leaves. The
+// tests are ballots (wave-uniform), so exactly one s_setprio runs.
+__device__ __forceinline__ void wave_priority(int left) {
+    if (__ballot(left >= 24)) __builtin_amdgcn_s_setprio(3);
+    else if (__ballot(left >= 16)) __builtin_amdgcn_s_setprio(2);
+    else if (__ballot(left >= 8)) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 // Scenes of up to TRACE_LDS_TRIS triangles, TRACE_FLAT_SPHERES spheres and
 // TRACE_LDS_MATERIALS materials and lights each are traced by the flat loop,
 // with their triangle tables (the permuted geometry and the shading
@@ -289,6 +302,7 @@ __device__ __forceinline__ bool path_step(const TraceParams& p, const DevScene& 
     }
     if (st == ST_NEED) st = ST_DEAD;
     if (ballot64(st != ST_DEAD) == 0) return false;
+    wave_priority(left);
 
     // -- camera rays: main.cpp:691-705 ----------------------------------
     if (st == ST_CAMERA) {
@@ -966,6 +980,13 @@ int64_t plan_scan_scratch_elems(int n) { return (n + SCAN_BLOCK - 1) / SCAN_BLOC
 // across materials). This rewrites the active list of a plan in tile order
 // (8x8 tiles row by row, pixels row by row within a tile): the same pixels,
 // counts, offsets and records, handed out in another order.
+//
+// With long_from > 0 the pixels of at least long_from samples go first (in tile
+// order), then the others. A pixel's samples are one chain on one lane and an
+// adaptive pass gives some pixels up to eight times the mean, so those chains
+// start at once; the trace's wave priority then runs them ahead of the rest.
+// Both classes are counted in one scan: class 0 in the low 32 bits of the
+// int64 prefix, class 1 in the high 32 bits (each count stays below 2^26).
 constexpr int PLAN_TILE = 8;
 __host__ __device__ inline int64_t plan_tile_positions(int W, int H) {
     return int64_t((W + PLAN_TILE - 1) / PLAN_TILE) * int64_t((H + PLAN_TILE - 1) / PLAN_TILE) * PLAN_TILE *
@@ -978,15 +999,20 @@ __device__ __forceinline__ int tile_pixel(int64_t k, int W, int H) {
     const int x = int(t % tiles_x) * PLAN_TILE + w % PLAN_TILE, y = int(t / tiles_x) * PLAN_TILE + w / PLAN_TILE;
     return (x < W && y < H) ? y * W + x : -1;
 }
-__global__ void __launch_bounds__(SCAN_BLOCK) k_tord_blocks(int W, int H, int64_t K, const int* counts,
+__device__ __forceinline__ int64_t tord_class_bit(int c, int long_from) {
+    if (c <= 0) return 0;
+    return (long_from > 0 && c >= long_from) ? int64_t(1) : (int64_t(1) << 32);
+}
+__global__ void __launch_bounds__(SCAN_BLOCK) k_tord_blocks(int W, int H, int64_t K, const int* counts, int long_from,
                                                             int64_t* block_sums) {
     __shared__ int64_t sh[SCAN_BLOCK];
     const int64_t k = int64_t(blockIdx.x) * SCAN_BLOCK + threadIdx.x;
     const int p = k < K ? tile_pixel(k, W, H) : -1;
-    const int64_t v = (p >= 0 && counts[p] > 0) ? 1 : 0;
+    const int64_t v = p >= 0 ? tord_class_bit(counts[p], long_from) : 0;
     const int64_t s = block_incl_scan(v, sh);
     if (threadIdx.x == SCAN_BLOCK - 1) block_sums[blockIdx.x] = s;
 }
+// Exclusive prefix of the block sums in place; block_sums[nb] gets the total.
 __global__ void __launch_bounds__(SCAN_BLOCK) k_tord_totals(int nb, int64_t* block_sums) {
     __shared__ int64_t sh[SCAN_BLOCK];
     int64_t carry = 0;
@@ -999,24 +1025,27 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_tord_totals(int nb, int64_t* blo
         carry += sh[SCAN_BLOCK - 1];
         __syncthreads();
     }
+    if (threadIdx.x == 0) block_sums[nb] = carry;
 }
-__global__ void __launch_bounds__(SCAN_BLOCK) k_tord_apply(int W, int H, int64_t K, const int* counts,
-                                                           const int64_t* block_sums, int* active) {
+__global__ void __launch_bounds__(SCAN_BLOCK) k_tord_apply(int W, int H, int64_t K, const int* counts, int long_from,
+                                                           int nb, const int64_t* block_sums, int* active) {
     __shared__ int64_t sh[SCAN_BLOCK];
     const int64_t k = int64_t(blockIdx.x) * SCAN_BLOCK + threadIdx.x;
     const int p = k < K ? tile_pixel(k, W, H) : -1;
-    const bool on = p >= 0 && counts[p] > 0;
-    const int64_t v = on ? 1 : 0;
+    const int64_t v = p >= 0 ? tord_class_bit(counts[p], long_from) : 0;
     const int64_t s = block_incl_scan(v, sh) - v + block_sums[blockIdx.x];
-    if (on) active[s] = p;
+    if (v == 1) active[s & 0xffffffffll] = p;
+    else if (v != 0) active[(block_sums[nb] & 0xffffffffll) + (s >> 32)] = p;
 }
 int64_t plan_tile_scratch_elems(int W, int H) { return (plan_tile_positions(W, H) + SCAN_BLOCK - 1) / SCAN_BLOCK + 1; }
-void launch_plan_tile_order(int W, int H, const int* counts, int* active, int64_t* scratch, hipStream_t st) {
+void launch_plan_tile_order(int W, int H, const int* counts, int* active, int64_t* scratch, int long_from,
+                            hipStream_t st) {
     const int64_t K = plan_tile_positions(W, H);
     const int nb = int((K + SCAN_BLOCK - 1) / SCAN_BLOCK);
-    hipLaunchKernelGGL(k_tord_blocks, dim3(nb), dim3(SCAN_BLOCK), 0, st, W, H, K, counts, scratch);
+    hipLaunchKernelGGL(k_tord_blocks, dim3(nb), dim3(SCAN_BLOCK), 0, st, W, H, K, counts, long_from, scratch);
     hipLaunchKernelGGL(k_tord_totals, dim3(1), dim3(SCAN_BLOCK), 0, st, nb, scratch);
-    hipLaunchKernelGGL(k_tord_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, W, H, K, counts, scratch, active);
+    hipLaunchKernelGGL(k_tord_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, W, H, K, counts, long_from, nb, scratch,
+                       active);
 }
 void launch_plan_scan(int n, const int* counts, int* offsets, int* active, int* totals, int64_t* scratch,
                       int64_t capacity, hipStream_t st) {

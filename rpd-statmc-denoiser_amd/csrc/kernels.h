@@ -332,10 +332,11 @@ void launch_plan_scan(int n_pixels, const int* counts, int* offsets, int* active
 void launch_fill_i32(int* p, int value, int64_t n, hipStream_t st);
 int64_t plan_scan_scratch_elems(int n_pixels);
 // rewrites the active list of a plan (counts set) in 8x8 tile order, the order
-// the trace hands pixels out in and the fold walks them in (scratch:
+// the trace hands pixels out in and the fold walks them in; with long_from > 0
+// the pixels of at least long_from samples come first (scratch:
 // plan_tile_scratch_elems entries)
 void launch_plan_tile_order(int width, int height, const int* counts, int* active, int64_t* scratch,
-                            hipStream_t st);
+                            int long_from, hipStream_t st);
 int64_t plan_tile_scratch_elems(int width, int height);
 
 void launch_features(const Batch& b, hipStream_t st);

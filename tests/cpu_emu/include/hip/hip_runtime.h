@@ -182,6 +182,8 @@ template <class T> inline T __shfl_up(T v, unsigned delta, int width = 64) {
 }
 // v of lane `lane` (a wave-uniform index, as the device instruction requires)
 inline int __builtin_amdgcn_readlane(int v, int lane) { return __shfl(v, lane); }
+// issue priority of the wave: a scheduling hint with no effect on results
+inline void __builtin_amdgcn_s_setprio(short) {}
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 inline int __clz(unsigned v) { return v ? __builtin_clz(v) : 32; }
 inline int __clz(int v) { return __clz(unsigned(v)); }
