@@ -138,9 +138,9 @@ __device__ __forceinline__ bool is_zero3(v3 a) { return a.x == 0.0f && a.y == 0.
 // of those of waves with shorter chains, which fill the cycles it leaves. The
 // tests are ballots (wave-uniform), so exactly one s_setprio runs.
 __device__ __forceinline__ void wave_priority(int left) {
-    if (__ballot(left >= 24)) __builtin_amdgcn_s_setprio(3);
-    else if (__ballot(left >= 16)) __builtin_amdgcn_s_setprio(2);
-    else if (__ballot(left >= 8)) __builtin_amdgcn_s_setprio(1);
+    if (__ballot(left >= 16)) __builtin_amdgcn_s_setprio(3);
+    else if (__ballot(left >= 8)) __builtin_amdgcn_s_setprio(2);
+    else if (__ballot(left >= 4)) __builtin_amdgcn_s_setprio(1);
     else __builtin_amdgcn_s_setprio(0);
 }
 
