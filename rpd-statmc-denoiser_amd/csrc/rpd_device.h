@@ -448,6 +448,7 @@ RPD_HD HitRec trace_query(const DevScene& S, const float4* tp, v3 o, v3 d, float
     float best = t_max;
     int bi = -1;
     const float4* e = tp + tri_perm_entry(0, rp.kz);
+#pragma unroll 4
     for (int i = 0; i < S.n_tris; ++i, e += 9) {
         float t;
         if (tri_hit(e[0], e[1], e[2], rp, 0.0f, best, t) && t < best) { best = t; bi = i; }
