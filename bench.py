@@ -146,6 +146,8 @@ def new_context(args):
         ctx.set_trace_wavefront(args.trace_wavefront)
     if getattr(args, "gather_overlap", None) is not None:
         ctx.set_gather_overlap(args.gather_overlap)
+    if getattr(args, "statmc_pairs", None) is not None:
+        ctx.set_statmc_pairs(args.statmc_pairs)
     return ctx
 
 
@@ -534,6 +536,9 @@ def main():
                          "default: the engine's default)")
     ap.add_argument("--gather-overlap", type=int, choices=[0, 1], default=None,
                     help="grid gather of a pass beside the next pass's trace (1) or in line (0); "
+                         "default: the engine's default")
+    ap.add_argument("--statmc-pairs", type=int, choices=[0, 1], default=None,
+                    help="StatMC window as the symmetric pair and sum passes (1) or one pass (0); "
                          "default: the engine's default")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy (counter calibration) and exit")

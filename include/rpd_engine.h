@@ -408,6 +408,13 @@ int rpd_ctx_set_trace_wavefront(rpd_context* ctx, float rounds_per_sample);
    bits as with every gather in line, which 0 selects (and graph passes
    always use). */
 int rpd_ctx_set_gather_overlap(rpd_context* ctx, int on);
+/* StatMC window (main.cpp:1032-1097) in two passes, on (the default) or off
+   (one pass). A pair's weight is the same bits with its two pixels exchanged,
+   so the pair pass computes each pair of the window once into a table held in
+   the job's tape buffer (free once the passes are done; radius up to 13) and
+   the sum pass adds each pixel's full window in the reference's order. Same
+   bits as one pass; denoise-only jobs and batches always take one pass. */
+int rpd_ctx_set_statmc_pairs(rpd_context* ctx, int on);
 /* Device bytes a job of this configuration needs (its slab size; admission by
    HBM footprint), the second plan and record set of the gather overlap
    included wherever its configuration allows it (an upper bound for scenes

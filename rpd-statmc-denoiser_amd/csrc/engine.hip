@@ -160,6 +160,9 @@ struct rpd_context {
     // jobs created next run a pass's grid gather beside the next pass's trace
     // (second plan and record set, second stream)
     bool gather_overlap = true;
+    // jobs' StatMC windows run as the symmetric pair and sum passes where the
+    // job has room for the pair weights (render jobs, radius up to 13)
+    bool statmc_pairs = true;
     // streams of released jobs, for the next jobs: creating a stream costs
     // host time, and with more hardware queues than the default (the
     // GPU_MAX_HW_QUEUES environment variable) a new stream can also set up a
@@ -576,7 +579,7 @@ int do_recompute_sensitivity(rpd_job* j) {
 uint64_t tune_key(const Batch& b) {
     int size_class = 0;
     for (int64_t px = b.total[BK_PIXEL]; px > 1; px >>= 1) ++size_class;
-    return (uint64_t(b.max_radius) << 8) | uint64_t(size_class);
+    return (uint64_t(b.pairs ? 1 : 0) << 16) | (uint64_t(b.max_radius) << 8) | uint64_t(size_class);
 }
 
 int run_statmc_window(rpd_context* ctx, Batch& b, hipStream_t st, bool can_time,
@@ -647,7 +650,12 @@ int do_statmc(rpd_job* j) {
             launch_set_tile_prefix1(b, b.total[BK_TILE], j->stream);
             return RPD_OK;
         };
-        st = run_statmc_window(j->ctx, j->batch1, j->stream, !j->capturing, set_shape);
+        // the pair and sum passes where the job has room for the pair weights
+        Batch b = j->batch1;
+        b.pairs = (j->ctx->statmc_pairs && j->desc.pair_w && b.n == 1) ? 1 : 0;
+        b.width1 = j->W;
+        b.height1 = j->H;
+        st = run_statmc_window(j->ctx, b, j->stream, !j->capturing, set_shape);
     }
     if (st) return st;
     HIP_TRY(hipGetLastError());
@@ -866,6 +874,13 @@ ImageDesc make_desc(const rpd_job* j) {
     d.fc.rpd_scale = j->cfg.rpf_shrinkage_scale;
     d.conf_n0 = j->cfg.rpf_confidence_samples;
     d.dbg = j->dbg;
+    // The StatMC pair weights take the tape buffer, which no stage after the
+    // last pass reads (a later pass writes its tapes anew): a render job's
+    // tapes hold MT_TAPE_WORDS words per pixel, the weights
+    // statmc_forward_count(radius) doubles.
+    d.pair_w = nullptr;
+    const int r = j->cfg.color_window_radius;
+    if (!j->denoiser && j->tape && statmc_pairs_radius_ok(r)) d.pair_w = reinterpret_cast<double*>(j->tape);
     // grid accumulators this job's samples can reach: a render job records no
     // environment sample without an environment and no lens sample without an
     // aperture (k_trace); records given by the caller may carry any
@@ -2071,9 +2086,14 @@ int rpd_ctx_statmc_tile(rpd_context* ctx, int radius, int64_t pixels, int* tile_
     b.max_radius = radius;
     b.total[BK_PIXEL] = pixels;
     int k = ctx->forced_shape;
-    if (k < 0) {
+    if (k < 0) {   // the pair form's shape if it was tuned, else the one-pass form's
         std::lock_guard<std::mutex> lk(ctx->tune_mu);
+        b.pairs = 1;
         auto it = ctx->tuned.find(tune_key(b));
+        if (it == ctx->tuned.end()) {
+            b.pairs = 0;
+            it = ctx->tuned.find(tune_key(b));
+        }
         if (it != ctx->tuned.end()) k = it->second;
     }
     *tile_w = k < 0 ? 0 : STATMC_SHAPES[k][0];
@@ -2092,6 +2112,12 @@ int rpd_ctx_set_trace_wavefront(rpd_context* ctx, float rounds_per_sample) {
     if (!(rounds_per_sample >= 0.0f) || rounds_per_sample > 64.0f)
         return fail(RPD_ERR_INVALID_ARG, "rounds_per_sample must lie in [0, 64]");
     ctx->wf_rounds = rounds_per_sample;
+    return RPD_OK;
+}
+
+int rpd_ctx_set_statmc_pairs(rpd_context* ctx, int on) {
+    if (!ctx) return fail(RPD_ERR_INVALID_ARG, "null context");
+    ctx->statmc_pairs = on != 0;
     return RPD_OK;
 }
 

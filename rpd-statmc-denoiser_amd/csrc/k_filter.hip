@@ -157,6 +157,48 @@ struct GateTile {
     int ox, oy;      // image coordinates of its first element
 };
 
+struct StatmcConst {
+    int table;   // t-critical table: 0 (alpha 0.05), 1 (alpha 0.01), 2 (none)
+    float z;
+    double zz;   // z * z
+};
+__device__ __forceinline__ StatmcConst statmc_const(const FilterConfig& fc) {
+    StatmcConst k;
+    k.table = 2;
+    if (fabsf(fmaxf(fminf(fc.compat_alpha, 0.999999f), RPD_EPS_SMALL) - 0.05f) < RPD_EPS) k.table = 0;
+    else if (fabsf(fmaxf(fminf(fc.compat_alpha, 0.999999f), RPD_EPS_SMALL) - 0.01f) < RPD_EPS) k.table = 1;
+    const float a = fmaxf(fminf(fc.compat_alpha, 0.999999f), RPD_EPS_SMALL);
+    if (a <= 0.001f) k.z = 3.29f;
+    else if (a <= 0.005f) k.z = 2.81f;
+    else if (a <= 0.01f) k.z = 2.58f;
+    else if (a <= 0.02f) k.z = 2.33f;
+    else if (a <= 0.05f) k.z = 1.96f;
+    else if (a <= 0.10f) k.z = 1.64f;
+    else k.z = 1.28f;
+    k.zz = double(k.z) * double(k.z);
+    return k;
+}
+
+// The pair tests of main.cpp:1040-1062 on neighbour j's P0 (normal, depth) and
+// P1 (albedo, n) against pixel i's normal ni, depth di and albedo ai. Each test
+// gives the same result with i and j exchanged: the dot product's products
+// commute and are summed in one order, |a - b| = |b - a| and (a - b)^2 =
+// (b - a)^2 exactly, and max and min commute.
+__device__ __forceinline__ bool statmc_gate0(const FilterConfig& fc, const v3 ni, const float di, const float4 q0) {
+    if (q0.w < 0.0f) return false;
+    if (dot3(ni, mk3(q0.x, q0.y, q0.z)) < fc.normal_threshold) return false;
+    return !(relative_depth_delta(di, q0.w) > fc.depth_threshold);
+}
+__device__ __forceinline__ bool statmc_gate1(const v3 ai, const float4 q1) {
+    return !(norm3(ai - mk3(q1.x, q1.y, q1.z)) > 0.12f);
+}
+
+// The relaxed Welch statistic q of a pair that passed the gates
+// (main.cpp:1063-1097): pixel i's P2, P3, Welch-Satterthwaite terms and
+// reliable components against neighbour j's. Every operation is symmetric in
+// i and j (a difference only squared, commutative sums, max and min), so q is
+// the same bits with the two exchanged.
+//
 // Two operations of the reference's per-channel loop are replaced by ones that
 // give the same bits:
 //  * the t critical value depends on df only through float(df), and is the
@@ -166,60 +208,169 @@ struct GateTile {
 //  * when every channel's critical value is the same tc, the channel maximum of
 //    RN(t2_c / RN(tc tc)) is RN(max_c t2_c / RN(tc tc)): correctly rounded
 //    division by one divisor is monotone. One FP64 division instead of three.
+__device__ __forceinline__ double statmc_q(const FilterConfig& fc, const StatmcConst& K, const int n_i, const float4 p2,
+                                           const float4 p3, const double (&df_i)[3], const float (&ci)[6],
+                                           const int n_j, const float4 q2, const float4 q3, const double (&df_j)[3],
+                                           const float4 q5, const float4 q6) {
+    const float smi[3] = {p2.x, p2.y, p2.z}, smj[3] = {q2.x, q2.y, q2.z};
+    const float vsi[3] = {p3.x, p3.y, p3.z}, vsj[3] = {q3.x, q3.y, q3.z};
+    double t2[3], tcs[3];
+    bool all_z = true;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        const double vi = double(vsi[ch]), vj = double(vsj[ch]);
+        const double vs = vi + vj;
+        const double md = double(smi[ch] - smj[ch]);
+        t2[ch] = md * md / fmax(double(RPD_EPS_SMALL), vs);
+        const double den_df = df_i[ch] + df_j[ch];
+        tcs[ch] = double(K.z);
+        if (den_df > double(RPD_EPS_SMALL)) {
+            const float vsf = float(vs);
+            if (!(vsf * vsf >= 31.0f * float(den_df))) {
+                tcs[ch] = double(t_critical(float((vs * vs) / den_df), K.table, K.z));
+                all_z = false;
+            }
+        } else {
+            tcs[ch] = double(t_critical(float(double(n_i + n_j - 2)), K.table, K.z));
+            all_z = false;
+        }
+    }
+    double mx;
+    if (all_z) {
+        mx = fmax(fmax(t2[0], t2[1]), t2[2]) / K.zz;
+    } else {
+        mx = 0.0;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) mx = fmax(mx, t2[ch] / (tcs[ch] * tcs[ch]));
+    }
+    const float cj[6] = {q5.x, q5.y, q5.z, q5.w, q6.x, q6.y};
+    float shared = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) shared = fmaxf(shared, fminf(ci[k], cj[k]));
+    shared = clampf(shared, 0.0f, 1.0f);
+    const float adj = clampf(fc.rpd_scale * shared, 0.0f, 1.0f);
+    return mx / (1.0 + double(adj));
+}
+
+__device__ __forceinline__ double statmc_sp(const double* sp_tab, int d2) {
+    return d2 <= STATMC_SP_MAX ? sp_tab[d2] : 1.0 / (1.0 + double(d2));
+}
+
+// A pixel's own filter inputs (the i side of every pair).
+struct StatmcSelf {
+    float4 p0, p1, p2, p3, p4, p5, p6;
+    v3 ni, ai;
+    float di;
+    int n_i;
+    float ci[6];
+    double df_i[3];
+};
+__device__ __forceinline__ void statmc_self(const ImageDesc& d, const int i, StatmcSelf& S) {
+    const int n = d.N;
+    const float4* __restrict__ P = reinterpret_cast<const float4*>(d.filter_scratch + 4 * size_t(n));
+    S.p0 = P[i];
+    S.p1 = P[n + i]; S.p2 = P[2 * n + i]; S.p3 = P[3 * n + i]; S.p4 = P[4 * n + i]; S.p5 = P[5 * n + i];
+    S.p6 = P[6 * n + i];
+    S.ni = mk3(S.p0.x, S.p0.y, S.p0.z);
+    S.di = S.p0.w;
+    S.ai = mk3(S.p1.x, S.p1.y, S.p1.z);
+    S.n_i = int(S.p1.w);
+    S.ci[0] = S.p5.x; S.ci[1] = S.p5.y; S.ci[2] = S.p5.z; S.ci[3] = S.p5.w; S.ci[4] = S.p6.x; S.ci[5] = S.p6.y;
+    const double2* __restrict__ D = filter_ws_planes(d);
+    const double2 s01 = D[i], s2 = D[n + i];
+    S.df_i[0] = s01.x; S.df_i[1] = s01.y; S.df_i[2] = s2.x;
+}
+
+// Weight of neighbour q of pixel i at offset (dx, dy) (main.cpp:1032-1097), or
+// -1 when a gate or the Welch test rejects it; q0 and q1 are q's P0 and P1.
+__device__ __forceinline__ double statmc_pair_weight(const ImageDesc& d, const StatmcConst& K, const StatmcSelf& S,
+                                                     const double* sp_tab, const int q, const float4 q0,
+                                                     const float4 q1, const int dx, const int dy) {
+    const FilterConfig& fc = d.fc;
+    if (!statmc_gate0(fc, S.ni, S.di, q0)) return -1.0;
+    if (!statmc_gate1(S.ai, q1)) return -1.0;
+    const int n = d.N;
+    const float4* __restrict__ P = reinterpret_cast<const float4*>(d.filter_scratch + 4 * size_t(n));
+    const double2* __restrict__ D = filter_ws_planes(d);
+    const float4 q2 = P[2 * n + q], q3 = P[3 * n + q];
+    const double2 sj01 = D[q], sj2 = D[n + q];
+    const double df_j[3] = {sj01.x, sj01.y, sj2.x};
+    const float4 q5 = P[5 * n + q], q6 = P[6 * n + q];
+    const double qv = statmc_q(fc, K, S.n_i, S.p2, S.p3, S.df_i, S.ci, int(q1.w), q2, q3, df_j, q5, q6);
+    if (qv > 25.0) return -1.0;
+    return statmc_sp(sp_tab, dx * dx + dy * dy) * exp(-0.5 * qv);
+}
+
+// A pixel without a primary hit keeps its mean (main.cpp:1000-1010).
+__device__ __forceinline__ void statmc_keep_mean(const ImageDesc& d, const int i) {
+    const int n = d.N;
+    const PixelBuffers& pb = d.pb;
+    const int cnt = pb.n[i];
+    d.den[i] = pb.color_mean[i]; d.den[n + i] = pb.color_mean[n + i]; d.den[2 * n + i] = pb.color_mean[2 * n + i];
+    const float var = cnt > 1 ? pb.lum_m2[i] / float(cnt - 1) : 0.0f;
+    d.unc[i] = var / float(max(1, cnt));
+    d.alp[i] = 1.0f;
+}
+
+// The window's sums of one pixel, in the window's row-major order.
+struct StatmcSums {
+    double ws = 0, wss = 0, wvs = 0, c0 = 0, c1 = 0, c2 = 0, accepted = 0;
+    __device__ __forceinline__ void add(const double wj, const double sp, const float4 q4, const float q3w,
+                                        const bool centre) {
+        c0 += double(q4.x) * wj; c1 += double(q4.y) * wj; c2 += double(q4.z) * wj;
+        ws += wj;
+        wss += wj * wj;
+        wvs += wj * wj * double(q3w);
+        if (!centre && wj > 0.05 * sp) accepted += 1.0;
+    }
+};
+
+// The blend, uncertainty and diagnostics from the window's sums
+// (main.cpp:1099-1126).
+__device__ __forceinline__ void statmc_finish(const ImageDesc& d, const int i, const StatmcSelf& S, const StatmcSums& A) {
+    const int n = d.N;
+    const float var_i = S.p2.w, vm_i = S.p3.w;
+    d.vtot[i] = var_i;
+    d.veff[i] = vm_i;
+    d.vrat[i] = (var_i > RPD_EPS_SMALL) ? (vm_i * float(S.n_i)) / var_i : 0.0f;
+    const float4 p4 = S.p4;
+    v3 mn = mk3(p4.x, p4.y, p4.z);
+    double vn = double(vm_i);
+    if (A.ws > double(RPD_EPS_SMALL)) {
+        mn = mk3(float(A.c0 / A.ws), float(A.c1 / A.ws), float(A.c2 / A.ws));
+        vn = A.wvs / (A.ws * A.ws);
+    }
+    const double neff = A.ws * A.ws / fmax(double(RPD_EPS_SMALL), A.wss);
+    const double blend = (A.accepted >= 2.0 && A.ws >= 1.1) ? fmin(fmax(1.0 - 1.0 / sqrt(neff), 0.0), 0.8) : 0.0;
+    const float a = float(1.0 - blend);
+    const float bf = float(blend);
+    d.den[i] = p4.x + (mn.x - p4.x) * bf;
+    d.den[n + i] = p4.y + (mn.y - p4.y) * bf;
+    d.den[2 * n + i] = p4.z + (mn.z - p4.z) * bf;
+    d.unc[i] = float(double(a) * double(a) * double(vm_i) + blend * blend * vn +
+                     2.0 * double(a) * blend * (1.0 / fmax(A.ws, double(RPD_EPS_SMALL))) * double(vm_i));
+    d.alp[i] = a;
+    if (d.dbg) {
+        d.dbg[i] = float(A.accepted);
+        d.dbg[n + i] = float(A.ws);
+        d.dbg[2 * n + i] = float(neff);
+    }
+}
+
 __device__ __forceinline__ void statmc_pixel(const ImageDesc& d, const int x, const int y, const double* sp_tab,
                                              const GateTile& gt) {
     const int W = d.W, H = d.H, n = d.N;
-    const FilterConfig fc = d.fc;
     const float4* __restrict__ P = reinterpret_cast<const float4*>(d.filter_scratch + 4 * size_t(n));
-    const PixelBuffers& pb = d.pb;
-    float *den = d.den, *unc = d.unc, *alp = d.alp, *vtot = d.vtot, *veff = d.veff, *vrat = d.vrat;
     const int i = y * W + x;
-    const float4 p0 = P[i];
-    if (p0.w < 0.0f) {  // no primary hit: keep the mean
-        const int cnt = pb.n[i];
-        den[i] = pb.color_mean[i]; den[n + i] = pb.color_mean[n + i]; den[2 * n + i] = pb.color_mean[2 * n + i];
-        const float var = cnt > 1 ? pb.lum_m2[i] / float(cnt - 1) : 0.0f;
-        unc[i] = var / float(max(1, cnt));
-        alp[i] = 1.0f;
+    if (P[i].w < 0.0f) {  // no primary hit: keep the mean
+        statmc_keep_mean(d, i);
         return;
     }
-    int table = 2;
-    if (fabsf(fmaxf(fminf(fc.compat_alpha, 0.999999f), RPD_EPS_SMALL) - 0.05f) < RPD_EPS) table = 0;
-    else if (fabsf(fmaxf(fminf(fc.compat_alpha, 0.999999f), RPD_EPS_SMALL) - 0.01f) < RPD_EPS) table = 1;
-    float z;
-    {
-        const float a = fmaxf(fminf(fc.compat_alpha, 0.999999f), RPD_EPS_SMALL);
-        if (a <= 0.001f) z = 3.29f;
-        else if (a <= 0.005f) z = 2.81f;
-        else if (a <= 0.01f) z = 2.58f;
-        else if (a <= 0.02f) z = 2.33f;
-        else if (a <= 0.05f) z = 1.96f;
-        else if (a <= 0.10f) z = 1.64f;
-        else z = 1.28f;
-    }
-    const double zz = double(z) * double(z);
-    const v3 ni = mk3(p0.x, p0.y, p0.z);
-    const float di = p0.w;
-    const float4 p1 = P[n + i], p2 = P[2 * n + i], p3 = P[3 * n + i], p4 = P[4 * n + i], p5 = P[5 * n + i],
-                 p6 = P[6 * n + i];
-    const v3 ai = mk3(p1.x, p1.y, p1.z);
-    const int n_i = int(p1.w);
-    const float var_i = p2.w;
-    const float vm_i = p3.w;
-    const float ci[6] = {p5.x, p5.y, p5.z, p5.w, p6.x, p6.y};
-    vtot[i] = var_i;
-    veff[i] = vm_i;
-    vrat[i] = (var_i > RPD_EPS_SMALL) ? (vm_i * float(n_i)) / var_i : 0.0f;
-    // each pixel's term of each channel's Welch-Satterthwaite denominator
-    const double2* __restrict__ D = filter_ws_planes(d);
-    double df_i[3];
-    {
-        const double2 s01 = D[i], s2 = D[n + i];
-        df_i[0] = s01.x; df_i[1] = s01.y; df_i[2] = s2.x;
-    }
-
-    double ws = 0, wss = 0, wvs = 0, c0 = 0, c1 = 0, c2 = 0, accepted = 0;
-    const int R = fc.radius;
+    const StatmcConst K = statmc_const(d.fc);
+    StatmcSelf S;
+    statmc_self(d, i, S);
+    StatmcSums A;
+    const int R = d.fc.radius;
     for (int dy = -R; dy <= R; ++dy) {
         const int yy = y + dy;
         if (yy < 0 || yy >= H) continue;
@@ -230,95 +381,16 @@ __device__ __forceinline__ void statmc_pixel(const ImageDesc& d, const int x, co
             float4 q0, q1;
             if (gt.g0) {
                 const int k = (yy - gt.oy) * gt.span_w + (xx - gt.ox);
-                q0 = gt.g0[k];
-                if (q0.w < 0.0f) continue;
-                if (dot3(ni, mk3(q0.x, q0.y, q0.z)) < fc.normal_threshold) continue;
-                if (relative_depth_delta(di, q0.w) > fc.depth_threshold) continue;
-                q1 = gt.g1[k];
+                q0 = gt.g0[k]; q1 = gt.g1[k];
             } else {
-                q0 = P[q];
-                if (q0.w < 0.0f) continue;
-                if (dot3(ni, mk3(q0.x, q0.y, q0.z)) < fc.normal_threshold) continue;
-                if (relative_depth_delta(di, q0.w) > fc.depth_threshold) continue;
-                q1 = P[n + q];
+                q0 = P[q]; q1 = P[n + q];
             }
-            if (norm3(ai - mk3(q1.x, q1.y, q1.z)) > 0.12f) continue;
-            const float4 q2 = P[2 * n + q], q3 = P[3 * n + q];
-            const int n_j = int(q1.w);
-            const double2 sj01 = D[q], sj2 = D[n + q];
-            const double df_j[3] = {sj01.x, sj01.y, sj2.x};
-            const float smi[3] = {p2.x, p2.y, p2.z}, smj[3] = {q2.x, q2.y, q2.z};
-            const float vsi[3] = {p3.x, p3.y, p3.z}, vsj[3] = {q3.x, q3.y, q3.z};
-            double t2[3], tcs[3];
-            bool all_z = true;
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                const double vi = double(vsi[ch]), vj = double(vsj[ch]);
-                const double vs = vi + vj;
-                const double md = double(smi[ch] - smj[ch]);
-                t2[ch] = md * md / fmax(double(RPD_EPS_SMALL), vs);
-                const double den_df = df_i[ch] + df_j[ch];
-                tcs[ch] = double(z);
-                if (den_df > double(RPD_EPS_SMALL)) {
-                    const float vsf = float(vs);
-                    if (!(vsf * vsf >= 31.0f * float(den_df))) {
-                        tcs[ch] = double(t_critical(float((vs * vs) / den_df), table, z));
-                        all_z = false;
-                    }
-                } else {
-                    tcs[ch] = double(t_critical(float(double(n_i + n_j - 2)), table, z));
-                    all_z = false;
-                }
-            }
-            double mx;
-            if (all_z) {
-                mx = fmax(fmax(t2[0], t2[1]), t2[2]) / zz;
-            } else {
-                mx = 0.0;
-#pragma unroll
-                for (int ch = 0; ch < 3; ++ch) mx = fmax(mx, t2[ch] / (tcs[ch] * tcs[ch]));
-            }
-            const float4 q5 = P[5 * n + q], q6 = P[6 * n + q];
-            const float cj[6] = {q5.x, q5.y, q5.z, q5.w, q6.x, q6.y};
-            float shared = 0.0f;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) shared = fmaxf(shared, fminf(ci[k], cj[k]));
-            shared = clampf(shared, 0.0f, 1.0f);
-            const float adj = clampf(fc.rpd_scale * shared, 0.0f, 1.0f);
-            const double qv = mx / (1.0 + double(adj));
-            if (qv > 25.0) continue;
-            const int d2 = dx * dx + dy * dy;
-            const double sp = d2 <= STATMC_SP_MAX ? sp_tab[d2] : 1.0 / (1.0 + double(d2));
-            const double wj = sp * exp(-0.5 * qv);
-            const float4 q4 = P[4 * n + q];
-            c0 += double(q4.x) * wj; c1 += double(q4.y) * wj; c2 += double(q4.z) * wj;
-            ws += wj;
-            wss += wj * wj;
-            wvs += wj * wj * double(q3.w);
-            if ((dx != 0 || dy != 0) && wj > 0.05 * sp) accepted += 1.0;
+            const double wj = statmc_pair_weight(d, K, S, sp_tab, q, q0, q1, dx, dy);
+            if (wj < 0.0) continue;
+            A.add(wj, statmc_sp(sp_tab, dx * dx + dy * dy), P[4 * n + q], P[3 * n + q].w, dx == 0 && dy == 0);
         }
     }
-    v3 mn = mk3(p4.x, p4.y, p4.z);
-    double vn = double(vm_i);
-    if (ws > double(RPD_EPS_SMALL)) {
-        mn = mk3(float(c0 / ws), float(c1 / ws), float(c2 / ws));
-        vn = wvs / (ws * ws);
-    }
-    const double neff = ws * ws / fmax(double(RPD_EPS_SMALL), wss);
-    const double blend = (accepted >= 2.0 && ws >= 1.1) ? fmin(fmax(1.0 - 1.0 / sqrt(neff), 0.0), 0.8) : 0.0;
-    const float a = float(1.0 - blend);
-    const float bf = float(blend);
-    den[i] = p4.x + (mn.x - p4.x) * bf;
-    den[n + i] = p4.y + (mn.y - p4.y) * bf;
-    den[2 * n + i] = p4.z + (mn.z - p4.z) * bf;
-    unc[i] = float(double(a) * double(a) * double(vm_i) + blend * blend * vn +
-                   2.0 * double(a) * blend * (1.0 / fmax(ws, double(RPD_EPS_SMALL))) * double(vm_i));
-    alp[i] = a;
-    if (d.dbg) {
-        d.dbg[i] = float(accepted);
-        d.dbg[n + i] = float(ws);
-        d.dbg[2 * n + i] = float(neff);
-    }
+    statmc_finish(d, i, S, A);
 }
 
 // Tile shapes of the window kernel: TW x TH pixels, one thread each. Every
@@ -330,6 +402,10 @@ const int STATMC_SHAPES[STATMC_SHAPE_COUNT][2] = {{16, 16}, {32, 8}, {8, 32}, {6
 // LDS for the staged gate inputs of one tile and halo: two float4 per pixel.
 __host__ __device__ inline size_t statmc_gate_bytes(int tw, int th, int radius) {
     return size_t(tw + 2 * radius) * size_t(th + 2 * radius) * 2 * sizeof(float4);
+}
+// the pair pass stages the tile and its forward halo only (no rows above)
+__host__ __device__ inline size_t statmc_gate_bytes_forward(int tw, int th, int radius) {
+    return size_t(tw + 2 * radius) * size_t(th + radius) * 2 * sizeof(float4);
 }
 constexpr size_t STATMC_GATE_LDS_MAX = 48 * 1024;   // larger halos read the gate inputs from global memory
 
@@ -378,6 +454,187 @@ __global__ void __launch_bounds__(TW * TH) k_statmc(Batch b, int stage) {
     }
 }
 
+// ---- the symmetric two-pass window -------------------------------------------
+// A pair's weight is the same bits with its two pixels exchanged (the gates
+// and q above; the spatial term depends on dx^2 + dy^2). So the pair pass
+// computes each pair once, for the forward half of the window (dy > 0, or
+// dy == 0 and dx > 0), into plane k of `pair_w` (plane k holds offset k's
+// weight of every pixel: N doubles, -1 for a rejected or missing neighbour),
+// and the sum pass walks each pixel's full window in the reference's order,
+// taking a forward neighbour's weight from its own row, a backward one's from
+// the neighbour's row, and computing the centre's. Half the pairs' FP64 work
+// of the one-pass form, the same sums in the same order.
+__device__ __forceinline__ int statmc_forward_index(int dx, int dy, int r) {
+    return dy == 0 ? dx - 1 : r + (dy - 1) * (2 * r + 1) + (dx + r);
+}
+__device__ __forceinline__ void statmc_pairs_pixel(const ImageDesc& d, const int x, const int y, const double* sp_tab,
+                                                   const GateTile& gt) {
+    const int W = d.W, H = d.H, n = d.N, R = d.fc.radius;
+    const float4* __restrict__ P = reinterpret_cast<const float4*>(d.filter_scratch + 4 * size_t(n));
+    double* __restrict__ Wp = d.pair_w;
+    const int i = y * W + x;
+    if (P[i].w < 0.0f) {   // a neighbour without a primary hit is skipped either way
+        for (int k = 0; k < statmc_forward_count(R); ++k) Wp[size_t(k) * n + i] = -1.0;
+        return;
+    }
+    const StatmcConst K = statmc_const(d.fc);
+    StatmcSelf S;
+    statmc_self(d, i, S);
+    for (int dy = 0; dy <= R; ++dy) {
+        const int yy = y + dy;
+        for (int dx = dy == 0 ? 1 : -R; dx <= R; ++dx) {
+            const int xx = x + dx;
+            double wj = -1.0;
+            if (yy < H && xx >= 0 && xx < W) {
+                const int q = yy * W + xx;
+                float4 q0, q1;
+                if (gt.g0) {
+                    const int k = (yy - gt.oy) * gt.span_w + (xx - gt.ox);
+                    q0 = gt.g0[k]; q1 = gt.g1[k];
+                } else {
+                    q0 = P[q]; q1 = P[n + q];
+                }
+                wj = statmc_pair_weight(d, K, S, sp_tab, q, q0, q1, dx, dy);
+            }
+            Wp[size_t(statmc_forward_index(dx, dy, R)) * n + i] = wj;
+        }
+    }
+}
+
+template <int TW, int TH>
+__global__ void __launch_bounds__(TW * TH) k_statmc_pairs(Batch b, int stage) {
+    __shared__ int64_t s_item;
+    __shared__ double s_sp[STATMC_SP_MAX + 1];
+    extern __shared__ float4 s_gate[];
+    for (int k = threadIdx.x; k <= STATMC_SP_MAX; k += blockDim.x) s_sp[k] = 1.0 / (1.0 + double(k));
+    int cursor = 0;
+    const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+    while (true) {
+        if (threadIdx.x == 0) s_item = batch_take(b, BK_TILE, cursor);
+        __syncthreads();
+        const int64_t item = s_item;
+        __syncthreads();
+        if (item < 0) break;
+        int64_t lt;
+        const ImageDesc& d = b.img[batch_find(b, BK_TILE, item, lt)];
+        const int tiles_x = (d.W + TW - 1) / TW;
+        const int x0 = int(lt % tiles_x) * TW, y0 = int(lt / tiles_x) * TH;
+        const int x = x0 + tx, y = y0 + ty;
+        GateTile gt;
+        gt.g0 = nullptr; gt.g1 = nullptr; gt.span_w = 0; gt.ox = 0; gt.oy = 0;
+        if (stage) {   // the tile and its forward halo: rows y0 .. y0 + TH - 1 + r
+            const int r = d.fc.radius;
+            const int sw = TW + 2 * r, sh = TH + r, cnt = sw * sh;
+            const float4* P = reinterpret_cast<const float4*>(d.filter_scratch + 4 * size_t(d.N));
+            for (int k = threadIdx.x; k < cnt; k += TW * TH) {
+                const int gx = x0 - r + k % sw, gy = y0 + k / sw;
+                if (gx >= 0 && gx < d.W && gy < d.H) {
+                    const int q = gy * d.W + gx;
+                    s_gate[k] = P[q];
+                    s_gate[cnt + k] = P[d.N + q];
+                }
+            }
+            __syncthreads();
+            gt.g0 = s_gate; gt.g1 = s_gate + cnt; gt.span_w = sw; gt.ox = x0 - r; gt.oy = y0;
+        }
+        if (x < d.W && y < d.H) statmc_pairs_pixel(d, x, y, s_sp, gt);
+        if (stage) __syncthreads();   // the next tile's copy overwrites the staged inputs
+    }
+}
+
+// One 16 x 16 tile of pixels per block, so the two reads of a pair's weight,
+// by each of its pixels, often fall in one block. The neighbours' colour means
+// and variances of the mean (P4.xyz, P3.w) of the tile and its halo are
+// copied to LDS first. A window row is walked with its bounds hoisted,
+// neighbours before the pixel reading the neighbour's row of the table, those
+// after it the pixel's own row, and the weights of four neighbours are loaded
+// before any of them is added, in order.
+constexpr int STATMC_SUM_TILE = 16;
+constexpr int STATMC_SUM_MAX_R = 13;   // the pair form's largest radius (the table must fit the tapes)
+constexpr int STATMC_SUM_SPAN = STATMC_SUM_TILE + 2 * STATMC_SUM_MAX_R;
+bool statmc_pairs_radius_ok(int r) {
+    return r >= 1 && r <= STATMC_SUM_MAX_R &&
+           size_t(statmc_forward_count(r)) * sizeof(double) <= size_t(MT_TAPE_WORDS) * sizeof(uint32_t);
+}
+__global__ void __launch_bounds__(STATMC_SUM_TILE * STATMC_SUM_TILE) k_statmc_sum(Batch b) {
+    __shared__ double s_sp[STATMC_SP_MAX + 1];
+    __shared__ float4 s_c[STATMC_SUM_SPAN * STATMC_SUM_SPAN];   // neighbour's (P4.xyz, P3.w)
+    const ImageDesc& d = b.img[0];   // the pair form runs on one image
+    const int W = d.W, H = d.H, n = d.N, R = d.fc.radius;
+    const int tiles_x = (W + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE;
+    const int x0 = int(blockIdx.x % tiles_x) * STATMC_SUM_TILE, y0 = int(blockIdx.x / tiles_x) * STATMC_SUM_TILE;
+    const float4* __restrict__ P = reinterpret_cast<const float4*>(d.filter_scratch + 4 * size_t(n));
+    const float4* __restrict__ P3 = P + 3 * size_t(n);
+    const float4* __restrict__ P4 = P + 4 * size_t(n);
+    const int sw = STATMC_SUM_TILE + 2 * R;
+    for (int k = threadIdx.x; k <= STATMC_SP_MAX; k += blockDim.x) s_sp[k] = 1.0 / (1.0 + double(k));
+    for (int k = threadIdx.x; k < sw * sw; k += blockDim.x) {
+        const int gx = x0 - R + k % sw, gy = y0 - R + k / sw;
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const int q = gy * W + gx;
+            const float4 c = P4[q];
+            s_c[k] = make_float4(c.x, c.y, c.z, P3[q].w);
+        }
+    }
+    __syncthreads();
+    const int x = x0 + int(threadIdx.x) % STATMC_SUM_TILE, y = y0 + int(threadIdx.x) / STATMC_SUM_TILE;
+    if (x >= W || y >= H) return;
+    const int i = y * W + x;
+    const double* __restrict__ Wp = d.pair_w;
+    if (P[i].w < 0.0f) {
+        statmc_keep_mean(d, i);
+        return;
+    }
+    // the centre's weight first (its inputs are not needed during the walk)
+    double w_self;
+    {
+        const StatmcConst K = statmc_const(d.fc);
+        StatmcSelf S;
+        statmc_self(d, i, S);
+        w_self = statmc_pair_weight(d, K, S, s_sp, i, S.p0, S.p1, 0, 0);
+    }
+    StatmcSums A;
+    const int xa = max(-R, -x), xb = min(R, W - 1 - x);
+    const int lx = x - x0 + R;   // the pixel's column in s_c
+    for (int dy = -R; dy <= R; ++dy) {
+        const int yy = y + dy;
+        if (yy < 0 || yy >= H) continue;
+        const float4* crow = s_c + (yy - y0 + R) * sw + lx;
+        if (dy == 0) {
+            for (int dx = xa; dx <= xb; ++dx) {
+                double wj;
+                if (dx < 0) wj = Wp[size_t(statmc_forward_index(-dx, 0, R)) * n + (i + dx)];
+                else if (dx > 0) wj = Wp[size_t(statmc_forward_index(dx, 0, R)) * n + i];
+                else wj = w_self;
+                if (wj < 0.0) continue;
+                const float4 c = crow[dx];
+                A.add(wj, statmc_sp(s_sp, dx * dx), c, c.w, dx == 0);
+            }
+            continue;
+        }
+        // the neighbour's row (dy < 0: plane of offset (-dx, -dy), one pixel and one plane back per dx)
+        // or the pixel's own (dy > 0: plane of offset (dx, dy), one plane on per dx)
+        const double* w0 = dy < 0 ? Wp + size_t(statmc_forward_index(-xa, -dy, R)) * n + (i + dy * W + xa)
+                                  : Wp + size_t(statmc_forward_index(xa, dy, R)) * n + i;
+        const int64_t step = dy < 0 ? 1 - int64_t(n) : int64_t(n);
+        const int d2y = dy * dy;
+        for (int dx = xa; dx <= xb; dx += 4) {
+            double w4[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) w4[u] = dx + u <= xb ? w0[(dx - xa + u) * step] : -1.0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (w4[u] < 0.0) continue;
+                const float4 c = crow[dx + u];
+                A.add(w4[u], statmc_sp(s_sp, (dx + u) * (dx + u) + d2y), c, c.w, false);
+            }
+        }
+    }
+    StatmcSelf S;
+    statmc_self(d, i, S);
+    statmc_finish(d, i, S, A);
+}
+
 void launch_statmc_prepare(const Batch& b, hipStream_t st) {
     const int64_t np = b.total[BK_PIXEL];
     hipLaunchKernelGGL(k_sqrt_var_mean, dim3((np + 255) / 256), dim3(256), 0, st, b);
@@ -389,12 +646,16 @@ static void launch_statmc_t(const Batch& b, hipStream_t st) {
     // resident blocks over the device (a function-local static is initialised once
     // even when several job threads launch at the same time), counted without the
     // staged inputs' LDS; a launch that stages them keeps to what fits beside it
-    static const int cap = device_resident_blocks(reinterpret_cast<const void*>(&k_statmc<TW, TH>), TW * TH);
-    const size_t lds = statmc_gate_bytes(TW, TH, b.max_radius);
+    const void* kern = b.pairs ? reinterpret_cast<const void*>(&k_statmc_pairs<TW, TH>)
+                               : reinterpret_cast<const void*>(&k_statmc<TW, TH>);
+    static const int cap_one = device_resident_blocks(reinterpret_cast<const void*>(&k_statmc<TW, TH>), TW * TH);
+    static const int cap_pairs =
+        device_resident_blocks(reinterpret_cast<const void*>(&k_statmc_pairs<TW, TH>), TW * TH);
+    const int cap = b.pairs ? cap_pairs : cap_one;
+    const size_t lds = b.pairs ? statmc_gate_bytes_forward(TW, TH, b.max_radius) : statmc_gate_bytes(TW, TH, b.max_radius);
     const int stage = b.max_radius > 0 && lds <= STATMC_GATE_LDS_MAX ? 1 : 0;
     int per_cu = 0, dev = 0, cus = 1;
-    if (stage && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_statmc<TW, TH>),
-                                                              TW * TH, lds) == hipSuccess && per_cu > 0) {
+    if (stage && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, TW * TH, lds) == hipSuccess && per_cu > 0) {
         hipGetDevice(&dev);
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
     } else {
@@ -403,7 +664,15 @@ static void launch_statmc_t(const Batch& b, hipStream_t st) {
     const int64_t limit = stage && per_cu > 0 ? int64_t(per_cu) * cus : cap;
     const int64_t blocks = std::min<int64_t>(b.total[BK_TILE], limit);
     if (blocks <= 0) return;
-    hipLaunchKernelGGL((k_statmc<TW, TH>), dim3(unsigned(blocks)), dim3(TW * TH), stage ? lds : 0, st, b, stage);
+    if (!b.pairs) {
+        hipLaunchKernelGGL((k_statmc<TW, TH>), dim3(unsigned(blocks)), dim3(TW * TH), stage ? lds : 0, st, b, stage);
+        return;
+    }
+    hipLaunchKernelGGL((k_statmc_pairs<TW, TH>), dim3(unsigned(blocks)), dim3(TW * TH), stage ? lds : 0, st, b, stage);
+    // the sum pass: one block per 16 x 16 tile of the batch's one image
+    const int64_t sum_tiles = int64_t((b.width1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE) *
+                              ((b.height1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE);
+    hipLaunchKernelGGL(k_statmc_sum, dim3(unsigned(sum_tiles)), dim3(STATMC_SUM_TILE * STATMC_SUM_TILE), 0, st, b);
 }
 
 int statmc_shape_of(int tile_w, int tile_h) {

@@ -245,6 +245,8 @@ struct ImageDesc {
     float* dbg;              // null, or 3 planes the window filter fills per pixel with a primary hit:
                              // accepted neighbours, weight sum, effective neighbours (main.cpp:991-1108)
     uint32_t acc_mask;       // bit k: grid accumulator k can receive samples (GRID_ACC_ALL: every one)
+    double* pair_w;          // the StatMC pair pass's forward-window weights (statmc_forward_count(radius)
+                             // planes of N doubles), or null where the image has no room for them
 };
 constexpr uint32_t GRID_ACC_ALL = (1u << GRID_ACC) - 1u;
 enum : int { BK_PIXEL = 0, BK_NODE = 1, BK_TILE = 2, BK_KINDS = 3 };
@@ -263,7 +265,15 @@ struct Batch {
     int max_step;                    // host: largest grid step of the images (0: not known)
     int tile_w, tile_h;              // StatMC tile shape the BK_TILE row counts
     int max_radius;                  // host: largest StatMC window radius of the images
+    int pairs;                       // host: StatMC as the symmetric pair and sum passes (a batch of one
+                                     // image, which has pair_w)
+    int width1, height1;             // host: that image's size (the sum pass's grid)
 };
+// pairs in the forward half of a StatMC window of radius r (dy > 0, or dy == 0 and dx > 0)
+__host__ __device__ constexpr int statmc_forward_count(int r) { return 2 * r * (r + 1); }
+// radii the pair form takes: its table fits a render job's tapes and its sum
+// pass's LDS tile holds the window
+bool statmc_pairs_radius_ok(int r);
 __host__ __device__ inline int64_t image_items(int kind, int W, int H, GridGeom g, int tile_w = 16, int tile_h = 16) {
     return kind == BK_PIXEL ? int64_t(W) * H
          : kind == BK_NODE  ? int64_t(g.nx) * g.ny

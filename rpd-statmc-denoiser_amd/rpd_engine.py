@@ -221,6 +221,7 @@ def lib():
             ("rpd_ctx_set_graph_passes", [vp, C.c_int]),
             ("rpd_ctx_set_trace_wavefront", [vp, C.c_float]),
             ("rpd_ctx_set_gather_overlap", [vp, C.c_int]),
+            ("rpd_ctx_set_statmc_pairs", [vp, C.c_int]),
             ("rpd_ctx_set_debug_outputs", [vp, C.c_int]),
             ("rpd_statmc_tile_shapes", [vp, C.c_int, C.POINTER(C.c_int)]),
             ("rpd_ctx_set_statmc_tile", [vp, C.c_int, C.c_int]),
@@ -323,6 +324,10 @@ class Context:
 
     def set_graph_passes(self, on):
         check(self.L.rpd_ctx_set_graph_passes(self.h, int(on)))
+
+    def set_statmc_pairs(self, on):
+        """StatMC window as the symmetric pair and sum passes (on, the default) or one pass; same bits."""
+        check(self.L.rpd_ctx_set_statmc_pairs(self.h, int(on)))
 
     def set_gather_overlap(self, on):
         """Jobs created next gather a pass into the RPF grid beside the next
