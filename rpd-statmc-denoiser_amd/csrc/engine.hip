@@ -895,19 +895,18 @@ ImageDesc make_desc(const rpd_job* j) {
 
 int build_batch1(rpd_job* j, ImageDesc* dd, int64_t* dp, unsigned int* dq) {
     j->desc = make_desc(j);
-    int64_t pf[BK_KINDS * 2];
+    BatchPrefix1 pf;
     for (int k = 0; k < BK_KINDS; ++k) {
-        pf[2 * k] = 0;
-        pf[2 * k + 1] = image_items(k, j->W, j->H, j->geom);
-        j->batch1.total[k] = pf[2 * k + 1];
+        pf.v[2 * k] = 0;
+        pf.v[2 * k + 1] = image_items(k, j->W, j->H, j->geom);
+        j->batch1.total[k] = pf.v[2 * k + 1];
     }
     j->batch1.max_step = j->geom.step;
     j->batch1.tile_w = STATMC_SHAPES[0][0];
     j->batch1.tile_h = STATMC_SHAPES[0][1];
     j->batch1.max_radius = j->cfg.color_window_radius;
-    HIP_TRY(sync_memcpy(dd, &j->desc, sizeof(ImageDesc), hipMemcpyHostToDevice));
-    HIP_TRY(sync_memcpy(dp, pf, sizeof(pf), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(dq, 0, sizeof(unsigned int) * BK_KINDS * BATCH_QUEUES));
+    launch_put_batch1(dd, j->desc, dp, pf, dq, j->stream);
+    HIP_TRY(hipGetLastError());
     j->batch1.img = dd;
     j->batch1.prefix = dp;
     j->batch1.queues = dq;
@@ -1085,11 +1084,10 @@ int job_alloc(rpd_job* j) {
         int* p2 = j->plan_sets[1];
         d.plan.counts = p2; d.plan.offsets = p2 + N; d.plan.active = p2 + 2 * N; d.plan.totals = p2 + 3 * N;
         d.rec = j->rec_sets[1];
-        int64_t pf[BK_KINDS * 2];
-        for (int k = 0; k < BK_KINDS; ++k) { pf[2 * k] = 0; pf[2 * k + 1] = j->batch1.total[k]; }
-        HIP_TRY(sync_memcpy(j->b2_img, &d, sizeof(ImageDesc), hipMemcpyHostToDevice));
-        HIP_TRY(sync_memcpy(j->b2_prefix, pf, sizeof(pf), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemset(j->b2_queues, 0, sizeof(unsigned int) * BK_KINDS * BATCH_QUEUES));
+        BatchPrefix1 pf;
+        for (int k = 0; k < BK_KINDS; ++k) { pf.v[2 * k] = 0; pf.v[2 * k + 1] = j->batch1.total[k]; }
+        launch_put_batch1(j->b2_img, d, j->b2_prefix, pf, j->b2_queues, j->stream);
+        HIP_TRY(hipGetLastError());
         Batch b = j->batch1;
         b.img = j->b2_img; b.prefix = j->b2_prefix; b.queues = j->b2_queues;
         j->batch_sets[1] = b;
@@ -1154,8 +1152,10 @@ void job_free(rpd_job* j) {
     }
 }
 
+// reset: clear the accumulators now (a job run stage by stage); a job queued
+// for its whole pipeline at once is cleared by run_passes, its first step.
 int job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam, uint32_t seed,
-               rpd_job** out) {
+               rpd_job** out, bool reset = true) {
     if (!ctx || !scene || !cfg || !cam || !out) return fail(RPD_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     if (scene->ctx != ctx) return fail(RPD_ERR_INVALID_ARG, "scene belongs to another context or to none (host-only)");
@@ -1176,7 +1176,7 @@ int job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg,
         return fail(RPD_ERR_OUT_OF_MEMORY, "job does not fit the context's memory budget");
     HIP_TRY(job_stream_take(ctx, &j->stream));
     if ((st = job_alloc(j.get()))) { job_free(j.get()); return st; }
-    if ((st = reset_state(j.get()))) { job_free(j.get()); return st; }
+    if (reset && (st = reset_state(j.get()))) { job_free(j.get()); return st; }
     *out = j.release();
     return RPD_OK;
 }
@@ -1687,7 +1687,7 @@ int rpd_job_create(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* 
 
 int rpd_job_submit(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam,
                    uint32_t seed, rpd_job** out) {
-    int st = job_create(ctx, scene, cfg, cam, seed, out);
+    int st = job_create(ctx, scene, cfg, cam, seed, out, false);
     if (st) return st;
     rpd_job* j = *out;
     const int dev = ctx->device;
@@ -1703,7 +1703,7 @@ int rpd_job_submit(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* 
 
 int rpd_job_submit_passes(rpd_context* ctx, rpd_scene* scene, const rpd_render_config* cfg, const rpd_camera* cam,
                           uint32_t seed, rpd_job** out) {
-    int st = job_create(ctx, scene, cfg, cam, seed, out);
+    int st = job_create(ctx, scene, cfg, cam, seed, out, false);
     if (st) return st;
     rpd_job* j = *out;
     const int dev = ctx->device;

@@ -51,6 +51,19 @@ void launch_set_tile_prefix1(const Batch& b, int64_t tiles, hipStream_t st) {
                        tiles);
 }
 
+// A batch of one's device copies, written on the stream: the image's
+// descriptor (passed by value in the kernel arguments), its prefix rows and
+// zeroed work counters. No host round trip, unlike a synchronous copy.
+__global__ void k_put_batch1(ImageDesc* dd, ImageDesc d, int64_t* dp, BatchPrefix1 pf, unsigned int* dq) {
+    if (threadIdx.x == 0) *dd = d;
+    if (threadIdx.x < BK_KINDS * 2) dp[threadIdx.x] = pf.v[threadIdx.x];
+    if (threadIdx.x < BK_KINDS * BATCH_QUEUES) dq[threadIdx.x] = 0u;
+}
+void launch_put_batch1(ImageDesc* dd, const ImageDesc& d, int64_t* dp, const BatchPrefix1& pf, unsigned int* dq,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_put_batch1, dim3(1), dim3(64), 0, st, dd, d, dp, pf, dq);
+}
+
 void launch_batch_queues_reset(const Batch& b, hipStream_t st) {
     hipLaunchKernelGGL(k_queues_zero, dim3(1), dim3(64), 0, st, b.queues);
 }

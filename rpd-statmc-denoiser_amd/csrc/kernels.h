@@ -269,6 +269,12 @@ struct Batch {
                                      // image, which has pair_w)
     int width1, height1;             // host: that image's size (the sum pass's grid)
 };
+// prefix rows of a batch of one: {0, items} per item kind
+struct BatchPrefix1 { int64_t v[BK_KINDS * 2]; };
+static_assert(BK_KINDS * BATCH_QUEUES <= 64 && BK_KINDS * 2 <= 64, "k_put_batch1 writes with one wave");
+// writes a batch of one's device copies (descriptor, prefix rows, zeroed work counters) on the stream
+void launch_put_batch1(ImageDesc* dd, const ImageDesc& d, int64_t* dp, const BatchPrefix1& pf, unsigned int* dq,
+                       hipStream_t st);
 // pairs in the forward half of a StatMC window of radius r (dy > 0, or dy == 0 and dx > 0)
 __host__ __device__ constexpr int statmc_forward_count(int r) { return 2 * r * (r + 1); }
 // radii the pair form takes: its table fits a render job's tapes and its sum
