@@ -305,8 +305,11 @@ def run_c3(args):
         "batched_device_ms": float(np.median(batch_dev_ms)), "speedup_vs_plain": plain_med / batch,
         "identical_to_plain": identical,
         "relmse": {"denoised": err_den, "input_mean": err_in},
-        "filter_tile": {"radius": radius, "tile": list(ctx.statmc_tile(radius, n_px)),
-                        "chosen_by": "autotuner (every tile shape timed on the first batch)"},
+        "filter_tile": ({"radius": radius, "form": "pair and sum passes", "tile": [16, 16],
+                         "chosen_by": "fixed: the batched pair form walks 16x16 tiles"}
+                        if getattr(args, "statmc_pairs", None) != 0 else
+                        {"radius": radius, "form": "one pass", "tile": list(ctx.statmc_tile(radius, n_px)),
+                         "chosen_by": "autotuner (every tile shape timed on the first batch)"}),
     }
     print(json.dumps(result))
     ctx.close()

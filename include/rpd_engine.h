@@ -413,7 +413,9 @@ int rpd_ctx_set_gather_overlap(rpd_context* ctx, int on);
    so the pair pass computes each pair of the window once into a table held in
    the job's tape buffer (free once the passes are done; radius up to 13) and
    the sum pass adds each pixel's full window in the reference's order. Same
-   bits as one pass; denoise-only jobs and batches always take one pass. */
+   bits as one pass. Batches take it with the tables in a context workspace
+   of at most 16 GiB (allocated on first use, freed with the context);
+   denoise-only jobs denoised alone and radii above 13 take one pass. */
 int rpd_ctx_set_statmc_pairs(rpd_context* ctx, int on);
 /* Device bytes a job of this configuration needs (its slab size; admission by
    HBM footprint), the second plan and record set of the gather overlap

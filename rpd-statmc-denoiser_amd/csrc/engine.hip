@@ -113,6 +113,13 @@ struct rpd_context {
     int64_t* b_prefix = nullptr;
     unsigned int* b_queues = nullptr;
     int b_cap = 0;
+    // batched StatMC pair form: the pair tables of a group of images, and the
+    // group's descriptor and prefix arrays (grown on demand)
+    double* pw = nullptr;
+    size_t pw_bytes = 0;
+    rpd::ImageDesc* p_img = nullptr;
+    int64_t* p_prefix = nullptr;
+    int p_cap = 0;
     hipStream_t b_stream = nullptr;
     hipEvent_t b_ev[2] = {nullptr, nullptr};
     bool graph_passes = false;  // capture each job's adaptive-pass sequence as a hipGraph and replay it
@@ -1366,6 +1373,9 @@ int rpd_ctx_destroy(rpd_context* ctx) {
     if (ctx->b_stream) hipStreamSynchronize(ctx->b_stream);
     if (ctx->b_img) dev_free(ctx->b_img);
     if (ctx->b_prefix) dev_free(ctx->b_prefix);
+    if (ctx->pw) dev_free(ctx->pw);
+    if (ctx->p_img) dev_free(ctx->p_img);
+    if (ctx->p_prefix) dev_free(ctx->p_prefix);
     if (ctx->b_queues) dev_free(ctx->b_queues);
     for (hipEvent_t e : ctx->b_ev) if (e) event_destroy(e);
     if (ctx->b_stream) stream_destroy(ctx->b_stream);
@@ -1918,6 +1928,83 @@ static int check_batch_list(rpd_context* ctx, rpd_job* const* jobs, int n) {
     return RPD_OK;
 }
 
+// StatMC of a batch in the pair form (the symmetric pair pass and the in-order
+// sum pass, the same bits as one pass), in groups of consecutive images whose
+// pair tables fit one workspace of at most STATMC_PAIRS_WORKSPACE bytes; both
+// passes take 16 x 16 tiles. Returns RPD_ERR_UNSUPPORTED, with nothing
+// launched, when an image's radius does not take the pair form or no
+// workspace holds its table: the caller then runs one pass.
+constexpr size_t STATMC_PAIRS_WORKSPACE = size_t(16) << 30;
+int batch_statmc_pairs(rpd_context* ctx, const std::vector<ImageDesc>& descs, const Batch& base, hipStream_t s,
+                       std::vector<std::vector<int64_t>>& keep, std::vector<std::vector<ImageDesc>>& keep_d) {
+    const int nb = int(descs.size());
+    std::vector<size_t> bytes(nb);
+    size_t most = 0, total = 0;
+    for (int k = 0; k < nb; ++k) {
+        if (!statmc_pairs_radius_ok(descs[k].fc.radius)) return RPD_ERR_UNSUPPORTED;
+        bytes[k] = (size_t(descs[k].N) * statmc_forward_count(descs[k].fc.radius) * sizeof(double) + 255) / 256 * 256;
+        most = std::max(most, bytes[k]);
+        total += bytes[k];
+    }
+    const size_t want = std::min(total, std::max(most, STATMC_PAIRS_WORKSPACE));
+    if (want > STATMC_PAIRS_WORKSPACE) return RPD_ERR_UNSUPPORTED;
+    if (ctx->pw_bytes < want) {
+        HIP_TRY(hipStreamSynchronize(s));   // earlier groups may still read the old workspace
+        if (ctx->pw) HIP_TRY(dev_free(ctx->pw));
+        ctx->pw = nullptr; ctx->pw_bytes = 0;
+        if (dev_malloc(&ctx->pw, want) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->pw = nullptr;
+            return RPD_ERR_UNSUPPORTED;
+        }
+        ctx->pw_bytes = want;
+    }
+    if (nb > ctx->p_cap) {
+        HIP_TRY(hipStreamSynchronize(s));
+        if (ctx->p_img) HIP_TRY(dev_free(ctx->p_img));
+        if (ctx->p_prefix) HIP_TRY(dev_free(ctx->p_prefix));
+        ctx->p_img = nullptr; ctx->p_prefix = nullptr; ctx->p_cap = 0;
+        HIP_TRY(dev_malloc(&ctx->p_img, sizeof(ImageDesc) * nb));
+        HIP_TRY(dev_malloc(&ctx->p_prefix, sizeof(int64_t) * BK_KINDS * (nb + 1)));
+        ctx->p_cap = nb;
+    }
+    for (int g0 = 0; g0 < nb;) {
+        int g1 = g0;
+        size_t used = 0;
+        while (g1 < nb && used + bytes[g1] <= ctx->pw_bytes) used += bytes[g1++];
+        const int ng = g1 - g0;
+        keep_d.emplace_back(descs.begin() + g0, descs.begin() + g1);
+        std::vector<ImageDesc>& gd = keep_d.back();
+        keep.emplace_back(size_t(BK_KINDS) * (ng + 1), 0);
+        std::vector<int64_t>& pf = keep.back();
+        Batch bg = base;
+        bg.tile_w = bg.tile_h = 16;
+        bg.max_radius = 0;
+        size_t off = 0;
+        for (int k = 0; k < ng; ++k) {
+            gd[k].pair_w = reinterpret_cast<double*>(reinterpret_cast<char*>(ctx->pw) + off);
+            off += bytes[g0 + k];
+            bg.max_radius = std::max(bg.max_radius, gd[k].fc.radius);
+        }
+        for (int kind = 0; kind < BK_KINDS; ++kind) {
+            for (int k = 0; k < ng; ++k)
+                pf[size_t(kind) * (ng + 1) + k + 1] =
+                    pf[size_t(kind) * (ng + 1) + k] + image_items(kind, gd[k].W, gd[k].H, gd[k].g, 16, 16);
+            bg.total[kind] = pf[size_t(kind) * (ng + 1) + ng];
+        }
+        HIP_TRY(hipMemcpyAsync(ctx->p_img, gd.data(), sizeof(ImageDesc) * ng, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(ctx->p_prefix, pf.data(), sizeof(int64_t) * pf.size(), hipMemcpyHostToDevice, s));
+        bg.img = ctx->p_img; bg.prefix = ctx->p_prefix; bg.n = ng;
+        bg.pairs = 1;
+        bg.width1 = bg.height1 = 0;
+        launch_batch_queues_reset(bg, s);
+        launch_statmc_window(bg, s);
+        HIP_TRY(hipGetLastError());
+        g0 = g1;
+    }
+    return RPD_OK;
+}
+
 // The denoise stages of `jobs` as one batch on the context's batch stream
 // (ctx->batch_mu held): with `records`, the fold of each job's loaded pass
 // and the grid gather first (main.cpp:707-717); then features, sensitivity
@@ -1985,7 +2072,10 @@ static int batch_denoise_stages(rpd_context* ctx, const std::vector<rpd_job*>& j
                                sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice, s));
         return RPD_OK;
     };
-    const int st = run_statmc_window(ctx, b, s, true, set_shape);
+    // the pair form where every image takes it and the workspace holds each table
+    std::vector<std::vector<ImageDesc>> group_descs;
+    int st = ctx->statmc_pairs ? batch_statmc_pairs(ctx, descs, b, s, tile_rows, group_descs) : RPD_ERR_UNSUPPORTED;
+    if (st == RPD_ERR_UNSUPPORTED) st = run_statmc_window(ctx, b, s, true, set_shape);
     if (st) return st;
     // the host-side descriptor and prefix rows are read by asynchronous copies
     HIP_TRY(hipStreamSynchronize(s));

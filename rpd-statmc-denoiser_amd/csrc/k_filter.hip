@@ -559,10 +559,13 @@ bool statmc_pairs_radius_ok(int r) {
 __global__ void __launch_bounds__(STATMC_SUM_TILE * STATMC_SUM_TILE) k_statmc_sum(Batch b) {
     __shared__ double s_sp[STATMC_SP_MAX + 1];
     __shared__ float4 s_c[STATMC_SUM_SPAN * STATMC_SUM_SPAN];   // neighbour's (P4.xyz, P3.w)
-    const ImageDesc& d = b.img[0];   // the pair form runs on one image
+    // a job's own image (width1 set: the pair pass may have used another tile
+    // shape), or a batch whose BK_TILE row counts 16 x 16 tiles
+    int64_t lt = blockIdx.x;
+    const ImageDesc& d = b.width1 > 0 ? b.img[0] : b.img[batch_find(b, BK_TILE, blockIdx.x, lt)];
     const int W = d.W, H = d.H, n = d.N, R = d.fc.radius;
     const int tiles_x = (W + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE;
-    const int x0 = int(blockIdx.x % tiles_x) * STATMC_SUM_TILE, y0 = int(blockIdx.x / tiles_x) * STATMC_SUM_TILE;
+    const int x0 = int(lt % tiles_x) * STATMC_SUM_TILE, y0 = int(lt / tiles_x) * STATMC_SUM_TILE;
     const float4* __restrict__ P = reinterpret_cast<const float4*>(d.filter_scratch + 4 * size_t(n));
     const float4* __restrict__ P3 = P + 3 * size_t(n);
     const float4* __restrict__ P4 = P + 4 * size_t(n);
@@ -669,9 +672,10 @@ static void launch_statmc_t(const Batch& b, hipStream_t st) {
         return;
     }
     hipLaunchKernelGGL((k_statmc_pairs<TW, TH>), dim3(unsigned(blocks)), dim3(TW * TH), stage ? lds : 0, st, b, stage);
-    // the sum pass: one block per 16 x 16 tile of the batch's one image
-    const int64_t sum_tiles = int64_t((b.width1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE) *
-                              ((b.height1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE);
+    // the sum pass: one block per 16 x 16 tile (of the job's image, or of the batch)
+    const int64_t sum_tiles = b.width1 > 0 ? int64_t((b.width1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE) *
+                                                 ((b.height1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE)
+                                           : b.total[BK_TILE];
     hipLaunchKernelGGL(k_statmc_sum, dim3(unsigned(sum_tiles)), dim3(STATMC_SUM_TILE * STATMC_SUM_TILE), 0, st, b);
 }
 

@@ -267,7 +267,8 @@ struct Batch {
     int max_radius;                  // host: largest StatMC window radius of the images
     int pairs;                       // host: StatMC as the symmetric pair and sum passes (a batch of one
                                      // image, which has pair_w)
-    int width1, height1;             // host: that image's size (the sum pass's grid)
+    int width1, height1;             // host: that image's size (the sum pass's grid); 0 for a batch
+                                     // whose BK_TILE row counts 16 x 16 tiles
 };
 // prefix rows of a batch of one: {0, items} per item kind
 struct BatchPrefix1 { int64_t v[BK_KINDS * 2]; };
