@@ -550,6 +550,7 @@ __global__ void __launch_bounds__(TW * TH) k_statmc_pairs(Batch b, int stage) {
 // after it the pixel's own row, and the weights of four neighbours are loaded
 // before any of them is added, in order.
 constexpr int STATMC_SUM_TILE = 16;
+constexpr int STATMC_SUM_BAND = 8;     // tile rows walked column by column
 constexpr int STATMC_SUM_MAX_R = 13;   // the pair form's largest radius (the table must fit the tapes)
 constexpr int STATMC_SUM_SPAN = STATMC_SUM_TILE + 2 * STATMC_SUM_MAX_R;
 bool statmc_pairs_radius_ok(int r) {
@@ -565,7 +566,25 @@ __global__ void __launch_bounds__(STATMC_SUM_TILE * STATMC_SUM_TILE) k_statmc_su
     const ImageDesc& d = b.width1 > 0 ? b.img[0] : b.img[batch_find(b, BK_TILE, blockIdx.x, lt)];
     const int W = d.W, H = d.H, n = d.N, R = d.fc.radius;
     const int tiles_x = (W + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE;
-    const int x0 = int(lt % tiles_x) * STATMC_SUM_TILE, y0 = int(lt / tiles_x) * STATMC_SUM_TILE;
+    int tx = int(lt % tiles_x), ty = int(lt / tiles_x);
+    if (b.width1 > 0) {
+        // A pixel reads the weights its neighbours above stored, so a tile's
+        // reads follow those of the tile above it. Blocks are dealt to the 8
+        // XCDs in turn (block b on XCD b % 8), so XCD x takes positions
+        // x * per .. of one order that walks bands of 8 tile rows column by
+        // column: the tile above runs one step earlier on the same XCD, whose
+        // L2 still holds its weights. The grid is 8 * per blocks.
+        const int tiles_y = (H + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE;
+        const int64_t T = int64_t(tiles_x) * tiles_y, per = (T + 7) / 8;
+        const int64_t m = int64_t(blockIdx.x % 8) * per + blockIdx.x / 8;
+        if (m >= T) return;
+        const int band = int(m / (int64_t(STATMC_SUM_BAND) * tiles_x));
+        const int rows = min(STATMC_SUM_BAND, tiles_y - band * STATMC_SUM_BAND);
+        const int64_t within = m - int64_t(band) * STATMC_SUM_BAND * tiles_x;
+        tx = int(within / rows);
+        ty = band * STATMC_SUM_BAND + int(within % rows);
+    }
+    const int x0 = tx * STATMC_SUM_TILE, y0 = ty * STATMC_SUM_TILE;
     const float4* __restrict__ P = reinterpret_cast<const float4*>(d.filter_scratch + 4 * size_t(n));
     const float4* __restrict__ P3 = P + 3 * size_t(n);
     const float4* __restrict__ P4 = P + 4 * size_t(n);
@@ -673,9 +692,9 @@ static void launch_statmc_t(const Batch& b, hipStream_t st) {
     }
     hipLaunchKernelGGL((k_statmc_pairs<TW, TH>), dim3(unsigned(blocks)), dim3(TW * TH), stage ? lds : 0, st, b, stage);
     // the sum pass: one block per 16 x 16 tile (of the job's image, or of the batch)
-    const int64_t sum_tiles = b.width1 > 0 ? int64_t((b.width1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE) *
-                                                 ((b.height1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE)
-                                           : b.total[BK_TILE];
+    const int64_t own_tiles = int64_t((b.width1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE) *
+                              ((b.height1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE);
+    const int64_t sum_tiles = b.width1 > 0 ? (own_tiles + 7) / 8 * 8 : b.total[BK_TILE];
     hipLaunchKernelGGL(k_statmc_sum, dim3(unsigned(sum_tiles)), dim3(STATMC_SUM_TILE * STATMC_SUM_TILE), 0, st, b);
 }
 
