@@ -560,30 +560,28 @@ bool statmc_pairs_radius_ok(int r) {
 __global__ void __launch_bounds__(STATMC_SUM_TILE * STATMC_SUM_TILE) k_statmc_sum(Batch b) {
     __shared__ double s_sp[STATMC_SP_MAX + 1];
     __shared__ float4 s_c[STATMC_SUM_SPAN * STATMC_SUM_SPAN];   // neighbour's (P4.xyz, P3.w)
-    // a job's own image (width1 set: the pair pass may have used another tile
-    // shape), or a batch whose BK_TILE row counts 16 x 16 tiles
-    int64_t lt = blockIdx.x;
-    const ImageDesc& d = b.width1 > 0 ? b.img[0] : b.img[batch_find(b, BK_TILE, blockIdx.x, lt)];
+    // The tiles: a job's own image (width1 set: the pair pass may have used
+    // another tile shape), or a batch whose BK_TILE row counts 16 x 16 tiles.
+    // A pixel reads the weights its neighbours above stored, so a tile's reads
+    // follow those of the tile above it. Blocks are dealt to the 8 XCDs in turn
+    // (block b on XCD b % 8), so XCD x takes positions x * per .. of one order
+    // that goes image by image and, in an image, walks bands of 8 tile rows
+    // column by column: the tile above runs one step earlier on the same XCD,
+    // whose L2 still holds its weights. The grid is 8 * per blocks.
+    const int64_t T = b.width1 > 0 ? int64_t((b.width1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE) *
+                                         ((b.height1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE)
+                                   : b.total[BK_TILE];
+    const int64_t per = (T + 7) / 8, m = int64_t(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (m >= T) return;
+    int64_t lt = m;
+    const ImageDesc& d = b.width1 > 0 ? b.img[0] : b.img[batch_find(b, BK_TILE, m, lt)];
     const int W = d.W, H = d.H, n = d.N, R = d.fc.radius;
     const int tiles_x = (W + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE;
-    int tx = int(lt % tiles_x), ty = int(lt / tiles_x);
-    if (b.width1 > 0) {
-        // A pixel reads the weights its neighbours above stored, so a tile's
-        // reads follow those of the tile above it. Blocks are dealt to the 8
-        // XCDs in turn (block b on XCD b % 8), so XCD x takes positions
-        // x * per .. of one order that walks bands of 8 tile rows column by
-        // column: the tile above runs one step earlier on the same XCD, whose
-        // L2 still holds its weights. The grid is 8 * per blocks.
-        const int tiles_y = (H + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE;
-        const int64_t T = int64_t(tiles_x) * tiles_y, per = (T + 7) / 8;
-        const int64_t m = int64_t(blockIdx.x % 8) * per + blockIdx.x / 8;
-        if (m >= T) return;
-        const int band = int(m / (int64_t(STATMC_SUM_BAND) * tiles_x));
-        const int rows = min(STATMC_SUM_BAND, tiles_y - band * STATMC_SUM_BAND);
-        const int64_t within = m - int64_t(band) * STATMC_SUM_BAND * tiles_x;
-        tx = int(within / rows);
-        ty = band * STATMC_SUM_BAND + int(within % rows);
-    }
+    const int tiles_y = (H + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE;
+    const int band = int(lt / (int64_t(STATMC_SUM_BAND) * tiles_x));
+    const int rows = min(STATMC_SUM_BAND, tiles_y - band * STATMC_SUM_BAND);
+    const int64_t within = lt - int64_t(band) * STATMC_SUM_BAND * tiles_x;
+    const int tx = int(within / rows), ty = band * STATMC_SUM_BAND + int(within % rows);
     const int x0 = tx * STATMC_SUM_TILE, y0 = ty * STATMC_SUM_TILE;
     const float4* __restrict__ P = reinterpret_cast<const float4*>(d.filter_scratch + 4 * size_t(n));
     const float4* __restrict__ P3 = P + 3 * size_t(n);
@@ -694,7 +692,7 @@ static void launch_statmc_t(const Batch& b, hipStream_t st) {
     // the sum pass: one block per 16 x 16 tile (of the job's image, or of the batch)
     const int64_t own_tiles = int64_t((b.width1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE) *
                               ((b.height1 + STATMC_SUM_TILE - 1) / STATMC_SUM_TILE);
-    const int64_t sum_tiles = b.width1 > 0 ? (own_tiles + 7) / 8 * 8 : b.total[BK_TILE];
+    const int64_t sum_tiles = ((b.width1 > 0 ? own_tiles : b.total[BK_TILE]) + 7) / 8 * 8;
     hipLaunchKernelGGL(k_statmc_sum, dim3(unsigned(sum_tiles)), dim3(STATMC_SUM_TILE * STATMC_SUM_TILE), 0, st, b);
 }
 
